@@ -1,0 +1,251 @@
+"""Python mirror of the reference's per-population interface over libcbrkc.so.
+
+The reference drives each cell type through
+    xx_initialize / xx_solve_initialize      (pkj.cu:240, pkj_solve.cu:286)
+    xx_set_current                           (pkj.cu:333)
+    xx_solve_update_v                        (pkj_solve.cu:395)
+    xx_output_file                           (pkj.cu:370)
+    xx_finalize / xx_solve_finalize          (pkj.cu:308, pkj_solve.cu:368)
+This module exposes the same steps on a `Population` object (and as
+reference-named functions for Purkinje cells).  Every call goes through the
+C-ABI declared in include/cbrkc.h; there is no fallback: if the HIP library
+is missing or no GPU is present, calls fail loudly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcbrkc.so")
+
+CB_OK = 0
+GR, GO, PKJ, IO = 0, 1, 2, 3
+BE, CN, RKC = 0, 1, 2
+ENGINE_GLOBAL_FP64, ENGINE_CELL_FP64, ENGINE_CELL_MIXED = 0, 1, 2
+F_VNEW, F_V, F_CA, F_REV, F_B, F_IEXT, F_Y, F_YN, F_FN = range(9)
+F_GATE0 = 100
+H_MAX = 0.125
+
+# every symbol include/cbrkc.h declares
+EXPORTS = [
+    "cb_config_default", "cb_pop_create", "cb_pop_destroy", "cb_pop_set_current", "cb_pop_advance",
+    "cb_pop_run", "cb_pop_read", "cb_pop_size", "cb_pop_stats", "cb_pop_controller",
+    "cb_synth_pkj_morph", "cb_philox4x32_10", "cb_last_error", "cb_pkj_initialize",
+    "cb_pkj_solve_update_v", "cb_pkj_set_current", "cb_pkj_output_file", "cb_pkj_finalize",
+]
+
+
+class CbError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("cell_type", ctypes.c_int),
+        ("solver", ctypes.c_int),
+        ("engine", ctypes.c_int),
+        ("nx", ctypes.c_int),
+        ("ny", ctypes.c_int),
+        ("morph_path", ctypes.c_char_p),
+        ("gj_path", ctypes.c_char_p),
+        ("atol", ctypes.c_double),
+        ("rtol", ctypes.c_double),
+        ("v0", ctypes.POINTER(ctypes.c_double)),
+        ("stim_comp", ctypes.c_int),
+        ("stim_t0", ctypes.c_double),
+        ("stim_t1", ctypes.c_double),
+        ("stim_amp", ctypes.POINTER(ctypes.c_double)),
+        ("poisson_on", ctypes.c_int),
+        ("poisson_rate_hz", ctypes.c_double),
+        ("poisson_g_mS", ctypes.c_double),
+        ("poisson_tau_ms", ctypes.c_double),
+        ("poisson_e_mV", ctypes.c_double),
+        ("poisson_seed", ctypes.c_uint64),
+        ("ion_interval_ms", ctypes.c_double),
+        ("device", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libcbrkc.so (built in-tree); raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CbError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        vp = ctypes.c_void_p
+        L.cb_last_error.restype = ctypes.c_char_p
+        L.cb_config_default.argtypes = [ctypes.POINTER(Config), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.cb_pop_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(vp)]
+        L.cb_pop_destroy.argtypes = [vp]
+        L.cb_pop_set_current.argtypes = [vp, ctypes.c_double]
+        L.cb_pop_advance.argtypes = [vp]
+        L.cb_pop_run.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp]
+        L.cb_pop_read.argtypes = [vp, ctypes.c_int, dp]
+        L.cb_pop_size.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.cb_pop_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+        L.cb_pop_controller.argtypes = [vp, dp, dp]
+        L.cb_synth_pkj_morph.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        L.cb_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.cb_philox4x32_10.restype = None
+        L.cb_pkj_initialize.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.cb_pkj_solve_update_v.argtypes = [vp]
+        L.cb_pkj_set_current.argtypes = [vp, ctypes.c_double]
+        L.cb_pkj_output_file.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p]
+        L.cb_pkj_finalize.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != CB_OK:
+        raise CbError(f"cbrkc error {rc}: {lib().cb_last_error().decode()}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def philox4x32_10(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    lib().cb_philox4x32_10(c.ctypes.data_as(u32p), k.ctypes.data_as(u32p), o.ctypes.data_as(u32p))
+    return o
+
+
+def synth_pkj_morph(path, seed, ncell, soma_r_um=10.0, lmin_um=5.0, lmax_um=20.0, d0_um=3.0, d1_um=0.5,
+                    dscale=None):
+    """Writes ncell seeded synthetic Purkinje trees (Pkj.txt format) to path."""
+    ds = None
+    if dscale is not None:
+        dscale = np.ascontiguousarray(dscale, dtype=np.float64)
+        ds = _dp(dscale)
+    _check(lib().cb_synth_pkj_morph(path.encode(), seed, ncell, soma_r_um, lmin_um, lmax_um, d0_um, d1_um, ds))
+
+
+class Population:
+    """One cell population with its solver (the reference's neuron_t + neuron_solve_t)."""
+
+    def __init__(self, cell_type=PKJ, nx=1, ny=1, solver=RKC, engine=ENGINE_GLOBAL_FP64, morph=None, gj=None,
+                 atol=None, rtol=None, v0=None, stim=None, poisson=None, ion_interval=None, device=0):
+        L = lib()
+        cfg = Config()
+        _check(L.cb_config_default(ctypes.byref(cfg), cell_type, solver, nx, ny))
+        cfg.engine = engine
+        self._keep = []
+        if morph:
+            cfg.morph_path = morph.encode()
+        if gj:
+            cfg.gj_path = gj.encode()
+        if atol:
+            cfg.atol = atol
+        if rtol:
+            cfg.rtol = rtol
+        if v0 is not None:
+            v0 = np.ascontiguousarray(v0, dtype=np.float64)
+            self._keep.append(v0)
+            cfg.v0 = _dp(v0)
+        if stim is not None:
+            comp, t0, t1, amp = stim
+            amp = np.ascontiguousarray(np.broadcast_to(np.asarray(amp, dtype=np.float64), (nx * ny,)))
+            self._keep.append(amp)
+            cfg.stim_comp, cfg.stim_t0, cfg.stim_t1 = comp, t0, t1
+            cfg.stim_amp = _dp(amp)
+        if poisson is not None:
+            cfg.poisson_on = 1
+            cfg.poisson_rate_hz = poisson.get("rate_hz", 5.0)
+            cfg.poisson_g_mS = poisson.get("g_mS", 1e-6)
+            cfg.poisson_tau_ms = poisson.get("tau_ms", 2.0)
+            cfg.poisson_e_mV = poisson.get("e_mV", 0.0)
+            cfg.poisson_seed = poisson.get("seed", 0)
+        if ion_interval:
+            cfg.ion_interval_ms = ion_interval
+        cfg.device = device
+        h = ctypes.c_void_p()
+        _check(L.cb_pop_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        n = ctypes.c_int()
+        nc = ctypes.c_int()
+        _check(L.cb_pop_size(h, ctypes.byref(n), ctypes.byref(nc)))
+        self.n, self.nc = n.value, nc.value
+        self.comp = self.nc // max(self.n, 1)
+        self.engine = engine
+
+    def close(self):
+        if getattr(self, "h", None):
+            _check(lib().cb_pop_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_current(self, t):
+        _check(lib().cb_pop_set_current(self.h, t))
+
+    def advance(self):
+        _check(lib().cb_pop_advance(self.h))
+
+    def run(self, nsteps, t0=0.0, rec_comp=1599, record=True):
+        out = np.zeros((nsteps, self.n)) if record else None
+        _check(lib().cb_pop_run(self.h, nsteps, t0, rec_comp, _dp(out) if record else None))
+        return out
+
+    def read(self, field=F_VNEW):
+        a = np.empty(self.nc)
+        _check(lib().cb_pop_read(self.h, field, _dp(a)))
+        return a
+
+    def stats(self):
+        s = (ctypes.c_int64 * 6)()
+        _check(lib().cb_pop_stats(self.h, s))
+        return dict(steps=s[0], rejects=s[1], stages=s[2], rho_iters=s[3], ion_updates=s[4], macro_steps=s[5])
+
+    def controller(self):
+        o = np.zeros(14)
+        hw = np.zeros(8)
+        _check(lib().cb_pop_controller(self.h, _dp(o), _dp(hw)))
+        return o, hw
+
+
+# ---- reference-named entry points (Purkinje) ----
+
+def pkj_initialize(nx, ny, solver="RKC"):
+    """pkj_initialize + pkj_solve_initialize (pkj.cu:240, pkj_solve.cu:286); reads ./Pkj.txt."""
+    h = ctypes.c_void_p()
+    _check(lib().cb_pkj_initialize(nx, ny, solver.encode(), ctypes.byref(h)))
+    p = Population.__new__(Population)
+    p.h = h
+    n = ctypes.c_int()
+    nc = ctypes.c_int()
+    _check(lib().cb_pop_size(h, ctypes.byref(n), ctypes.byref(nc)))
+    p.n, p.nc, p.comp, p.engine, p._keep = n.value, nc.value, nc.value // max(n.value, 1), ENGINE_GLOBAL_FP64, []
+    return p
+
+
+def pkj_set_current(p, t):
+    _check(lib().cb_pkj_set_current(p.h, t))
+
+
+def pkj_solve_update_v(p):
+    _check(lib().cb_pkj_solve_update_v(p.h))
+
+
+def pkj_output_file(p, memv, raster_prev, t, v_csv, raster_csv):
+    memv = np.ascontiguousarray(memv, dtype=np.float64)
+    _check(lib().cb_pkj_output_file(p.h, _dp(memv), _dp(raster_prev), t, v_csv.encode(), raster_csv.encode()))
+
+
+def pkj_finalize(p):
+    p.close()

@@ -1,0 +1,283 @@
+// extern "C" boundary of libcbrkc.so (declared in include/cbrkc.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/cbrkc.h"
+#include "engine.hpp"
+#include "morph.hpp"
+#include "philox.hpp"
+#include "pkj_model.hpp"
+
+using namespace cbrkc;
+
+struct cb_pop {
+  std::unique_ptr<Engine> eng;
+  int cell_type = CB_PKJ;
+  int solver = CB_SOLVER_RKC;
+  int engine = CB_ENGINE_GLOBAL_FP64;
+  int comp = 0;
+  int device = 0;
+};
+
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+template <typename F>
+int guarded(F &&f) {
+  try {
+    g_err.clear();
+    return f();
+  } catch (const Error &e) {
+    g_err = e.msg;
+    if (e.msg.rfind("HIP error", 0) == 0) return CB_E_HIP;
+    if (e.code > 0) return CB_E_SOLVER;
+    return CB_E_ARG;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return CB_E_ARG;
+  }
+}
+
+int solver_from_name(const char *s) {
+  if (!s) return -1;
+  if (std::strncmp(s, "BE", 2) == 0) return CB_SOLVER_BE;
+  if (std::strncmp(s, "CN", 2) == 0) return CB_SOLVER_CN;
+  if (std::strncmp(s, "RKC", 3) == 0) return CB_SOLVER_RKC;
+  return -1;
+}
+}  // namespace
+
+extern "C" {
+
+const char *cb_last_error(void) { return g_err.c_str(); }
+
+void cb_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  u32x4 r = philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
+  for (int i = 0; i < 4; i++) out[i] = r.v[i];
+}
+
+int cb_config_default(cb_pop_config *cfg, int cell_type, int solver, int nx, int ny) {
+  if (!cfg) return fail(CB_E_ARG, "null config");
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->cell_type = cell_type;
+  cfg->solver = solver;
+  cfg->engine = CB_ENGINE_GLOBAL_FP64;
+  cfg->nx = nx;
+  cfg->ny = ny;
+  cfg->atol = -1.0;
+  cfg->rtol = -1.0;
+  cfg->stim_comp = -1;
+  cfg->poisson_rate_hz = 5.0;
+  cfg->poisson_g_mS = 1e-6;
+  cfg->poisson_tau_ms = 2.0;
+  cfg->poisson_e_mV = 0.0;
+  return CB_OK;
+}
+
+int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
+  return guarded([&]() -> int {
+    if (!cfg || !out) return fail(CB_E_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->nx < 0 || cfg->ny < 0) return fail(CB_E_ARG, "negative grid size");
+    if (cfg->cell_type != CB_PKJ) return fail(CB_E_UNSUP, "cell type not provided by this build");
+    if (cfg->solver != CB_SOLVER_RKC) return fail(CB_E_UNSUP, "solver not provided by this build");
+    CB_HIP(hipSetDevice(cfg->device));
+    std::vector<MorphRow> rows;
+    std::string err;
+    const std::string mpath = cfg->morph_path ? cfg->morph_path : "./Pkj.txt";
+    if (!read_pkj_morph(mpath, pkj::COMP, rows, err)) return fail(CB_E_IO, err);
+    std::vector<int> gpre, gpost;
+    const int n = cfg->nx * cfg->ny;
+    if (n == 0) return fail(CB_E_ARG, "empty population");
+    bool use_gj = n > 1;
+    if (use_gj) {
+      const std::string gpath = cfg->gj_path ? cfg->gj_path : "./Pkj_GJ.csv";
+      if (!read_pkj_gj(gpath, gpre, gpost, err)) {
+        if (cfg->gj_path) return fail(CB_E_IO, err);
+        use_gj = false;   // the default file is optional
+      }
+    }
+    PkjHost host;
+    if (!build_pkj_host(rows, cfg->nx, cfg->ny, use_gj ? &gpre : nullptr, use_gj ? &gpost : nullptr, cfg->v0, host,
+                        err))
+      return fail(CB_E_IO, err);
+    StimSpec stim;
+    if (cfg->stim_comp >= 0) {
+      if (cfg->stim_comp >= pkj::COMP) return fail(CB_E_ARG, "stimulus compartment out of range");
+      stim.comp = cfg->stim_comp;
+      stim.t0 = cfg->stim_t0;
+      stim.t1 = cfg->stim_t1;
+    }
+    stim.amp.assign(n, pkj::CURRENT);
+    if (cfg->stim_amp)
+      for (int c = 0; c < n; c++) stim.amp[c] = cfg->stim_amp[c];
+    PoissonSpec syn;
+    syn.on = cfg->poisson_on != 0;
+    syn.rate_hz = cfg->poisson_rate_hz;
+    syn.g_event = cfg->poisson_g_mS;
+    syn.tau_ms = cfg->poisson_tau_ms;
+    syn.e_syn = cfg->poisson_e_mV;
+    syn.seed = cfg->poisson_seed;
+    const double tol = 1.0e-6;   // TOL_PKJ, solve_rkc.cuh:29
+    SolverSpec sol;
+    sol.atol = cfg->atol > 0 ? cfg->atol : tol;
+    sol.rtol = cfg->rtol > 0 ? cfg->rtol : tol;
+    sol.ion_interval = cfg->ion_interval_ms;
+    std::unique_ptr<cb_pop> p(new cb_pop);
+    p->cell_type = cfg->cell_type;
+    p->solver = cfg->solver;
+    p->engine = cfg->engine;
+    p->comp = pkj::COMP;
+    p->device = cfg->device;
+    switch (cfg->engine) {
+      case CB_ENGINE_GLOBAL_FP64:
+        p->eng.reset(make_global_pkj(host, sol, stim, syn));
+        break;
+      case CB_ENGINE_CELL_FP64:
+      case CB_ENGINE_CELL_MIXED:
+        if (host.n_gj > 0) return fail(CB_E_UNSUP, "per-cell engines need cells without intercell gap junctions");
+        p->eng.reset(make_cell_pkj(host, sol, stim, syn, cfg->engine == CB_ENGINE_CELL_MIXED));
+        break;
+      default:
+        return fail(CB_E_ARG, "unknown engine");
+    }
+    *out = p.release();
+    return CB_OK;
+  });
+}
+
+int cb_pop_destroy(cb_pop *p) {
+  return guarded([&]() -> int {
+    if (!p) return CB_OK;
+    CB_HIP(hipSetDevice(p->device));
+    delete p;
+    return CB_OK;
+  });
+}
+
+int cb_pop_set_current(cb_pop *p, double t) {
+  return guarded([&]() -> int {
+    if (!p) return fail(CB_E_ARG, "null population");
+    p->eng->set_current(t);
+    return CB_OK;
+  });
+}
+
+int cb_pop_advance(cb_pop *p) {
+  return guarded([&]() -> int {
+    if (!p) return fail(CB_E_ARG, "null population");
+    p->eng->advance();
+    return CB_OK;
+  });
+}
+
+int cb_pop_run(cb_pop *p, int nsteps, double t0, int rec_comp, double *rec_out) {
+  return guarded([&]() -> int {
+    if (!p) return fail(CB_E_ARG, "null population");
+    if (nsteps < 0) return fail(CB_E_ARG, "negative step count");
+    if (rec_out && (rec_comp < 0 || rec_comp >= p->comp)) return fail(CB_E_ARG, "record compartment out of range");
+    p->eng->run(nsteps, t0, rec_comp, rec_out);
+    return CB_OK;
+  });
+}
+
+int cb_pop_read(cb_pop *p, int field, double *dst) {
+  return guarded([&]() -> int {
+    if (!p || !dst) return fail(CB_E_ARG, "null argument");
+    p->eng->read(field, dst);
+    return CB_OK;
+  });
+}
+
+int cb_pop_size(const cb_pop *p, int *ncell, int *nc) {
+  if (!p) return fail(CB_E_ARG, "null population");
+  if (ncell) *ncell = p->eng->ncell();
+  if (nc) *nc = p->eng->nc();
+  return CB_OK;
+}
+
+int cb_pop_stats(const cb_pop *p, int64_t *out6) {
+  if (!p || !out6) return fail(CB_E_ARG, "null argument");
+  Stats s = p->eng->stats();
+  out6[0] = s.steps;
+  out6[1] = s.rejects;
+  out6[2] = s.stages;
+  out6[3] = s.rho_iters;
+  out6[4] = s.ion_updates;
+  out6[5] = s.macro_steps;
+  return CB_OK;
+}
+
+int cb_pop_controller(const cb_pop *p, double *o, double *hw) {
+  if (!p || !o || !hw) return fail(CB_E_ARG, "null argument");
+  if (p->engine != CB_ENGINE_GLOBAL_FP64) return fail(CB_E_UNSUP, "per-cell engines have no global controller");
+  p->eng->controller(o, hw);
+  return CB_OK;
+}
+
+int cb_synth_pkj_morph(const char *path, uint64_t seed, int ncell, double soma_r_um, double lmin_um, double lmax_um,
+                       double d0_um, double d1_um, const double *dscale) {
+  return guarded([&]() -> int {
+    if (!path || ncell <= 0) return fail(CB_E_ARG, "bad argument");
+    if (!(lmin_um > 0) || lmax_um < lmin_um || !(d1_um > 0) || !(d0_um > 0) || !(soma_r_um > 0))
+      return fail(CB_E_ARG, "bad geometry");
+    std::vector<MorphRow> all, one;
+    all.reserve((size_t)ncell * pkj::COMP);
+    for (int c = 0; c < ncell; c++) {
+      synth_pkj_tree(seed + (uint64_t)c, pkj::COMP, soma_r_um, lmin_um, lmax_um, d0_um, d1_um, dscale ? dscale[c] : 1.0,
+                     one);
+      all.insert(all.end(), one.begin(), one.end());
+    }
+    std::string err;
+    if (!write_pkj_morph(path, all, err)) return fail(CB_E_IO, err);
+    return CB_OK;
+  });
+}
+
+int cb_pkj_initialize(int nx, int ny, const char *solver, cb_pop **out) {
+  int s = solver_from_name(solver);
+  if (s < 0) return fail(CB_E_ARG, "error in pkj_initialize: unknown solver");
+  cb_pop_config cfg;
+  cb_config_default(&cfg, CB_PKJ, s, nx, ny);
+  return cb_pop_create(&cfg, out);
+}
+
+int cb_pkj_solve_update_v(cb_pop *p) { return cb_pop_advance(p); }
+int cb_pkj_set_current(cb_pop *p, double t) { return cb_pop_set_current(p, t); }
+int cb_pkj_finalize(cb_pop *p) { return cb_pop_destroy(p); }
+
+int cb_pkj_output_file(const cb_pop *p, const double *memv, double *raster_prev, double t, const char *v_csv,
+                       const char *raster_csv) {
+  if (!p || !memv || !raster_prev) return fail(CB_E_ARG, "null argument");
+  const int n = p->eng->ncell(), C = p->comp;
+  if (v_csv) {
+    FILE *f = std::fopen(v_csv, "a");
+    if (!f) return fail(CB_E_IO, std::string("cannot open ") + v_csv);
+    std::fprintf(f, "%lf,", t);
+    for (int j = 0; j < n; j++) std::fprintf(f, "%lf,", memv[1599 + (size_t)j * C]);
+    std::fprintf(f, "\n");
+    std::fclose(f);
+  }
+  FILE *fr = raster_csv ? std::fopen(raster_csv, "a") : nullptr;
+  if (raster_csv && !fr) return fail(CB_E_IO, std::string("cannot open ") + raster_csv);
+  for (int j = 0; j < n; j++) {
+    double vj = memv[1599 + (size_t)j * C];
+    if (vj > -10.0 && raster_prev[j] <= -10.0 && fr) std::fprintf(fr, "%lf,%d\n", t, j);
+    raster_prev[j] = vj;
+  }
+  if (fr) std::fclose(fr);
+  return CB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,85 @@
+// Engine interface behind the C-ABI population handle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "morph.hpp"
+
+#define CB_HIP(call)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      throw cbrkc::Error(std::string("HIP error ") + hipGetErrorString(e_) + " at " +   \
+                         __FILE__ + ":" + std::to_string(__LINE__));                      \
+    }                                                                                     \
+  } while (0)
+
+namespace cbrkc {
+
+struct Error {
+  std::string msg;
+  int code = -1;
+  explicit Error(std::string m, int c = -1) : msg(std::move(m)), code(c) {}
+};
+
+// field ids shared with the C-ABI (cb_pop_read)
+enum Field {
+  F_VNEW = 0,     // interpolated V at the end of the last macro step (vec[v_new])
+  F_V = 1,        // V at the last ion update (elem[v])
+  F_CA = 2,
+  F_REV = 3,
+  F_B = 4,
+  F_IEXT = 5,
+  F_Y = 6,        // RKC vec[y]
+  F_YN = 7,       // RKC vec[yn_r]
+  F_FN = 8,       // RKC vec[fn]
+  F_GATE0 = 100,  // + gate index
+};
+
+struct Stats {
+  long steps = 0, rejects = 0, stages = 0, rho_iters = 0, ion_updates = 0, macro_steps = 0;
+};
+
+struct StimSpec {
+  int comp = 1599;        // compartment (local index) receiving the step current
+  double t0 = 550.0, t1 = 1300.0;
+  std::vector<double> amp;  // per cell, uA
+};
+
+// Poisson conductance drive on every compartment: rate_hz events per second,
+// each adding g_event (mS) that decays with tau_ms; reversal e_syn (mV).
+struct PoissonSpec {
+  bool on = false;
+  double rate_hz = 5.0, g_event = 1e-6, tau_ms = 2.0, e_syn = 0.0;
+  unsigned long long seed = 0;
+};
+
+class Engine {
+ public:
+  virtual ~Engine() {}
+  virtual void set_current(double t) = 0;      // xx_set_current
+  virtual void advance() = 0;                  // one H_MAX macro step (xx_solve_by_rkc)
+  virtual void read(int field, double *dst) = 0;  // nc values
+  virtual Stats stats() const = 0;
+  virtual int nc() const = 0;
+  virtual int ncell() const = 0;
+  virtual void controller(double *others14, double *hwork8) const { (void)others14; (void)hwork8; }
+  virtual void sync() {}
+  // many macro steps without host round trips; soma V and spikes are
+  // buffered on the device.  Default: loop over advance().
+  virtual void run(int nsteps, double t0, int rec_comp, double *rec_out) ;
+};
+
+// Tolerances and the operator-splitting interval of one population.
+struct SolverSpec {
+  double atol = 1e-6, rtol = 1e-6;
+  double ion_interval = -1.0;   // <= 0: 2^log10(atol) (solve_rkc.cu:1265)
+};
+
+Engine *make_global_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);
+Engine *make_cell_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn,
+                      bool mixed);
+
+}  // namespace cbrkc

@@ -1,0 +1,703 @@
+// Plain FP64 RKC with one population-global controller: the reference's
+// semantics (solve_rkc.cu) on MI355X.
+//
+// One thread per compartment, structure-of-arrays FP64 state in HBM.  The
+// differences from the reference are in how the work is laid out, not in
+// what is computed:
+//   - the RHS (dydt) and the stage combination (calcY) are one kernel, so a
+//     stage reads each vector once (solve_rkc.cu:857-900 uses two launches);
+//   - the error map, f(Y_m) and the first reduction level are one kernel;
+//   - every norm is a deterministic 256-wide block tree (no atomics, no
+//     library dot products), so runs are bit-reproducible;
+//   - the ion update and the frozen-current rebuild are one kernel;
+//   - the host controller reads back one double per step (the error norm)
+//     and one per power-method iteration, nothing else.
+// The controller (rkclow/step/rkcrho) is restated on the host, in the
+// reference's scalar arithmetic order, with the same quirks (see the oracle
+// header for the list).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "detsum.hpp"
+#include "engine.hpp"
+#include "philox.hpp"
+#include "pkj_model.hpp"
+
+namespace cbrkc {
+
+__global__ void __launch_bounds__(256) k_reduce256(const double *__restrict__ in, double *__restrict__ out, int n) {
+  __shared__ double s[256];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double x = (i < n) ? in[i] : 0.0;
+  double r = block_tree_sum(x, s);
+  if (threadIdx.x == 0) out[blockIdx.x] = r;
+}
+
+namespace {
+
+constexpr double UROUND = 2.22e-16;   // solve_rkc.cuh:22
+constexpr double H_MAX = 0.125;       // solve_rkc.cuh:23
+
+__device__ inline double rhs(int i, const int *__restrict__ row, const int *__restrict__ col,
+                             const double *__restrict__ val, const double *__restrict__ b,
+                             const double *__restrict__ cm, const double *__restrict__ y) {
+  // (-b_i + sum_j val_j (y_col_j - y_i)) / Cm_i in CSR column order (solve_rkc.cu:177-196)
+  double d = -b[i];
+  const double yi = y[i];
+  const int e = row[i + 1];
+  for (int j = row[i]; j < e; j++) d += val[j] * (y[col[j]] - yi);
+  return d / cm[i];
+}
+
+struct Csr {
+  const int *row, *col;
+  const double *val;
+};
+
+// vt2 = yn; vt1 = (h mus) fn + yn   (solve_rkc.cu:843-847)
+__global__ void __launch_bounds__(256) k_first(const double *__restrict__ fn, const double *__restrict__ yn,
+                                               double *__restrict__ vt1, double *__restrict__ vt2, double hmus, int nc) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nc) {
+    double y0 = yn[i];
+    vt2[i] = y0;
+    vt1[i] = hmus * fn[i] + y0;
+  }
+}
+
+// Y_j = mu Y_{j-1} + nu Y_{j-2} + (1-mu-nu) Y_0 + h mus (F(Y_{j-1}) - a_{j-1} F_0)
+__global__ void __launch_bounds__(256) k_stage(Csr A, const double *__restrict__ b, const double *__restrict__ cm,
+                                               const double *__restrict__ vt1, const double *__restrict__ vt2,
+                                               const double *__restrict__ yn, const double *__restrict__ fn,
+                                               double *__restrict__ y, double mu, double nu, double omn, double hmus,
+                                               double ajm1, int nc) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nc) {
+    double F = rhs(i, A.row, A.col, A.val, b, cm, vt1);
+    y[i] = mu * vt1[i] + nu * vt2[i] + omn * yn[i] + hmus * (F - ajm1 * fn[i]);
+  }
+}
+
+// vt1 = F(y); partial[block] = sum ((0.8 (yn - y) + 0.4 h (fn + vt1)) / wt)^2
+// (solve_rkc.cu:979-982, 58-73)
+__global__ void __launch_bounds__(256) k_err(Csr A, const double *__restrict__ b, const double *__restrict__ cm,
+                                             const double *__restrict__ y, const double *__restrict__ yn,
+                                             const double *__restrict__ fn, double *__restrict__ vt1, double at,
+                                             double rt, double h, double *__restrict__ partial, int *__restrict__ bad,
+                                             int nc) {
+  __shared__ double s[256];
+  int i = blockIdx.x * 256 + threadIdx.x;
+  double term = 0.0;
+  if (i < nc) {
+    double F = rhs(i, A.row, A.col, A.val, b, cm, y);
+    vt1[i] = F;
+    double yi = y[i], yni = yn[i];
+    double ay = fabs(yi), ayn = fabs(yni);
+    double wt = at + rt * ((ay > ayn) ? ay : ayn);
+    double est = 0.8 * (yni - yi) + 0.4 * h * (fn[i] + F);
+    if (wt == 0.0) *bad = 1;
+    term = (est / wt) * (est / wt);
+  }
+  double r = block_tree_sum(term, s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+// partial[block] = sum x^2
+__global__ void __launch_bounds__(256) k_sumsq(const double *__restrict__ x, double *__restrict__ partial, int nc) {
+  __shared__ double s[256];
+  int i = blockIdx.x * 256 + threadIdx.x;
+  double t = 0.0;
+  if (i < nc) {
+    double v = x[i];
+    t = v * v;
+  }
+  double r = block_tree_sum(t, s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+// C = s A + B (vec_sA_plus_B, solve_rkc.cu:143-148)
+__global__ void __launch_bounds__(256) k_saxpb(double s, const double *__restrict__ A, const double *__restrict__ B,
+                                               double *__restrict__ C, int nc) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nc) C[i] = s * A[i] + B[i];
+}
+
+__global__ void __launch_bounds__(256) k_scale(double s, const double *__restrict__ A, double *__restrict__ C, int nc) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nc) C[i] = s * A[i];
+}
+
+__global__ void __launch_bounds__(256) k_fill(double s, double *__restrict__ C, int nc) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nc) C[i] = s;
+}
+
+// one power-method iteration: vt2 = F(vt1); dammy = -fn + vt2;
+// partial = sum dammy^2 (solve_rkc.cu:645-661)
+__global__ void __launch_bounds__(256) k_rho_iter(Csr A, const double *__restrict__ b, const double *__restrict__ cm,
+                                                  const double *__restrict__ vt1, const double *__restrict__ fn,
+                                                  double *__restrict__ vt2, double *__restrict__ dammy,
+                                                  double *__restrict__ partial, int nc) {
+  __shared__ double s[256];
+  int i = blockIdx.x * 256 + threadIdx.x;
+  double t = 0.0;
+  if (i < nc) {
+    double F = rhs(i, A.row, A.col, A.val, b, cm, vt1);
+    vt2[i] = F;
+    double d = -1.0 * fn[i] + F;
+    dammy[i] = d;
+    t = d * d;
+  }
+  double r = block_tree_sum(t, s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+// vt1 = alpha (-vt2 + fn) + yn  (the two vec_sA_plus_B of solve_rkc.cu:725-726)
+__global__ void __launch_bounds__(256) k_rho_next(double alpha, const double *__restrict__ vt2,
+                                                  const double *__restrict__ fn, const double *__restrict__ yn,
+                                                  double *__restrict__ vt1, int nc) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nc) {
+    double t = -1.0 * vt2[i] + fn[i];
+    vt1[i] = alpha * t + yn[i];
+  }
+}
+
+__global__ void k_flip1(double *__restrict__ v, const double *__restrict__ y, int id) {
+  v[id] = 2.0 * y[id] - v[id];   // rand_elements_change, solve_rkc.cu:22-27
+}
+
+// set_initial_absh weights: dammy = (vt2 - fn) / (at + rt |yn|) as the
+// reference's host loop + Hadamard_product (solve_rkc.cu:525-543)
+__global__ void __launch_bounds__(256) k_absh_w(const double *__restrict__ vt2, const double *__restrict__ fn,
+                                                const double *__restrict__ yn, double at, double rt,
+                                                double *__restrict__ dammy, int *__restrict__ bad, int nc) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nc) {
+    double wt = at + rt * fabs(yn[i]);
+    if (wt == 0.0) *bad = 1;
+    double w = 1.0 / wt;
+    dammy[i] = (vt2[i] - fn[i]) * w;
+  }
+}
+
+// rkcint, solve_rkc.cu:794-813
+__global__ void __launch_bounds__(256) k_rkcint(double *__restrict__ vnew, double a1, double a2, double b1, double b2,
+                                                const double *__restrict__ vt1, const double *__restrict__ vt2,
+                                                const double *__restrict__ yn, const double *__restrict__ fn, int nc) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nc) vnew[i] = a1 * vt1[i] + a2 * yn[i] + b1 * vt2[i] + b2 * fn[i];
+}
+
+struct PkjState {
+  double *v, *ca, *rev, *iext, *shell;
+  const double *cond;   // [NCOND][nc]
+  double *gate;         // [NGATE][nc]
+  double *b;
+  const double *syn_g;  // may be null
+  double syn_e;
+  int nc;
+};
+
+// pkj_update_ion_RKC (pkj_ion.cu:304-358) followed by pkj_update_matrix
+// (solve_rkc.cu:369-395), fused: one pass over the compartment state.
+// With do_ion == 0 only the current is rebuilt (RKC_vec_initialize).
+__global__ void __launch_bounds__(256) k_pkj_ion_b(PkjState S, const double *__restrict__ vnew, double DT, int do_ion) {
+  using namespace pkj;
+  const int nc = S.nc;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nc) return;
+  double g[NGATE], c[NCOND];
+#pragma unroll
+  for (int k = 0; k < NGATE; k++) g[k] = S.gate[(size_t)k * nc + i];
+#pragma unroll
+  for (int k = 0; k < NCOND; k++) c[k] = S.cond[(size_t)k * nc + i];
+  const double vn = vnew[i];
+  double rev = S.rev[i];
+  if (do_ion) {
+    const double vo = S.v[i];
+    const double vm = 0.5 * (vn + vo);
+    double ca = S.ca[i];
+    double caq = ca_rk4(vo, ca, rev, S.shell[i], c[g_CaP] * g[m_CaP] * g[h_CaP], c[g_CaT] * g[m_CaT] * g[h_CaT], DT, i);
+    gates_cn<double>(vm, caq, DT, g);
+#pragma unroll
+    for (int k = 0; k < NGATE; k++) S.gate[(size_t)k * nc + i] = g[k];
+    S.ca[i] = ca;
+    S.rev[i] = rev;
+    S.v[i] = vn;
+  }
+  double bb = current<double>(vn, S.iext[i], rev, c, g);
+  if (S.syn_g) bb += S.syn_g[i] * (vn - S.syn_e);
+  S.b[i] = bb;
+}
+
+// xx_set_current shape: one compartment per cell, a time window (pkj.cu:333-344)
+__global__ void k_set_current(double *__restrict__ iext, const double *__restrict__ amp, int comp, int stim_comp,
+                              double t0, double t1, double t, int n) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) {
+    double *x = &iext[stim_comp + (size_t)c * comp];
+    *x = 0;
+    if (t0 <= t && t < t1) *x = amp[c];
+  }
+}
+
+// Poisson conductance drive, advanced once per macro step k:
+//   g <- g exp(-H/tau) + g_event [u < p],  u from Philox(ctr = (i, k, 0x5359, 0), key = seed)
+__global__ void __launch_bounds__(256) k_poisson(double *__restrict__ g, double decay, double p, double g_event,
+                                                 uint32_t k0, uint32_t k1, uint32_t step, int nc) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nc) {
+    u32x4 r = philox4x32_10((uint32_t)i, step, 0x5359u, 0u, k0, k1);
+    double u = u01_53(r.v[0], r.v[1]);
+    g[i] = g[i] * decay + ((u < p) ? g_event : 0.0);
+  }
+}
+
+inline int nblk(int n) { return (n + 255) / 256; }
+
+template <typename T>
+T *dalloc(size_t n) {
+  void *p = nullptr;
+  CB_HIP(hipMalloc(&p, sizeof(T) * (n ? n : 1)));
+  return (T *)p;
+}
+
+template <typename T>
+void h2d(T *d, const std::vector<T> &h) {
+  CB_HIP(hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+}
+
+double fmax_ori(double a, double b) { return (a > b) ? a : b; }
+int nint_ori(double x) {
+  if (x - (double)(int)x < 0.5) return (int)(double)(int)x;
+  return (int)(double)(int)(x + 0.9);
+}
+
+enum { Y = 0, YN, FN, VT1, VT2, EDE, VNEW, NVEC };
+enum { MMAX = 0, ABSH, SPRAD, NACCPT, ATOL, RTOL, IDID, GT, GT_OLD, LT, TEND, ERROLD, HOLD, NSTSIG, NOTHERS };
+
+class GlobalPkj : public Engine {
+ public:
+  GlobalPkj(const PkjHost &H, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn)
+      : n_(H.n), nc_(H.nc), comp_(H.comp), stim_(stim), syn_(syn) {
+    using namespace pkj;
+    CB_HIP(hipStreamCreate(&st_));
+    v_ = dalloc<double>(nc_); h2d(v_, H.v);
+    ca_ = dalloc<double>(nc_); h2d(ca_, H.ca);
+    rev_ = dalloc<double>(nc_); h2d(rev_, H.rev);
+    iext_ = dalloc<double>(nc_); h2d(iext_, H.iext);
+    shell_ = dalloc<double>(nc_); h2d(shell_, H.shell);
+    cm_ = dalloc<double>(nc_); h2d(cm_, H.cm);
+    cond_ = dalloc<double>((size_t)NCOND * nc_); h2d(cond_, H.cond);
+    gate_ = dalloc<double>((size_t)NGATE * nc_); h2d(gate_, H.gate);
+    b_ = dalloc<double>(nc_);
+    dammy_ = dalloc<double>(nc_);
+    row_ = dalloc<int>(nc_ + 1); h2d(row_, H.row);
+    col_ = dalloc<int>(H.col.size()); h2d(col_, H.col);
+    val_ = dalloc<double>(H.val.size()); h2d(val_, H.val);
+    for (int k = 0; k < NVEC; k++) vec_[k] = dalloc<double>(nc_);
+    red_a_ = dalloc<double>(nblk(nc_) + 1);
+    red_b_ = dalloc<double>(nblk(nc_) + 1);
+    bad_ = dalloc<int>(1);
+    CB_HIP(hipMemset(bad_, 0, sizeof(int)));
+    CB_HIP(hipHostMalloc((void **)&hres_, sizeof(double) * 2, hipHostMallocDefault));
+    if (stim_.amp.size() != (size_t)n_) stim_.amp.assign(n_, pkj::CURRENT);
+    amp_ = dalloc<double>(n_); h2d(amp_, stim_.amp);
+    if (syn_.on) {
+      syn_g_ = dalloc<double>(nc_);
+      CB_HIP(hipMemset(syn_g_, 0, sizeof(double) * nc_));
+    }
+    hmax_ = H_MAX;
+    hmin_ = std::pow(2.0, -10.0);   // H_MIN_PKJ, solve_rkc.cuh:26
+    init(sol.atol, sol.rtol);
+    if (sol.ion_interval > 0) integ_ = sol.ion_interval;
+  }
+
+  ~GlobalPkj() override {
+    (void)hipStreamSynchronize(st_);
+    for (void *p : {(void *)v_, (void *)ca_, (void *)rev_, (void *)iext_, (void *)shell_, (void *)cm_, (void *)cond_,
+                    (void *)gate_, (void *)b_, (void *)dammy_, (void *)row_, (void *)col_, (void *)val_,
+                    (void *)red_a_, (void *)red_b_, (void *)bad_, (void *)amp_, (void *)syn_g_, (void *)zero_})
+      if (p) (void)hipFree(p);
+    for (int k = 0; k < NVEC; k++) (void)hipFree(vec_[k]);
+    (void)hipHostFree(hres_);
+    (void)hipStreamDestroy(st_);
+  }
+
+  int nc() const override { return nc_; }
+  int ncell() const override { return n_; }
+  Stats stats() const override { return st; }
+  void controller(double *o, double *hw) const override {
+    std::memcpy(o, o_, sizeof(o_));
+    std::memcpy(hw, hw_, sizeof(hw_));
+  }
+  void sync() override { CB_HIP(hipStreamSynchronize(st_)); }
+
+  void set_current(double t) override {
+    k_set_current<<<(n_ + 127) / 128, 128, 0, st_>>>(iext_, amp_, comp_, stim_.comp, stim_.t0, stim_.t1, t, n_);
+  }
+
+  void advance() override {
+    if (syn_.on) {
+      double lam = syn_.rate_hz * H_MAX * 1e-3;
+      double p = 1.0 - std::exp(-lam);
+      k_poisson<<<nblk(nc_), 256, 0, st_>>>(syn_g_, std::exp(-H_MAX / syn_.tau_ms), p, syn_.g_event,
+                                            (uint32_t)syn_.seed, (uint32_t)(syn_.seed >> 32),
+                                            (uint32_t)st.macro_steps, nc_);
+    }
+    if (sum_dt_ != 0.0) {
+      ion_and_b(sum_dt_);
+      sum_dt_ = 0.0;
+    }
+    while (true) {
+      rkclow();
+      sum_dt_ += o_[GT] - o_[GT_OLD];
+      o_[GT_OLD] = o_[GT];
+      if (sum_dt_ >= integ_ && o_[IDID] == 2.0) {   // '>=' for PKJ, solve_rkc.cu:1284
+        ion_and_b(sum_dt_);
+        sum_dt_ = 0.0;
+      }
+      if (o_[IDID] == 2.0) continue;
+      if (o_[IDID] == 1.0) break;
+      throw Error("RKC failed: idid = " + std::to_string((int)o_[IDID]), (int)o_[IDID]);
+    }
+    // rkcint at gt_old (solve_rkc.cu:794-813)
+    double hlast = hw_[0], lt = hw_[1], gtv = o_[GT_OLD];
+    double tlast = lt - hlast;
+    double s = (gtv - tlast) / hlast;
+    double a1 = (1.0 + 2.0 * s) * (s - 1.0) * (s - 1.0);
+    double a2 = (3.0 - 2.0 * s) * s * s;
+    double b1 = hlast * s * (s - 1.0) * (s - 1.0);
+    double b2 = hlast * (s - 1.0) * s * s;
+    k_rkcint<<<nblk(nc_), 256, 0, st_>>>(vec_[VNEW], a1, a2, b1, b2, vec_[VT1], vec_[VT2], vec_[YN], vec_[FN], nc_);
+    o_[TEND] += H_MAX;
+    st.macro_steps++;
+  }
+
+  void read(int field, double *dst) override {
+    const double *src = nullptr;
+    switch (field) {
+      case F_VNEW: src = vec_[VNEW]; break;
+      case F_V: src = v_; break;
+      case F_CA: src = ca_; break;
+      case F_REV: src = rev_; break;
+      case F_B: src = b_; break;
+      case F_IEXT: src = iext_; break;
+      case F_Y: src = vec_[Y]; break;
+      case F_YN: src = vec_[YN]; break;
+      case F_FN: src = vec_[FN]; break;
+      default:
+        if (field >= F_GATE0 && field < F_GATE0 + pkj::NGATE) src = gate_ + (size_t)(field - F_GATE0) * nc_;
+    }
+    if (!src) throw Error("unknown field " + std::to_string(field));
+    CB_HIP(hipMemcpyAsync(dst, src, sizeof(double) * nc_, hipMemcpyDeviceToHost, st_));
+    CB_HIP(hipStreamSynchronize(st_));
+  }
+
+ private:
+  Csr csr() const { return Csr{row_, col_, val_}; }
+
+  void ion_and_b(double dt) {
+    st.ion_updates++;
+    PkjState S{v_, ca_, rev_, iext_, shell_, cond_, gate_, b_, syn_g_, syn_.e_syn, nc_};
+    k_pkj_ion_b<<<nblk(nc_), 256, 0, st_>>>(S, vec_[Y], dt, 1);
+  }
+
+  // deterministic total of the per-block partials left in red_a_
+  double finish_sum(int nparts) {
+    double *in = red_a_, *out = red_b_;
+    while (nparts > 1) {
+      int nb = nblk(nparts);
+      k_reduce256<<<nb, 256, 0, st_>>>(in, out, nparts);
+      std::swap(in, out);
+      nparts = nb;
+    }
+    CB_HIP(hipMemcpyAsync(hres_, in, sizeof(double), hipMemcpyDeviceToHost, st_));
+    CB_HIP(hipStreamSynchronize(st_));
+    return hres_[0];
+  }
+
+  double sumsq(const double *x) {
+    k_sumsq<<<nblk(nc_), 256, 0, st_>>>(x, red_a_, nc_);
+    return finish_sum(nblk(nc_));
+  }
+
+  void init(double atol, double rtol) {
+    // RKC_vec_initialize (solve_rkc.cu:756-791)
+    PkjState S{v_, ca_, rev_, iext_, shell_, cond_, gate_, b_, syn_g_, syn_.e_syn, nc_};
+    k_pkj_ion_b<<<nblk(nc_), 256, 0, st_>>>(S, v_, 0.0, 0);
+    // set_initial_vec (solve_rkc.cu:437-487)
+    CB_HIP(hipMemcpyAsync(vec_[VNEW], v_, sizeof(double) * nc_, hipMemcpyDeviceToDevice, st_));
+    CB_HIP(hipMemcpyAsync(vec_[Y], v_, sizeof(double) * nc_, hipMemcpyDeviceToDevice, st_));
+    CB_HIP(hipMemcpyAsync(vec_[YN], v_, sizeof(double) * nc_, hipMemcpyDeviceToDevice, st_));
+    for (int k : {FN, EDE, VT1, VT2}) k_fill<<<nblk(nc_), 256, 0, st_>>>(0.0, vec_[k], nc_);
+    for (double &x : hw_) x = 0.0;
+    for (double &x : o_) x = 0.0;
+    hw_[2] = nc_;
+    hw_[3] = UROUND;
+    hw_[4] = std::sqrt(UROUND);
+    hw_[5] = hmax_;
+    hw_[6] = hmin_;
+    o_[ATOL] = atol;
+    o_[RTOL] = rtol;
+    o_[TEND] = hmax_;
+    last_ = true;
+    newspc_ = true;
+    jacatt_ = false;
+    k_stage_rhs(vec_[YN], vec_[FN]);
+    CB_HIP(hipMemcpyAsync(vec_[EDE], vec_[FN], sizeof(double) * nc_, hipMemcpyDeviceToDevice, st_));
+    rkcrho();
+    // set_initial_absh (solve_rkc.cu:489-560)
+    double absh = hmax_;
+    if (o_[SPRAD] * absh > 1.0) absh = 1.0 / o_[SPRAD];
+    absh = fmax_ori(absh, hw_[6]);
+    k_saxpb<<<nblk(nc_), 256, 0, st_>>>(absh, vec_[FN], vec_[YN], vec_[VT1], nc_);
+    k_stage_rhs(vec_[VT1], vec_[VT2]);
+    k_absh_w<<<nblk(nc_), 256, 0, st_>>>(vec_[VT2], vec_[FN], vec_[YN], o_[ATOL], o_[RTOL], dammy_, bad_, nc_);
+    if (read_bad()) throw Error("RKC initial step: zero error weight (idid = 3)", 3);
+    double est = sumsq(dammy_);
+    est = absh * std::sqrt(est / nc_);
+    if (0.1 * absh < hmax_ * std::sqrt(est))
+      absh = fmax_ori(0.1 * absh / std::sqrt(est), hw_[6]);
+    else
+      absh = hmax_;
+    o_[ABSH] = absh;
+    o_[MMAX] = nint_ori(std::sqrt(o_[RTOL] / (10.0 * UROUND)));
+    sum_dt_ = 0.0;
+    integ_ = std::pow(2.0, std::log10(o_[ATOL]));   // solve_rkc.cu:1265
+  }
+
+  int read_bad() {
+    int h = 0;
+    CB_HIP(hipMemcpyAsync(&h, bad_, sizeof(int), hipMemcpyDeviceToHost, st_));
+    CB_HIP(hipStreamSynchronize(st_));
+    return h;
+  }
+
+  // dy = F(y) alone: a stage with mu = 0 is not the same arithmetic, so use
+  // the power-iteration kernel's RHS with fn = 0 and discard the norm.
+  void k_stage_rhs(const double *y, double *dy) {
+    k_rho_iter<<<nblk(nc_), 256, 0, st_>>>(csr(), b_, cm_, y, zero_fn(), dy, dammy_, red_a_, nc_);
+  }
+
+  const double *zero_fn() {
+    if (!zero_) {
+      zero_ = dalloc<double>(nc_);
+      CB_HIP(hipMemsetAsync(zero_, 0, sizeof(double) * nc_, st_));
+    }
+    return zero_;
+  }
+
+  // rkcrho, solve_rkc.cu:562-754
+  void rkcrho() {
+    const int itmax = 1000;
+    double sprad_old[3] = {0.0, 0.0, 0.0};
+    const double sqrtu = std::sqrt(UROUND);
+    const double small = 1.0 / 0.125;
+    double ynrm = std::sqrt(sumsq(vec_[YN]));
+    double vnrm = std::sqrt(sumsq(vec_[EDE]));
+    double dynrm;
+    if (ynrm != 0.0 && vnrm != 0.0) {
+      dynrm = ynrm * sqrtu;
+      k_saxpb<<<nblk(nc_), 256, 0, st_>>>(dynrm / vnrm, vec_[EDE], vec_[YN], vec_[VT1], nc_);
+    } else if (ynrm != 0.0) {
+      dynrm = ynrm * sqrtu;
+      k_saxpb<<<nblk(nc_), 256, 0, st_>>>(sqrtu, vec_[YN], vec_[YN], vec_[VT1], nc_);
+    } else if (vnrm != 0.0) {
+      dynrm = UROUND;
+      k_scale<<<nblk(nc_), 256, 0, st_>>>(dynrm / vnrm, vec_[EDE], vec_[VT1], nc_);
+    } else {
+      dynrm = UROUND;
+      k_fill<<<nblk(nc_), 256, 0, st_>>>(dynrm, vec_[VT1], nc_);
+    }
+    double sigma = 0.0, sigmal;
+    for (int iter = 0; iter < itmax; iter++) {
+      st.rho_iters++;
+      k_rho_iter<<<nblk(nc_), 256, 0, st_>>>(csr(), b_, cm_, vec_[VT1], vec_[FN], vec_[VT2], dammy_, red_a_, nc_);
+      double dfnrm = std::sqrt(finish_sum(nblk(nc_)));
+      sigmal = sigma;
+      sigma = dfnrm / dynrm;
+      sprad_old[0] = sprad_old[1];
+      sprad_old[1] = sprad_old[2];
+      sprad_old[2] = o_[SPRAD];
+      o_[SPRAD] = 1.2 * sigma;
+      if (iter >= 1 && std::fabs(sigma - sigmal) <= fmax_ori(sigma, small) * 0.001) {
+        k_saxpb<<<nblk(nc_), 256, 0, st_>>>(-1.0, vec_[YN], vec_[VT1], vec_[EDE], nc_);
+        return;
+      }
+      if (iter > itmax / 2) {
+        if ((std::fabs(sprad_old[2] - sprad_old[0]) < 0.00001) && (std::fabs(sprad_old[1] - o_[SPRAD]) < 0.00001)) {
+          o_[SPRAD] = (o_[SPRAD] + sprad_old[2]) / 2.0;
+          k_saxpb<<<nblk(nc_), 256, 0, st_>>>(-1.0, vec_[YN], vec_[VT1], vec_[EDE], nc_);
+          return;
+        }
+      }
+      if (dfnrm != 0.0) {
+        k_rho_next<<<nblk(nc_), 256, 0, st_>>>(dynrm / dfnrm, vec_[VT2], vec_[FN], vec_[YN], vec_[VT1], nc_);
+      } else {
+        k_flip1<<<1, 1, 0, st_>>>(vec_[VT1], vec_[YN], iter % nc_);
+      }
+    }
+    o_[IDID] = 6;
+  }
+
+  // step, solve_rkc.cu:815-906
+  void step(double h, int m) {
+    double ajm1, arg, bj, bjm1, bjm2, dzj, dzjm1, dzjm2, d2zj, d2zjm1, d2zjm2, mu, mus, nu, temp1, temp2, w0, w1, zj,
+        zjm1, zjm2;
+    w0 = 1.0 + 2.0 / (13.0 * m * m);
+    temp1 = w0 * w0 - 1.0;
+    temp2 = std::sqrt(temp1);
+    arg = m * std::log(w0 + temp2);
+    w1 = std::sinh(arg) * temp1 / (std::cosh(arg) * m * temp2 - w0 * std::sinh(arg));
+    bjm1 = 1.0 / ((2.0 * w0) * (2.0 * w0));
+    bjm2 = bjm1;
+    mus = w1 * bjm1;
+    k_first<<<nblk(nc_), 256, 0, st_>>>(vec_[FN], vec_[YN], vec_[VT1], vec_[VT2], h * mus, nc_);
+    st.stages++;
+    zjm1 = w0; zjm2 = 1.0; dzjm1 = 1.0; dzjm2 = 0.0; d2zjm1 = 0.0; d2zjm2 = 0.0;
+    for (int j = 1; j < m; j++) {
+      zj = 2.0 * w0 * zjm1 - zjm2;
+      dzj = 2.0 * w0 * dzjm1 - dzjm2 + 2.0 * zjm1;
+      d2zj = 2.0 * w0 * d2zjm1 - d2zjm2 + 4.0 * dzjm1;
+      bj = d2zj / (dzj * dzj);
+      ajm1 = 1.0 - zjm1 * bjm1;
+      mu = 2.0 * w0 * bj / bjm1;
+      nu = -bj / bjm2;
+      mus = mu * w1 / w0;
+      k_stage<<<nblk(nc_), 256, 0, st_>>>(csr(), b_, cm_, vec_[VT1], vec_[VT2], vec_[YN], vec_[FN], vec_[Y], mu, nu,
+                                          (1.0 - mu - nu), h * mus, ajm1, nc_);
+      st.stages++;
+      double *tmp = vec_[VT2];
+      vec_[VT2] = vec_[VT1];
+      vec_[VT1] = vec_[Y];
+      vec_[Y] = tmp;
+      bjm2 = bjm1; bjm1 = bj;
+      zjm2 = zjm1; zjm1 = zj;
+      dzjm2 = dzjm1; dzjm1 = dzj;
+      d2zjm2 = d2zjm1; d2zjm1 = d2zj;
+    }
+    double *tmp = vec_[Y];
+    vec_[Y] = vec_[VT1];
+    vec_[VT1] = vec_[VT2];
+    vec_[VT2] = tmp;
+  }
+
+  // rkclow, solve_rkc.cu:908-1096
+  void rkclow() {
+    const double hmin = hw_[6];
+    int loop_num = 0;
+    while (true) {
+      if (++loop_num > 1000) return;
+      if (newspc_) {
+        rkcrho();
+        jacatt_ = true;
+      }
+      last_ = false;
+      if (1.1 * o_[ABSH] >= std::fabs(o_[TEND] - o_[LT])) {
+        o_[ABSH] = std::fabs(o_[TEND] - o_[LT]);
+        last_ = true;
+      }
+      int m = 1 + (int)(std::sqrt(1.54 * o_[ABSH] * o_[SPRAD] + 1.0));
+      if (m > o_[MMAX]) {
+        m = (int)o_[MMAX];
+        o_[ABSH] = (m * m - 1) / (1.54 * o_[SPRAD]);
+        last_ = false;
+      }
+      double h = o_[ABSH];
+      if (h <= hmin) h = hmin;
+      step(h, m);
+      const int nb = nblk(nc_);
+      k_err<<<nb, 256, 0, st_>>>(csr(), b_, cm_, vec_[Y], vec_[YN], vec_[FN], vec_[VT1], o_[ATOL], o_[RTOL], h, red_a_,
+                                 bad_, nc_);
+      st.stages++;
+      double err = finish_sum(nb);
+      if (read_bad()) {
+        o_[IDID] = 3;
+        return;
+      }
+      err = std::sqrt(err / (double)nc_);
+      if (err > 1.0) {
+        st.rejects++;
+        o_[ABSH] = 0.8 * o_[ABSH] / (std::pow(err, 1.0 / 3.0));
+        if (o_[ABSH] < hmin) {
+          o_[IDID] = 4;   // and fall through to the accept branch, as the reference does
+        } else {
+          newspc_ = !jacatt_;
+          continue;
+        }
+      }
+      st.steps++;
+      o_[NACCPT] += 1;
+      o_[LT] += h;
+      o_[GT] = o_[LT];
+      jacatt_ = false;
+      int l_nstsig = ((int)(o_[NSTSIG]) + 1) % 25;   // nstsig is never advanced
+      newspc_ = false;
+      if (l_nstsig == 0) newspc_ = !jacatt_;
+      hw_[0] = h;
+      hw_[1] = o_[LT];
+      double *tmp = vec_[VT2];
+      vec_[VT2] = vec_[FN];
+      vec_[FN] = vec_[VT1];
+      vec_[VT1] = vec_[YN];
+      vec_[YN] = vec_[Y];
+      vec_[Y] = tmp;
+      double fac = 10.0, temp1, temp2;
+      if (o_[NACCPT] == 1) {
+        temp2 = std::pow(err, 1.0 / 3.0);
+        if (0.8 < fac * temp2) fac = 0.8 / temp2;
+      } else {
+        temp1 = 0.8 * o_[ABSH] * std::pow(o_[ERROLD], 1.0 / 3.0);
+        temp2 = std::fabs(o_[HOLD]) * std::pow(err, 2.0 / 3.0);
+        if (temp1 < fac * temp2) fac = temp1 / temp2;
+      }
+      o_[ABSH] = fmax_ori(0.1, fac) * o_[ABSH];
+      o_[ABSH] = fmax_ori(hmin, std::fmin(hmax_, o_[ABSH]));
+      o_[ERROLD] = err;
+      o_[HOLD] = h;
+      o_[IDID] = last_ ? 1 : 2;
+      return;
+    }
+  }
+
+  int n_, nc_, comp_;
+  StimSpec stim_;
+  PoissonSpec syn_;
+  hipStream_t st_;
+  double *v_, *ca_, *rev_, *iext_, *shell_, *cm_, *cond_, *gate_, *b_, *dammy_, *val_;
+  int *row_, *col_;
+  double *vec_[NVEC];
+  double *red_a_, *red_b_, *hres_, *amp_;
+  double *syn_g_ = nullptr;
+  double *zero_ = nullptr;
+  int *bad_;
+  double hw_[8], o_[NOTHERS];
+  bool last_, newspc_, jacatt_;
+  double sum_dt_ = 0.0, integ_ = 0.0, hmin_, hmax_;
+  Stats st;
+};
+
+}  // namespace
+
+void Engine::run(int nsteps, double t0, int rec_comp, double *rec_out) {
+  std::vector<double> buf(nc());
+  for (int k = 0; k < nsteps; k++) {
+    set_current(t0 + k * H_MAX);
+    advance();
+    if (rec_out) {
+      read(F_VNEW, buf.data());
+      for (int c = 0; c < ncell(); c++) rec_out[(size_t)k * ncell() + c] = buf[(size_t)c * (nc() / ncell()) + rec_comp];
+    }
+  }
+}
+
+Engine *make_global_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn) {
+  return new GlobalPkj(host, sol, stim, syn);
+}
+
+}  // namespace cbrkc

@@ -1,0 +1,58 @@
+// Host-side population builder: morphology loading, synthetic tree
+// generation, per-compartment parameters and the cable matrix.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace cbrkc {
+
+// One line of a Pkj.txt-format morphology: idx parent x y z diam type
+// (pkj.cu:30-34).  Lengths are micrometres; parent -1 marks the root.
+struct MorphRow {
+  int idx, parent, type;
+  double x, y, z, d;
+};
+
+// Reads every whitespace-separated 7-column row of a Pkj.txt-format file.
+// Returns false (with a message in err) when the file is missing or its row
+// count is not a multiple of comp.
+bool read_pkj_morph(const std::string &path, int comp, std::vector<MorphRow> &rows, std::string &err);
+
+// Reads the 26 "pre, post" pairs of Pkj_GJ.csv (pkj_solve.cu:86-97).
+bool read_pkj_gj(const std::string &path, std::vector<int> &pre, std::vector<int> &post, std::string &err);
+
+// Seeded synthetic Purkinje tree in Pkj.txt layout: soma (type 0) is the last
+// index and every parent has a higher index than its children.  Soma sphere
+// radius soma_r_um, binary branching, segment length U(lmin, lmax) um,
+// diameter tapering linearly with depth from d0 to d1 um, scaled by dscale.
+void synth_pkj_tree(uint64_t seed, int comp, double soma_r_um, double lmin, double lmax, double d0,
+                    double d1, double dscale, std::vector<MorphRow> &rows);
+
+bool write_pkj_morph(const std::string &path, const std::vector<MorphRow> &rows, std::string &err);
+
+// Structure-of-arrays description of a Purkinje population on the host: the
+// reference's neuron_t fields (param.h:27-35) plus the cable CSR
+// (pkj_solve.cu:28-254) in RKC sign convention.
+struct PkjHost {
+  int n = 0, nx = 0, ny = 0, comp = 0, nc = 0;
+  std::vector<double> cm, area, shell, iext, v, ca, rev;
+  std::vector<int8_t> compart;
+  std::vector<double> cond;   // [NCOND][nc]
+  std::vector<double> gate;   // [NGATE][nc]
+  std::vector<int> parent;    // global index of the parent compartment, -1 for roots
+  std::vector<double> gpar;   // axial conductance to the parent [mS]
+  // CSR with stored diagonal: off-diagonal +g, diagonal -sum(g) (RKC sign)
+  std::vector<int> row, col;
+  std::vector<double> val;
+  int n_gj = 0;
+};
+
+// Builds the population from a morphology (one block replicated, or one
+// block per cell), optional gap-junction list (only used when nx > 1) and
+// per-cell initial voltages (nullptr: draw -68 + 2 (U - 0.5) from libc rand()
+// per cell in order, as pkj_ion.cu:367-369 does).
+bool build_pkj_host(const std::vector<MorphRow> &rows, int nx, int ny, const std::vector<int> *gj_pre,
+                    const std::vector<int> *gj_post, const double *v0, PkjHost &out, std::string &err);
+
+}  // namespace cbrkc

@@ -1,0 +1,186 @@
+// Purkinje-cell channel model on the RKC path, shared by the FP64 and mixed
+// engines.  Written for CDNA: every function is a straight-line __host__
+// __device__ expression, templated on the arithmetic type so the same text
+// serves the FP64 population engine and the FP32 per-cell engine.
+//
+// Model constants and formulas follow the reference cell model:
+//   constants            pkj.cuh:11-82
+//   rate functions       pkj_ion.cu:4-91
+//   ion update (RKC)     pkj_ion.cu:304-358
+//   frozen current b     solve_rkc.cu:369-395
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace cbrkc {
+namespace pkj {
+
+constexpr int COMP = 1600;          // pkj.cuh:11
+constexpr int SOMA = 0, MAIN = 1, SMOOTH = 2, SPINY = 3;
+constexpr double V_INIT = -68.0, V_LEAK = -80.0, V_NA = 45.0, V_CA = 135.0,
+                 V_KH = -30.0, V_K = -85.0;
+constexpr double G_GJ = 2.5e-6;     // mS, pkj.cuh:26
+constexpr int GJ_EACH = 26;
+constexpr double SPINE_DENS = 13.0e4, SPINE_AREA = 1.33e-8;
+constexpr double CM = 1.64, RM_S = 10.0, RM_D = 30.0, RI = 0.25;
+constexpr double B_CA = 10.0, SHELLD = 0.2e-4, CA_OUT = 2.4, CA_0 = 4.0e-5, FARADAY = 9.6485e4;
+constexpr double CURRENT = 0.1e-3;  // uA (0.1 nA), pkj.cuh:15
+
+// gate slots, pkj.cuh:86-87
+enum Gate { m_NaF, h_NaF, m_NaP, m_CaP, h_CaP, m_CaT, h_CaT, m_KA, h_KA, m_KC, m_Kh1, m_Kh2,
+            m_Kdr, h_Kdr, m_KM, z_KC, m_K2, z_K2, NGATE };
+// conductance slots, pkj.cuh:90-91
+enum Cond { g_leak, g_NaF, g_NaP, g_CaP, g_CaT, g_Kh, g_Kdr, g_KM, g_KA, g_KC, g_K2, NCOND };
+
+// densities [mS/cm^2] by compartment class (soma, main dendrite, rest),
+// order NaF NaP CaP CaT KH KDR KM KA KC K2 (pkj.cuh:28-62)
+constexpr double DENSITY[3][10] = {
+    {7500.0, 1.0, 0.0, 0.5, 0.3, 600.0, 0.040, 15.0, 0.0, 0.0},
+    {0.0, 0.0, 4.5, 0.5, 0.0, 60.0, 0.01, 2.0, 80.0, 0.39},
+    {0.0, 0.0, 4.5, 0.5, 0.0, 0.0, 0.013, 0.0, 80.0, 0.39},
+};
+
+template <typename T> __host__ __device__ inline T ex(T x);
+template <> __host__ __device__ inline double ex<double>(double x) { return exp(x); }
+template <> __host__ __device__ inline float ex<float>(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __expf(x);
+#else
+  return expf(x);
+#endif
+}
+
+// A / (B + exp((v + C) / D)), the reference's generic rate shape
+template <typename T>
+__host__ __device__ inline T lf(T v, T A, T B, T C, T D) { return A / (B + ex<T>((v + C) / D)); }
+
+// alpha/beta pair of every two-rate gate, evaluated once; the reference
+// evaluates them separately for tau and inf but the values are identical.
+template <typename T> struct Rates { T a, b; };
+
+template <typename T>
+__host__ __device__ inline void cn_gate(T &x, T inf, T tau, T DT) {
+  // (2 dt inf + (2 tau - dt) x) / (2 tau + dt): CN step of dx/dt = (inf - x)/tau
+  x = (T(2.0) * DT * inf + (T(2.0) * tau - DT) * x) / (T(2.0) * tau + DT);
+}
+
+template <typename T>
+__host__ __device__ inline void ab_gate(T &x, T a, T b, T DT) {
+  T tau = T(1.0) / (a + b);
+  T inf = a / (a + b);
+  cn_gate<T>(x, inf, tau, DT);
+}
+
+template <typename T> __host__ __device__ inline T kdr_alpha(T u) {
+  // -0.0235 u / (exp(-u/12) - 1), removable singularity at u = 0 (pkj_ion.cu:53-54)
+  return (fabs(u) > T(1e-6)) ? (T(-0.0235) * u / (ex<T>(-u / T(12.0)) - T(1.0))) : T(0.0235 * 12.0);
+}
+
+// Steady state of every gate at voltage v and raw Ca (pkj_ion.cu:360-395).
+template <typename T>
+__host__ __device__ inline void steady_state(T v, T ca, T *g) {
+  auto inf2 = [](T a, T b) { return a / (a + b); };
+  g[m_NaF] = inf2(lf<T>(v, 35.0, 0.0, 5.0, -10.0), lf<T>(v, 7.0, 0.0, 65.0, 20.0));
+  g[h_NaF] = inf2(lf<T>(v, 0.225, 1.0, 80.0, 10.0), lf<T>(v, 7.5, 0.0, -3.0, -18.0));
+  g[m_NaP] = inf2(lf<T>(v, 200.0, 1.0, -18.0, -16.0), lf<T>(v, 25.0, 1.0, 58.0, 8.0));
+  g[m_CaP] = inf2(lf<T>(v, 8.5, 1.0, -8.0, -12.5), lf<T>(v, 35.0, 1.0, 74.0, 14.5));
+  g[h_CaP] = inf2(lf<T>(v, 0.0015, 1.0, 29.0, 8.0), lf<T>(v, 0.0055, 1.0, 23.0, -8.0));
+  g[m_CaT] = inf2(lf<T>(v, 2.60, 1.0, 21.0, -8.0), lf<T>(v, 0.180, 1.0, 40.0, 4.0));
+  g[h_CaT] = inf2(lf<T>(v, 0.0025, 1.0, 40.0, 8.0), lf<T>(v, 0.190, 1.0, 50.0, -10.0));
+  g[m_Kh1] = T(0.8) / (T(1.0) + ex<T>((v + T(82.0)) / T(7.0)));
+  g[m_Kh2] = T(0.2) / (T(1.0) + ex<T>((v + T(82.0)) / T(7.0)));
+  {
+    T a = kdr_alpha<T>(v - T(8.0));
+    T b = T(5.0) * ex<T>(-(v + T(127.0)) / T(30.0));
+    g[m_Kdr] = a / (a + b);
+  }
+  g[h_Kdr] = T(1.0) / (T(1.0) + ex<T>((v + T(25.0)) / T(4.0)));
+  g[m_KM] = T(1.0) / (T(1.0) + ex<T>(-(v + T(35.0)) / T(10.0)));
+  g[m_KA] = inf2(lf<T>(v, 1.40, 1.0, 27.0, -12.0), lf<T>(v, 0.490, 1.0, 30.0, 4.0));
+  g[h_KA] = inf2(lf<T>(v, 0.0175, 1.0, 50.0, 8.0), lf<T>(v, 1.30, 1.0, 13.0, -10.0));
+  g[m_KC] = inf2(T(7.5), lf<T>(v, 0.110, 0.0, -35.0, 14.9));
+  g[z_KC] = T(1.0) / (T(1.0) + T(4.00) / ca);
+  g[m_K2] = inf2(T(25.0), lf<T>(v, 0.075, 0.0, 25.0, 6.0));
+  g[z_K2] = T(1.0) / (T(1.0) + T(0.20) / ca);
+}
+
+// Crank-Nicolson update of all 18 gates at the interval midpoint voltage
+// vm over dt, with the quantised Ca caq (pkj_ion.cu:337-354).
+template <typename T>
+__host__ __device__ inline void gates_cn(T vm, T caq, T DT, T *g) {
+  ab_gate<T>(g[m_NaF], lf<T>(vm, 35.0, 0.0, 5.0, -10.0), lf<T>(vm, 7.0, 0.0, 65.0, 20.0), DT);
+  ab_gate<T>(g[h_NaF], lf<T>(vm, 0.225, 1.0, 80.0, 10.0), lf<T>(vm, 7.5, 0.0, -3.0, -18.0), DT);
+  ab_gate<T>(g[m_NaP], lf<T>(vm, 200.0, 1.0, -18.0, -16.0), lf<T>(vm, 25.0, 1.0, 58.0, 8.0), DT);
+  ab_gate<T>(g[m_CaP], lf<T>(vm, 8.5, 1.0, -8.0, -12.5), lf<T>(vm, 35.0, 1.0, 74.0, 14.5), DT);
+  ab_gate<T>(g[h_CaP], lf<T>(vm, 0.0015, 1.0, 29.0, 8.0), lf<T>(vm, 0.0055, 1.0, 23.0, -8.0), DT);
+  ab_gate<T>(g[m_CaT], lf<T>(vm, 2.60, 1.0, 21.0, -8.0), lf<T>(vm, 0.180, 1.0, 40.0, 4.0), DT);
+  ab_gate<T>(g[h_CaT], lf<T>(vm, 0.0025, 1.0, 40.0, 8.0), lf<T>(vm, 0.190, 1.0, 50.0, -10.0), DT);
+  {
+    T e = ex<T>((vm + T(82.0)) / T(7.0));
+    cn_gate<T>(g[m_Kh1], T(0.8) / (T(1.0) + e), T(7.6), DT);
+    cn_gate<T>(g[m_Kh2], T(0.2) / (T(1.0) + e), T(36.8), DT);
+  }
+  {
+    T a = kdr_alpha<T>(vm - T(8.0));
+    T b = T(5.0) * ex<T>(-(vm + T(127.0)) / T(30.0));
+    T inf = a / (a + b);
+    T a2 = kdr_alpha<T>(vm + T(12.0));
+    T b2 = T(5.0) * ex<T>(-(vm + T(147.0)) / T(30.0));
+    T tau = T(1.0) / (a2 + b2);
+    cn_gate<T>(g[m_Kdr], inf, tau, DT);
+  }
+  cn_gate<T>(g[h_Kdr], T(1.0) / (T(1.0) + ex<T>((vm + T(25.0)) / T(4.0))),
+             (vm < T(-25.0)) ? T(1200.0) : T(10.0), DT);
+  cn_gate<T>(g[m_KM], T(1.0) / (T(1.0) + ex<T>(-(vm + T(35.0)) / T(10.0))),
+             T(200.0) / (T(3.3) * (ex<T>((vm + T(35.0)) / T(20.0))) + ex<T>(-(vm + T(35.0)) / T(20.0))), DT);
+  ab_gate<T>(g[m_KA], lf<T>(vm, 1.40, 1.0, 27.0, -12.0), lf<T>(vm, 0.490, 1.0, 30.0, 4.0), DT);
+  ab_gate<T>(g[h_KA], lf<T>(vm, 0.0175, 1.0, 50.0, 8.0), lf<T>(vm, 1.30, 1.0, 13.0, -10.0), DT);
+  ab_gate<T>(g[m_KC], T(7.5), lf<T>(vm, 0.110, 0.0, -35.0, 14.9), DT);
+  cn_gate<T>(g[z_KC], T(1.0) / (T(1.0) + T(4.00) / caq), T(10.0), DT);
+  ab_gate<T>(g[m_K2], T(25.0), lf<T>(vm, 0.075, 0.0, 25.0, 6.0), DT);
+  cn_gate<T>(g[z_K2], T(1.0) / (T(1.0) + T(0.20) / caq), T(10.0), DT);
+}
+
+// Ca shell update (RK4 over dt of dCa/dt = I - B (Ca - Ca0)), the reversal
+// potential and the quantised Ca for KC/K2 (pkj_ion.cu:312-335).  Always
+// FP64: the quantisation floor() is discontinuous in Ca.
+//   gid: global compartment index (the reference keys the fixed somatic
+//        reversal on gid % 1599 == 0).
+// Returns the quantised Ca; updates ca and rev.
+__host__ __device__ inline double ca_rk4(double v_old, double &ca, double &rev, double shell,
+                                         double gCaP_open, double gCaT_open, double DT, long gid) {
+  double Ca_val = ca;
+  double I_Ca1 = 1e-3 * (v_old - rev) * (gCaP_open + gCaT_open);
+  (I_Ca1 > 0.0) ? (I_Ca1 = 0.0) : (I_Ca1 = -I_Ca1 / (2.0 * FARADAY * shell));
+  double k1 = DT * (I_Ca1 - B_CA * (Ca_val - CA_0));
+  double k2 = DT * (I_Ca1 - B_CA * (Ca_val + k1 / 2.0 - CA_0));
+  double k3 = DT * (I_Ca1 - B_CA * (Ca_val + k2 / 2.0 - CA_0));
+  double k4 = DT * (I_Ca1 - B_CA * (Ca_val + k3 - CA_0));
+  k1 = Ca_val + (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
+  ca = k1;
+  Ca_val = 0.5 * (Ca_val + k1);
+  rev = (gid % 1599 == 0) ? 12.5 * log(CA_OUT / CA_0) : 13.361624877 * log(CA_OUT / Ca_val);
+  return floor((Ca_val * 1000 - 0.04) / 0.099986667) * 0.099986667 + 0.04;
+}
+
+// Frozen membrane current b at voltage v (solve_rkc.cu:369-395), summed in
+// the reference's term order.
+template <typename T>
+__host__ __device__ inline T current(T v, T iext, T rev, const T *c, const T *g) {
+  return (-iext
+          + c[g_leak] * (v - T(V_LEAK))
+          + c[g_NaF] * g[m_NaF] * g[m_NaF] * g[m_NaF] * g[h_NaF] * (v - T(V_NA))
+          + c[g_NaP] * g[m_NaP] * g[m_NaP] * g[m_NaP] * (v - T(V_NA))
+          + c[g_CaP] * g[m_CaP] * g[h_CaP] * (v - rev)
+          + c[g_CaT] * g[m_CaT] * g[h_CaT] * (v - rev)
+          + c[g_Kh] * g[m_Kh1] * (v - T(V_KH))
+          + c[g_Kh] * g[m_Kh2] * (v - T(V_KH))
+          + c[g_Kdr] * g[m_Kdr] * g[m_Kdr] * g[h_Kdr] * (v - T(V_K))
+          + c[g_KM] * g[m_KM] * (v - T(V_K))
+          + c[g_KA] * g[m_KA] * g[m_KA] * g[m_KA] * g[m_KA] * g[h_KA] * (v - T(V_K))
+          + c[g_KC] * g[m_KC] * g[z_KC] * g[z_KC] * (v - T(V_K))
+          + c[g_K2] * g[m_K2] * g[z_K2] * g[z_K2] * (v - T(V_K)));
+}
+
+}  // namespace pkj
+}  // namespace cbrkc

@@ -1,0 +1,151 @@
+/*
+ * cbrkc — MI355X-native RKC time stepping for cerebellar multi-compartment
+ * neuron populations.  C-ABI boundary: plain pointers, sizes and status
+ * codes, no device or framework types.
+ *
+ * A population handle (cb_pop) stands for the reference's pair
+ * (neuron_t, neuron_solve_t) of one cell type (param.h:27-46): it owns the
+ * compartment state, the cable matrix and the solver state.  Unlike the
+ * reference, which keeps solver state in function-local statics
+ * (solve_rkc.cu:1258-1265), every handle is independent.
+ *
+ * All functions return CB_OK (0) or a negative CB_E_* code; the message of
+ * the last failure on the calling thread is available from cb_last_error().
+ */
+#ifndef CBRKC_H
+#define CBRKC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CB_OK 0
+#define CB_E_ARG (-1)      /* bad argument or configuration */
+#define CB_E_IO (-2)       /* missing or malformed input file */
+#define CB_E_HIP (-3)      /* HIP runtime failure */
+#define CB_E_SOLVER (-4)   /* RKC controller failure (idid 3, 4 or 6) */
+#define CB_E_UNSUP (-5)    /* combination not provided */
+
+/* cell types, param.h:15 */
+enum { CB_GR = 0, CB_GO = 1, CB_PKJ = 2, CB_IO = 3 };
+/* solvers, param.h:14 (argv names "BE", "CN", "RKC") */
+enum { CB_SOLVER_BE = 0, CB_SOLVER_CN = 1, CB_SOLVER_RKC = 2 };
+/* engines for the RKC solver */
+enum {
+  CB_ENGINE_GLOBAL_FP64 = 0,  /* one controller for the whole population, FP64 (reference semantics) */
+  CB_ENGINE_CELL_FP64 = 1,    /* one controller per cell, persistent launch, FP64 */
+  CB_ENGINE_CELL_MIXED = 2    /* one controller per cell, FP32 stages/gates, compensated V, FP64 control */
+};
+/* fields readable with cb_pop_read (nc doubles each) */
+enum {
+  CB_F_VNEW = 0,   /* V at the end of the last macro step (vec[v_new]) */
+  CB_F_V = 1,      /* V at the last ion update (elem[v]) */
+  CB_F_CA = 2,
+  CB_F_REV = 3,    /* Ca reversal potential */
+  CB_F_B = 4,      /* frozen current */
+  CB_F_IEXT = 5,
+  CB_F_Y = 6,
+  CB_F_YN = 7,
+  CB_F_FN = 8,
+  CB_F_GATE0 = 100 /* + gate index in the cell type's gate order */
+};
+
+typedef struct cb_pop cb_pop;
+
+typedef struct cb_pop_config {
+  int cell_type;            /* CB_PKJ, ... */
+  int solver;               /* CB_SOLVER_* */
+  int engine;               /* CB_ENGINE_* (RKC only) */
+  int nx, ny;               /* cell grid; nx * ny cells */
+  const char *morph_path;   /* morphology file; NULL: "./Pkj.txt" */
+  const char *gj_path;      /* gap-junction pair list; NULL: "./Pkj_GJ.csv" (grid with nx > 1 only) */
+  double atol, rtol;        /* <= 0: the reference's value for the cell type */
+  const double *v0;         /* per-cell initial V [mV]; NULL: -68 + 2 (U - 0.5) from libc rand() */
+  int stim_comp;            /* local compartment of the step current; < 0: reference protocol */
+  double stim_t0, stim_t1;  /* [ms] window of the step current */
+  const double *stim_amp;   /* per-cell amplitude [uA]; NULL: reference amplitude */
+  int poisson_on;           /* dendritic Poisson conductance drive on every compartment */
+  double poisson_rate_hz, poisson_g_mS, poisson_tau_ms, poisson_e_mV;
+  uint64_t poisson_seed;
+  double ion_interval_ms;   /* accepted time between ion updates; <= 0: 2^log10(atol) (solve_rkc.cu:1265) */
+  int device;               /* HIP device ordinal */
+} cb_pop_config;
+
+/* Fills a config with the reference defaults for one cell type and solver. */
+int cb_config_default(cb_pop_config *cfg, int cell_type, int solver, int nx, int ny);
+
+/* Creates a population and initialises its solver: xx_initialize +
+ * xx_solve_initialize + RKC_vec_initialize (pkj.cu:240-306,
+ * pkj_solve.cu:286-366, solve_rkc.cu:756-791). */
+int cb_pop_create(const cb_pop_config *cfg, cb_pop **out);
+
+/* Releases every resource of the population: xx_finalize + xx_solve_finalize
+ * (pkj.cu:308-331, pkj_solve.cu:368-393). */
+int cb_pop_destroy(cb_pop *p);
+
+/* Sets the external current for time t [ms]: xx_set_current (pkj.cu:333-344). */
+int cb_pop_set_current(cb_pop *p, double t);
+
+/* Advances one macro step (H_MAX = 0.125 ms for RKC): xx_solve_update_v
+ * (pkj_solve.cu:395-409 -> solve_rkc.cu:1252-1299). */
+int cb_pop_advance(cb_pop *p);
+
+/* nsteps x (set_current(t0 + k H_MAX); advance), recording compartment
+ * rec_comp of every cell after each macro step into rec_out[k * ncell + c]
+ * (rec_out may be NULL).  The main loop of main.cu:182-311 for one
+ * population, without a host round trip per step where the engine allows. */
+int cb_pop_run(cb_pop *p, int nsteps, double t0, int rec_comp, double *rec_out);
+
+/* Copies nc values of a state field to host memory. */
+int cb_pop_read(cb_pop *p, int field, double *dst);
+
+/* Population sizes. */
+int cb_pop_size(const cb_pop *p, int *ncell, int *nc);
+
+/* Counters: accepted steps, rejected steps, RHS stage sweeps, power-method
+ * iterations, ion updates, macro steps (6 values). */
+int cb_pop_stats(const cb_pop *p, int64_t *out6);
+
+/* Global-controller scalars: h_others[14] and h_work[8] (param.h:22,
+ * solve_rkc.cu:453-476).  CB_E_UNSUP for per-cell engines. */
+int cb_pop_controller(const cb_pop *p, double *others14, double *hwork8);
+
+/* Writes one seeded synthetic Purkinje tree per cell, in the Pkj.txt format,
+ * to path: soma sphere of radius soma_r_um, binary branching, segment
+ * lengths U(lmin_um, lmax_um), diameters tapering with depth from d0_um to
+ * d1_um; cell c uses seed + c and its diameters are scaled by dscale[c]
+ * (dscale NULL: 1). */
+int cb_synth_pkj_morph(const char *path, uint64_t seed, int ncell, double soma_r_um, double lmin_um,
+                       double lmax_um, double d0_um, double d1_um, const double *dscale);
+
+/* Philox4x32-10 block (Salmon et al., SC'11), the generator behind every
+ * seeded synthetic input. */
+void cb_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *cb_last_error(void);
+
+/* ---- entry points named after the reference's per-type interface ---- */
+
+/* pkj_initialize + pkj_solve_initialize (pkj.cu:240, pkj_solve.cu:286):
+ * solver "BE", "CN" or "RKC" matched by prefix as main.cu:60-102 does;
+ * reads ./Pkj.txt and ./Pkj_GJ.csv. */
+int cb_pkj_initialize(int nx, int ny, const char *solver, cb_pop **out);
+/* pkj_solve_update_v (pkj_solve.cu:395-409) without network synapses. */
+int cb_pkj_solve_update_v(cb_pop *p);
+/* pkj_set_current (pkj.cu:333-344). */
+int cb_pkj_set_current(cb_pop *p, double t);
+/* pkj_output_file (pkj.cu:370-412): appends "t,V_1599(cell0),...,\n" to
+ * v_csv and "t,cell\n" for every upward crossing of -10 mV to raster_csv;
+ * memv is the population V (nc values), raster_prev one value per cell. */
+int cb_pkj_output_file(const cb_pop *p, const double *memv, double *raster_prev, double t, const char *v_csv,
+                       const char *raster_csv);
+/* pkj_finalize + pkj_solve_finalize. */
+int cb_pkj_finalize(cb_pop *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

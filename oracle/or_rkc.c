@@ -1,0 +1,613 @@
+/*
+ * Oracle: population-global adaptive RKC (TEST INFRASTRUCTURE, see oracle.h).
+ *
+ * Restates solve_rkc.cu of the reference:
+ *   dydt               solve_rkc.cu:177-196
+ *   step               solve_rkc.cu:815-906  (Chebyshev stage recurrence)
+ *   rkclow             solve_rkc.cu:908-1096 (one accepted step + controller)
+ *   rkcrho             solve_rkc.cu:562-754  (nonlinear power method)
+ *   err_estimate       solve_rkc.cu:58-128
+ *   set_initial_vec    solve_rkc.cu:437-487
+ *   set_initial_absh   solve_rkc.cu:489-560
+ *   RKC_vec_initialize solve_rkc.cu:756-791
+ *   rkcint             solve_rkc.cu:794-813
+ *   xx_solve_by_rkc    solve_rkc.cu:1121-1362 (operator-split driver)
+ *
+ * Differences from the reference that are deliberate and exact-by-design:
+ *   - every vector norm (cuBLAS Ddot / the tree reduction in reduction.cu)
+ *     is computed by or_det_sum: fixed 256-wide pairwise blocks, the same
+ *     order the HIP kernels use.  The reference's own summation order belongs
+ *     to closed libraries and is not reproducible (parity unpinned at the
+ *     bit level for the norms; pinned by tolerance instead).
+ *
+ * Reference behaviour reproduced on purpose (not bugs of this restatement):
+ *   - after an accepted step vec[y] holds the stage Y_{m-2}, and that is what
+ *     the ion update consumes (pointer rotations solve_rkc.cu:885-888,
+ *     901-904, 1031-1037);
+ *   - h_others[nstsig] is never incremented, so the spectral radius is only
+ *     re-estimated after a rejected step (solve_rkc.cu:1024-1026);
+ *   - a rejected step whose shrunk step size falls below hmin sets idid=4
+ *     and then falls through to the accept branch (solve_rkc.cu:1006-1019),
+ *     so steps at hmin are always accepted;
+ *   - F_0 (fn) is not recomputed after an ion update changes b.
+ *
+ * Pinning: or_det_sum against exact integer sums and math.fsum; the stage
+ * recurrence against the closed-form RKC stability polynomial on y'=lambda*y;
+ * rkcrho against a dense eigen-solve; rkcint s=1 identity; step-size control
+ * by a scalar replay (tests/test_oracle_rkc.py).
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define UROUND (2.22e-16)          /* solve_rkc.cuh:22 */
+#define H_MAX (0.125)              /* solve_rkc.cuh:23, param.h:12 */
+
+/* ---------------- deterministic reduction ---------------- */
+
+static double blk256(const double *x, long nvalid)
+{
+  double s[256];
+  for (int i = 0; i < 256; i++) s[i] = (i < nvalid) ? x[i] : 0.0;
+  for (int stride = 128; stride > 0; stride >>= 1)
+    for (int i = 0; i < stride; i++) s[i] = s[i] + s[i + stride];
+  return s[0];
+}
+
+double or_det_sum(const double *x, long n)
+{
+  if (n <= 0) return 0.0;
+  long nb = (n + 255) / 256;
+  if (nb == 1) return blk256(x, n);
+  double *part = (double *)malloc(sizeof(double) * nb);
+#pragma omp parallel for schedule(static)
+  for (long b = 0; b < nb; b++) {
+    long rem = n - b * 256;
+    part[b] = blk256(x + b * 256, rem < 256 ? rem : 256);
+  }
+  double r = or_det_sum(part, nb);
+  free(part);
+  return r;
+}
+
+/* sum of squares of a vector through the deterministic reduction */
+static double dot_self(const double *x, long n, double *scratch)
+{
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; i++) scratch[i] = x[i] * x[i];
+  return or_det_sum(scratch, n);
+}
+
+static double fmax_ori(double a, double b) { return (a > b) ? a : b; }  /* solve_rkc.cu:36-39 */
+
+static int nint_ori(double x)                                           /* solve_rkc.cu:30-34 */
+{
+  if (x - (double)(int)x < 0.5) return (int)(double)(int)x;
+  return (int)(double)(int)(x + 0.9);
+}
+
+/* ---------------- model dispatch ---------------- */
+extern void or_gr_update_b(or_pop *p, const double *vv);
+extern void or_go_update_b(or_pop *p, const double *vv);
+extern void or_io_update_b(or_pop *p, const double *vv);
+extern void or_model_update_ion_and_b(or_pop *p, double sum_dt);
+
+static void update_b(or_pop *p, const double *vv)
+{
+  switch (p->type) {
+    case OR_PKJ: or_pkj_update_b(p, vv); break;
+    default: break;
+  }
+  if (p->syn_g) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < p->nc; i++) p->b[i] += p->syn_g[i] * (vv[i] - p->syn_e);
+  }
+}
+
+/* ---------------- dydt, solve_rkc.cu:177-196 ---------------- */
+void or_dydt(or_pop *p, const double *y, double *dy)
+{
+  const int *row = p->row, *col = p->col;
+  const double *val = p->val, *b = p->b, *cm = p->cm;
+#pragma omp parallel for schedule(static)
+  for (int id = 0; id < p->nc; id++) {
+    double d_ = (-(b[id]));
+    const double y_id = y[id];
+    for (int j = row[id]; j < row[id + 1]; j++) d_ += val[j] * (y[col[j]] - y_id);
+    dy[id] = d_ / cm[id];
+  }
+}
+
+static void vec_sA_plus_B(double s, const double *A, const double *B, double *C, int nc)  /* :143-148 */
+{
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nc; i++) C[i] = s * A[i] + B[i];
+}
+
+/* ---------------- rkcrho, solve_rkc.cu:562-754 ---------------- */
+static void rkcrho(or_pop *p)
+{
+  double **vec = p->vec;
+  double *dammy = p->dammy;
+  const int nc = p->nc;
+  const int itmax = 1000;
+  double sprad_old[3] = {0.0, 0.0, 0.0};
+  const double uround = UROUND;
+  const double sqrtu = sqrt(UROUND);
+  const double small = 1.0 / 0.125;
+  double ynrm, sigma, sigmal, dynrm, dfnrm, vnrm;
+
+  ynrm = dot_self(vec[OR_YN], nc, dammy);
+  vnrm = dot_self(vec[OR_EDE], nc, dammy);
+  ynrm = sqrt(ynrm);
+  vnrm = sqrt(vnrm);
+
+  if (ynrm != 0.0 && vnrm != 0.0) {
+    dynrm = ynrm * sqrtu;
+    double alpha = dynrm / vnrm;
+    vec_sA_plus_B(alpha, vec[OR_EDE], vec[OR_YN], vec[OR_VT1], nc);
+  } else if (ynrm != 0.0) {
+    dynrm = ynrm * sqrtu;
+    vec_sA_plus_B(sqrtu, vec[OR_YN], vec[OR_YN], vec[OR_VT1], nc);
+  } else if (vnrm != 0.0) {
+    dynrm = uround;
+    double alpha = dynrm / vnrm;
+    for (int i = 0; i < nc; i++) vec[OR_VT1][i] = alpha * vec[OR_EDE][i];
+  } else {
+    dynrm = uround;
+    for (int i = 0; i < nc; i++) vec[OR_VT1][i] = dynrm;
+  }
+
+  sigma = 0.0;
+  for (int iter = 0; iter < itmax; iter++) {
+    p->n_rho_iters++;
+    or_dydt(p, vec[OR_VT1], vec[OR_VT2]);
+    vec_sA_plus_B(-1.0, vec[OR_FN], vec[OR_VT2], dammy, nc);
+    /* Ddot(dammy, dammy): square in place then reduce (dammy is scratch) */
+    dfnrm = dot_self(dammy, nc, dammy);
+    dfnrm = sqrt(dfnrm);
+    sigmal = sigma;
+    sigma = dfnrm / dynrm;
+    sprad_old[0] = sprad_old[1];
+    sprad_old[1] = sprad_old[2];
+    sprad_old[2] = p->others[OR_SPRAD];
+    p->others[OR_SPRAD] = 1.2 * sigma;
+    if (iter >= 1 && fabs(sigma - sigmal) <= fmax_ori(sigma, small) * 0.001) {
+      vec_sA_plus_B(-1.0, vec[OR_YN], vec[OR_VT1], vec[OR_EDE], nc);
+      return;
+    }
+    if (iter > itmax / 2) {
+      if ((fabs(sprad_old[2] - sprad_old[0]) < 0.00001) &&
+          (fabs(sprad_old[1] - p->others[OR_SPRAD]) < 0.00001)) {
+        p->others[OR_SPRAD] = (p->others[OR_SPRAD] + sprad_old[2]) / 2.0;
+        vec_sA_plus_B(-1.0, vec[OR_YN], vec[OR_VT1], vec[OR_EDE], nc);
+        return;
+      }
+    }
+    if (dfnrm != 0.0) {
+      double alpha = dynrm / dfnrm;
+      vec_sA_plus_B(-1.0, vec[OR_VT2], vec[OR_FN], vec[OR_VT1], nc);
+      vec_sA_plus_B(alpha, vec[OR_VT1], vec[OR_YN], vec[OR_VT1], nc);
+    } else {
+      int index = iter % nc;
+      vec[OR_VT1][index] = 2.0 * vec[OR_YN][index] - vec[OR_VT1][index];
+    }
+  }
+  p->others[OR_IDID] = 6;
+}
+
+/* ---------------- step, solve_rkc.cu:815-906 ---------------- */
+static void step(or_pop *p, double h, int m)
+{
+  double **vec = p->vec;
+  const int nc = p->nc;
+  double ajm1, arg, bj, bjm1, bjm2, dzj, dzjm1, dzjm2, d2zj, d2zjm1, d2zjm2, mu, mus, nu,
+      temp1, temp2, thj, thjm1, thjm2, w0, w1, zj, zjm1, zjm2;
+  double *tmp;
+
+  w0 = 1.0 + 2.0 / (13.0 * m * m);
+  temp1 = w0 * w0 - 1.0;
+  temp2 = sqrt(temp1);
+  arg = m * log(w0 + temp2);
+  w1 = sinh(arg) * temp1 / (cosh(arg) * m * temp2 - w0 * sinh(arg));
+  bjm1 = 1.0 / ((2.0 * w0) * (2.0 * w0));
+  bjm2 = bjm1;
+  mus = w1 * bjm1;
+
+  memcpy(vec[OR_VT2], vec[OR_YN], sizeof(double) * nc);
+  double l_hmus = h * mus;
+  vec_sA_plus_B(l_hmus, vec[OR_FN], vec[OR_YN], vec[OR_VT1], nc);
+  thjm2 = 0.0; thjm1 = mus; zjm1 = w0;
+  zjm2 = 1.0; dzjm1 = 1.0; dzjm2 = 0.0;
+  d2zjm1 = 0.0; d2zjm2 = 0.0;
+  p->n_stages += 1;
+
+  for (int j = 1; j < m; j++) {
+    zj = 2.0 * w0 * zjm1 - zjm2;
+    dzj = 2.0 * w0 * dzjm1 - dzjm2 + 2.0 * zjm1;
+    d2zj = 2.0 * w0 * d2zjm1 - d2zjm2 + 4.0 * dzjm1;
+    bj = d2zj / (dzj * dzj);
+    ajm1 = 1.0 - zjm1 * bjm1;
+    mu = 2.0 * w0 * bj / bjm1;
+    nu = -bj / bjm2;
+    mus = mu * w1 / w0;
+
+    or_dydt(p, vec[OR_VT1], vec[OR_Y]);
+    {
+      const double one_mu_nu = (1.0 - mu - nu), h_mus = h * mus;
+      double *Y = vec[OR_Y];
+      const double *V1 = vec[OR_VT1], *V2 = vec[OR_VT2], *YN = vec[OR_YN], *FN = vec[OR_FN];
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < nc; i++)
+        Y[i] = mu * V1[i] + nu * V2[i] + one_mu_nu * YN[i] + h_mus * (Y[i] - ajm1 * FN[i]);
+    }
+    p->n_stages += 1;
+    thj = mu * thjm1 + nu * thjm2 + mus * (1.0 - ajm1);
+    (void)thj;
+    tmp = vec[OR_VT2];
+    vec[OR_VT2] = vec[OR_VT1];
+    vec[OR_VT1] = vec[OR_Y];
+    vec[OR_Y] = tmp;
+    thjm2 = thjm1; thjm1 = thj;
+    bjm2 = bjm1; bjm1 = bj;
+    zjm2 = zjm1; zjm1 = zj;
+    dzjm2 = dzjm1; dzjm1 = dzj;
+    d2zjm2 = d2zjm1; d2zjm1 = d2zj;
+  }
+  tmp = vec[OR_Y];
+  vec[OR_Y] = vec[OR_VT1];
+  vec[OR_VT1] = vec[OR_VT2];
+  vec[OR_VT2] = tmp;
+}
+
+/* ---------------- err_estimate, solve_rkc.cu:58-73 + reduction ---------------- */
+static int err_estimate(or_pop *p, double h, double *sum_out)
+{
+  double **vec = p->vec;
+  const double at = p->others[OR_ATOL], rt = p->others[OR_RTOL];
+  const double *Y = vec[OR_Y], *YN = vec[OR_YN], *FN = vec[OR_FN], *V1 = vec[OR_VT1];
+  double *d = p->dammy;
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (int id = 0; id < p->nc; id++) {
+    double wt = at + rt * fmax_ori(fabs(Y[id]), fabs(YN[id]));
+    double est = 0.8 * (YN[id] - Y[id]) + 0.4 * h * (FN[id] + V1[id]);
+    if (wt == 0.0) bad = 1;
+    d[id] = (est / wt) * (est / wt);
+  }
+  *sum_out = or_det_sum(d, p->nc);
+  return bad ? 3 : 0;
+}
+
+/* ---------------- rkclow, solve_rkc.cu:908-1096 ---------------- */
+static void rkclow(or_pop *p)
+{
+  double **vec = p->vec;
+  double *o = p->others;
+  int m;
+  double err, fac, h, temp1, temp2;
+  const double tdir = 1.0;
+  const double hmax = p->hmax;
+  const double hmin = p->h_work[6];
+  int loop_num = 0;
+
+  while (1) {
+    loop_num++;
+    if (loop_num > 1000) { return; }
+    if (p->newspc) {
+      rkcrho(p);
+      p->jacatt = 1;
+    }
+    p->last = 0;
+    if (1.1 * o[OR_ABSH] >= fabs(o[OR_TEND] - o[OR_LT])) {
+      o[OR_ABSH] = fabs(o[OR_TEND] - o[OR_LT]);
+      p->last = 1;
+    }
+    m = 1 + (int)(sqrt(1.54 * o[OR_ABSH] * o[OR_SPRAD] + 1.0));
+    if (m > o[OR_MMAX]) {
+      m = (int)o[OR_MMAX];
+      o[OR_ABSH] = (m * m - 1) / (1.54 * o[OR_SPRAD]);
+      p->last = 0;
+    }
+    h = tdir * o[OR_ABSH];
+    if (h <= hmin) { h = hmin; }
+
+    step(p, h, m);
+    or_dydt(p, vec[OR_Y], vec[OR_VT1]);
+    p->n_stages += 1;
+    err = 0.0;
+    if (err_estimate(p, h, &err) == 3) { o[OR_IDID] = 3; return; }
+    if (o[OR_IDID] == 3.0) { o[OR_IDID] = 3; return; }
+    err = sqrt(err / (double)p->nc);
+
+    if (err > 1.0) {
+      p->n_rejects++;
+      o[OR_ABSH] = 0.8 * o[OR_ABSH] / (pow(err, 1.0 / 3.0));
+      if (o[OR_ABSH] < hmin) {
+        o[OR_IDID] = 4;
+      } else {
+        p->newspc = !(p->jacatt);
+        continue;
+      }
+    }
+    p->n_steps++;
+    o[OR_NACCPT] += 1;
+    o[OR_LT] += h;
+    o[OR_GT] = o[OR_LT];
+    p->jacatt = 0;
+    int l_nstsig = ((int)(o[OR_NSTSIG]) + 1) % 25;
+    p->newspc = 0;
+    if (l_nstsig == 0) { p->newspc = !(p->jacatt); }
+
+    p->h_work[0] = h;
+    p->h_work[1] = o[OR_LT];
+    double *tmp = vec[OR_VT2];
+    vec[OR_VT2] = vec[OR_FN];
+    vec[OR_FN] = vec[OR_VT1];
+    vec[OR_VT1] = vec[OR_YN];
+    vec[OR_YN] = vec[OR_Y];
+    vec[OR_Y] = tmp;
+
+    fac = 10.0;
+    if (o[OR_NACCPT] == 1) {
+      temp2 = pow(err, 1.0 / 3.0);
+      if (0.8 < fac * temp2) { fac = 0.8 / temp2; }
+    } else {
+      temp1 = 0.8 * o[OR_ABSH] * pow(o[OR_ERROLD], 1.0 / 3.0);
+      temp2 = fabs(o[OR_HOLD]) * pow(err, 2.0 / 3.0);
+      if (temp1 < fac * temp2) { fac = temp1 / temp2; }
+    }
+    o[OR_ABSH] = fmax_ori(0.1, fac) * o[OR_ABSH];
+    o[OR_ABSH] = fmax_ori(hmin, fmin(hmax, o[OR_ABSH]));
+    o[OR_ERROLD] = err;
+    o[OR_HOLD] = h;
+    h = tdir * o[OR_ABSH];
+    o[OR_IDID] = p->last ? 1 : 2;
+    return;
+  }
+}
+
+/* ---------------- initialisation, solve_rkc.cu:437-560, 756-791 ---------------- */
+static void set_initial_vec(or_pop *p)
+{
+  const int nc = p->nc;
+  double **vec = p->vec;
+  for (int i = 0; i < nc; i++) {
+    vec[OR_VNEW][i] = p->v[i];
+    vec[OR_Y][i] = p->v[i];
+    vec[OR_YN][i] = p->v[i];
+    vec[OR_FN][i] = 0.0;
+    vec[OR_EDE][i] = 0.0;
+    vec[OR_VT1][i] = 0.0;
+    vec[OR_VT2][i] = 0.0;
+  }
+  for (int i = 0; i < 8; i++) p->h_work[i] = 0.0;
+  for (int i = 0; i < OR_NOTHERS; i++) p->others[i] = 0.0;
+  p->h_work[2] = nc;
+  p->h_work[3] = UROUND;
+  p->h_work[4] = sqrt(UROUND);
+  p->h_work[5] = p->hmax;
+  p->h_work[6] = p->hmin;
+  p->others[OR_ATOL] = p->others[OR_RTOL] = 0.0;  /* set by caller below */
+  p->others[OR_TEND] = p->hmax;
+  p->last = 1;
+  p->newspc = 1;
+  p->jacatt = 0;
+  or_dydt(p, vec[OR_YN], vec[OR_FN]);
+  memcpy(vec[OR_EDE], vec[OR_FN], sizeof(double) * nc);
+}
+
+static int set_initial_absh(or_pop *p)
+{
+  const int nc = p->nc;
+  double **vec = p->vec;
+  double *dammy = p->dammy;
+  double absh_init = p->hmax;
+  if (p->others[OR_SPRAD] * absh_init > 1.0) absh_init = 1.0 / p->others[OR_SPRAD];
+  absh_init = fmax_ori(absh_init, p->h_work[6]);
+  /* vtemp1 = yn; vtemp1 += absh_init * fn   (cublasDcopy + cublasDaxpy) */
+  for (int i = 0; i < nc; i++) vec[OR_VT1][i] = absh_init * vec[OR_FN][i] + vec[OR_YN][i];
+  or_dydt(p, vec[OR_VT1], vec[OR_VT2]);
+  const double at = p->others[OR_ATOL];
+  for (int i = 0; i < nc; i++) {
+    double wt = at + p->others[OR_RTOL] * fabs(vec[OR_YN][i]);
+    if (wt == 0.0) { p->others[OR_IDID] = 3.0; return 3; }
+    dammy[i] = 1.0 / wt;
+  }
+  for (int i = 0; i < nc; i++) {   /* Hadamard_product, solve_rkc.cu:130-141 */
+    double wt = dammy[i];
+    dammy[i] = (vec[OR_VT2][i] - vec[OR_FN][i]) * wt;
+  }
+  double est = dot_self(dammy, nc, dammy);
+  est = absh_init * sqrt(est / nc);
+  if (0.1 * absh_init < p->hmax * sqrt(est)) {
+    absh_init = fmax_ori(0.1 * absh_init / sqrt(est), p->h_work[6]);
+  } else {
+    absh_init = p->hmax;
+  }
+  p->others[OR_ABSH] = absh_init;
+  p->others[OR_MMAX] = nint_ori(sqrt(p->others[OR_RTOL] / (10.0 * UROUND)));
+  return 0;
+}
+
+int or_rkc_init(or_pop *p)
+{
+  /* caller has stored atol/rtol into others[] before calling; keep them */
+  double at = p->others[OR_ATOL], rt = p->others[OR_RTOL];
+  update_b(p, p->v);
+  set_initial_vec(p);
+  p->others[OR_ATOL] = at;
+  p->others[OR_RTOL] = rt;
+  rkcrho(p);
+  if (set_initial_absh(p) == 3) return 3;
+  p->sum_dt = 0.0;
+  p->integ = pow(2.0, log10(p->others[OR_ATOL]));   /* solve_rkc.cu:1265 */
+  return 0;
+}
+
+/* ---------------- macro-step driver, solve_rkc.cu:1252-1299 (and GR/GO/IO analogues) ---- */
+static void rkcint(or_pop *p, double *d_v_new, double hlast, double lt, double d_gt)
+{
+  double **vec = p->vec;
+  double a1, a2, b1, b2, s, tlast;
+  tlast = lt - hlast;
+  s = (d_gt - tlast) / hlast;
+  a1 = (1.0 + 2.0 * s) * (s - 1.0) * (s - 1.0);
+  a2 = (3.0 - 2.0 * s) * s * s;
+  b1 = hlast * s * (s - 1.0) * (s - 1.0);
+  b2 = hlast * (s - 1.0) * s * s;
+  for (int id = 0; id < p->nc; id++)
+    d_v_new[id] = a1 * vec[OR_VT1][id] + a2 * vec[OR_YN][id] + b1 * vec[OR_VT2][id] + b2 * vec[OR_FN][id];
+}
+
+static int trigger(const or_pop *p)
+{
+  /* strict '>' for GR and IO (solve_rkc.cu:1148, :1347), '>=' for GO and PKJ (:1212, :1284) */
+  if (p->type == OR_GR || p->type == OR_IO) return p->sum_dt > p->integ;
+  return p->sum_dt >= p->integ;
+}
+
+int or_rkc_advance(or_pop *p)
+{
+  double **vec = p->vec;
+  if (p->sum_dt != 0.0) {
+    or_model_update_ion_and_b(p, p->sum_dt);
+    p->sum_dt = 0.0;
+  }
+  while (1) {
+    rkclow(p);
+    p->sum_dt += p->others[OR_GT] - p->others[OR_GT_OLD];
+    p->others[OR_GT_OLD] = p->others[OR_GT];
+    if (trigger(p) && p->others[OR_IDID] == 2.0) {
+      or_model_update_ion_and_b(p, p->sum_dt);
+      p->sum_dt = 0.0;
+    }
+    if (p->others[OR_IDID] == 2.0) continue;
+    else if (p->others[OR_IDID] == 1.0) break;
+    else return (int)p->others[OR_IDID];
+  }
+  rkcint(p, vec[OR_VNEW], p->h_work[0], p->h_work[1], p->others[OR_GT_OLD]);
+  p->others[OR_TEND] += H_MAX;
+  return 0;
+}
+
+/* ion update + b rebuild: xx_update_ion_and_matrix (solve_rkc.cu:1229-1250 for PKJ) */
+void or_model_update_ion_and_b(or_pop *p, double sum_dt)
+{
+  p->n_ion_updates++;
+  switch (p->type) {
+    case OR_PKJ: or_pkj_update_ion(p, p->vec[OR_Y], sum_dt); break;
+    default: break;
+  }
+  update_b(p, p->vec[OR_Y]);
+}
+
+/* ---------------- stimulus ---------------- */
+void or_set_stim(or_pop *p, int comp, double t0, double t1, const double *amp)
+{
+  p->stim_comp = comp;
+  p->stim_t0 = t0;
+  p->stim_t1 = t1;
+  free(p->stim_amp);
+  p->stim_amp = (double *)malloc(sizeof(double) * (p->n > 0 ? p->n : 1));
+  for (int i = 0; i < p->n; i++) p->stim_amp[i] = amp[i];
+}
+
+void or_set_current(or_pop *p, double t)  /* pkj.cu:333-344 shape */
+{
+  if (!p->stim_amp) return;
+  for (int c = 0; c < p->n; c++) {
+    double *x = &p->iext[p->stim_comp + c * p->comp];
+    *x = 0;
+    if (p->stim_t0 <= t && t < p->stim_t1) *x = p->stim_amp[c];
+  }
+}
+
+/* ---------------- accessors ---------------- */
+void or_pop_set_integ(or_pop *p, double integ) { p->integ = integ; }
+int or_pop_nc(const or_pop *p) { return p->nc; }
+int or_pop_nnz(const or_pop *p) { return p->nnz; }
+
+static double *field(const or_pop *p, int what)
+{
+  if (what >= 0 && what < OR_NVEC) return p->vec[what];
+  switch (what) {
+    case 100: return p->v;
+    case 101: return p->ca;
+    case 102: return p->rev;
+    case 103: return p->b;
+    case 104: return p->cm;
+    case 105: return p->iext;
+    case 106: return p->shell;
+    case 107: return p->area;
+  }
+  if (what >= 200 && what < 200 + p->nion) return p->ion[what - 200];
+  if (what >= 300 && what < 300 + p->ncond) return p->cond[what - 300];
+  return NULL;
+}
+
+void or_pop_get(const or_pop *p, int what, double *dst)
+{
+  double *f = field(p, what);
+  if (f) memcpy(dst, f, sizeof(double) * p->nc);
+}
+
+void or_pop_set(or_pop *p, int what, const double *src)
+{
+  double *f = field(p, what);
+  if (f) memcpy(f, src, sizeof(double) * p->nc);
+}
+
+void or_pop_controller(const or_pop *p, double *o14, double *hw8, int *fl3, double *sumdt)
+{
+  memcpy(o14, p->others, sizeof(double) * OR_NOTHERS);
+  memcpy(hw8, p->h_work, sizeof(double) * 8);
+  fl3[0] = p->last; fl3[1] = p->newspc; fl3[2] = p->jacatt;
+  *sumdt = p->sum_dt;
+}
+
+void or_pop_stats(const or_pop *p, long *out)
+{
+  out[0] = p->n_steps; out[1] = p->n_rejects; out[2] = p->n_stages;
+  out[3] = p->n_rho_iters; out[4] = p->n_ion_updates;
+}
+
+void or_pop_set_threads(or_pop *p, int nthreads)
+{
+  p->nthreads = nthreads;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+}
+
+void or_pop_csr(const or_pop *p, int *row, int *col, double *val)
+{
+  memcpy(row, p->row, sizeof(int) * (p->nc + 1));
+  memcpy(col, p->col, sizeof(int) * p->nnz);
+  memcpy(val, p->val, sizeof(double) * p->nnz);
+}
+
+void or_pop_enable_syn(or_pop *p, double e_syn)
+{
+  if (!p->syn_g) p->syn_g = (double *)calloc(p->nc, sizeof(double));
+  p->syn_e = e_syn;
+}
+
+void or_pop_free(or_pop *p)
+{
+  if (!p) return;
+  free(p->v); free(p->ca); free(p->cm); free(p->area); free(p->iext); free(p->shell);
+  free(p->rev); free(p->compart);
+  for (int i = 0; i < p->ncond; i++) free(p->cond[i]);
+  for (int i = 0; i < p->nion; i++) free(p->ion[i]);
+  free(p->row); free(p->col); free(p->val); free(p->b); free(p->dammy);
+  for (int i = 0; i < OR_NVEC; i++) free(p->vec[i]);
+  free(p->stim_amp);
+  free(p->syn_g);
+  free(p);
+}

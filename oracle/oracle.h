@@ -1,0 +1,113 @@
+/*
+ * CPU FP64 oracle for the RKC time-stepping path of the cerebellar
+ * multi-compartment simulator.
+ *
+ * THIS IS TEST INFRASTRUCTURE.  It is a plain-C restatement of the reference
+ * algorithm, written from the reference sources read as text, and it is used
+ * only as the checker (tests, the smoke check and the CPU-baseline leg of the
+ * benchmark).  Nothing in the product links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates.  The arithmetic
+ * order of each expression follows the reference expression, with
+ * floating-point contraction disabled (-ffp-contract=off), so that the HIP
+ * FP64 path can be compared against it at the ulp level for short runs.
+ *
+ * Pinning status of each part is recorded in the header of the file that
+ * implements it.
+ */
+#ifndef CB_ORACLE_H
+#define CB_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* cell types, param.h:15 */
+enum { OR_GR = 0, OR_GO = 1, OR_PKJ = 2, OR_IO = 3 };
+
+/* RKC vector slots, param.h:21 */
+enum { OR_Y = 0, OR_YN, OR_FN, OR_VT1, OR_VT2, OR_EDE, OR_VNEW, OR_NVEC };
+/* scalar controller slots, param.h:22 */
+enum { OR_MMAX = 0, OR_ABSH, OR_SPRAD, OR_NACCPT, OR_ATOL, OR_RTOL, OR_IDID,
+       OR_GT, OR_GT_OLD, OR_LT, OR_TEND, OR_ERROLD, OR_HOLD, OR_NSTSIG,
+       OR_NOTHERS };
+
+#define OR_MAX_COND 16
+#define OR_MAX_ION 24
+
+typedef struct or_pop {
+  int type;           /* OR_PKJ, ... */
+  int n, nx, ny;      /* cells */
+  int comp;           /* compartments per cell */
+  int nc;             /* n * comp */
+  int ncond, nion;
+  /* compartment state, structure of arrays (param.h:27-35 elem/ion/cond) */
+  double *v, *ca, *cm, *area, *iext, *shell, *rev;
+  int *compart;       /* compartment type */
+  double *cond[OR_MAX_COND];
+  double *ion[OR_MAX_ION];
+  /* cable matrix in CSR (signed for RKC: off-diagonal +g, diagonal -sum g) */
+  int nnz;
+  int *row, *col;
+  double *val;
+  double *b;          /* frozen ionic + external + synaptic current */
+  double *dammy;
+  double *vec[OR_NVEC];
+  double *syn_g;      /* optional per-compartment synaptic conductance [mS] */
+  double syn_e;       /* its reversal potential [mV] */
+  /* RKC controller (solve_rkc.cu:437-560) */
+  double h_work[8];
+  double others[OR_NOTHERS];
+  int last, newspc, jacatt;
+  double sum_dt, integ;
+  double hmin, hmax;
+  /* stimulus (pkj.cu:333-344 shape: one compartment per cell, a window) */
+  int stim_comp;
+  double stim_t0, stim_t1;
+  double *stim_amp;   /* per cell */
+  /* statistics */
+  long n_steps, n_rejects, n_stages, n_rho_iters, n_ion_updates;
+  int nthreads;
+} or_pop;
+
+/* ---- deterministic sums (or_rkc.c) ---- */
+double or_det_sum(const double *x, long n);
+
+/* ---- generic RKC (or_rkc.c), solve_rkc.cu ---- */
+void or_dydt(or_pop *p, const double *y, double *dy);
+int  or_rkc_init(or_pop *p);                /* RKC_vec_initialize */
+int  or_rkc_advance(or_pop *p);             /* xx_solve_by_rkc, one H_MAX macro step */
+void or_pop_free(or_pop *p);
+
+/* ---- Purkinje (or_pkj.c) ---- */
+or_pop *or_pkj_create(const char *morph_file, const char *gj_file, int nx, int ny,
+                      const double *v0_per_cell, double atol, double rtol,
+                      int use_libc_rand);
+void or_pkj_update_b(or_pop *p, const double *vv);
+void or_pkj_update_ion(or_pop *p, const double *vnew, double dt);
+double or_pkj_rate(int which, double v, double ca);
+
+/* ---- stimulus ---- */
+void or_set_stim(or_pop *p, int comp, double t0, double t1, const double *amp_per_cell);
+void or_set_current(or_pop *p, double t);
+
+/* ---- accessors for tests ---- */
+int  or_pop_nc(const or_pop *p);
+void or_pop_set_integ(or_pop *p, double integ);
+void or_pop_get(const or_pop *p, int what, double *dst);
+void or_pop_set(or_pop *p, int what, const double *src);
+void or_pop_controller(const or_pop *p, double *others14, double *hwork8, int *flags3, double *sumdt);
+void or_pop_stats(const or_pop *p, long *out5);
+void or_pop_set_threads(or_pop *p, int nthreads);
+int  or_pop_nnz(const or_pop *p);
+void or_pop_csr(const or_pop *p, int *row, int *col, double *val);
+void or_pop_enable_syn(or_pop *p, double e_syn);
+
+/* ---- Philox4x32-10 (or_philox.c), Salmon et al., SC'11 ---- */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
