@@ -63,6 +63,7 @@ class Config(ctypes.Structure):
         ("poisson_e_mV", ctypes.c_double),
         ("poisson_seed", ctypes.c_uint64),
         ("ion_interval_ms", ctypes.c_double),
+        ("refresh_fn", ctypes.c_int),
         ("device", ctypes.c_int),
     ]
 
@@ -136,7 +137,8 @@ class Population:
     """One cell population with its solver (the reference's neuron_t + neuron_solve_t)."""
 
     def __init__(self, cell_type=PKJ, nx=1, ny=1, solver=RKC, engine=ENGINE_GLOBAL_FP64, morph=None, gj=None,
-                 atol=None, rtol=None, v0=None, stim=None, poisson=None, ion_interval=None, device=0):
+                 atol=None, rtol=None, v0=None, stim=None, poisson=None, ion_interval=None, refresh_fn=False,
+                 device=0):
         L = lib()
         cfg = Config()
         _check(L.cb_config_default(ctypes.byref(cfg), cell_type, solver, nx, ny))
@@ -169,6 +171,7 @@ class Population:
             cfg.poisson_seed = poisson.get("seed", 0)
         if ion_interval:
             cfg.ion_interval_ms = ion_interval
+        cfg.refresh_fn = 1 if refresh_fn else 0
         cfg.device = device
         h = ctypes.c_void_p()
         _check(L.cb_pop_create(ctypes.byref(cfg), ctypes.byref(h)))

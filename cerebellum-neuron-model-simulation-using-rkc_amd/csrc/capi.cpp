@@ -134,6 +134,10 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     sol.atol = cfg->atol > 0 ? cfg->atol : tol;
     sol.rtol = cfg->rtol > 0 ? cfg->rtol : tol;
     sol.ion_interval = cfg->ion_interval_ms;
+    sol.refresh_fn = cfg->refresh_fn != 0;
+    // atol > 0 keeps every error weight positive (the reference asserts on a
+    // zero weight, idid = 3, solve_rkc.cu:50-53)
+    if (!(sol.atol > 0.0) || !(sol.rtol >= 0.0)) return fail(CB_E_ARG, "atol must be > 0 and rtol >= 0");
     std::unique_ptr<cb_pop> p(new cb_pop);
     p->cell_type = cfg->cell_type;
     p->solver = cfg->solver;

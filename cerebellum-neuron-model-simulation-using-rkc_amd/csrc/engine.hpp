@@ -35,6 +35,7 @@ enum Field {
   F_Y = 6,        // RKC vec[y]
   F_YN = 7,       // RKC vec[yn_r]
   F_FN = 8,       // RKC vec[fn]
+  F_CTL = 50,     // per-cell engines: the per-cell controller records (ncell x 32 doubles)
   F_GATE0 = 100,  // + gate index
 };
 
@@ -76,6 +77,7 @@ class Engine {
 struct SolverSpec {
   double atol = 1e-6, rtol = 1e-6;
   double ion_interval = -1.0;   // <= 0: 2^log10(atol) (solve_rkc.cu:1265)
+  bool refresh_fn = false;      // recompute fn = F(y_n) after each ion update (off in the reference)
 };
 
 Engine *make_global_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);

@@ -1,7 +1,1065 @@
-// placeholder, replaced by the per-cell engine
+// Per-cell adaptive RKC in one persistent launch: one workgroup advances one
+// Purkinje cell through many macro steps with its own step-size controller.
+//
+// Each cell is the reference's population of one cell (solve_rkc.cu run on
+// a single neuron): the same Chebyshev recurrence (step, :815-906), the same
+// error estimate and step control (rkclow, :908-1096), the same nonlinear
+// power method (rkcrho, :562-754), the same operator-split ion update
+// cadence (pkj_solve_by_rkc, :1252-1299) and quirks (Y_{m-2} as the voltage
+// handed to the ion update; spectral radius refreshed only after a reject;
+// steps at hmin always accepted).  What changes is the mapping:
+//
+//   * compartments are renumbered breadth-first from the soma, so the
+//     children of a compartment are contiguous and the cable stencil is
+//     parent + child range; thread t owns compartments t, t+NT, ...
+//     (coalesced when the gate planes are streamed);
+//   * the stage recurrence runs on increments D_j = Y_j - Y_0 kept in
+//     registers, with D_{j-1} staged in LDS (double buffered, one barrier
+//     per stage) for the neighbour gather; nothing touches HBM inside the
+//     stage loop;
+//   * y_n is a compensated pair (hi + lo) updated with TwoSum, so the
+//     mixed-precision engine keeps ~48 bits of voltage while every stage,
+//     gate and current is FP32;
+//   * every norm is a fixed-order workgroup reduction in FP64, and the
+//     controller arithmetic is FP64;
+//   * gates, Ca and the reversal potential stay in HBM and are streamed
+//     only at ion updates; Ca and its quantisation stay FP64.
+//
+// T = double gives the per-cell FP64 engine; T = float the mixed engine.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <vector>
+
 #include "engine.hpp"
+#include "philox.hpp"
+#include "pkj_model.hpp"
+
 namespace cbrkc {
-Engine *make_cell_pkj(const PkjHost &, const SolverSpec &, const StimSpec &, const PoissonSpec &, bool) {
-  throw Error("per-cell engine not built");
+namespace {
+
+constexpr double UROUND = 2.22e-16;
+constexpr double H_MAX = 0.125;
+constexpr int NO_PARENT = 0x7FF;
+
+// per-cell controller record (doubles)
+enum Ctl {
+  C_ABSH, C_SPRAD, C_NACCPT, C_ERROLD, C_HOLD, C_LT, C_TEND, C_GT, C_GTOLD, C_SUMDT, C_MMAX, C_HLAST, C_LTLAST,
+  C_NEWSPC, C_JACATT, C_LAST, C_IDID, C_STEPS, C_REJECTS, C_STAGES, C_RHO, C_IONS, C_MACRO, C_STATUS, NCTL = 32
+};
+
+template <typename T>
+struct CellArgs {
+  int ncell, comp;
+  double atol, rtol, hmin, hmax, integ;
+  int refresh_fn;         // 1: fn = F(y_n) after each ion update
+  // static, per compartment, cell-major, breadth-first order
+  const uint32_t *topo;   // parent | first_child << 11 | nchild << 22
+  const T *gpar;          // conductance to the parent [mS]
+  const T *invc;          // 1 / Cm
+  const int8_t *cls;      // compartment type
+  const T *area, *leak_area, *shell;
+  const int *orig;        // original local index
+  // state
+  T *y0hi, *y0lo, *f0, *b, *vold, *vy, *ede;
+  T *gate;                // [NGATE][ncell * comp]
+  double *ca;
+  T *rev;
+  T *syn_g;               // may be null
+  double *ctl;            // [ncell][NCTL]
+  double *vnew_out;       // [ncell * comp] original order, written at launch end
+  // stimulus
+  int stim_orig;          // original local index of the stimulated compartment
+  double stim_t0, stim_t1;
+  const double *amp;      // per cell
+  // Poisson drive
+  int syn_on;
+  T syn_decay, syn_g_event, syn_e;
+  double syn_p;
+  uint32_t syn_k0, syn_k1;
+  // recording
+  int rec_orig;           // original local index of the recorded compartment
+  double *rec;            // [nsteps][ncell] or null
+  int *queue;             // work counter
+};
+
+template <typename T> __device__ inline T absT(T x) { return x < T(0) ? -x : x; }
+
+// The workgroup-wide values (reduction results, the cell index) are equal in
+// every lane; saying so lets the compiler keep the controller in SGPRs.
+__device__ inline double uniform(double x) {
+  const int lo = __builtin_amdgcn_readfirstlane(__double2loint(x));
+  const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(x));
+  return __hiloint2double(hi, lo);
 }
+
+// fixed-order workgroup sum in FP64: xor butterfly inside each wave, then
+// the wave totals in index order.  Returns the same value in every thread.
+template <int NT>
+__device__ inline double wg_sum(double x, double *red, int &slot) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off);
+  const int w = threadIdx.x >> 6;
+  double *r = red + slot * (NT / 64);
+  slot ^= 1;
+  if ((threadIdx.x & 63) == 0) r[w] = x;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < NT / 64; i++) s += r[i];
+  return uniform(s);
 }
+
+template <typename T>
+struct Lds {
+  T *d0, *d1;        // stage buffers
+  T *gp;             // gpar
+  T *ic;             // invc
+  uint32_t *tp;      // topology
+};
+
+// sum_nb g (D_nb - D_i) with D in LDS buffer cur
+template <typename T>
+__device__ inline T lap(const Lds<T> &L, const T *cur, int i, T di) {
+  const uint32_t tp = L.tp[i];
+  const int p = tp & 0x7FF, fc = (tp >> 11) & 0x7FF, nch = (tp >> 22) & 3;
+  T s = T(0);
+  if (p != NO_PARENT) s += L.gp[i] * (cur[p] - di);
+  for (int c = 0; c < nch; c++) s += L.gp[fc + c] * (cur[fc + c] - di);
+  return s;
+}
+
+template <typename T>
+__device__ inline void two_sum_acc(T &hi, T &lo, T d) {
+  if constexpr (sizeof(T) == 8) {
+    hi = hi + d;     // FP64 engine: plain accumulation, as the reference
+    (void)lo;
+  } else {
+    T s = hi + d;
+    T bp = s - hi;
+    T e = (hi - (s - bp)) + (d - bp);
+    T l = lo + e;
+    hi = s + l;
+    lo = l - (hi - s);
+  }
+}
+
+// Occupancy target: 4 waves per SIMD for FP32 (4 cells per CU), 2 for FP64.
+template <typename T, int NT, int K>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2, 8)))
+k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
+  using namespace pkj;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int C = a.comp;
+  Lds<T> L;
+  L.d0 = (T *)smem;
+  L.d1 = L.d0 + C;
+  L.gp = L.d1 + C;
+  L.ic = L.gp + C;
+  L.tp = (uint32_t *)(L.ic + C);
+  double *red = (double *)(((uintptr_t)(L.tp + C) + 15) & ~(uintptr_t)15);
+  __shared__ int s_cell;
+  const int tid = threadIdx.x;
+  int slot = 0;
+
+  while (true) {
+    if (tid == 0) s_cell = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int cell = __builtin_amdgcn_readfirstlane(s_cell);
+    __syncthreads();
+    if (cell >= a.ncell) break;
+    const size_t base = (size_t)cell * C;
+    const size_t NCt = (size_t)a.ncell * C;
+    double *ctl = a.ctl + (size_t)cell * NCTL;
+
+    for (int i = tid; i < C; i += NT) {
+      L.gp[i] = a.gpar[base + i];
+      L.ic[i] = a.invc[base + i];
+      L.tp[i] = a.topo[base + i];
+    }
+
+    // register-resident per-compartment state
+    T hi[K], lo[K], F0[K], A[K], D1[K], D2[K], VY[K];
+    uint32_t rec_mask = 0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int i = tid + k * NT;
+      if (i < C) {
+        if (a.orig[base + i] == a.rec_orig) rec_mask |= 1u << k;
+        hi[k] = a.y0hi[base + i];
+        lo[k] = a.y0lo[base + i];
+        F0[k] = a.f0[base + i];
+        VY[k] = a.vy[base + i];
+      } else {
+        hi[k] = lo[k] = F0[k] = VY[k] = T(0);
+      }
+      A[k] = D1[k] = D2[k] = T(0);
+    }
+    __syncthreads();
+
+    // ---- controller state (uniform across the workgroup) ----
+    double absh = ctl[C_ABSH], sprad = ctl[C_SPRAD], naccpt = ctl[C_NACCPT], errold = ctl[C_ERROLD],
+           hold = ctl[C_HOLD], lt = ctl[C_LT], tend = ctl[C_TEND], gt = ctl[C_GT], gt_old = ctl[C_GTOLD],
+           sum_dt = ctl[C_SUMDT], mmax = ctl[C_MMAX], hlast = ctl[C_HLAST], ltlast = ctl[C_LTLAST];
+    bool newspc = ctl[C_NEWSPC] != 0.0, jacatt = ctl[C_JACATT] != 0.0, last = ctl[C_LAST] != 0.0;
+    double idid = ctl[C_IDID];
+    double n_steps = ctl[C_STEPS], n_rej = ctl[C_REJECTS], n_stage = ctl[C_STAGES], n_rho = ctl[C_RHO],
+           n_ion = ctl[C_IONS], n_macro = ctl[C_MACRO];
+    int status = (int)ctl[C_STATUS];
+    const double at = a.atol, rt = a.rtol, hmin = a.hmin, hmax = a.hmax;
+    double t_cur = t0;            // time of the last set_current
+    bool a_valid = false;         // A = F(y_n) with the current b
+
+    // A = F(y_n) = (-b + L y_n) / C with y_n = hi + lo
+    auto compute_A = [&]() {
+      T *h = L.d0, *l = L.d1;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int i = tid + k * NT;
+        if (i < C) {
+          h[i] = hi[k];
+          if constexpr (sizeof(T) == 4) l[i] = lo[k];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int i = tid + k * NT;
+        if (i < C) {
+          T s = lap<T>(L, h, i, hi[k]);
+          if constexpr (sizeof(T) == 4) s += lap<T>(L, l, i, lo[k]);
+          A[k] = (-a.b[base + i] + s) * L.ic[i];
+        }
+      }
+      __syncthreads();
+      a_valid = true;
+    };
+
+    // pkj_update_ion_RKC (pkj_ion.cu:304-358) when do_ion, then the frozen
+    // current b (pkj_update_matrix, solve_rkc.cu:369-395).  Streams the
+    // compartment's gates from HBM; one compartment per iteration so the
+    // channel arithmetic does not multiply register pressure by K.
+    auto ion_and_b = [&](double dt, bool do_ion, bool stim_live) {
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int i = tid + k * NT;
+        if (i < C) a.vy[base + i] = VY[k];
+      }
+#pragma unroll 1
+      for (int i = tid; i < C; i += NT) {
+        const size_t gi = base + i;
+        T g[NGATE];
+#pragma unroll
+        for (int q = 0; q < NGATE; q++) g[q] = a.gate[(size_t)q * NCt + gi];
+        const int cl = a.cls[gi];
+        const int row = (cl == SOMA) ? 0 : (cl == MAIN) ? 1 : 2;
+        const T ar = a.area[gi];
+        T c[NCOND];
+        c[g_leak] = (T)(1.0 / ((cl == SOMA) ? RM_S : RM_D)) * a.leak_area[gi];
+#pragma unroll
+        for (int q = 0; q < 10; q++) c[g_NaF + q] = (T)DENSITY[row][q] * ar;
+        T rev = a.rev[gi];
+        const T vn = a.vy[gi];
+        const int og = a.orig[gi];
+        if (do_ion) {
+          const T vo = a.vold[gi];
+          double ca = a.ca[gi];
+          double rv = (double)rev;
+          const long gid = (long)cell * COMP + og;
+          double caq = ca_rk4((double)vo, ca, rv, (double)a.shell[gi], (double)(c[g_CaP] * g[m_CaP] * g[h_CaP]),
+                              (double)(c[g_CaT] * g[m_CaT] * g[h_CaT]), dt, gid);
+          rev = (T)rv;
+          gates_cn<T>(T(0.5) * (vn + vo), (T)caq, (T)dt, g);
+#pragma unroll
+          for (int q = 0; q < NGATE; q++) a.gate[(size_t)q * NCt + gi] = g[q];
+          a.ca[gi] = ca;
+          a.rev[gi] = rev;
+          a.vold[gi] = vn;
+        }
+        const T iext = (stim_live && og == a.stim_orig && a.stim_t0 <= t_cur && t_cur < a.stim_t1)
+                           ? (T)a.amp[cell] : T(0);
+        T bb = current<T>(vn, iext, rev, c, g);
+        if (a.syn_on) bb += a.syn_g[gi] * (vn - a.syn_e);
+        a.b[gi] = bb;
+      }
+      a_valid = false;
+    };
+
+    // rkcrho, solve_rkc.cu:562-754, carried on the perturbation
+    // delta = v - y_n (in D1); D2 holds G = F(y_n + delta) - fn.
+    auto rkcrho = [&]() {
+      const int itmax = 1000;
+      double sprad_old[3] = {0.0, 0.0, 0.0};
+      const double sqrtu = sqrt(UROUND), small = 1.0 / 0.125;
+      double sy = 0.0, se = 0.0;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int i = tid + k * NT;
+        D1[k] = (i < C) ? a.ede[base + i] : T(0);
+        double y = (double)hi[k] + (double)lo[k];
+        sy += y * y;
+        se += (double)D1[k] * (double)D1[k];
+      }
+      double ynrm = sqrt(wg_sum<NT>(sy, red, slot));
+      double vnrm = sqrt(wg_sum<NT>(se, red, slot));
+      double dynrm;
+      if (ynrm != 0.0 && vnrm != 0.0) {
+        dynrm = ynrm * sqrtu;
+        const T al = (T)(dynrm / vnrm);
+#pragma unroll
+        for (int k = 0; k < K; k++) D1[k] = al * D1[k];
+      } else if (ynrm != 0.0) {
+        dynrm = ynrm * sqrtu;
+#pragma unroll
+        for (int k = 0; k < K; k++) D1[k] = (T)sqrtu * hi[k];
+      } else if (vnrm != 0.0) {
+        dynrm = UROUND;
+        const T al = (T)(dynrm / vnrm);
+#pragma unroll
+        for (int k = 0; k < K; k++) D1[k] = al * D1[k] - hi[k];
+      } else {
+        dynrm = UROUND;
+#pragma unroll
+        for (int k = 0; k < K; k++) D1[k] = (T)dynrm - hi[k];
+      }
+      T *cur = L.d0, *nxt = L.d1;
+      double sigma = 0.0, sigmal;
+      for (int iter = 0; iter < itmax; iter++) {
+        n_rho += 1;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          const int i = tid + k * NT;
+          if (i < C) cur[i] = D1[k];
+        }
+        __syncthreads();
+        double sq = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          const int i = tid + k * NT;
+          D2[k] = T(0);
+          if (i < C) {
+            const T G = (A[k] - F0[k]) + lap<T>(L, cur, i, D1[k]) * L.ic[i];
+            D2[k] = G;
+            sq += (double)G * (double)G;
+          }
+        }
+        const double dfnrm = sqrt(wg_sum<NT>(sq, red, slot));
+        T *tmp = cur; cur = nxt; nxt = tmp;
+        sigmal = sigma;
+        sigma = dfnrm / dynrm;
+        sprad_old[0] = sprad_old[1];
+        sprad_old[1] = sprad_old[2];
+        sprad_old[2] = sprad;
+        sprad = 1.2 * sigma;
+        bool done = iter >= 1 && fabs(sigma - sigmal) <= fmax(sigma, small) * 0.001;
+        if (!done && iter > itmax / 2 && fabs(sprad_old[2] - sprad_old[0]) < 0.00001 &&
+            fabs(sprad_old[1] - sprad) < 0.00001) {
+          sprad = (sprad + sprad_old[2]) / 2.0;
+          done = true;
+        }
+        if (done) {
+#pragma unroll
+          for (int k = 0; k < K; k++) {
+            const int i = tid + k * NT;
+            if (i < C) a.ede[base + i] = D1[k];
+          }
+          return;
+        }
+        if (dfnrm != 0.0) {
+          const T al = (T)(dynrm / dfnrm);
+#pragma unroll
+          for (int k = 0; k < K; k++) D1[k] = al * (-D2[k]);   // v = y_n + alpha (fn - F(v))
+        } else {
+          const int idx = iter % C;
+#pragma unroll
+          for (int k = 0; k < K; k++)
+            if (tid + k * NT == idx) D1[k] = -D1[k];
+        }
+      }
+      idid = 6;
+    };
+
+    // rkclow, solve_rkc.cu:908-1096
+    auto rkclow = [&]() {
+      int loop_num = 0;
+      while (true) {
+        if (++loop_num > 1000) return;
+        if (!a_valid) compute_A();
+        if (newspc) {
+          rkcrho();
+          jacatt = true;
+        }
+        last = false;
+        if (1.1 * absh >= fabs(tend - lt)) {
+          absh = fabs(tend - lt);
+          last = true;
+        }
+        int m = 1 + (int)(sqrt(1.54 * absh * sprad + 1.0));
+        if (m > mmax) {
+          m = (int)mmax;
+          absh = (m * m - 1) / (1.54 * sprad);
+          last = false;
+        }
+        double h = absh;
+        if (h <= hmin) h = hmin;
+        // ---- step(h, m), solve_rkc.cu:815-906, on increments D_j = Y_j - Y_0 ----
+        const double w0 = 1.0 + 2.0 / (13.0 * m * m);
+        const double temp1 = w0 * w0 - 1.0, temp2 = sqrt(temp1);
+        const double arg = m * log(w0 + temp2);
+        const double w1 = sinh(arg) * temp1 / (cosh(arg) * m * temp2 - w0 * sinh(arg));
+        double bjm1 = 1.0 / ((2.0 * w0) * (2.0 * w0)), bjm2 = bjm1;
+        T *cur = L.d0, *nxt = L.d1;
+        {
+          const T hm = (T)(h * (w1 * bjm1));
+#pragma unroll
+          for (int k = 0; k < K; k++) {
+            D2[k] = T(0);
+            D1[k] = hm * F0[k];
+            VY[k] = hi[k];   // Y_{m-2} when m == 2
+            const int i = tid + k * NT;
+            if (i < C) cur[i] = D1[k];
+          }
+        }
+        __syncthreads();
+        double zjm1 = w0, zjm2 = 1.0, dzjm1 = 1.0, dzjm2 = 0.0, d2zjm1 = 0.0, d2zjm2 = 0.0;
+        for (int j = 1; j < m; j++) {
+          const double zj = 2.0 * w0 * zjm1 - zjm2;
+          const double dzj = 2.0 * w0 * dzjm1 - dzjm2 + 2.0 * zjm1;
+          const double d2zj = 2.0 * w0 * d2zjm1 - d2zjm2 + 4.0 * dzjm1;
+          const double bj = d2zj / (dzj * dzj);
+          const double ajm1 = 1.0 - zjm1 * bjm1;
+          const double mu = 2.0 * w0 * bj / bjm1;
+          const double nu = -bj / bjm2;
+          const double musj = mu * w1 / w0;
+          const T tmu = (T)mu, tnu = (T)nu, thm = (T)(h * musj), taj = (T)ajm1;
+          const bool lastj = (j == m - 1);
+#pragma unroll
+          for (int k = 0; k < K; k++) {
+            const int i = tid + k * NT;
+            if (i < C) {
+              const T F = A[k] + lap<T>(L, cur, i, D1[k]) * L.ic[i];
+              const T Dn = tmu * D1[k] + tnu * D2[k] + thm * (F - taj * F0[k]);
+              if (lastj) VY[k] = hi[k] + (lo[k] + D2[k]);   // Y_{m-2}
+              D2[k] = D1[k];
+              D1[k] = Dn;
+              nxt[i] = Dn;
+            }
+          }
+          __syncthreads();
+          T *tmp = cur; cur = nxt; nxt = tmp;
+          bjm2 = bjm1; bjm1 = bj;
+          zjm2 = zjm1; zjm1 = zj;
+          dzjm2 = dzjm1; dzjm1 = dzj;
+          d2zjm2 = d2zjm1; d2zjm1 = d2zj;
+        }
+        n_stage += m + 1;
+        // ---- f(Y_m) (into D2) and the error estimate, solve_rkc.cu:58-73 ----
+        // (every weight is positive: atol > 0 is enforced at creation)
+        double sq = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          const int i = tid + k * NT;
+          D2[k] = T(0);
+          if (i < C) {
+            const T Fm = A[k] + lap<T>(L, cur, i, D1[k]) * L.ic[i];
+            D2[k] = Fm;
+            const T yn = hi[k] + lo[k];
+            const T y = hi[k] + (lo[k] + D1[k]);
+            const T ay = absT(y), ayn = absT(yn);
+            const double wt = at + rt * (double)((ay > ayn) ? ay : ayn);
+            const double est = 0.8 * (double)(-D1[k]) + 0.4 * h * ((double)F0[k] + (double)Fm);
+            const double q = est / wt;
+            sq += q * q;
+          }
+        }
+        double err = wg_sum<NT>(sq, red, slot);
+        err = sqrt(err / (double)C);
+        if (err > 1.0) {
+          n_rej += 1;
+          absh = 0.8 * absh / pow(err, 1.0 / 3.0);
+          if (absh < hmin) {
+            idid = 4;   // falls through to the accept branch, as the reference does
+          } else {
+            newspc = !jacatt;
+            continue;
+          }
+        }
+        // ---- accept ----
+        n_steps += 1;
+        naccpt += 1;
+        lt += h;
+        gt = lt;
+        jacatt = false;
+        newspc = false;   // nstsig is never advanced in the reference
+        hlast = h;
+        ltlast = lt;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          two_sum_acc<T>(hi[k], lo[k], D1[k]);
+          F0[k] = D2[k];
+        }
+        a_valid = false;   // A is rebuilt from y_{n+1}, so rounding does not accumulate
+        double fac = 10.0, tp1, tp2;
+        if (naccpt == 1) {
+          tp2 = pow(err, 1.0 / 3.0);
+          if (0.8 < fac * tp2) fac = 0.8 / tp2;
+        } else {
+          tp1 = 0.8 * absh * pow(errold, 1.0 / 3.0);
+          tp2 = fabs(hold) * pow(err, 2.0 / 3.0);
+          if (tp1 < fac * tp2) fac = tp1 / tp2;
+        }
+        absh = fmax(0.1, fac) * absh;
+        absh = fmax(hmin, fmin(hmax, absh));
+        errold = err;
+        hold = h;
+        idid = last ? 1 : 2;
+        return;
+      }
+    };
+
+    if (mode == 1) {
+      // ---- RKC_vec_initialize (solve_rkc.cu:756-791): b with i_ext = 0 ----
+      ion_and_b(0.0, false, false);
+      compute_A();
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        F0[k] = A[k];
+        const int i = tid + k * NT;
+        if (i < C) a.ede[base + i] = F0[k];
+      }
+      rkcrho();
+      // set_initial_absh (solve_rkc.cu:489-560)
+      double ab = hmax;
+      if (sprad * ab > 1.0) ab = 1.0 / sprad;
+      ab = fmax(ab, hmin);
+      T *cur = L.d0;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int i = tid + k * NT;
+        D1[k] = (T)ab * F0[k];
+        if (i < C) cur[i] = D1[k];
+      }
+      __syncthreads();
+      double sq = 0.0;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int i = tid + k * NT;
+        if (i < C) {
+          const T F = A[k] + lap<T>(L, cur, i, D1[k]) * L.ic[i];
+          const double yn = (double)hi[k] + (double)lo[k];
+          const double wt = at + rt * fabs(yn);
+          const double d = (double)(F - F0[k]) * (1.0 / wt);
+          sq += d * d;
+        }
+      }
+      double est = wg_sum<NT>(sq, red, slot);
+      est = ab * sqrt(est / C);
+      if (0.1 * ab < hmax * sqrt(est))
+        ab = fmax(0.1 * ab / sqrt(est), hmin);
+      else
+        ab = hmax;
+      absh = ab;
+      __syncthreads();
+    } else {
+      for (int s = 0; s < nsteps && status == 0; s++) {
+        t_cur = t0 + s * H_MAX;
+        if (a.syn_on) {
+          for (int i = tid; i < C; i += NT) {
+            const size_t gi = base + i;
+            const u32x4 r = philox4x32_10((uint32_t)(cell * COMP + a.orig[gi]), (uint32_t)(step0 + s), 0x5359u,
+                                          0u, a.syn_k0, a.syn_k1);
+            const double u = u01_53(r.v[0], r.v[1]);
+            a.syn_g[gi] = a.syn_g[gi] * a.syn_decay + ((u < a.syn_p) ? a.syn_g_event : T(0));
+          }
+        }
+        // ---- pkj_solve_by_rkc, solve_rkc.cu:1252-1299 ----
+        if (sum_dt != 0.0) {
+          n_ion += 1;
+          ion_and_b(sum_dt, true, true);
+          if (a.refresh_fn) {   // fn = F(y_n) with the new b
+            compute_A();
+#pragma unroll
+            for (int k = 0; k < K; k++) F0[k] = A[k];
+          }
+          sum_dt = 0.0;
+        }
+        while (true) {
+          rkclow();
+          sum_dt += gt - gt_old;
+          gt_old = gt;
+          if (sum_dt >= a.integ && idid == 2.0) {
+            n_ion += 1;
+            ion_and_b(sum_dt, true, true);
+          if (a.refresh_fn) {   // fn = F(y_n) with the new b
+            compute_A();
+#pragma unroll
+            for (int k = 0; k < K; k++) F0[k] = A[k];
+          }
+            sum_dt = 0.0;
+          }
+          if (idid == 2.0) continue;
+          if (idid == 1.0) break;
+          status = (int)idid;
+          break;
+        }
+        tend += H_MAX;
+        n_macro += 1;
+        // v_new = y_{n+1} (rkcint at s = 1); record the chosen compartment
+        if (a.rec) {
+#pragma unroll
+          for (int k = 0; k < K; k++)
+            if (rec_mask >> k & 1u) a.rec[(size_t)s * a.ncell + cell] = (double)hi[k] + (double)lo[k];
+        }
+      }
+    }
+
+    // ---- write back ----
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int i = tid + k * NT;
+      if (i < C) {
+        const size_t gi = base + i;
+        a.y0hi[gi] = hi[k];
+        a.y0lo[gi] = lo[k];
+        a.f0[gi] = F0[k];
+        a.vy[gi] = VY[k];
+        a.vnew_out[base + a.orig[gi]] = (double)hi[k] + (double)lo[k];
+      }
+    }
+    if (tid == 0) {
+      ctl[C_ABSH] = absh; ctl[C_SPRAD] = sprad; ctl[C_NACCPT] = naccpt; ctl[C_ERROLD] = errold;
+      ctl[C_HOLD] = hold; ctl[C_LT] = lt; ctl[C_TEND] = tend; ctl[C_GT] = gt; ctl[C_GTOLD] = gt_old;
+      ctl[C_SUMDT] = sum_dt; ctl[C_MMAX] = mmax; ctl[C_HLAST] = hlast; ctl[C_LTLAST] = ltlast;
+      ctl[C_NEWSPC] = newspc; ctl[C_JACATT] = jacatt; ctl[C_LAST] = last; ctl[C_IDID] = idid;
+      ctl[C_STEPS] = n_steps; ctl[C_REJECTS] = n_rej; ctl[C_STAGES] = n_stage; ctl[C_RHO] = n_rho;
+      ctl[C_IONS] = n_ion; ctl[C_MACRO] = n_macro; ctl[C_STATUS] = status;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+T *dalloc(size_t n) {
+  void *p = nullptr;
+  CB_HIP(hipMalloc(&p, sizeof(T) * (n ? n : 1)));
+  return (T *)p;
+}
+
+template <typename T>
+void h2d(T *d, const std::vector<T> &h) {
+  CB_HIP(hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+}
+
+template <typename T>
+class CellPkj : public Engine {
+ public:
+  struct Variant {
+    int nt, k;
+  };
+
+  CellPkj(const PkjHost &H, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn)
+      : n_(H.n), C_(H.comp), syn_(syn) {
+    using namespace pkj;
+    CB_HIP(hipStreamCreate(&st_));
+    const int n = n_, C = C_;
+    const size_t NC = (size_t)n * C;
+    nc_ = (int)NC;
+    // breadth-first renumbering per cell: soma first, children contiguous
+    std::vector<int> perm(NC), inv(NC);   // perm[bfs] = original global, inv[orig] = bfs
+    std::vector<std::vector<int>> kids(C);
+    for (int c = 0; c < n; c++) {
+      for (int i = 0; i < C; i++) kids[i].clear();
+      int root = -1;
+      for (int i = 0; i < C; i++) {
+        int p = H.parent[(size_t)c * C + i];
+        if (p < 0) {
+          if (root >= 0) throw Error("cell " + std::to_string(c) + " has more than one root");
+          root = i;
+        } else {
+          kids[p - c * C].push_back(i);
+        }
+      }
+      if (root < 0) throw Error("cell " + std::to_string(c) + " has no root");
+      std::deque<int> q{root};
+      int nb = 0;
+      while (!q.empty()) {
+        int u = q.front();
+        q.pop_front();
+        perm[(size_t)c * C + nb] = c * C + u;
+        inv[(size_t)c * C + u] = c * C + nb;
+        nb++;
+        for (int v : kids[u]) q.push_back(v);
+      }
+      if (nb != C) throw Error("cell " + std::to_string(c) + " is not a single tree");
+    }
+    std::vector<uint32_t> topo(NC);
+    std::vector<T> gpar(NC), invc(NC), area(NC), leak(NC), shell(NC), y0(NC), zero(NC, T(0)), vv(NC), rev(NC);
+    std::vector<int8_t> cls(NC);
+    std::vector<int> orig(NC);
+    std::vector<double> ca(NC);
+    std::vector<T> gate((size_t)NGATE * NC);
+    for (int c = 0; c < n; c++) {
+      // children count / first child in bfs order
+      std::vector<int> fc(C, 0), nch(C, 0);
+      for (int b = 0; b < C; b++) {
+        int og = perm[(size_t)c * C + b];
+        int p = H.parent[og];
+        if (p >= 0) {
+          int pb = inv[p] - c * C;
+          if (nch[pb] == 0) fc[pb] = b;
+          nch[pb]++;
+        }
+      }
+      for (int b = 0; b < C; b++) {
+        const size_t g = (size_t)c * C + b;
+        const int og = perm[g];
+        int p = H.parent[og];
+        int pb = p >= 0 ? inv[p] - c * C : NO_PARENT;
+        if (nch[b] > 3) throw Error("compartment with more than 3 children");
+        if (C > 2047) throw Error("cell too large for the per-cell engine");
+        topo[g] = (uint32_t)pb | ((uint32_t)fc[b] << 11) | ((uint32_t)nch[b] << 22);
+        gpar[g] = (T)H.gpar[og];
+        invc[g] = (T)(1.0 / H.cm[og]);
+        cls[g] = H.compart[og];
+        area[g] = (T)H.area[og];
+        // leak conductance / (1/RM) recovers the spine-corrected area
+        leak[g] = (T)(H.cond[(size_t)g_leak * NC + og] / (1.0 / ((cls[g] == SOMA) ? RM_S : RM_D)));
+        shell[g] = (T)H.shell[og];
+        orig[g] = og - c * C;
+        y0[g] = (T)H.v[og];
+        vv[g] = (T)H.v[og];
+        rev[g] = (T)H.rev[og];
+        ca[g] = H.ca[og];
+        for (int q = 0; q < NGATE; q++) gate[(size_t)q * NC + g] = (T)H.gate[(size_t)q * NC + og];
+      }
+    }
+    perm_ = perm;
+    a_.ncell = n;
+    a_.comp = C;
+    a_.atol = sol.atol;
+    a_.rtol = sol.rtol;
+    a_.hmin = std::pow(2.0, -10.0);
+    a_.hmax = H_MAX;
+    a_.integ = sol.ion_interval > 0 ? sol.ion_interval : std::pow(2.0, std::log10(sol.atol));
+    a_.refresh_fn = sol.refresh_fn ? 1 : 0;
+    a_.topo = put(topo);
+    a_.gpar = put(gpar);
+    a_.invc = put(invc);
+    a_.cls = put(cls);
+    a_.area = put(area);
+    a_.leak_area = put(leak);
+    a_.shell = put(shell);
+    a_.orig = put(orig);
+    a_.y0hi = put(y0);
+    a_.y0lo = put(zero);
+    a_.f0 = put(zero);
+    a_.b = put(zero);
+    a_.vold = put(vv);
+    a_.vy = put(vv);
+    a_.ede = put(zero);
+    a_.gate = put(gate);
+    a_.ca = put(ca);
+    a_.rev = put(rev);
+    a_.syn_g = nullptr;
+    if (syn.on) {
+      a_.syn_g = put(zero);
+    }
+    std::vector<double> ctl((size_t)n * NCTL, 0.0);
+    for (int c = 0; c < n; c++) {
+      double *q = &ctl[(size_t)c * NCTL];
+      q[C_TEND] = H_MAX;
+      q[C_LAST] = 1;
+      q[C_NEWSPC] = 1;
+      q[C_JACATT] = 0;
+      q[C_MMAX] = nint_ori(std::sqrt(sol.rtol / (10.0 * UROUND)));
+    }
+    a_.ctl = put(ctl);
+    vnew_ = dalloc<double>(NC);
+    a_.vnew_out = vnew_;
+    a_.stim_orig = stim.comp;
+    a_.stim_t0 = stim.t0;
+    a_.stim_t1 = stim.t1;
+    std::vector<double> amp = stim.amp;
+    amp.resize(n, pkj::CURRENT);
+    a_.amp = put(amp);
+    a_.syn_on = syn.on ? 1 : 0;
+    a_.syn_decay = (T)std::exp(-H_MAX / syn.tau_ms);
+    a_.syn_g_event = (T)syn.g_event;
+    a_.syn_e = (T)syn.e_syn;
+    a_.syn_p = 1.0 - std::exp(-syn.rate_hz * H_MAX * 1e-3);
+    a_.syn_k0 = (uint32_t)syn.seed;
+    a_.syn_k1 = (uint32_t)(syn.seed >> 32);
+    a_.rec_orig = 1599;
+    a_.rec = nullptr;
+    a_.queue = dalloc<int>(1);
+    variants_ = variants();
+    choose_default();
+    launch(1, 0, 0.0);
+    check_status();
+  }
+
+  ~CellPkj() override {
+    (void)hipStreamSynchronize(st_);
+    for (void *p : bufs_) (void)hipFree(p);
+    (void)hipFree(vnew_);
+    (void)hipFree(a_.queue);
+    if (rec_) (void)hipFree(rec_);
+    (void)hipStreamDestroy(st_);
+  }
+
+  int nc() const override { return nc_; }
+  int ncell() const override { return n_; }
+  Stats stats() const override {
+    std::vector<double> ctl((size_t)n_ * NCTL);
+    CB_HIP(hipMemcpy(ctl.data(), a_.ctl, sizeof(double) * ctl.size(), hipMemcpyDeviceToHost));
+    Stats s;
+    for (int c = 0; c < n_; c++) {
+      const double *q = &ctl[(size_t)c * NCTL];
+      s.steps += (long)q[C_STEPS];
+      s.rejects += (long)q[C_REJECTS];
+      s.stages += (long)q[C_STAGES];
+      s.rho_iters += (long)q[C_RHO];
+      s.ion_updates += (long)q[C_IONS];
+    }
+    s.macro_steps = macro_;
+    return s;
+  }
+
+  void set_current(double t) override { t_next_ = t; }
+
+  void advance() override { run(1, t_next_, -1, nullptr); }
+
+  void run(int nsteps, double t0, int rec_comp, double *rec_out) override {
+    if (nsteps <= 0) return;
+    double *rec = nullptr;
+    if (rec_out) {
+      if (rec_cap_ < (size_t)nsteps * n_) {
+        if (rec_) CB_HIP(hipFree(rec_));
+        rec_cap_ = (size_t)nsteps * n_;
+        rec_ = dalloc<double>(rec_cap_);
+      }
+      rec = rec_;
+      a_.rec_orig = rec_comp;
+    }
+    a_.rec = rec;
+    launch(0, nsteps, t0);
+    a_.rec = nullptr;
+    macro_ += nsteps;
+    if (rec_out) {
+      CB_HIP(hipMemcpyAsync(rec_out, rec, sizeof(double) * (size_t)nsteps * n_, hipMemcpyDeviceToHost, st_));
+    }
+    CB_HIP(hipStreamSynchronize(st_));
+    check_status();
+    t_next_ = t0 + nsteps * H_MAX;
+  }
+
+  void read(int field, double *dst) override {
+    using namespace pkj;
+    const size_t NC = nc_;
+    if (field == F_VNEW) {
+      CB_HIP(hipMemcpy(dst, vnew_, sizeof(double) * NC, hipMemcpyDeviceToHost));
+      return;
+    }
+    std::vector<double> tmp(NC);
+    auto fetchT = [&](const T *src) {
+      std::vector<T> h(NC);
+      CB_HIP(hipMemcpy(h.data(), src, sizeof(T) * NC, hipMemcpyDeviceToHost));
+      for (size_t g = 0; g < NC; g++) dst[perm_[g]] = (double)h[g];
+    };
+    switch (field) {
+      case F_V: fetchT(a_.vold); return;
+      case F_REV: fetchT(a_.rev); return;
+      case F_Y: fetchT(a_.vy); return;
+      case F_FN: fetchT(a_.f0); return;
+      case F_YN: {
+        std::vector<T> h(NC), l(NC);
+        CB_HIP(hipMemcpy(h.data(), a_.y0hi, sizeof(T) * NC, hipMemcpyDeviceToHost));
+        CB_HIP(hipMemcpy(l.data(), a_.y0lo, sizeof(T) * NC, hipMemcpyDeviceToHost));
+        for (size_t g = 0; g < NC; g++) dst[perm_[g]] = (double)h[g] + (double)l[g];
+        return;
+      }
+      case F_CTL:
+        CB_HIP(hipMemcpy(dst, a_.ctl, sizeof(double) * (size_t)n_ * NCTL, hipMemcpyDeviceToHost));
+        return;
+      case F_B: fetchT(a_.b); return;
+      case F_CA: {
+        CB_HIP(hipMemcpy(tmp.data(), a_.ca, sizeof(double) * NC, hipMemcpyDeviceToHost));
+        for (size_t g = 0; g < NC; g++) dst[perm_[g]] = tmp[g];
+        return;
+      }
+      default:
+        if (field >= F_GATE0 && field < F_GATE0 + NGATE) {
+          fetchT(a_.gate + (size_t)(field - F_GATE0) * NC);
+          return;
+        }
+    }
+    throw Error("field not available from the per-cell engine");
+  }
+
+  // Picks the (threads per cell, compartments per thread) mapping: every
+  // variant runs the same short trial from the same state and the fastest
+  // is kept.  State is restored after each trial.
+  void autotune(int trial_steps, double t0) {
+    if (variants_.size() < 2) return;
+    Snapshot snap = snapshot();
+    double best = 1e30;
+    int bi = cur_;
+    for (size_t v = 0; v < variants_.size(); v++) {
+      cur_ = (int)v;
+      launch(0, 1, t0);   // warm
+      restore(snap);
+      CB_HIP(hipStreamSynchronize(st_));
+      auto t1 = std::chrono::steady_clock::now();
+      launch(0, trial_steps, t0);
+      CB_HIP(hipStreamSynchronize(st_));
+      double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+      restore(snap);
+      if (dt < best) {
+        best = dt;
+        bi = (int)v;
+      }
+      tune_log_.push_back({variants_[v].nt, variants_[v].k, dt});
+    }
+    cur_ = bi;
+    release(snap);
+  }
+
+  struct TuneRec {
+    int nt, k;
+    double sec;
+  };
+  std::vector<TuneRec> tune_log_;
+  int chosen_nt() const { return variants_[cur_].nt; }
+
+ private:
+  struct Snapshot {
+    std::vector<std::pair<void *, std::pair<void *, size_t>>> saved;
+  };
+
+  Snapshot snapshot() {
+    Snapshot s;
+    for (auto &m : mutable_bufs()) {
+      void *p = nullptr;
+      CB_HIP(hipMalloc(&p, m.second));
+      CB_HIP(hipMemcpyAsync(p, m.first, m.second, hipMemcpyDeviceToDevice, st_));
+      s.saved.push_back({m.first, {p, m.second}});
+    }
+    return s;
+  }
+  void restore(const Snapshot &s) {
+    for (auto &e : s.saved)
+      CB_HIP(hipMemcpyAsync(e.first, e.second.first, e.second.second, hipMemcpyDeviceToDevice, st_));
+  }
+  void release(Snapshot &s) {
+    CB_HIP(hipStreamSynchronize(st_));
+    for (auto &e : s.saved) (void)hipFree(e.second.first);
+    s.saved.clear();
+  }
+  std::vector<std::pair<void *, size_t>> mutable_bufs() {
+    const size_t NC = nc_;
+    std::vector<std::pair<void *, size_t>> v = {
+        {a_.y0hi, sizeof(T) * NC}, {a_.y0lo, sizeof(T) * NC}, {a_.f0, sizeof(T) * NC}, {a_.b, sizeof(T) * NC},
+        {a_.vold, sizeof(T) * NC}, {a_.vy, sizeof(T) * NC},   {a_.ede, sizeof(T) * NC},
+        {a_.gate, sizeof(T) * NC * pkj::NGATE},               {a_.ca, sizeof(double) * NC},
+        {a_.rev, sizeof(T) * NC},  {a_.ctl, sizeof(double) * (size_t)n_ * NCTL}};
+    if (a_.syn_g) v.push_back({a_.syn_g, sizeof(T) * NC});
+    return v;
+  }
+
+  static int nint_ori(double x) {
+    if (x - (double)(int)x < 0.5) return (int)(double)(int)x;
+    return (int)(double)(int)(x + 0.9);
+  }
+
+  template <typename U>
+  U *put(const std::vector<U> &h) {
+    U *d = dalloc<U>(h.size());
+    h2d(d, h);
+    bufs_.push_back((void *)d);
+    return d;
+  }
+
+  static std::vector<Variant> variants() {
+    std::vector<Variant> v;
+    if (sizeof(T) == 4) {
+      v.push_back({256, 7});
+      v.push_back({320, 5});
+      v.push_back({512, 4});
+    } else {
+      v.push_back({256, 7});
+      v.push_back({320, 5});
+    }
+    return v;
+  }
+
+  size_t lds_bytes() const { return (size_t)C_ * (4 * sizeof(T) + 4) + 16 + 2 * 16 * sizeof(double); }
+
+  void choose_default() {
+    cur_ = 0;
+    for (size_t v = 0; v < variants_.size(); v++)
+      if (variants_[v].nt == (sizeof(T) == 4 ? 320 : 256)) cur_ = (int)v;
+  }
+
+  template <int NT, int K>
+  void go(int mode, int nsteps, double t0, int grid) {
+    hipLaunchKernelGGL((k_cell<T, NT, K>), dim3(grid), dim3(NT), lds_bytes(), st_, a_, mode, nsteps, t0,
+                       (int)macro_);
+  }
+
+  void launch(int mode, int nsteps, double t0) {
+    const Variant &v = variants_[cur_];
+    if ((long)v.nt * v.k < C_) throw Error("per-cell mapping too small for the cell");
+    CB_HIP(hipMemsetAsync(a_.queue, 0, sizeof(int), st_));
+    int dev = 0;
+    CB_HIP(hipGetDevice(&dev));
+    int ncu = 0;
+    CB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    int per_cu = std::max(1, (int)(160 * 1024 / (lds_bytes() + 1024)));
+    int grid = std::min(n_, ncu * per_cu);
+    switch (v.nt) {
+      case 256: go<256, 7>(mode, nsteps, t0, grid); break;
+      case 320: go<320, 5>(mode, nsteps, t0, grid); break;
+      case 512: go<512, 4>(mode, nsteps, t0, grid); break;
+      default: throw Error("unknown mapping");
+    }
+    CB_HIP(hipGetLastError());
+  }
+
+  void check_status() {
+    std::vector<double> ctl((size_t)n_ * NCTL);
+    CB_HIP(hipMemcpyAsync(ctl.data(), a_.ctl, sizeof(double) * ctl.size(), hipMemcpyDeviceToHost, st_));
+    CB_HIP(hipStreamSynchronize(st_));
+    for (int c = 0; c < n_; c++) {
+      int s = (int)ctl[(size_t)c * NCTL + C_STATUS];
+      if (s != 0) throw Error("RKC failed in cell " + std::to_string(c) + ": idid = " + std::to_string(s), s);
+    }
+  }
+
+  int n_, C_, nc_ = 0;
+  PoissonSpec syn_;
+  hipStream_t st_;
+  CellArgs<T> a_{};
+  std::vector<void *> bufs_;
+  double *vnew_ = nullptr;
+  double *rec_ = nullptr;
+  size_t rec_cap_ = 0;
+  std::vector<int> perm_;
+  std::vector<Variant> variants_;
+  int cur_ = 0;
+  long macro_ = 0;
+  double t_next_ = 0.0;
+};
+
+}  // namespace
+
+Engine *make_cell_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn,
+                      bool mixed) {
+  if (mixed) return new CellPkj<float>(host, sol, stim, syn);
+  return new CellPkj<double>(host, sol, stim, syn);
+}
+
+}  // namespace cbrkc

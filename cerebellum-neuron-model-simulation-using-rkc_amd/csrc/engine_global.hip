@@ -315,6 +315,7 @@ class GlobalPkj : public Engine {
     hmin_ = std::pow(2.0, -10.0);   // H_MIN_PKJ, solve_rkc.cuh:26
     init(sol.atol, sol.rtol);
     if (sol.ion_interval > 0) integ_ = sol.ion_interval;
+    refresh_fn_ = sol.refresh_fn;
   }
 
   ~GlobalPkj() override {
@@ -405,6 +406,7 @@ class GlobalPkj : public Engine {
     st.ion_updates++;
     PkjState S{v_, ca_, rev_, iext_, shell_, cond_, gate_, b_, syn_g_, syn_.e_syn, nc_};
     k_pkj_ion_b<<<nblk(nc_), 256, 0, st_>>>(S, vec_[Y], dt, 1);
+    if (refresh_fn_) k_stage_rhs(vec_[YN], vec_[FN]);
   }
 
   // deterministic total of the per-block partials left in red_a_
@@ -677,7 +679,7 @@ class GlobalPkj : public Engine {
   double *zero_ = nullptr;
   int *bad_;
   double hw_[8], o_[NOTHERS];
-  bool last_, newspc_, jacatt_;
+  bool last_, newspc_, jacatt_, refresh_fn_ = false;
   double sum_dt_ = 0.0, integ_ = 0.0, hmin_, hmax_;
   Stats st;
 };

@@ -70,6 +70,8 @@ typedef struct cb_pop_config {
   double poisson_rate_hz, poisson_g_mS, poisson_tau_ms, poisson_e_mV;
   uint64_t poisson_seed;
   double ion_interval_ms;   /* accepted time between ion updates; <= 0: 2^log10(atol) (solve_rkc.cu:1265) */
+  int refresh_fn;           /* 1: recompute fn = F(y_n) after each ion update (the reference keeps
+                               the stale value, solve_rkc.cu:1229-1250); 0: reference behaviour */
   int device;               /* HIP device ordinal */
 } cb_pop_config;
 

@@ -418,7 +418,7 @@ void or_pkj_update_ion(or_pop *p, const double *elem_vnew, double DT)
     k1 = Ca_val + (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
     p->ca[id] = k1;
     Ca_val = 0.5 * (Ca_val + k1);
-    (id % 1599 == 0) ? (p->rev[id] = 12.5 * log(Ca1OUT_PKJ / Ca1_0_PKJ))
+    ((id + p->gid_offset) % 1599 == 0) ? (p->rev[id] = 12.5 * log(Ca1OUT_PKJ / Ca1_0_PKJ))
                      : (p->rev[id] = 13.361624877 * log(Ca1OUT_PKJ / Ca_val));
     Ca_val = floor((Ca_val * 1000 - 0.04) / 0.099986667) * 0.099986667 + 0.04;
     CNU(m_NaF, inf_m_NaF(v_val), tau_m_NaF(v_val));

@@ -505,7 +505,12 @@ void or_model_update_ion_and_b(or_pop *p, double sum_dt)
     default: break;
   }
   update_b(p, p->vec[OR_Y]);
+  /* optional (off in the reference): F_0 = F(y_n) with the new b, so the
+     next step does not start from a derivative of the old current */
+  if (p->refresh_fn) or_dydt(p, p->vec[OR_YN], p->vec[OR_FN]);
 }
+
+void or_pop_set_refresh_fn(or_pop *p, int on) { p->refresh_fn = on; }
 
 /* ---------------- stimulus ---------------- */
 void or_set_stim(or_pop *p, int comp, double t0, double t1, const double *amp)
@@ -530,6 +535,7 @@ void or_set_current(or_pop *p, double t)  /* pkj.cu:333-344 shape */
 
 /* ---------------- accessors ---------------- */
 void or_pop_set_integ(or_pop *p, double integ) { p->integ = integ; }
+void or_pop_set_gid_offset(or_pop *p, long off) { p->gid_offset = off; }
 int or_pop_nc(const or_pop *p) { return p->nc; }
 int or_pop_nnz(const or_pop *p) { return p->nnz; }
 
@@ -596,6 +602,23 @@ void or_pop_enable_syn(or_pop *p, double e_syn)
 {
   if (!p->syn_g) p->syn_g = (double *)calloc(p->nc, sizeof(double));
   p->syn_e = e_syn;
+}
+
+/* Poisson conductance drive on every compartment for macro step `step`:
+ *   g <- g * decay + g_event * [u < prob],
+ *   u = 53-bit uniform from Philox4x32-10(ctr = (gid, step, 0x5359, 0), key)
+ * (the synthetic dendritic input of the 1024-cell precision workload; the
+ * reference's own network noise is cuRAND, syn.cu:37-42, not reproduced). */
+void or_pop_poisson_step(or_pop *p, uint32_t step, uint32_t k0, uint32_t k1, double decay, double prob,
+                         double g_event)
+{
+  uint32_t key[2] = {k0, k1};
+  for (int i = 0; i < p->nc; i++) {
+    uint32_t ctr[4] = {(uint32_t)(p->gid_offset + i), step, 0x5359u, 0u}, r[4];
+    or_philox4x32_10(ctr, key, r);
+    double u = ((double)(r[0] >> 5) * 67108864.0 + (double)(r[1] >> 6)) * (1.0 / 9007199254740992.0);
+    p->syn_g[i] = p->syn_g[i] * decay + ((u < prob) ? g_event : 0.0);
+  }
 }
 
 void or_pop_free(or_pop *p)

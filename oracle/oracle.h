@@ -69,6 +69,11 @@ typedef struct or_pop {
   /* statistics */
   long n_steps, n_rejects, n_stages, n_rho_iters, n_ion_updates;
   int nthreads;
+  /* global index of compartment 0 of this population: a one-cell oracle
+     standing in for cell c of a larger population uses c * comp, so the
+     index-keyed quirk (pkj_ion.cu:329) and the noise counters line up */
+  long gid_offset;
+  int refresh_fn;     /* 1: recompute fn = F(y_n) after every ion update */
 } or_pop;
 
 /* ---- deterministic sums (or_rkc.c) ---- */
@@ -95,6 +100,8 @@ void or_set_current(or_pop *p, double t);
 /* ---- accessors for tests ---- */
 int  or_pop_nc(const or_pop *p);
 void or_pop_set_integ(or_pop *p, double integ);
+void or_pop_set_gid_offset(or_pop *p, long off);
+void or_pop_set_refresh_fn(or_pop *p, int on);
 void or_pop_get(const or_pop *p, int what, double *dst);
 void or_pop_set(or_pop *p, int what, const double *src);
 void or_pop_controller(const or_pop *p, double *others14, double *hwork8, int *flags3, double *sumdt);
@@ -103,6 +110,8 @@ void or_pop_set_threads(or_pop *p, int nthreads);
 int  or_pop_nnz(const or_pop *p);
 void or_pop_csr(const or_pop *p, int *row, int *col, double *val);
 void or_pop_enable_syn(or_pop *p, double e_syn);
+void or_pop_poisson_step(or_pop *p, uint32_t step, uint32_t k0, uint32_t k1, double decay, double prob,
+                         double g_event);
 
 /* ---- Philox4x32-10 (or_philox.c), Salmon et al., SC'11 ---- */
 void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

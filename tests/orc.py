@@ -54,6 +54,10 @@ def lib():
         L.or_pkj_rate.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.or_dydt.argtypes = [ctypes.c_void_p, dp, dp]
         L.or_pop_set_integ.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.or_pop_set_gid_offset.argtypes = [ctypes.c_void_p, ctypes.c_long]
+        L.or_pop_set_refresh_fn.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_pop_poisson_step.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_double]
         u32p = ctypes.POINTER(ctypes.c_uint32)
         L.or_philox4x32_10.argtypes = [u32p, u32p, u32p]
         L.srand.argtypes = [ctypes.c_uint]
@@ -124,11 +128,6 @@ class Pop:
     def set_current(self, t):
         lib().or_set_current(self.h, t)
 
-    def advance(self):
-        r = lib().or_rkc_advance(self.h)
-        if r != 0:
-            raise RuntimeError(f"oracle RKC failed: idid={r}")
-
     def controller(self):
         o = np.zeros(14)
         hw = np.zeros(8)
@@ -159,3 +158,39 @@ class Pop:
 
     def threads(self, n):
         lib().or_pop_set_threads(self.h, n)
+
+    def gid_offset(self, off):
+        lib().or_pop_set_gid_offset(self.h, off)
+
+    def stable_split(self, ion_interval=2.0 ** -8):
+        """ion interval + fn refresh used for the synthetic Purkinje workloads"""
+        lib().or_pop_set_integ(self.h, ion_interval)
+        lib().or_pop_set_refresh_fn(self.h, 1)
+
+    def enable_poisson(self, rate_hz=5.0, g_mS=1e-6, tau_ms=2.0, e_mV=0.0, seed=0):
+        lib().or_pop_enable_syn(self.h, e_mV)
+        self._poisson = (seed & 0xffffffff, seed >> 32, np.exp(-0.125 / tau_ms),
+                         1.0 - np.exp(-rate_hz * 0.125 * 1e-3), g_mS)
+        self._step = 0
+
+    def advance(self):
+        if getattr(self, "_poisson", None):
+            k0, k1, decay, prob, g = self._poisson
+            lib().or_pop_poisson_step(self.h, self._step, k0, k1, decay, prob, g)
+            self._step += 1
+        r = lib().or_rkc_advance(self.h)
+        if r != 0:
+            raise RuntimeError(f"oracle RKC failed: idid={r}")
+
+
+def split_morph(path, ncell, comp, outdir):
+    """Writes one single-cell morphology file per cell block of `path`."""
+    lines = [l for l in open(path) if l.strip()]
+    assert len(lines) == ncell * comp
+    out = []
+    for c in range(ncell):
+        p = os.path.join(outdir, f"cell{c}.txt")
+        with open(p, "w") as f:
+            f.writelines(lines[c * comp:(c + 1) * comp])
+        out.append(p)
+    return out
