@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark: compartment-steps/s of the batched mixed-precision RKC engine on
+1024 synthetic Purkinje cells (1600 compartments each), with the voltage and
+spike error against the per-cell FP64 engine on the same run, the plain FP64
+population-global RKC beside it, and the CPU oracle on a bounded sample.
+
+One bench step = one macro step (H_MAX = 0.125 ms, the reference's output
+interval) of all cells.  A compartment-step is one compartment advanced over
+one macro step, independent of the adaptive internal steps.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "cerebellum-neuron-model-simulation-using-rkc_amd")
+sys.path.insert(0, PKG)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "compartment-steps/s @1024 PCs RKC: mixed-FP32 vs FP64, ≤0.1 mV error"
+H_MAX = 0.125
+COMP = 1600
+# algorithmic work per unit (mixed engine; see DESIGN.md "Roofline accounting")
+FLOP_STAGE = 16.0        # per compartment-stage: 2-3 neighbour terms, /C, stage combination
+FLOP_ION = 335.0         # per compartment-ion-update: ~45 exp/log + ~40 div + ~250 other
+FLOP_RHO = 12.0          # per compartment-power-iteration
+BYTES_ION = 197.0        # per compartment-ion-update: gates r/w (144), Ca (16), rev, V, b, params
+BYTES_STEP = 8.0         # per compartment-accepted-step: b read for A, ion voltage write
+BYTES_LAUNCH = 64.0      # per compartment-launch: state load + store
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+
+
+def philox_params(seed, n):
+    """Per-cell initial V (-68 + N(0,1)) and soma current U(0.2, 1.0) nA from
+    Philox4x32-10 (ctr = (cell, 0, 0, 0), key = (seed, 0x43454c4c))."""
+    import cbrkc
+    v0 = np.zeros(n)
+    amp = np.zeros(n)
+    for c in range(n):
+        r = cbrkc.philox4x32_10([c, 0, 0, 0], [seed & 0xffffffff, 0x43454C4C])
+        u1 = ((r[0] >> 5) * 67108864.0 + (r[1] >> 6)) / 9007199254740992.0
+        u2 = ((r[2] >> 5) * 67108864.0 + (r[3] >> 6)) / 9007199254740992.0
+        g = math.sqrt(-2.0 * math.log(1.0 - u1)) * math.cos(2.0 * math.pi * u2)
+        v0[c] = -68.0 + g
+        r2 = cbrkc.philox4x32_10([c, 1, 0, 0], [seed & 0xffffffff, 0x43454C4C])
+        amp[c] = (0.2 + 0.8 * (((r2[0] >> 5) * 67108864.0 + (r2[1] >> 6)) / 9007199254740992.0)) * 1e-3
+    return v0, amp
+
+
+def make_pop(cbrkc, engine, morph, n, v0, amp, args):
+    return cbrkc.Population(
+        cbrkc.PKJ, n, 1, morph=morph, v0=v0, atol=args.tol, rtol=args.tol, stim=(1599, 0.0, 1e9, amp),
+        engine=engine, poisson=dict(rate_hz=5.0, g_mS=1e-6, tau_ms=2.0, e_mV=0.0, seed=args.seed),
+        **cbrkc.STABLE)
+
+
+def timed_run(p, steps, chunk, t0, record=True):
+    """steps macro steps in launches of `chunk`; returns (trace, wall seconds)."""
+    tr = np.zeros((steps, p.n)) if record else None
+    done = 0
+    t = time.perf_counter()
+    while done < steps:
+        k = min(chunk, steps - done)
+        out = p.run(k, t0 + done * H_MAX, 1599, record=record)
+        if record:
+            tr[done:done + k] = out
+        done += k
+    return tr, time.perf_counter() - t
+
+
+def work(s0, s1):
+    return {k: s1[k] - s0[k] for k in s0}
+
+
+def cpu_baseline(morph_file, v0, amp, args):
+    """The oracle (FP64 C restatement, population-global controller, OpenMP) on
+    the first cells of the same population for a bounded wall time."""
+    import orc
+    ncell = args.cpu_cells
+    lines = []
+    with open(morph_file) as f:
+        for i, l in enumerate(f):
+            if i >= ncell * COMP:
+                break
+            lines.append(l)
+    sample = morph_file + f".first{ncell}"
+    with open(sample, "w") as f:
+        f.writelines(lines)
+    o = orc.Pop.pkj(sample, ncell, 1, v0=v0[:ncell], atol=args.tol, rtol=args.tol, libc_rand=False)
+    o.stable_split()
+    o.enable_poisson(seed=args.seed)
+    o.set_stim(1599, 0.0, 1e9, amp[:ncell])
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o.threads(threads)
+    t = time.perf_counter()
+    steps = 0
+    while steps < args.steps and time.perf_counter() - t < args.cpu_seconds:
+        o.set_current(steps * H_MAX)
+        o.advance()
+        steps += 1
+    dt = time.perf_counter() - t
+    return {"value": ncell * COMP * steps / dt, "unit": "compartment-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{ncell} of the 1024 cells as one population (reference global controller, FP64), "
+                      f"{steps} macro steps ({steps * H_MAX:g} ms simulated) in {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1600, help="timed macro steps (1600 = 200 ms)")
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--chunk", type=int, default=80, help="macro steps per persistent launch")
+    ap.add_argument("--cells", type=int, default=1024)
+    ap.add_argument("--tol", type=float, default=1e-6, help="atol = rtol (the reference's TOL_PKJ)")
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--plain-steps", type=int, default=40)
+    ap.add_argument("--cpu-cells", type=int, default=8)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--morph", default=None, help="reuse an existing morphology file")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import cbrkc
+    from trace_metrics import compare
+
+    n = args.cells
+    morph = args.morph or f"/tmp/cbrkc_bench_pkj{n}_s{args.seed}.txt"
+    if not os.path.exists(morph):
+        cbrkc.synth_pkj_morph(morph, seed=args.seed, ncell=n)
+    v0, amp = philox_params(args.seed, n)
+    t0 = args.warmup * H_MAX
+
+    # ---- mixed engine (the measured value) ----
+    print(f"[bench] mixed engine: {n} cells, warmup {args.warmup}, timed {args.steps} macro steps", file=sys.stderr,
+          flush=True)
+    pm = make_pop(cbrkc, cbrkc.ENGINE_CELL_MIXED, morph, n, v0, amp, args)
+    nt = pm.autotune(4)
+    pm.run(args.warmup, 0.0, 1599, record=False)
+    pm.kernel_time(reset=True)
+    s0 = pm.stats()
+    tr_m, wall_m = timed_run(pm, args.steps, args.chunk, t0)
+    kms, nlaunch = pm.kernel_time(reset=True)
+    wm = work(s0, pm.stats())
+    value = n * COMP * args.steps / wall_m
+
+    # roofline of the persistent kernel over the timed region
+    flops = COMP * (FLOP_STAGE * wm["stages"] + FLOP_ION * wm["ion_updates"] + FLOP_RHO * wm["rho_iters"])
+    bytes_ = COMP * (BYTES_ION * wm["ion_updates"] + BYTES_STEP * wm["steps"]) + BYTES_LAUNCH * n * COMP * nlaunch
+    ksec = kms * 1e-3
+    gbs = bytes_ / nlaunch / (ksec / nlaunch) / 1e9
+    tfl = flops / ksec / 1e12
+    roof = {"bound": "hbm", "achieved": round(gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(gbs / PEAK_HBM_GBS, 5), "traffic": None,
+            "kernel": f"k_cell<float,{nt}> (persistent, one workgroup per cell)",
+            "kernel_ms_per_launch": round(kms / nlaunch, 4), "launches": nlaunch,
+            "algorithmic_bytes_per_launch": round(bytes_ / nlaunch)}
+    roof_valu = {"achieved": round(tfl, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                 "frac": round(tfl / PEAK_FP32_TFLOPS, 5),
+                 "algorithmic_flops_per_launch": round(flops / nlaunch)}
+    if roof_valu["frac"] > roof["frac"]:
+        roof["binding"] = "fp32-valu"
+    else:
+        roof["binding"] = "hbm"
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "compartment-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall_m / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "precision": "f32 stages/gates/currents, compensated f32 voltage, f64 step control and Ca",
+        "data": "synthetic",
+        "config": {"workload": f"{n} synthetic Purkinje cells x 1600 compartments (Pkj.txt format, per-cell seeded "
+                               f"trees), RKC atol=rtol={args.tol:g}, soma current U(0.2,1.0) nA, dendritic Poisson "
+                               "5 Hz x 1 nS tau 2 ms; safeguards: ion interval 2^-8 ms, fn refresh, spectral-radius floor",
+                   "cells": n, "compartments": n * COMP, "sim_ms": args.steps * H_MAX, "threads_per_cell": nt},
+        "sim_ms_per_s": args.steps * H_MAX / wall_m,
+        "mixed": {"wall_s": wall_m, "work": wm},
+        "roofline": roof, "roofline_valu": roof_valu,
+    }
+
+    # ---- per-cell FP64 engine: error of the mixed run ----
+    if not args.no_fp64:
+        print("[bench] per-cell FP64 engine on the same run", file=sys.stderr, flush=True)
+        pd = make_pop(cbrkc, cbrkc.ENGINE_CELL_FP64, morph, n, v0, amp, args)
+        pd.run(args.warmup, 0.0, 1599, record=False)
+        s0 = pd.stats()
+        tr_d, wall_d = timed_run(pd, args.steps, args.chunk, t0)
+        counts, dt, sub = compare(tr_d, tr_m)
+        out["fp64_cell"] = {"value": n * COMP * args.steps / wall_d, "wall_s": wall_d, "work": work(s0, pd.stats())}
+        out["error_vs_fp64"] = {
+            "spike_counts_identical": bool(np.all(counts[:, 0] == counts[:, 1])),
+            "spikes_fp64": int(counts[:, 0].sum()), "spikes_mixed": int(counts[:, 1].sum()),
+            "cells_with_count_mismatch": int(np.sum(counts[:, 0] != counts[:, 1])),
+            "max_spike_time_diff_ms": float(dt.max()),
+            "max_subthreshold_abs_dV_mV": float(sub.max()),
+            "median_per_cell_subthreshold_abs_dV_mV": float(np.median(sub)),
+            "max_abs_dV_mV_all_samples": float(np.max(np.abs(tr_d - tr_m)))}
+        del pd
+
+    # ---- plain FP64 population-global RKC ----
+    if not args.no_plain:
+        print(f"[bench] plain FP64 global engine, {args.plain_steps} macro steps", file=sys.stderr, flush=True)
+        pg = make_pop(cbrkc, cbrkc.ENGINE_GLOBAL_FP64, morph, n, v0, amp, args)
+        pg.run(2, 0.0, 1599, record=False)
+        s0 = pg.stats()
+        _, wall_g = timed_run(pg, args.plain_steps, args.plain_steps, 2 * H_MAX, record=False)
+        vg = n * COMP * args.plain_steps / wall_g
+        out["plain_fp64_global"] = {"value": vg, "wall_s": wall_g, "macro_steps": args.plain_steps,
+                                    "work": work(s0, pg.stats())}
+        out["speedup_vs_plain_fp64"] = value / vg
+        del pg
+
+    # ---- CPU oracle ----
+    if not args.no_cpu:
+        print("[bench] CPU oracle sample", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(morph, v0, amp, args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

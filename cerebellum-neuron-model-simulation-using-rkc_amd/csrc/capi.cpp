@@ -135,6 +135,12 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     sol.rtol = cfg->rtol > 0 ? cfg->rtol : tol;
     sol.ion_interval = cfg->ion_interval_ms;
     sol.refresh_fn = cfg->refresh_fn != 0;
+    if (cfg->rho_floor) {
+      std::vector<double> rho;
+      cable_spectral_radius(host, rho);
+      sol.rho_floor.resize(rho.size());
+      for (size_t c = 0; c < rho.size(); c++) sol.rho_floor[c] = 1.2 * rho[c];
+    }
     // atol > 0 keeps every error weight positive (the reference asserts on a
     // zero weight, idid = 3, solve_rkc.cu:50-53)
     if (!(sol.atol > 0.0) || !(sol.rtol >= 0.0)) return fail(CB_E_ARG, "atol must be > 0 and rtol >= 0");
@@ -223,6 +229,25 @@ int cb_pop_stats(const cb_pop *p, int64_t *out6) {
   return CB_OK;
 }
 
+int cb_pop_kernel_time(cb_pop *p, double *ms, int64_t *launches, int reset) {
+  return guarded([&]() -> int {
+    if (!p || !ms || !launches) return fail(CB_E_ARG, "null argument");
+    long n = 0;
+    p->eng->kernel_time(ms, &n, reset != 0);
+    *launches = n;
+    return CB_OK;
+  });
+}
+
+int cb_pop_autotune(cb_pop *p, int trial_steps, int *threads) {
+  return guarded([&]() -> int {
+    if (!p) return fail(CB_E_ARG, "null population");
+    int t = p->eng->autotune(trial_steps, -1.0);   // trial from the current time
+    if (threads) *threads = t;
+    return CB_OK;
+  });
+}
+
 int cb_pop_controller(const cb_pop *p, double *o, double *hw) {
   if (!p || !o || !hw) return fail(CB_E_ARG, "null argument");
   if (p->engine != CB_ENGINE_GLOBAL_FP64) return fail(CB_E_UNSUP, "per-cell engines have no global controller");
@@ -245,6 +270,22 @@ int cb_synth_pkj_morph(const char *path, uint64_t seed, int ncell, double soma_r
     }
     std::string err;
     if (!write_pkj_morph(path, all, err)) return fail(CB_E_IO, err);
+    return CB_OK;
+  });
+}
+
+int cb_pkj_cable_spectral_radius(const char *morph_path, int ncell, double *rho) {
+  return guarded([&]() -> int {
+    if (!morph_path || !rho || ncell <= 0) return fail(CB_E_ARG, "bad argument");
+    std::vector<MorphRow> rows;
+    std::string err;
+    if (!read_pkj_morph(morph_path, pkj::COMP, rows, err)) return fail(CB_E_IO, err);
+    std::vector<double> v0(ncell, pkj::V_INIT);
+    PkjHost host;
+    if (!build_pkj_host(rows, ncell, 1, nullptr, nullptr, v0.data(), host, err)) return fail(CB_E_IO, err);
+    std::vector<double> r;
+    cable_spectral_radius(host, r);
+    for (int c = 0; c < ncell; c++) rho[c] = r[c];
     return CB_OK;
   });
 }

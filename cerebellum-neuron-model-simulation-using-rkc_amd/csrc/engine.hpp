@@ -68,6 +68,19 @@ class Engine {
   virtual int ncell() const = 0;
   virtual void controller(double *others14, double *hwork8) const { (void)others14; (void)hwork8; }
   virtual void sync() {}
+  // picks the launch mapping by timing short trials from the current state
+  // (the state is left unchanged); returns the chosen threads per cell
+  virtual int autotune(int trial_steps, double t0) {
+    (void)trial_steps;
+    (void)t0;
+    return 0;
+  }
+  // event-timed duration of the main kernel (ms) and its launch count
+  virtual void kernel_time(double *ms, long *launches, bool reset) {
+    *ms = 0.0;
+    *launches = 0;
+    (void)reset;
+  }
   // many macro steps without host round trips; soma V and spikes are
   // buffered on the device.  Default: loop over advance().
   virtual void run(int nsteps, double t0, int rec_comp, double *rec_out) ;
@@ -78,6 +91,7 @@ struct SolverSpec {
   double atol = 1e-6, rtol = 1e-6;
   double ion_interval = -1.0;   // <= 0: 2^log10(atol) (solve_rkc.cu:1265)
   bool refresh_fn = false;      // recompute fn = F(y_n) after each ion update (off in the reference)
+  std::vector<double> rho_floor;  // per cell: lower bound applied to every spectral-radius estimate (empty: none)
 };
 
 Engine *make_global_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);

@@ -57,6 +57,7 @@ struct CellArgs {
   int ncell, comp;
   double atol, rtol, hmin, hmax, integ;
   int refresh_fn;         // 1: fn = F(y_n) after each ion update
+  const double *rho_floor;  // per cell lower bound of the spectral radius (null: none)
   // static, per compartment, cell-major, breadth-first order
   const uint32_t *topo;   // parent | first_child << 11 | nchild << 22
   const T *gpar;          // conductance to the parent [mS]
@@ -391,6 +392,7 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
         if (!a_valid) compute_A();
         if (newspc) {
           rkcrho();
+          if (a.rho_floor) sprad = fmax(sprad, a.rho_floor[cell]);
           jacatt = true;
         }
         last = false;
@@ -532,6 +534,7 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
         if (i < C) a.ede[base + i] = F0[k];
       }
       rkcrho();
+      if (a.rho_floor) sprad = fmax(sprad, a.rho_floor[cell]);
       // set_initial_absh (solve_rkc.cu:489-560)
       double ab = hmax;
       if (sprad * ab > 1.0) ab = 1.0 / sprad;
@@ -665,6 +668,8 @@ class CellPkj : public Engine {
       : n_(H.n), C_(H.comp), syn_(syn) {
     using namespace pkj;
     CB_HIP(hipStreamCreate(&st_));
+    CB_HIP(hipEventCreate(&ev0_));
+    CB_HIP(hipEventCreate(&ev1_));
     const int n = n_, C = C_;
     const size_t NC = (size_t)n * C;
     nc_ = (int)NC;
@@ -746,6 +751,8 @@ class CellPkj : public Engine {
     a_.hmax = H_MAX;
     a_.integ = sol.ion_interval > 0 ? sol.ion_interval : std::pow(2.0, std::log10(sol.atol));
     a_.refresh_fn = sol.refresh_fn ? 1 : 0;
+    a_.rho_floor = nullptr;
+    if (!sol.rho_floor.empty()) a_.rho_floor = put(sol.rho_floor);
     a_.topo = put(topo);
     a_.gpar = put(gpar);
     a_.invc = put(invc);
@@ -808,6 +815,8 @@ class CellPkj : public Engine {
     (void)hipFree(vnew_);
     (void)hipFree(a_.queue);
     if (rec_) (void)hipFree(rec_);
+    (void)hipEventDestroy(ev0_);
+    (void)hipEventDestroy(ev1_);
     (void)hipStreamDestroy(st_);
   }
 
@@ -853,6 +862,7 @@ class CellPkj : public Engine {
       CB_HIP(hipMemcpyAsync(rec_out, rec, sizeof(double) * (size_t)nsteps * n_, hipMemcpyDeviceToHost, st_));
     }
     CB_HIP(hipStreamSynchronize(st_));
+    collect_time();
     check_status();
     t_next_ = t0 + nsteps * H_MAX;
   }
@@ -903,8 +913,9 @@ class CellPkj : public Engine {
   // Picks the (threads per cell, compartments per thread) mapping: every
   // variant runs the same short trial from the same state and the fastest
   // is kept.  State is restored after each trial.
-  void autotune(int trial_steps, double t0) {
-    if (variants_.size() < 2) return;
+  int autotune(int trial_steps, double t0) override {
+    if (variants_.size() < 2 || trial_steps <= 0) return variants_[cur_].nt;
+    if (t0 < 0) t0 = t_next_;
     Snapshot snap = snapshot();
     double best = 1e30;
     int bi = cur_;
@@ -913,19 +924,22 @@ class CellPkj : public Engine {
       launch(0, 1, t0);   // warm
       restore(snap);
       CB_HIP(hipStreamSynchronize(st_));
-      auto t1 = std::chrono::steady_clock::now();
+      timed_pending_ = false;
       launch(0, trial_steps, t0);
       CB_HIP(hipStreamSynchronize(st_));
-      double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+      float ms = 0.0f;
+      CB_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+      timed_pending_ = false;
       restore(snap);
-      if (dt < best) {
-        best = dt;
+      if (ms < best) {
+        best = ms;
         bi = (int)v;
       }
-      tune_log_.push_back({variants_[v].nt, variants_[v].k, dt});
+      tune_log_.push_back({variants_[v].nt, variants_[v].k, ms * 1e-3});
     }
     cur_ = bi;
     release(snap);
+    return variants_[cur_].nt;
   }
 
   struct TuneRec {
@@ -1020,6 +1034,7 @@ class CellPkj : public Engine {
     CB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     int per_cu = std::max(1, (int)(160 * 1024 / (lds_bytes() + 1024)));
     int grid = std::min(n_, ncu * per_cu);
+    if (mode == 0) CB_HIP(hipEventRecord(ev0_, st_));
     switch (v.nt) {
       case 256: go<256, 7>(mode, nsteps, t0, grid); break;
       case 320: go<320, 5>(mode, nsteps, t0, grid); break;
@@ -1027,7 +1042,33 @@ class CellPkj : public Engine {
       default: throw Error("unknown mapping");
     }
     CB_HIP(hipGetLastError());
+    if (mode == 0) {
+      CB_HIP(hipEventRecord(ev1_, st_));
+      timed_pending_ = true;
+    }
   }
+
+  // called after a stream synchronisation that follows a timed launch
+  void collect_time() {
+    if (!timed_pending_) return;
+    float ms = 0.0f;
+    CB_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    kt_ms_ += ms;
+    kt_n_ += 1;
+    timed_pending_ = false;
+  }
+
+ public:
+  void kernel_time(double *ms, long *launches, bool reset) override {
+    *ms = kt_ms_;
+    *launches = kt_n_;
+    if (reset) {
+      kt_ms_ = 0.0;
+      kt_n_ = 0;
+    }
+  }
+
+ private:
 
   void check_status() {
     std::vector<double> ctl((size_t)n_ * NCTL);
@@ -1050,6 +1091,10 @@ class CellPkj : public Engine {
   std::vector<int> perm_;
   std::vector<Variant> variants_;
   int cur_ = 0;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  bool timed_pending_ = false;
+  double kt_ms_ = 0.0;
+  long kt_n_ = 0;
   long macro_ = 0;
   double t_next_ = 0.0;
 };

@@ -313,6 +313,8 @@ class GlobalPkj : public Engine {
     }
     hmax_ = H_MAX;
     hmin_ = std::pow(2.0, -10.0);   // H_MIN_PKJ, solve_rkc.cuh:26
+    // one controller for the whole population: the floor is the largest cell's
+    for (double r : sol.rho_floor) rho_floor_ = std::max(rho_floor_, r);
     init(sol.atol, sol.rtol);
     if (sol.ion_interval > 0) integ_ = sol.ion_interval;
     refresh_fn_ = sol.refresh_fn;
@@ -495,7 +497,13 @@ class GlobalPkj : public Engine {
   }
 
   // rkcrho, solve_rkc.cu:562-754
+  // spectral radius estimate, raised to the floor when one is configured
   void rkcrho() {
+    rkcrho_power();
+    if (o_[SPRAD] < rho_floor_) o_[SPRAD] = rho_floor_;
+  }
+
+  void rkcrho_power() {
     const int itmax = 1000;
     double sprad_old[3] = {0.0, 0.0, 0.0};
     const double sqrtu = std::sqrt(UROUND);
@@ -680,6 +688,7 @@ class GlobalPkj : public Engine {
   int *bad_;
   double hw_[8], o_[NOTHERS];
   bool last_, newspc_, jacatt_, refresh_fn_ = false;
+  double rho_floor_ = 0.0;
   double sum_dt_ = 0.0, integ_ = 0.0, hmin_, hmax_;
   Stats st;
 };

@@ -10,6 +10,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <atomic>
+#include <thread>
 
 #include "philox.hpp"
 #include "pkj_model.hpp"
@@ -272,6 +274,103 @@ bool build_pkj_host(const std::vector<MorphRow> &rows, int nx, int ny, const std
     for (int k = 0; k < NGATE; k++) P.gate[(size_t)k * nc + i] = g[k];
   }
   return true;
+}
+
+namespace {
+
+// largest eigenvalue of the symmetric tridiagonal (a[0..k), b[0..k-1)) by
+// bisection on Sturm counts
+double tridiag_max_eig(const std::vector<double> &a, const std::vector<double> &b) {
+  const int k = (int)a.size();
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < k; i++) {
+    double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < k ? std::fabs(b[i]) : 0.0);
+    lo = std::min(lo, a[i] - r);
+    hi = std::max(hi, a[i] + r);
+  }
+  auto count_below = [&](double x) {   // eigenvalues < x
+    int c = 0;
+    double d = 1.0;
+    for (int i = 0; i < k; i++) {
+      double bb = i > 0 ? b[i - 1] * b[i - 1] : 0.0;
+      d = a[i] - x - (i > 0 ? bb / d : 0.0);
+      if (d == 0.0) d = -1e-300;
+      if (d < 0) c++;
+    }
+    return c;
+  };
+  for (int it = 0; it < 200 && hi - lo > 1e-13 * std::max(1.0, std::fabs(hi)); it++) {
+    double mid = 0.5 * (lo + hi);
+    if (count_below(mid) == k) hi = mid;
+    else lo = mid;
+  }
+  return hi;
+}
+
+}  // namespace
+
+void cable_spectral_radius(const PkjHost &P, std::vector<double> &rho) {
+  const int C = P.comp;
+  rho.assign(P.n, 0.0);
+  const int kmax = std::min(C, 80);
+  const int nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::atomic<int> next{0};
+  auto worker = [&]() {
+   for (int c = next++; c < P.n; c = next++) {
+    const int base = c * C;
+    std::vector<double> isq(C);
+    for (int i = 0; i < C; i++) isq[i] = 1.0 / std::sqrt(P.cm[base + i]);
+    // y = S x, S = C^-1/2 (-L) C^-1/2, (-L x)_i = sum_j g_ij (x_i - x_j) within the cell
+    auto apply = [&](const std::vector<double> &x, std::vector<double> &y) {
+      for (int i = 0; i < C; i++) {
+        const int gi = base + i;
+        double s = 0.0, xi = x[i] * isq[i];
+        for (int e = P.row[gi]; e < P.row[gi + 1]; e++) {
+          const int j = P.col[e] - base;
+          if (j == i || j < 0 || j >= C) continue;
+          s += P.val[e] * (xi - x[j] * isq[j]);
+        }
+        y[i] = s * isq[i];
+      }
+    };
+    std::vector<std::vector<double>> Q;
+    std::vector<double> al, be, q(C), w(C);
+    double nrm = 0.0;
+    for (int i = 0; i < C; i++) {
+      q[i] = ((i & 1) ? -1.0 : 1.0) * (1.0 + 0.1 * (double)((i * 7919) % 13) / 13.0);
+      nrm += q[i] * q[i];
+    }
+    nrm = std::sqrt(nrm);
+    for (double &x : q) x /= nrm;
+    double bprev = 0.0;
+    for (int k = 0; k < kmax; k++) {
+      Q.push_back(q);
+      apply(q, w);
+      double a = 0.0;
+      for (int i = 0; i < C; i++) a += q[i] * w[i];
+      al.push_back(a);
+      for (int i = 0; i < C; i++) w[i] -= a * q[i] + (k > 0 ? bprev * Q[k - 1][i] : 0.0);
+      for (int pass = 0; pass < 2; pass++)   // full reorthogonalisation
+        for (const auto &v : Q) {
+          double d = 0.0;
+          for (int i = 0; i < C; i++) d += v[i] * w[i];
+          for (int i = 0; i < C; i++) w[i] -= d * v[i];
+        }
+      double bnew = 0.0;
+      for (double x : w) bnew += x * x;
+      bnew = std::sqrt(bnew);
+      if (bnew < 1e-12 * std::fabs(a) || k + 1 == kmax) break;
+      be.push_back(bnew);
+      for (int i = 0; i < C; i++) q[i] = w[i] / bnew;
+      bprev = bnew;
+    }
+    be.resize(al.size() - 1);
+    rho[c] = tridiag_max_eig(al, be);
+   }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nth; t++) pool.emplace_back(worker);
+  for (auto &t : pool) t.join();
 }
 
 }  // namespace cbrkc

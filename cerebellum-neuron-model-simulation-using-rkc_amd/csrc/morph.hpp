@@ -55,4 +55,10 @@ struct PkjHost {
 bool build_pkj_host(const std::vector<MorphRow> &rows, int nx, int ny, const std::vector<int> *gj_pre,
                     const std::vector<int> *gj_post, const double *v0, PkjHost &out, std::string &err);
 
+// Spectral radius of each cell's cable operator C^-1 L (the RKC Jacobian
+// with the ionic current frozen), from the CSR rows of the cell without
+// intercell entries: Lanczos with full reorthogonalisation on the symmetric
+// form C^-1/2 L C^-1/2 (80 steps), largest Ritz value.
+void cable_spectral_radius(const PkjHost &P, std::vector<double> &rho);
+
 }  // namespace cbrkc

@@ -72,6 +72,8 @@ typedef struct cb_pop_config {
   double ion_interval_ms;   /* accepted time between ion updates; <= 0: 2^log10(atol) (solve_rkc.cu:1265) */
   int refresh_fn;           /* 1: recompute fn = F(y_n) after each ion update (the reference keeps
                                the stale value, solve_rkc.cu:1229-1250); 0: reference behaviour */
+  int rho_floor;            /* 1: spectral-radius estimates are raised to at least 1.2 x the exact
+                               cable spectral radius (see cb_pkj_cable_spectral_radius); 0: reference */
   int device;               /* HIP device ordinal */
 } cb_pop_config;
 
@@ -110,6 +112,19 @@ int cb_pop_size(const cb_pop *p, int *ncell, int *nc);
  * iterations, ion updates, macro steps (6 values). */
 int cb_pop_stats(const cb_pop *p, int64_t *out6);
 
+/* Device time of the engine's main kernel, measured with HIP events on the
+ * stream it is launched on: accumulated milliseconds and launch count since
+ * the last reset (reset != 0 clears them after reading).  No counterpart in
+ * the reference, which only times the whole run with clock() (main.cu:178-189). */
+int cb_pop_kernel_time(cb_pop *p, double *ms, int64_t *launches, int reset);
+
+/* Per-cell engines: times every launch mapping (threads per cell x
+ * compartments per thread) on a trial of trial_steps macro steps from the
+ * current state, keeps the fastest and restores the state; *threads gets the
+ * chosen threads per cell (0 for engines with a single mapping).  No
+ * counterpart in the reference. */
+int cb_pop_autotune(cb_pop *p, int trial_steps, int *threads);
+
 /* Global-controller scalars: h_others[14] and h_work[8] (param.h:22,
  * solve_rkc.cu:453-476).  CB_E_UNSUP for per-cell engines. */
 int cb_pop_controller(const cb_pop *p, double *others14, double *hwork8);
@@ -121,6 +136,15 @@ int cb_pop_controller(const cb_pop *p, double *others14, double *hwork8);
  * (dscale NULL: 1). */
 int cb_synth_pkj_morph(const char *path, uint64_t seed, int ncell, double soma_r_um, double lmin_um,
                        double lmax_um, double d0_um, double d1_um, const double *dscale);
+
+/* Host only: spectral radius of each cell's cable operator C^-1 L (the RKC
+ * Jacobian with the ionic current frozen) for the ncell cells of a Pkj.txt
+ * morphology (one block replicated, or one block per cell) into rho[ncell].
+ * The reference estimates it with a loosely converged power method
+ * (solve_rkc.cu:562-754), which can fall short of the true value on
+ * synthetic trees; 1.2 x this value is the floor applied when the
+ * configuration sets rho_floor. */
+int cb_pkj_cable_spectral_radius(const char *morph_path, int ncell, double *rho);
 
 /* Philox4x32-10 block (Salmon et al., SC'11), the generator behind every
  * seeded synthetic input. */

@@ -301,6 +301,7 @@ static void rkclow(or_pop *p)
     if (loop_num > 1000) { return; }
     if (p->newspc) {
       rkcrho(p);
+      if (p->others[OR_SPRAD] < p->rho_floor) p->others[OR_SPRAD] = p->rho_floor;
       p->jacatt = 1;
     }
     p->last = 0;
@@ -444,6 +445,7 @@ int or_rkc_init(or_pop *p)
   p->others[OR_ATOL] = at;
   p->others[OR_RTOL] = rt;
   rkcrho(p);
+  if (p->others[OR_SPRAD] < p->rho_floor) p->others[OR_SPRAD] = p->rho_floor;
   if (set_initial_absh(p) == 3) return 3;
   p->sum_dt = 0.0;
   p->integ = pow(2.0, log10(p->others[OR_ATOL]));   /* solve_rkc.cu:1265 */
@@ -511,6 +513,18 @@ void or_model_update_ion_and_b(or_pop *p, double sum_dt)
 }
 
 void or_pop_set_refresh_fn(or_pop *p, int on) { p->refresh_fn = on; }
+
+/* Sets the spectral-radius floor and re-runs the initial step-size choice
+   (solve_rkc.cu:489-560) with it, as a population created with the floor
+   would have done. */
+void or_pop_set_rho_floor(or_pop *p, double floor_)
+{
+  p->rho_floor = floor_;
+  if (p->others[OR_SPRAD] < floor_) {
+    p->others[OR_SPRAD] = floor_;
+    set_initial_absh(p);
+  }
+}
 
 /* ---------------- stimulus ---------------- */
 void or_set_stim(or_pop *p, int comp, double t0, double t1, const double *amp)

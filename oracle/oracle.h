@@ -74,6 +74,7 @@ typedef struct or_pop {
      index-keyed quirk (pkj_ion.cu:329) and the noise counters line up */
   long gid_offset;
   int refresh_fn;     /* 1: recompute fn = F(y_n) after every ion update */
+  double rho_floor;   /* lower bound applied to every spectral-radius estimate (0: none) */
 } or_pop;
 
 /* ---- deterministic sums (or_rkc.c) ---- */
@@ -102,6 +103,7 @@ int  or_pop_nc(const or_pop *p);
 void or_pop_set_integ(or_pop *p, double integ);
 void or_pop_set_gid_offset(or_pop *p, long off);
 void or_pop_set_refresh_fn(or_pop *p, int on);
+void or_pop_set_rho_floor(or_pop *p, double floor_);
 void or_pop_get(const or_pop *p, int what, double *dst);
 void or_pop_set(or_pop *p, int what, const double *src);
 void or_pop_controller(const or_pop *p, double *others14, double *hwork8, int *flags3, double *sumdt);

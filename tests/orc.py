@@ -56,6 +56,7 @@ def lib():
         L.or_pop_set_integ.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.or_pop_set_gid_offset.argtypes = [ctypes.c_void_p, ctypes.c_long]
         L.or_pop_set_refresh_fn.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_pop_set_rho_floor.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.or_pop_poisson_step.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_double]
         u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -162,10 +163,30 @@ class Pop:
     def gid_offset(self, off):
         lib().or_pop_set_gid_offset(self.h, off)
 
-    def stable_split(self, ion_interval=2.0 ** -8):
-        """ion interval + fn refresh used for the synthetic Purkinje workloads"""
+    def stable_split(self, ion_interval=2.0 ** -8, rho_floor=True):
+        """the safeguards used for the synthetic Purkinje workloads: ion interval,
+        fn refresh and the spectral-radius floor 1.2 x the exact cable radius
+        (computed here with scipy's ARPACK, independently of the product)"""
         lib().or_pop_set_integ(self.h, ion_interval)
         lib().or_pop_set_refresh_fn(self.h, 1)
+        if rho_floor:
+            lib().or_pop_set_rho_floor(self.h, 1.2 * max(self.cable_rho()))
+
+    def cable_rho(self):
+        """per-cell spectral radius of C^-1 L from the oracle's own CSR (scipy eigsh)"""
+        import scipy.sparse as sp
+        import scipy.sparse.linalg as sla
+        row, col, val = self.csr()
+        cm = self.get(CM)
+        out = []
+        for c in range(self.n):
+            s, e = c * self.comp, (c + 1) * self.comp
+            L = sp.csr_matrix((val, col, row), shape=(self.nc, self.nc))[s:e, s:e]
+            d = np.asarray(L.sum(axis=1)).ravel() - L.diagonal()
+            L = L - sp.diags(L.diagonal()) - sp.diags(d)   # cell-local diagonal (drops intercell terms)
+            S = sp.diags(cm[s:e] ** -0.5) @ L @ sp.diags(cm[s:e] ** -0.5)
+            out.append(float(sla.eigsh(-S, k=1, which="LA", return_eigenvectors=False, tol=1e-12)[0]))
+        return out
 
     def enable_poisson(self, rate_hz=5.0, g_mS=1e-6, tau_ms=2.0, e_mV=0.0, seed=0):
         lib().or_pop_enable_syn(self.h, e_mV)

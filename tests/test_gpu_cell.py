@@ -23,7 +23,7 @@ from trace_metrics import compare
 
 pytestmark = pytest.mark.gpu
 
-STABLE = dict(ion_interval=2.0 ** -8, refresh_fn=True)
+STABLE = cbrkc.STABLE
 
 
 def test_cell_fp64_matches_oracle_cell_by_cell(synth_dir):

@@ -19,14 +19,17 @@ from trace_metrics import compare
 pytestmark = pytest.mark.gpu
 
 
-def make_pair(morph, nx, ny, v0, stim, atol, rtol, ion_interval=None, refresh_fn=False):
+def make_pair(morph, nx, ny, v0, stim, atol, rtol, ion_interval=None, refresh_fn=False, rho_floor=False):
     o = orc.Pop.pkj(morph, nx, ny, v0=v0, atol=atol, rtol=rtol, libc_rand=False)
     o.set_stim(*stim)
     if ion_interval:
         orc.lib().or_pop_set_integ(o.h, ion_interval)
     orc.lib().or_pop_set_refresh_fn(o.h, 1 if refresh_fn else 0)
+    if rho_floor:
+        orc.lib().or_pop_set_rho_floor(o.h, 1.2 * max(o.cable_rho()))
     g = cbrkc.Population(cbrkc.PKJ, nx, ny, morph=morph, v0=v0, atol=atol, rtol=rtol, stim=stim,
-                         engine=cbrkc.ENGINE_GLOBAL_FP64, ion_interval=ion_interval, refresh_fn=refresh_fn)
+                         engine=cbrkc.ENGINE_GLOBAL_FP64, ion_interval=ion_interval, refresh_fn=refresh_fn,
+                         rho_floor=rho_floor)
     return o, g
 
 
@@ -112,8 +115,7 @@ def test_synthetic_trees_three_cells(synth_dir):
     v0 = np.array([-68.5, -67.0, -68.9])
     # rtol 1e-6 (TOL_PKJ); the synthetic trees need the 2^-8 ms ion interval and
     # the fn refresh to stay stable under the split scheme
-    o, g = make_pair(morph, 3, 1, v0, (1599, 1.0, 1e9, [5e-4, 1e-3, 2e-3]), 1e-6, 1e-6, ion_interval=2.0 ** -8,
-                     refresh_fn=True)
+    o, g = make_pair(morph, 3, 1, v0, (1599, 1.0, 1e9, [5e-4, 1e-3, 2e-3]), 1e-6, 1e-6, **cbrkc.STABLE)
     worst, (counts, dt, sub) = run_compare(o, g, 80, n=3)
     print("oracle", o.stats(), "gpu", g.stats(), "max|dV|", worst, "counts", counts.tolist(), "dt", dt, "sub", sub,
           flush=True)
