@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <type_traits>
 #include <vector>
 
 #include "engine.hpp"
@@ -45,6 +46,10 @@ namespace {
 constexpr double UROUND = 2.22e-16;
 constexpr double H_MAX = 0.125;
 constexpr int NO_PARENT = 0x7FF;
+// per-compartment flag byte: compartment type in the low bits, plus the
+// reference's index-keyed fixed somatic Ca reversal (pkj_ion.cu:329-331)
+constexpr int CLS_MASK = 0x7;
+constexpr int FIXED_REV = 0x10;
 
 // per-cell controller record (doubles)
 enum Ctl {
@@ -52,25 +57,37 @@ enum Ctl {
   C_NEWSPC, C_JACATT, C_LAST, C_IDID, C_STEPS, C_REJECTS, C_STAGES, C_RHO, C_IONS, C_MACRO, C_STATUS, NCTL = 32
 };
 
+// Per-compartment planes of one cell.  All of a cell's per-compartment
+// arrays sit in one slab: plane p of cell c starts at byte
+//   c * slab_stride + p * PITCH * sizeof(T)
+// in breadth-first compartment order, so a workgroup reaches every array of
+// its cell through one buffer resource held in SGPRs plus one 32-bit lane
+// offset, instead of one 64-bit vector address per array.  The pitch is a
+// compile-time constant (the largest cell the 256-thread mapping holds), so
+// every plane offset is a literal rather than a live register.
+constexpr int PITCH = 1792;
+enum Plane {
+  P_GATE = 0,                                    // NGATE gate planes
+  P_AREA = pkj::NGATE, P_LEAK, P_SHELL,          // static
+  P_META,                                        // uint32: orig | flags << 16
+  P_REV, P_VOLD, P_B, P_SYN,                     // ion and current state
+  P_Y0HI, P_Y0LO, P_F0, P_VY, P_EDE,             // integrator state at launch edges
+  P_CA                                           // FP64 Ca, 8 / sizeof(T) planes
+};
+template <typename T> constexpr int n_planes() { return P_CA + (int)(8 / sizeof(T)); }
+
 template <typename T>
 struct CellArgs {
   int ncell, comp;
   double atol, rtol, hmin, hmax, integ;
   int refresh_fn;         // 1: fn = F(y_n) after each ion update
   const double *rho_floor;  // per cell lower bound of the spectral radius (null: none)
-  // static, per compartment, cell-major, breadth-first order
+  // static, per compartment, cell-major, breadth-first order (read into LDS)
   const uint32_t *topo;   // parent | first_child << 11 | nchild << 22
   const T *gpar;          // conductance to the parent [mS]
   const T *invc;          // 1 / Cm
-  const int8_t *cls;      // compartment type
-  const T *area, *leak_area, *shell;
-  const int *orig;        // original local index
-  // state
-  T *y0hi, *y0lo, *f0, *b, *vold, *vy, *ede;
-  T *gate;                // [NGATE][ncell * comp]
-  double *ca;
-  T *rev;
-  T *syn_g;               // may be null
+  unsigned char *slab;    // per-cell compartment planes
+  uint32_t slab_stride;   // bytes per cell
   double *ctl;            // [ncell][NCTL]
   double *vnew_out;       // [ncell * comp] original order, written at launch end
   // stimulus
@@ -89,6 +106,39 @@ struct CellArgs {
 };
 
 template <typename T> __device__ inline T absT(T x) { return x < T(0) ? -x : x; }
+
+// gfx9 buffer resource word 3 (32-bit data format, raw addressing)
+constexpr int RSRC_WORD3 = 0x00020000;
+
+// One cell's slab as a buffer resource: plane offsets go in the scalar
+// offset, the lane's compartment index in the vector offset.
+template <typename T>
+struct Slab {
+  static constexpr int pl = PITCH * (int)sizeof(T);   // bytes per plane
+  __amdgpu_buffer_rsrc_t r;
+  __device__ Slab(unsigned char *base, int bytes) : r(__builtin_amdgcn_make_buffer_rsrc(base, 0, bytes, RSRC_WORD3)) {}
+  __device__ T ld(int p, int i) const {
+    if constexpr (sizeof(T) == 4)
+      return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, i * 4, p * pl, 0));
+    else
+      return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, i * 8, p * pl, 0));
+  }
+  __device__ void st(int p, int i, T v) const {
+    if constexpr (sizeof(T) == 4)
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, i * 4, p * pl, 0);
+    else
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v),
+                                            r, i * 8, p * pl, 0);
+  }
+  __device__ uint32_t meta(int i) const { return __builtin_amdgcn_raw_buffer_load_b32(r, i * 4, P_META * pl, 0); }
+  __device__ double ca(int i) const {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, i * 8, P_CA * pl, 0));
+  }
+  __device__ void set_ca(int i, double v) const {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v),
+                                          r, i * 8, P_CA * pl, 0);
+  }
+};
 
 // The workgroup-wide values (reduction results, the cell index) are equal in
 // every lane; saying so lets the compiler keep the controller in SGPRs.
@@ -164,6 +214,7 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
   L.tp = (uint32_t *)(L.ic + C);
   double *red = (double *)(((uintptr_t)(L.tp + C) + 15) & ~(uintptr_t)15);
   __shared__ int s_cell;
+  __shared__ double s_cnt[6];   // steps, rejects, stages, rho iterations, ion updates, macro steps
   const int tid = threadIdx.x;
   int slot = 0;
 
@@ -174,7 +225,7 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
     __syncthreads();
     if (cell >= a.ncell) break;
     const size_t base = (size_t)cell * C;
-    const size_t NCt = (size_t)a.ncell * C;
+    const Slab<T> S(a.slab + (size_t)cell * a.slab_stride, (int)a.slab_stride);
     double *ctl = a.ctl + (size_t)cell * NCTL;
 
     for (int i = tid; i < C; i += NT) {
@@ -190,11 +241,11 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
     for (int k = 0; k < K; k++) {
       const int i = tid + k * NT;
       if (i < C) {
-        if (a.orig[base + i] == a.rec_orig) rec_mask |= 1u << k;
-        hi[k] = a.y0hi[base + i];
-        lo[k] = a.y0lo[base + i];
-        F0[k] = a.f0[base + i];
-        VY[k] = a.vy[base + i];
+        if ((int)(S.meta(i) & 0xFFFFu) == a.rec_orig) rec_mask |= 1u << k;
+        hi[k] = S.ld(P_Y0HI, i);
+        lo[k] = S.ld(P_Y0LO, i);
+        F0[k] = S.ld(P_F0, i);
+        VY[k] = S.ld(P_VY, i);
       } else {
         hi[k] = lo[k] = F0[k] = VY[k] = T(0);
       }
@@ -208,8 +259,12 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
            sum_dt = ctl[C_SUMDT], mmax = ctl[C_MMAX], hlast = ctl[C_HLAST], ltlast = ctl[C_LTLAST];
     bool newspc = ctl[C_NEWSPC] != 0.0, jacatt = ctl[C_JACATT] != 0.0, last = ctl[C_LAST] != 0.0;
     double idid = ctl[C_IDID];
-    double n_steps = ctl[C_STEPS], n_rej = ctl[C_REJECTS], n_stage = ctl[C_STAGES], n_rho = ctl[C_RHO],
-           n_ion = ctl[C_IONS], n_macro = ctl[C_MACRO];
+    // work counters live in LDS, kept by thread 0 (they need no registers elsewhere)
+    if (tid == 0)
+      for (int q = 0; q < 6; q++) s_cnt[q] = ctl[C_STEPS + q];
+    auto count = [&](int q, double v) {
+      if (tid == 0) s_cnt[q] += v;
+    };
     int status = (int)ctl[C_STATUS];
     const double at = a.atol, rt = a.rtol, hmin = a.hmin, hmax = a.hmax;
     double t_cur = t0;            // time of the last set_current
@@ -233,7 +288,7 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
         if (i < C) {
           T s = lap<T>(L, h, i, hi[k]);
           if constexpr (sizeof(T) == 4) s += lap<T>(L, l, i, lo[k]);
-          A[k] = (-a.b[base + i] + s) * L.ic[i];
+          A[k] = (-S.ld(P_B, i) + s) * L.ic[i];
         }
       }
       __syncthreads();
@@ -245,47 +300,60 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
     // compartment's gates from HBM; one compartment per iteration so the
     // channel arithmetic does not multiply register pressure by K.
     auto ion_and_b = [&](double dt, bool do_ion, bool stim_live) {
+      // the voltage for the update goes through LDS: each thread reads back
+      // only what it wrote, so no barrier is needed
+      T *vyl = L.d1;
 #pragma unroll
       for (int k = 0; k < K; k++) {
         const int i = tid + k * NT;
-        if (i < C) a.vy[base + i] = VY[k];
+        if (i < C) vyl[i] = VY[k];
       }
 #pragma unroll 1
       for (int i = tid; i < C; i += NT) {
-        const size_t gi = base + i;
+        // the flag word first: the loads return in order, so the class
+        // select waits for it alone
+        const uint32_t meta = S.meta(i);
+        const T ar = S.ld(P_AREA, i);
+        const T leak = S.ld(P_LEAK, i);
         T g[NGATE];
 #pragma unroll
-        for (int q = 0; q < NGATE; q++) g[q] = a.gate[(size_t)q * NCt + gi];
-        const int cl = a.cls[gi];
+        for (int q = 0; q < NGATE; q++) g[q] = S.ld(P_GATE + q, i);
+        const int og = (int)(meta & 0xFFFFu);
+        const int fl = (int)(meta >> 16);
+        const int cl = fl & CLS_MASK;
         const int row = (cl == SOMA) ? 0 : (cl == MAIN) ? 1 : 2;
-        const T ar = a.area[gi];
         T c[NCOND];
-        c[g_leak] = (T)(1.0 / ((cl == SOMA) ? RM_S : RM_D)) * a.leak_area[gi];
-#pragma unroll
-        for (int q = 0; q < 10; q++) c[g_NaF + q] = (T)DENSITY[row][q] * ar;
-        T rev = a.rev[gi];
-        const T vn = a.vy[gi];
-        const int og = a.orig[gi];
+        c[g_leak] = (T)(1.0 / ((cl == SOMA) ? RM_S : RM_D)) * leak;
+        densities<T>(row, ar, c, std::make_index_sequence<10>{});
+        T rev = S.ld(P_REV, i);
+        const T vn = vyl[i];
         if (do_ion) {
-          const T vo = a.vold[gi];
-          double ca = a.ca[gi];
-          double rv = (double)rev;
-          const long gid = (long)cell * COMP + og;
-          double caq = ca_rk4((double)vo, ca, rv, (double)a.shell[gi], (double)(c[g_CaP] * g[m_CaP] * g[h_CaP]),
-                              (double)(c[g_CaT] * g[m_CaT] * g[h_CaT]), dt, gid);
-          rev = (T)rv;
-          gates_cn<T>(T(0.5) * (vn + vo), (T)caq, (T)dt, g);
+          const T vo = S.ld(P_VOLD, i);
+          double ca = S.ca(i);
+          if constexpr (sizeof(T) == 4) {
+            // the shell plane holds 1 / (2 F shell) for the mixed engine
+            const double caq = ca_rk4_mixed(vo, ca, rev, (double)S.ld(P_SHELL, i),
+                                            c[g_CaP] * g[m_CaP] * g[h_CaP] + c[g_CaT] * g[m_CaT] * g[h_CaT], dt,
+                                            (fl & FIXED_REV) != 0);
+            gates_cn_fast(0.5f * (vn + vo), (float)caq, (float)dt, g);
+          } else {
+            double rv = rev;
+            const double caq = ca_rk4(vo, ca, rv, S.ld(P_SHELL, i), c[g_CaP] * g[m_CaP] * g[h_CaP],
+                                      c[g_CaT] * g[m_CaT] * g[h_CaT], dt, (fl & FIXED_REV) != 0);
+            rev = rv;
+            gates_cn<T>(T(0.5) * (vn + vo), (T)caq, (T)dt, g);
+          }
 #pragma unroll
-          for (int q = 0; q < NGATE; q++) a.gate[(size_t)q * NCt + gi] = g[q];
-          a.ca[gi] = ca;
-          a.rev[gi] = rev;
-          a.vold[gi] = vn;
+          for (int q = 0; q < NGATE; q++) S.st(P_GATE + q, i, g[q]);
+          S.set_ca(i, ca);
+          S.st(P_REV, i, rev);
+          S.st(P_VOLD, i, vn);
         }
         const T iext = (stim_live && og == a.stim_orig && a.stim_t0 <= t_cur && t_cur < a.stim_t1)
                            ? (T)a.amp[cell] : T(0);
         T bb = current<T>(vn, iext, rev, c, g);
-        if (a.syn_on) bb += a.syn_g[gi] * (vn - a.syn_e);
-        a.b[gi] = bb;
+        if (a.syn_on) bb += S.ld(P_SYN, i) * (vn - a.syn_e);
+        S.st(P_B, i, bb);
       }
       a_valid = false;
     };
@@ -300,7 +368,7 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
 #pragma unroll
       for (int k = 0; k < K; k++) {
         const int i = tid + k * NT;
-        D1[k] = (i < C) ? a.ede[base + i] : T(0);
+        D1[k] = (i < C) ? S.ld(P_EDE, i) : T(0);
         double y = (double)hi[k] + (double)lo[k];
         sy += y * y;
         se += (double)D1[k] * (double)D1[k];
@@ -330,7 +398,7 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
       T *cur = L.d0, *nxt = L.d1;
       double sigma = 0.0, sigmal;
       for (int iter = 0; iter < itmax; iter++) {
-        n_rho += 1;
+        count(3, 1.0);
 #pragma unroll
         for (int k = 0; k < K; k++) {
           const int i = tid + k * NT;
@@ -366,7 +434,7 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
 #pragma unroll
           for (int k = 0; k < K; k++) {
             const int i = tid + k * NT;
-            if (i < C) a.ede[base + i] = D1[k];
+            if (i < C) S.st(P_EDE, i, D1[k]);
           }
           return;
         }
@@ -458,7 +526,7 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
           dzjm2 = dzjm1; dzjm1 = dzj;
           d2zjm2 = d2zjm1; d2zjm1 = d2zj;
         }
-        n_stage += m + 1;
+        count(2, m + 1);
         // ---- f(Y_m) (into D2) and the error estimate, solve_rkc.cu:58-73 ----
         // (every weight is positive: atol > 0 is enforced at creation)
         double sq = 0.0;
@@ -481,7 +549,7 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
         double err = wg_sum<NT>(sq, red, slot);
         err = sqrt(err / (double)C);
         if (err > 1.0) {
-          n_rej += 1;
+          count(1, 1.0);
           absh = 0.8 * absh / pow(err, 1.0 / 3.0);
           if (absh < hmin) {
             idid = 4;   // falls through to the accept branch, as the reference does
@@ -491,7 +559,7 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
           }
         }
         // ---- accept ----
-        n_steps += 1;
+        count(0, 1.0);
         naccpt += 1;
         lt += h;
         gt = lt;
@@ -531,7 +599,7 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
       for (int k = 0; k < K; k++) {
         F0[k] = A[k];
         const int i = tid + k * NT;
-        if (i < C) a.ede[base + i] = F0[k];
+        if (i < C) S.st(P_EDE, i, F0[k]);
       }
       rkcrho();
       if (a.rho_floor) sprad = fmax(sprad, a.rho_floor[cell]);
@@ -572,45 +640,39 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
         t_cur = t0 + s * H_MAX;
         if (a.syn_on) {
           for (int i = tid; i < C; i += NT) {
-            const size_t gi = base + i;
-            const u32x4 r = philox4x32_10((uint32_t)(cell * COMP + a.orig[gi]), (uint32_t)(step0 + s), 0x5359u,
+            const int og = (int)(S.meta(i) & 0xFFFFu);
+            const u32x4 r = philox4x32_10((uint32_t)(cell * COMP + og), (uint32_t)(step0 + s), 0x5359u,
                                           0u, a.syn_k0, a.syn_k1);
             const double u = u01_53(r.v[0], r.v[1]);
-            a.syn_g[gi] = a.syn_g[gi] * a.syn_decay + ((u < a.syn_p) ? a.syn_g_event : T(0));
+            S.st(P_SYN, i, S.ld(P_SYN, i) * a.syn_decay + ((u < a.syn_p) ? a.syn_g_event : T(0)));
           }
         }
         // ---- pkj_solve_by_rkc, solve_rkc.cu:1252-1299 ----
-        if (sum_dt != 0.0) {
-          n_ion += 1;
+        // A is rebuilt right after every update, so it is dead across the
+        // update's loop and holds no registers there
+        auto ion_step = [&]() {
+          count(4, 1.0);
           ion_and_b(sum_dt, true, true);
+          compute_A();
           if (a.refresh_fn) {   // fn = F(y_n) with the new b
-            compute_A();
 #pragma unroll
             for (int k = 0; k < K; k++) F0[k] = A[k];
           }
           sum_dt = 0.0;
-        }
+        };
+        if (sum_dt != 0.0) ion_step();
         while (true) {
           rkclow();
           sum_dt += gt - gt_old;
           gt_old = gt;
-          if (sum_dt >= a.integ && idid == 2.0) {
-            n_ion += 1;
-            ion_and_b(sum_dt, true, true);
-          if (a.refresh_fn) {   // fn = F(y_n) with the new b
-            compute_A();
-#pragma unroll
-            for (int k = 0; k < K; k++) F0[k] = A[k];
-          }
-            sum_dt = 0.0;
-          }
+          if (sum_dt >= a.integ && idid == 2.0) ion_step();
           if (idid == 2.0) continue;
           if (idid == 1.0) break;
           status = (int)idid;
           break;
         }
         tend += H_MAX;
-        n_macro += 1;
+        count(5, 1.0);
         // v_new = y_{n+1} (rkcint at s = 1); record the chosen compartment
         if (a.rec) {
 #pragma unroll
@@ -625,12 +687,11 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
     for (int k = 0; k < K; k++) {
       const int i = tid + k * NT;
       if (i < C) {
-        const size_t gi = base + i;
-        a.y0hi[gi] = hi[k];
-        a.y0lo[gi] = lo[k];
-        a.f0[gi] = F0[k];
-        a.vy[gi] = VY[k];
-        a.vnew_out[base + a.orig[gi]] = (double)hi[k] + (double)lo[k];
+        S.st(P_Y0HI, i, hi[k]);
+        S.st(P_Y0LO, i, lo[k]);
+        S.st(P_F0, i, F0[k]);
+        S.st(P_VY, i, VY[k]);
+        a.vnew_out[base + (S.meta(i) & 0xFFFFu)] = (double)hi[k] + (double)lo[k];
       }
     }
     if (tid == 0) {
@@ -638,8 +699,8 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
       ctl[C_HOLD] = hold; ctl[C_LT] = lt; ctl[C_TEND] = tend; ctl[C_GT] = gt; ctl[C_GTOLD] = gt_old;
       ctl[C_SUMDT] = sum_dt; ctl[C_MMAX] = mmax; ctl[C_HLAST] = hlast; ctl[C_LTLAST] = ltlast;
       ctl[C_NEWSPC] = newspc; ctl[C_JACATT] = jacatt; ctl[C_LAST] = last; ctl[C_IDID] = idid;
-      ctl[C_STEPS] = n_steps; ctl[C_REJECTS] = n_rej; ctl[C_STAGES] = n_stage; ctl[C_RHO] = n_rho;
-      ctl[C_IONS] = n_ion; ctl[C_MACRO] = n_macro; ctl[C_STATUS] = status;
+      for (int q = 0; q < 6; q++) ctl[C_STEPS + q] = s_cnt[q];
+      ctl[C_STATUS] = status;
     }
     __syncthreads();
   }
@@ -702,11 +763,14 @@ class CellPkj : public Engine {
       if (nb != C) throw Error("cell " + std::to_string(c) + " is not a single tree");
     }
     std::vector<uint32_t> topo(NC);
-    std::vector<T> gpar(NC), invc(NC), area(NC), leak(NC), shell(NC), y0(NC), zero(NC, T(0)), vv(NC), rev(NC);
-    std::vector<int8_t> cls(NC);
-    std::vector<int> orig(NC);
-    std::vector<double> ca(NC);
-    std::vector<T> gate((size_t)NGATE * NC);
+    std::vector<T> gpar(NC), invc(NC);
+    if (C > PITCH) throw Error("cell too large for the per-cell engine");
+    stride_ = (size_t)n_planes<T>() * PITCH * sizeof(T);
+    std::vector<unsigned char> slab((size_t)n * stride_, 0);
+    auto put_plane = [&](int c, int p, int b, auto v) {
+      std::memcpy(slab.data() + (size_t)c * stride_ + (size_t)p * PITCH * sizeof(T) + (size_t)b * sizeof(v), &v,
+                  sizeof(v));
+    };
     for (int c = 0; c < n; c++) {
       // children count / first child in bfs order
       std::vector<int> fc(C, 0), nch(C, 0);
@@ -729,17 +793,21 @@ class CellPkj : public Engine {
         topo[g] = (uint32_t)pb | ((uint32_t)fc[b] << 11) | ((uint32_t)nch[b] << 22);
         gpar[g] = (T)H.gpar[og];
         invc[g] = (T)(1.0 / H.cm[og]);
-        cls[g] = H.compart[og];
-        area[g] = (T)H.area[og];
+        const int cl = H.compart[og];
+        uint32_t flags = (uint32_t)cl;
+        if ((long)og % 1599 == 0) flags |= FIXED_REV;   // og is the global index here
+        put_plane(c, P_META, b, (uint32_t)(og - c * C) | flags << 16);
+        put_plane(c, P_AREA, b, (T)H.area[og]);
         // leak conductance / (1/RM) recovers the spine-corrected area
-        leak[g] = (T)(H.cond[(size_t)g_leak * NC + og] / (1.0 / ((cls[g] == SOMA) ? RM_S : RM_D)));
-        shell[g] = (T)H.shell[og];
-        orig[g] = og - c * C;
-        y0[g] = (T)H.v[og];
-        vv[g] = (T)H.v[og];
-        rev[g] = (T)H.rev[og];
-        ca[g] = H.ca[og];
-        for (int q = 0; q < NGATE; q++) gate[(size_t)q * NC + g] = (T)H.gate[(size_t)q * NC + og];
+        put_plane(c, P_LEAK, b, (T)(H.cond[(size_t)g_leak * NC + og] / (1.0 / ((cl == SOMA) ? RM_S : RM_D))));
+        // the mixed engine keeps 1 / (2 F shell); the FP64 engine the shell itself
+        put_plane(c, P_SHELL, b, (sizeof(T) == 4) ? (T)(1.0 / (2.0 * FARADAY * H.shell[og])) : (T)H.shell[og]);
+        put_plane(c, P_Y0HI, b, (T)H.v[og]);
+        put_plane(c, P_VOLD, b, (T)H.v[og]);
+        put_plane(c, P_VY, b, (T)H.v[og]);
+        put_plane(c, P_REV, b, (T)H.rev[og]);
+        put_plane(c, P_CA, b, H.ca[og]);
+        for (int q = 0; q < NGATE; q++) put_plane(c, P_GATE + q, b, (T)H.gate[(size_t)q * NC + og]);
       }
     }
     perm_ = perm;
@@ -756,25 +824,8 @@ class CellPkj : public Engine {
     a_.topo = put(topo);
     a_.gpar = put(gpar);
     a_.invc = put(invc);
-    a_.cls = put(cls);
-    a_.area = put(area);
-    a_.leak_area = put(leak);
-    a_.shell = put(shell);
-    a_.orig = put(orig);
-    a_.y0hi = put(y0);
-    a_.y0lo = put(zero);
-    a_.f0 = put(zero);
-    a_.b = put(zero);
-    a_.vold = put(vv);
-    a_.vy = put(vv);
-    a_.ede = put(zero);
-    a_.gate = put(gate);
-    a_.ca = put(ca);
-    a_.rev = put(rev);
-    a_.syn_g = nullptr;
-    if (syn.on) {
-      a_.syn_g = put(zero);
-    }
+    a_.slab = put(slab);
+    a_.slab_stride = (uint32_t)stride_;
     std::vector<double> ctl((size_t)n * NCTL, 0.0);
     for (int c = 0; c < n; c++) {
       double *q = &ctl[(size_t)c * NCTL];
@@ -874,36 +925,42 @@ class CellPkj : public Engine {
       CB_HIP(hipMemcpy(dst, vnew_, sizeof(double) * NC, hipMemcpyDeviceToHost));
       return;
     }
-    std::vector<double> tmp(NC);
-    auto fetchT = [&](const T *src) {
+    // one plane of every cell, in breadth-first order
+    auto plane = [&](int p, auto *h) {
+      using U = std::remove_pointer_t<decltype(h)>;
+      CB_HIP(hipMemcpy2D(h, sizeof(U) * C_, a_.slab + (size_t)p * PITCH * sizeof(T), stride_, sizeof(U) * C_, n_,
+                         hipMemcpyDeviceToHost));
+    };
+    auto fetchT = [&](int p) {
       std::vector<T> h(NC);
-      CB_HIP(hipMemcpy(h.data(), src, sizeof(T) * NC, hipMemcpyDeviceToHost));
+      plane(p, h.data());
       for (size_t g = 0; g < NC; g++) dst[perm_[g]] = (double)h[g];
     };
     switch (field) {
-      case F_V: fetchT(a_.vold); return;
-      case F_REV: fetchT(a_.rev); return;
-      case F_Y: fetchT(a_.vy); return;
-      case F_FN: fetchT(a_.f0); return;
+      case F_V: fetchT(P_VOLD); return;
+      case F_REV: fetchT(P_REV); return;
+      case F_Y: fetchT(P_VY); return;
+      case F_FN: fetchT(P_F0); return;
       case F_YN: {
         std::vector<T> h(NC), l(NC);
-        CB_HIP(hipMemcpy(h.data(), a_.y0hi, sizeof(T) * NC, hipMemcpyDeviceToHost));
-        CB_HIP(hipMemcpy(l.data(), a_.y0lo, sizeof(T) * NC, hipMemcpyDeviceToHost));
+        plane(P_Y0HI, h.data());
+        plane(P_Y0LO, l.data());
         for (size_t g = 0; g < NC; g++) dst[perm_[g]] = (double)h[g] + (double)l[g];
         return;
       }
       case F_CTL:
         CB_HIP(hipMemcpy(dst, a_.ctl, sizeof(double) * (size_t)n_ * NCTL, hipMemcpyDeviceToHost));
         return;
-      case F_B: fetchT(a_.b); return;
+      case F_B: fetchT(P_B); return;
       case F_CA: {
-        CB_HIP(hipMemcpy(tmp.data(), a_.ca, sizeof(double) * NC, hipMemcpyDeviceToHost));
-        for (size_t g = 0; g < NC; g++) dst[perm_[g]] = tmp[g];
+        std::vector<double> h(NC);
+        plane(P_CA, h.data());
+        for (size_t g = 0; g < NC; g++) dst[perm_[g]] = h[g];
         return;
       }
       default:
         if (field >= F_GATE0 && field < F_GATE0 + NGATE) {
-          fetchT(a_.gate + (size_t)(field - F_GATE0) * NC);
+          fetchT(P_GATE + (field - F_GATE0));
           return;
         }
     }
@@ -974,14 +1031,7 @@ class CellPkj : public Engine {
     s.saved.clear();
   }
   std::vector<std::pair<void *, size_t>> mutable_bufs() {
-    const size_t NC = nc_;
-    std::vector<std::pair<void *, size_t>> v = {
-        {a_.y0hi, sizeof(T) * NC}, {a_.y0lo, sizeof(T) * NC}, {a_.f0, sizeof(T) * NC}, {a_.b, sizeof(T) * NC},
-        {a_.vold, sizeof(T) * NC}, {a_.vy, sizeof(T) * NC},   {a_.ede, sizeof(T) * NC},
-        {a_.gate, sizeof(T) * NC * pkj::NGATE},               {a_.ca, sizeof(double) * NC},
-        {a_.rev, sizeof(T) * NC},  {a_.ctl, sizeof(double) * (size_t)n_ * NCTL}};
-    if (a_.syn_g) v.push_back({a_.syn_g, sizeof(T) * NC});
-    return v;
+    return {{a_.slab, stride_ * n_}, {a_.ctl, sizeof(double) * (size_t)n_ * NCTL}};
   }
 
   static int nint_ori(double x) {
@@ -1081,6 +1131,7 @@ class CellPkj : public Engine {
   }
 
   int n_, C_, nc_ = 0;
+  size_t stride_ = 0;   // slab bytes per cell
   PoissonSpec syn_;
   hipStream_t st_;
   CellArgs<T> a_{};

@@ -221,7 +221,8 @@ __global__ void __launch_bounds__(256) k_pkj_ion_b(PkjState S, const double *__r
     const double vo = S.v[i];
     const double vm = 0.5 * (vn + vo);
     double ca = S.ca[i];
-    double caq = ca_rk4(vo, ca, rev, S.shell[i], c[g_CaP] * g[m_CaP] * g[h_CaP], c[g_CaT] * g[m_CaT] * g[h_CaT], DT, i);
+    double caq = ca_rk4(vo, ca, rev, S.shell[i], c[g_CaP] * g[m_CaP] * g[h_CaP], c[g_CaT] * g[m_CaT] * g[h_CaT], DT,
+                        i % 1599 == 0);
     gates_cn<double>(vm, caq, DT, g);
 #pragma unroll
     for (int k = 0; k < NGATE; k++) S.gate[(size_t)k * nc + i] = g[k];

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <utility>
+
 namespace cbrkc {
 namespace pkj {
 
@@ -39,6 +41,19 @@ constexpr double DENSITY[3][10] = {
     {0.0, 0.0, 4.5, 0.5, 0.0, 60.0, 0.01, 2.0, 80.0, 0.39},
     {0.0, 0.0, 4.5, 0.5, 0.0, 0.0, 0.013, 0.0, 80.0, 0.39},
 };
+
+// c[g_NaF + Q] = DENSITY[row][Q] * area with the densities as immediates:
+// selecting among literals keeps the table out of memory, so no dependent
+// load sits in the ion update.
+template <typename T, int Q>
+__host__ __device__ inline T density_of(int row) {
+  constexpr T d0 = T(DENSITY[0][Q]), d1 = T(DENSITY[1][Q]), d2 = T(DENSITY[2][Q]);
+  return row == 0 ? d0 : (row == 1 ? d1 : d2);
+}
+template <typename T, size_t... Q>
+__host__ __device__ inline void densities(int row, T area, T *c, std::index_sequence<Q...>) {
+  ((c[1 + Q] = density_of<T, (int)Q>(row) * area), ...);
+}
 
 template <typename T> __host__ __device__ inline T ex(T x);
 template <> __host__ __device__ inline double ex<double>(double x) { return exp(x); }
@@ -144,11 +159,11 @@ __host__ __device__ inline void gates_cn(T vm, T caq, T DT, T *g) {
 // Ca shell update (RK4 over dt of dCa/dt = I - B (Ca - Ca0)), the reversal
 // potential and the quantised Ca for KC/K2 (pkj_ion.cu:312-335).  Always
 // FP64: the quantisation floor() is discontinuous in Ca.
-//   gid: global compartment index (the reference keys the fixed somatic
-//        reversal on gid % 1599 == 0).
+//   fixed_rev: the reference keys the fixed somatic reversal on the global
+//              compartment index being a multiple of 1599.
 // Returns the quantised Ca; updates ca and rev.
 __host__ __device__ inline double ca_rk4(double v_old, double &ca, double &rev, double shell,
-                                         double gCaP_open, double gCaT_open, double DT, long gid) {
+                                         double gCaP_open, double gCaT_open, double DT, bool fixed_rev) {
   double Ca_val = ca;
   double I_Ca1 = 1e-3 * (v_old - rev) * (gCaP_open + gCaT_open);
   (I_Ca1 > 0.0) ? (I_Ca1 = 0.0) : (I_Ca1 = -I_Ca1 / (2.0 * FARADAY * shell));
@@ -159,8 +174,104 @@ __host__ __device__ inline double ca_rk4(double v_old, double &ca, double &rev, 
   k1 = Ca_val + (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
   ca = k1;
   Ca_val = 0.5 * (Ca_val + k1);
-  rev = (gid % 1599 == 0) ? 12.5 * log(CA_OUT / CA_0) : 13.361624877 * log(CA_OUT / Ca_val);
+  rev = fixed_rev ? 12.5 * log(CA_OUT / CA_0) : 13.361624877 * log(CA_OUT / Ca_val);
   return floor((Ca_val * 1000 - 0.04) / 0.099986667) * 0.099986667 + 0.04;
+}
+
+// ---------------------------------------------------------------------------
+// FP32 fast path of the ion update for the mixed-precision engine.  Same
+// model, rearranged for the gfx950 VALU: every division is one hardware
+// reciprocal (v_rcp_f32, 1 ulp), every exp((v + C) / D) is one v_exp_f32 of
+// (v + C) * (log2 e / D) with the constant folded, and the Crank-Nicolson
+// step of a two-rate gate is written on the rates,
+//   x' = (2 dt a + (2 - dt s) x) / (2 + dt s),   s = a + b,
+// which is the reference's (2 dt inf + (2 tau - dt) x) / (2 tau + dt) with
+// tau = 1/s, inf = a/s multiplied through by s: one reciprocal instead of
+// three divisions per gate.
+#ifdef __HIP_DEVICE_COMPILE__
+__device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ inline float flog2(float x) { return __builtin_amdgcn_logf(x); }
+#else
+inline float frcp(float x) { return 1.0f / x; }
+inline float fexp2(float x) { return exp2f(x); }
+inline float flog2(float x) { return log2f(x); }
+#endif
+
+constexpr float LOG2E = 1.4426950408889634f;
+
+// A / (B + exp((v + C) / D))
+__host__ __device__ inline float lff(float v, float A, float B, float C, float D) {
+  return A * frcp(B + fexp2((v + C) * (LOG2E / D)));
+}
+
+__host__ __device__ inline void cn_rates(float &x, float a, float b, float dt) {
+  const float s = a + b;
+  x = (2.0f * dt * a + (2.0f - dt * s) * x) * frcp(2.0f + dt * s);
+}
+
+__host__ __device__ inline void cn_inf_tau(float &x, float inf, float tau, float dt) {
+  x = (2.0f * dt * inf + (2.0f * tau - dt) * x) * frcp(2.0f * tau + dt);
+}
+
+__host__ __device__ inline float kdr_alpha_f(float u) {
+  return (fabsf(u) > 1e-6f) ? -0.0235f * u * frcp(fexp2(-u * (LOG2E / 12.0f)) - 1.0f) : 0.0235f * 12.0f;
+}
+
+__host__ __device__ inline void gates_cn_fast(float vm, float caq, float dt, float *g) {
+  cn_rates(g[m_NaF], lff(vm, 35.0f, 0.0f, 5.0f, -10.0f), lff(vm, 7.0f, 0.0f, 65.0f, 20.0f), dt);
+  cn_rates(g[h_NaF], lff(vm, 0.225f, 1.0f, 80.0f, 10.0f), lff(vm, 7.5f, 0.0f, -3.0f, -18.0f), dt);
+  cn_rates(g[m_NaP], lff(vm, 200.0f, 1.0f, -18.0f, -16.0f), lff(vm, 25.0f, 1.0f, 58.0f, 8.0f), dt);
+  cn_rates(g[m_CaP], lff(vm, 8.5f, 1.0f, -8.0f, -12.5f), lff(vm, 35.0f, 1.0f, 74.0f, 14.5f), dt);
+  cn_rates(g[h_CaP], lff(vm, 0.0015f, 1.0f, 29.0f, 8.0f), lff(vm, 0.0055f, 1.0f, 23.0f, -8.0f), dt);
+  cn_rates(g[m_CaT], lff(vm, 2.60f, 1.0f, 21.0f, -8.0f), lff(vm, 0.180f, 1.0f, 40.0f, 4.0f), dt);
+  cn_rates(g[h_CaT], lff(vm, 0.0025f, 1.0f, 40.0f, 8.0f), lff(vm, 0.190f, 1.0f, 50.0f, -10.0f), dt);
+  {
+    const float r = frcp(1.0f + fexp2((vm + 82.0f) * (LOG2E / 7.0f)));
+    cn_inf_tau(g[m_Kh1], 0.8f * r, 7.6f, dt);
+    cn_inf_tau(g[m_Kh2], 0.2f * r, 36.8f, dt);
+  }
+  {
+    const float a = kdr_alpha_f(vm - 8.0f);
+    const float b = 5.0f * fexp2(-(vm + 127.0f) * (LOG2E / 30.0f));
+    const float a2 = kdr_alpha_f(vm + 12.0f);
+    const float b2 = 5.0f * fexp2(-(vm + 147.0f) * (LOG2E / 30.0f));
+    cn_inf_tau(g[m_Kdr], a * frcp(a + b), frcp(a2 + b2), dt);
+  }
+  cn_inf_tau(g[h_Kdr], frcp(1.0f + fexp2((vm + 25.0f) * (LOG2E / 4.0f))), (vm < -25.0f) ? 1200.0f : 10.0f, dt);
+  {
+    const float e = fexp2((vm + 35.0f) * (LOG2E / 20.0f));   // exp((v + 35) / 20)
+    const float ei = frcp(e);
+    cn_inf_tau(g[m_KM], frcp(1.0f + ei * ei), 200.0f * frcp(3.3f * e + ei), dt);
+  }
+  cn_rates(g[m_KA], lff(vm, 1.40f, 1.0f, 27.0f, -12.0f), lff(vm, 0.490f, 1.0f, 30.0f, 4.0f), dt);
+  cn_rates(g[h_KA], lff(vm, 0.0175f, 1.0f, 50.0f, 8.0f), lff(vm, 1.30f, 1.0f, 13.0f, -10.0f), dt);
+  cn_rates(g[m_KC], 7.5f, lff(vm, 0.110f, 0.0f, -35.0f, 14.9f), dt);
+  cn_inf_tau(g[z_KC], frcp(1.0f + 4.00f * frcp(caq)), 10.0f, dt);
+  cn_rates(g[m_K2], 25.0f, lff(vm, 0.075f, 0.0f, 25.0f, 6.0f), dt);
+  cn_inf_tau(g[z_K2], frcp(1.0f + 0.20f * frcp(caq)), 10.0f, dt);
+}
+
+// Ca shell update for the mixed engine: the Ca state, its RK4 and its
+// quantisation stay FP64 (the quantisation floor is discontinuous in Ca);
+// divisions become products with host-computed reciprocals and the
+// reversal potential uses an FP32 log.  inv2fs = 1 / (2 F shell).
+__host__ __device__ inline double ca_rk4_mixed(float v_old, double &ca, float &rev, double inv2fs, float gopen,
+                                               double DT, bool fixed_rev) {
+  double Ca_val = ca;
+  double I_Ca1 = 1e-3 * (double)((v_old - rev) * gopen);
+  I_Ca1 = (I_Ca1 > 0.0) ? 0.0 : -I_Ca1 * inv2fs;
+  const double k1 = DT * (I_Ca1 - B_CA * (Ca_val - CA_0));
+  const double k2 = DT * (I_Ca1 - B_CA * (Ca_val + 0.5 * k1 - CA_0));
+  const double k3 = DT * (I_Ca1 - B_CA * (Ca_val + 0.5 * k2 - CA_0));
+  const double k4 = DT * (I_Ca1 - B_CA * (Ca_val + k3 - CA_0));
+  const double cn = Ca_val + (k1 + k2 * 2.0 + k3 * 2.0 + k4) * (1.0 / 6.0);
+  ca = cn;
+  Ca_val = 0.5 * (Ca_val + cn);
+  // 13.361624877 ln(2.4 / Ca) = 13.361624877 ln 2 log2(2.4 / Ca)
+  rev = fixed_rev ? 137.52624801505297f /* 12.5 ln(2.4 / 4e-5) */
+                  : 9.261572611192175f * flog2((float)(CA_OUT / Ca_val));
+  return floor((Ca_val * 1000 - 0.04) * (1.0 / 0.099986667)) * 0.099986667 + 0.04;
 }
 
 // Frozen membrane current b at voltage v (solve_rkc.cu:369-395), summed in
