@@ -27,13 +27,19 @@ import numpy as np  # noqa: E402
 METRIC = "compartment-steps/s @1024 PCs RKC: mixed-FP32 vs FP64, ≤0.1 mV error"
 H_MAX = 0.125
 COMP = 1600
-# algorithmic work per unit (mixed engine; see DESIGN.md "Roofline accounting")
-FLOP_STAGE = 16.0        # per compartment-stage: 2-3 neighbour terms, /C, stage combination
-FLOP_ION = 335.0         # per compartment-ion-update: ~45 exp/log + ~40 div + ~250 other
-FLOP_RHO = 12.0          # per compartment-power-iteration
-BYTES_ION = 197.0        # per compartment-ion-update: gates r/w (144), Ca (16), rev, V, b, params
-BYTES_STEP = 8.0         # per compartment-accepted-step: b read for A, ion voltage write
-BYTES_LAUNCH = 64.0      # per compartment-launch: state load + store
+# Algorithmic work per unit, counted from the method (s = 4 bytes per FP32 value):
+#   compartment-RHS (one RHS evaluation of one compartment, fused with its stage
+#     update): reads Y_{j-1}, Y_{j-2}, Y_0, F_0, b, 1/C, g_parent + parent index,
+#     writes Y_j: 8s + 4 = 36 B; ~16 flop
+#   compartment-accepted-step (error norm): reads y, y_n, f_n, f(y): 4s = 16 B; ~5 flop
+#   compartment-ion-update: reads V_old, V_new, Ca, rev, shell, 18 gates, type
+#     byte; writes 18 gates, Ca, rev, b: 44s + 1 = 177 B; ~335 flop
+BYTES_RHS = 36.0
+BYTES_STEP = 16.0
+BYTES_ION = 177.0
+FLOP_RHS = 16.0
+FLOP_STEP = 5.0
+FLOP_ION = 335.0
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 
@@ -80,6 +86,40 @@ def work(s0, s1):
     return {k: s1[k] - s0[k] for k in s0}
 
 
+def error_report(compare, ref, tr, name, early_ms=25.0):
+    """Spike counts, spike-time shifts and subthreshold |dV| of trace `tr`
+    against `ref` over the whole run, plus the pointwise |dV| over the first
+    `early_ms` of the timed run (before chaotic divergence can grow)."""
+    counts, dt, sub = compare(ref, tr)
+    k = min(len(ref), int(round(early_ms / H_MAX)))
+    return {
+        "spike_counts_identical": bool(np.all(counts[:, 0] == counts[:, 1])),
+        "spikes_fp64": int(counts[:, 0].sum()), f"spikes_{name}": int(counts[:, 1].sum()),
+        "cells_with_count_mismatch": int(np.sum(counts[:, 0] != counts[:, 1])),
+        "max_spike_time_diff_ms": float(dt.max()),
+        "median_spike_time_diff_ms": float(np.median(dt)),
+        "max_subthreshold_abs_dV_mV": float(sub.max()),
+        "median_per_cell_subthreshold_abs_dV_mV": float(np.median(sub)),
+        "cells_subthreshold_within_0.1mV": int(np.sum(sub <= 0.1)),
+        f"max_abs_dV_mV_first_{early_ms:g}ms": float(np.max(np.abs(ref[:k] - tr[:k]))),
+        "max_abs_dV_mV_all_samples": float(np.max(np.abs(ref - tr)))}
+
+
+def measured_traffic(nt, cell_steps_per_launch):
+    """HBM bytes per launch of the run kernel: the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by a plain-copy calibration) gives
+    bytes per cell-macro-step for the 512-thread mapping; times this launch's
+    cell-steps.  None for another mapping or when the summary is absent."""
+    f = os.path.join(ROOT, "profiles", "k_cell_run_1024pkj_traffic.md")
+    if nt != 512 or not os.path.exists(f):
+        return None
+    with open(f) as fh:
+        for line in fh:
+            if line.startswith("| HBM bytes per cell-macro-step |"):
+                return round(float(line.split("|")[2]) * cell_steps_per_launch)
+    return None
+
+
 def cpu_baseline(morph_file, v0, amp, args):
     """The oracle (FP64 C restatement, population-global controller, OpenMP) on
     the first cells of the same population for a bounded wall time."""
@@ -124,11 +164,26 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=40)
     ap.add_argument("--cpu-cells", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--threads", type=int, default=0,
+                    help="threads per cell of the mixed engine (0: autotune among 256/320/512)")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-noise-floor", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--morph", default=None, help="reuse an existing morphology file")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (the reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": args.copy_calibration << 20, "repeats": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
@@ -148,8 +203,8 @@ def main():
     print(f"[bench] mixed engine: {n} cells, warmup {args.warmup}, timed {args.steps} macro steps", file=sys.stderr,
           flush=True)
     pm = make_pop(cbrkc, cbrkc.ENGINE_CELL_MIXED, morph, n, v0, amp, args)
-    nt = pm.autotune(4)
-    pm.run(args.warmup, 0.0, 1599, record=False)
+    nt = pm.autotune(4, threads=args.threads)
+    timed_run(pm, args.warmup, args.chunk, 0.0, record=False)
     pm.kernel_time(reset=True)
     s0 = pm.stats()
     tr_m, wall_m = timed_run(pm, args.steps, args.chunk, t0)
@@ -157,24 +212,27 @@ def main():
     wm = work(s0, pm.stats())
     value = n * COMP * args.steps / wall_m
 
-    # roofline of the persistent kernel over the timed region
-    flops = COMP * (FLOP_STAGE * wm["stages"] + FLOP_ION * wm["ion_updates"] + FLOP_RHO * wm["rho_iters"])
-    bytes_ = COMP * (BYTES_ION * wm["ion_updates"] + BYTES_STEP * wm["steps"]) + BYTES_LAUNCH * n * COMP * nlaunch
+    # roofline of the persistent kernel over the timed region: algorithmic
+    # bytes and flops of the work it did, over its event-timed duration
+    rhs = wm["stages"] - wm["steps"] - wm["rejects"] + wm["rho_iters"]
+    bytes_ = COMP * (BYTES_RHS * rhs + BYTES_STEP * wm["steps"] + BYTES_ION * wm["ion_updates"])
+    flops = COMP * (FLOP_RHS * rhs + FLOP_STEP * wm["steps"] + FLOP_ION * wm["ion_updates"])
     ksec = kms * 1e-3
-    gbs = bytes_ / nlaunch / (ksec / nlaunch) / 1e9
+    gbs = bytes_ / ksec / 1e9
     tfl = flops / ksec / 1e12
+    traffic = measured_traffic(nt, n * args.steps / nlaunch)
     roof = {"bound": "hbm", "achieved": round(gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(gbs / PEAK_HBM_GBS, 5), "traffic": None,
-            "kernel": f"k_cell<float,{nt}> (persistent, one workgroup per cell)",
+            "frac": round(gbs / PEAK_HBM_GBS, 5), "traffic": traffic,
+            "kernel": f"k_cell_run<float,{nt}> (persistent, one workgroup per cell)",
             "kernel_ms_per_launch": round(kms / nlaunch, 4), "launches": nlaunch,
-            "algorithmic_bytes_per_launch": round(bytes_ / nlaunch)}
+            "macro_steps_per_launch": args.steps / nlaunch,
+            "algorithmic_bytes_per_launch": round(bytes_ / nlaunch),
+            "compartment_rhs_per_launch": round(COMP * rhs / nlaunch),
+            "compartment_ion_updates_per_launch": round(COMP * wm["ion_updates"] / nlaunch)}
     roof_valu = {"achieved": round(tfl, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                  "frac": round(tfl / PEAK_FP32_TFLOPS, 5),
                  "algorithmic_flops_per_launch": round(flops / nlaunch)}
-    if roof_valu["frac"] > roof["frac"]:
-        roof["binding"] = "fp32-valu"
-    else:
-        roof["binding"] = "hbm"
+    roof["binding"] = "fp32-valu" if roof_valu["frac"] > roof["frac"] else "hbm"
 
     out = {
         "metric": METRIC, "value": value, "unit": "compartment-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -195,20 +253,24 @@ def main():
     if not args.no_fp64:
         print("[bench] per-cell FP64 engine on the same run", file=sys.stderr, flush=True)
         pd = make_pop(cbrkc, cbrkc.ENGINE_CELL_FP64, morph, n, v0, amp, args)
-        pd.run(args.warmup, 0.0, 1599, record=False)
+        pd.autotune(0, threads=256)
+        timed_run(pd, args.warmup, args.chunk, 0.0, record=False)
         s0 = pd.stats()
         tr_d, wall_d = timed_run(pd, args.steps, args.chunk, t0)
-        counts, dt, sub = compare(tr_d, tr_m)
         out["fp64_cell"] = {"value": n * COMP * args.steps / wall_d, "wall_s": wall_d, "work": work(s0, pd.stats())}
-        out["error_vs_fp64"] = {
-            "spike_counts_identical": bool(np.all(counts[:, 0] == counts[:, 1])),
-            "spikes_fp64": int(counts[:, 0].sum()), "spikes_mixed": int(counts[:, 1].sum()),
-            "cells_with_count_mismatch": int(np.sum(counts[:, 0] != counts[:, 1])),
-            "max_spike_time_diff_ms": float(dt.max()),
-            "max_subthreshold_abs_dV_mV": float(sub.max()),
-            "median_per_cell_subthreshold_abs_dV_mV": float(np.median(sub)),
-            "max_abs_dV_mV_all_samples": float(np.max(np.abs(tr_d - tr_m)))}
+        out["error_vs_fp64"] = error_report(compare, tr_d, tr_m, "mixed")
         del pd
+        if not args.no_noise_floor:
+            # the same FP64 run with another thread mapping: only the order of
+            # the norm reductions differs, so this is the FP64 model's own
+            # sensitivity to rounding over the same horizon
+            print("[bench] FP64 noise floor (FP64 engine, 320 threads per cell)", file=sys.stderr, flush=True)
+            pf = make_pop(cbrkc, cbrkc.ENGINE_CELL_FP64, morph, n, v0, amp, args)
+            pf.autotune(0, threads=320)
+            timed_run(pf, args.warmup, args.chunk, 0.0, record=False)
+            tr_f, _ = timed_run(pf, args.steps, args.chunk, t0)
+            out["fp64_noise_floor"] = error_report(compare, tr_d, tr_f, "fp64_reordered")
+            del pf
 
     # ---- plain FP64 population-global RKC ----
     if not args.no_plain:

@@ -240,8 +240,9 @@ class Population:
         _check(lib().cb_pop_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(n), 1 if reset else 0))
         return ms.value, n.value
 
-    def autotune(self, trial_steps=8):
-        t = ctypes.c_int()
+    def autotune(self, trial_steps=8, threads=0):
+        """Fastest launch mapping on a trial; threads > 0 selects one instead."""
+        t = ctypes.c_int(threads)
         _check(lib().cb_pop_autotune(self.h, trial_steps, ctypes.byref(t)))
         return t.value
 

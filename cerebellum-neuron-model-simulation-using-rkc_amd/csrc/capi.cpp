@@ -242,6 +242,10 @@ int cb_pop_kernel_time(cb_pop *p, double *ms, int64_t *launches, int reset) {
 int cb_pop_autotune(cb_pop *p, int trial_steps, int *threads) {
   return guarded([&]() -> int {
     if (!p) return fail(CB_E_ARG, "null population");
+    if (threads && *threads > 0) {
+      if (!p->eng->select_mapping(*threads)) return fail(CB_E_ARG, "no such mapping");
+      return CB_OK;
+    }
     int t = p->eng->autotune(trial_steps, -1.0);   // trial from the current time
     if (threads) *threads = t;
     return CB_OK;

@@ -75,6 +75,8 @@ class Engine {
     (void)t0;
     return 0;
   }
+  // selects the mapping with this many threads per cell; false if none
+  virtual bool select_mapping(int threads) { return threads == 0; }
   // event-timed duration of the main kernel (ms) and its launch count
   virtual void kernel_time(double *ms, long *launches, bool reset) {
     *ms = 0.0;
@@ -93,6 +95,11 @@ struct SolverSpec {
   bool refresh_fn = false;      // recompute fn = F(y_n) after each ion update (off in the reference)
   std::vector<double> rho_floor;  // per cell: lower bound applied to every spectral-radius estimate (empty: none)
 };
+
+// RKC stage coefficients per stage count m = 2..mtab: row m holds mtab
+// entries of 4 doubles; entry 0 = {mu~_1}, entry j = {mu_j, nu_j, mu~_j,
+// a_{j-1}} for j = 1..m-1 (solve_rkc.cu:815-906).
+std::vector<double> rkc_stage_table(int mtab);
 
 Engine *make_global_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);
 Engine *make_cell_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn,

@@ -54,8 +54,23 @@ constexpr int FIXED_REV = 0x10;
 // per-cell controller record (doubles)
 enum Ctl {
   C_ABSH, C_SPRAD, C_NACCPT, C_ERROLD, C_HOLD, C_LT, C_TEND, C_GT, C_GTOLD, C_SUMDT, C_MMAX, C_HLAST, C_LTLAST,
-  C_NEWSPC, C_JACATT, C_LAST, C_IDID, C_STEPS, C_REJECTS, C_STAGES, C_RHO, C_IONS, C_MACRO, C_STATUS, NCTL = 32
+  C_NEWSPC, C_JACATT, C_LAST, C_IDID, C_STEPS, C_REJECTS, C_STAGES, C_RHO, C_IONS, C_MACRO, C_STATUS,
+  C_PHASE0 = 24, NCTL = 32
 };
+
+// Optional per-phase cycle counters, built with -DCB_PHASE_CLOCKS: thread 0
+// of each workgroup accumulates clock64() deltas of its own wave into
+// C_PHASE0 + {A rebuild, ion update, power method, step set-up, stage loop,
+// error + controller}.  The default build compiles them out.
+enum Phase { PH_A, PH_ION, PH_RHO, PH_SETUP, PH_STAGES, PH_ERR };
+#ifdef CB_PHASE_CLOCKS
+#define PHASE_T0(v) const unsigned long long v = clock64()
+#define PHASE_ADD(q, v) \
+  if (threadIdx.x == 0) s_ph[q] += (double)(clock64() - (v))
+#else
+#define PHASE_T0(v) (void)0
+#define PHASE_ADD(q, v) (void)0
+#endif
 
 // Per-compartment planes of one cell.  All of a cell's per-compartment
 // arrays sit in one slab: plane p of cell c starts at byte
@@ -66,6 +81,10 @@ enum Ctl {
 // compile-time constant (the largest cell the 256-thread mapping holds), so
 // every plane offset is a literal rather than a live register.
 constexpr int PITCH = 1792;
+
+// stage counts up to this read their coefficients from the table (64 x 64
+// entries of 4 doubles, 128 KiB); larger ones run the recurrence in-kernel
+constexpr int COEF_MMAX = 64;
 enum Plane {
   P_GATE = 0,                                    // NGATE gate planes
   P_AREA = pkj::NGATE, P_LEAK, P_SHELL,          // static
@@ -82,6 +101,8 @@ struct CellArgs {
   double atol, rtol, hmin, hmax, integ;
   int refresh_fn;         // 1: fn = F(y_n) after each ion update
   const double *rho_floor;  // per cell lower bound of the spectral radius (null: none)
+  const double *coef;     // RKC stage coefficients by stage count (rkc_stage_table)
+  int coef_mmax;          // largest tabulated stage count
   // static, per compartment, cell-major, breadth-first order (read into LDS)
   const uint32_t *topo;   // parent | first_child << 11 | nchild << 22
   const T *gpar;          // conductance to the parent [mS]
@@ -106,6 +127,25 @@ struct CellArgs {
 };
 
 template <typename T> __device__ inline T absT(T x) { return x < T(0) ? -x : x; }
+
+// x^(e/3), e = 1 or 2, for the step-size factor (solve_rkc.cu:1006-1066).
+// The FP64 engine calls pow as the reference does.  The mixed engine takes a
+// hardware FP32 cube-root seed (relative error ~1e-6) through two Newton
+// steps in FP64 (~1e-12, then rounding level): three FP64 pow calls per step
+// in every wave were a fifth of the controller's time.  Arguments outside
+// (1e-30, 1e30), where the FP32 seed would under- or overflow, use pow.
+template <typename T>
+__device__ inline double ctl_pow(double x, int e) {
+  if constexpr (sizeof(T) == 8) {
+    return pow(x, e == 1 ? 1.0 / 3.0 : 2.0 / 3.0);
+  } else {
+    if (!(x > 1e-30 && x < 1e30)) return pow(x, e == 1 ? 1.0 / 3.0 : 2.0 / 3.0);
+    double y = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)x) * (1.0f / 3.0f));
+    y = (2.0 * y + x / (y * y)) * (1.0 / 3.0);
+    y = (2.0 * y + x / (y * y)) * (1.0 / 3.0);
+    return e == 1 ? y : y * y;
+  }
+}
 
 // gfx9 buffer resource word 3 (32-bit data format, raw addressing)
 constexpr int RSRC_WORD3 = 0x00020000;
@@ -180,7 +220,9 @@ __device__ inline T lap(const Lds<T> &L, const T *cur, int i, T di) {
   const int p = tp & 0x7FF, fc = (tp >> 11) & 0x7FF, nch = (tp >> 22) & 3;
   T s = T(0);
   if (p != NO_PARENT) s += L.gp[i] * (cur[p] - di);
-  for (int c = 0; c < nch; c++) s += L.gp[fc + c] * (cur[fc + c] - di);
+#pragma unroll
+  for (int c = 0; c < 3; c++)   // at most 3 children: straight-line, predicated
+    if (c < nch) s += L.gp[fc + c] * (cur[fc + c] - di);
   return s;
 }
 
@@ -199,10 +241,10 @@ __device__ inline void two_sum_acc(T &hi, T &lo, T d) {
   }
 }
 
-// Occupancy target: 4 waves per SIMD for FP32 (4 cells per CU), 2 for FP64.
-template <typename T, int NT, int K>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2, 8)))
-k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
+// The body of both kernels: INIT runs RKC_vec_initialize for every cell,
+// otherwise nsteps macro steps are advanced from t0.
+template <typename T, int NT, int K, bool INIT>
+__device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, double t0, int step0) {
   using namespace pkj;
   extern __shared__ __align__(16) unsigned char smem[];
   const int C = a.comp;
@@ -215,6 +257,9 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
   double *red = (double *)(((uintptr_t)(L.tp + C) + 15) & ~(uintptr_t)15);
   __shared__ int s_cell;
   __shared__ double s_cnt[6];   // steps, rejects, stages, rho iterations, ion updates, macro steps
+#ifdef CB_PHASE_CLOCKS
+  __shared__ double s_ph[8];    // cycles per phase, kept by thread 0
+#endif
   const int tid = threadIdx.x;
   int slot = 0;
 
@@ -265,6 +310,10 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
     auto count = [&](int q, double v) {
       if (tid == 0) s_cnt[q] += v;
     };
+#ifdef CB_PHASE_CLOCKS
+    if (tid == 0)
+      for (int q = 0; q < 8; q++) s_ph[q] = ctl[C_PHASE0 + q];
+#endif
     int status = (int)ctl[C_STATUS];
     const double at = a.atol, rt = a.rtol, hmin = a.hmin, hmax = a.hmax;
     double t_cur = t0;            // time of the last set_current
@@ -272,6 +321,7 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
 
     // A = F(y_n) = (-b + L y_n) / C with y_n = hi + lo
     auto compute_A = [&]() {
+      PHASE_T0(tA);
       T *h = L.d0, *l = L.d1;
 #pragma unroll
       for (int k = 0; k < K; k++) {
@@ -293,6 +343,7 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
       }
       __syncthreads();
       a_valid = true;
+      PHASE_ADD(PH_A, tA);
     };
 
     // pkj_update_ion_RKC (pkj_ion.cu:304-358) when do_ion, then the frozen
@@ -300,6 +351,7 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
     // compartment's gates from HBM; one compartment per iteration so the
     // channel arithmetic does not multiply register pressure by K.
     auto ion_and_b = [&](double dt, bool do_ion, bool stim_live) {
+      PHASE_T0(tI);
       // the voltage for the update goes through LDS: each thread reads back
       // only what it wrote, so no barrier is needed
       T *vyl = L.d1;
@@ -356,6 +408,7 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
         S.st(P_B, i, bb);
       }
       a_valid = false;
+      PHASE_ADD(PH_ION, tI);
     };
 
     // rkcrho, solve_rkc.cu:562-754, carried on the perturbation
@@ -459,10 +512,13 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
         if (++loop_num > 1000) return;
         if (!a_valid) compute_A();
         if (newspc) {
+          PHASE_T0(tR);
           rkcrho();
           if (a.rho_floor) sprad = fmax(sprad, a.rho_floor[cell]);
           jacatt = true;
+          PHASE_ADD(PH_RHO, tR);
         }
+        PHASE_T0(tS);
         last = false;
         if (1.1 * absh >= fabs(tend - lt)) {
           absh = fabs(tend - lt);
@@ -477,14 +533,22 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
         double h = absh;
         if (h <= hmin) h = hmin;
         // ---- step(h, m), solve_rkc.cu:815-906, on increments D_j = Y_j - Y_0 ----
-        const double w0 = 1.0 + 2.0 / (13.0 * m * m);
-        const double temp1 = w0 * w0 - 1.0, temp2 = sqrt(temp1);
-        const double arg = m * log(w0 + temp2);
-        const double w1 = sinh(arg) * temp1 / (cosh(arg) * m * temp2 - w0 * sinh(arg));
-        double bjm1 = 1.0 / ((2.0 * w0) * (2.0 * w0)), bjm2 = bjm1;
+        // the coefficients come from the per-m table; only a stage count
+        // beyond it runs the recurrence here
+        const double *ct = (m <= a.coef_mmax) ? a.coef + (size_t)m * a.coef_mmax * 4 : nullptr;
+        double w0 = 0.0, w1 = 0.0, mus1;
+        if (ct) {
+          mus1 = ct[0];
+        } else {
+          w0 = 1.0 + 2.0 / (13.0 * m * m);
+          const double temp1 = w0 * w0 - 1.0, temp2 = sqrt(temp1);
+          const double arg = m * log(w0 + temp2);
+          w1 = sinh(arg) * temp1 / (cosh(arg) * m * temp2 - w0 * sinh(arg));
+          mus1 = w1 * (1.0 / ((2.0 * w0) * (2.0 * w0)));
+        }
         T *cur = L.d0, *nxt = L.d1;
         {
-          const T hm = (T)(h * (w1 * bjm1));
+          const T hm = (T)(h * mus1);
 #pragma unroll
           for (int k = 0; k < K; k++) {
             D2[k] = T(0);
@@ -495,17 +559,10 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
           }
         }
         __syncthreads();
-        double zjm1 = w0, zjm2 = 1.0, dzjm1 = 1.0, dzjm2 = 0.0, d2zjm1 = 0.0, d2zjm2 = 0.0;
-        for (int j = 1; j < m; j++) {
-          const double zj = 2.0 * w0 * zjm1 - zjm2;
-          const double dzj = 2.0 * w0 * dzjm1 - dzjm2 + 2.0 * zjm1;
-          const double d2zj = 2.0 * w0 * d2zjm1 - d2zjm2 + 4.0 * dzjm1;
-          const double bj = d2zj / (dzj * dzj);
-          const double ajm1 = 1.0 - zjm1 * bjm1;
-          const double mu = 2.0 * w0 * bj / bjm1;
-          const double nu = -bj / bjm2;
-          const double musj = mu * w1 / w0;
-          const T tmu = (T)mu, tnu = (T)nu, thm = (T)(h * musj), taj = (T)ajm1;
+        PHASE_ADD(PH_SETUP, tS);
+        PHASE_T0(tL);
+        // stage j: D_j = mu D_{j-1} + nu D_{j-2} + h mu~ (F(Y_{j-1}) - a_{j-1} F_0)
+        auto stage = [&](int j, T tmu, T tnu, T thm, T taj) {
           const bool lastj = (j == m - 1);
 #pragma unroll
           for (int k = 0; k < K; k++) {
@@ -521,12 +578,34 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
           }
           __syncthreads();
           T *tmp = cur; cur = nxt; nxt = tmp;
-          bjm2 = bjm1; bjm1 = bj;
-          zjm2 = zjm1; zjm1 = zj;
-          dzjm2 = dzjm1; dzjm1 = dzj;
-          d2zjm2 = d2zjm1; d2zjm1 = d2zj;
+        };
+        if (ct) {
+          for (int j = 1; j < m; j++) {
+            const double *c4 = ct + 4 * j;
+            stage(j, (T)c4[0], (T)c4[1], (T)(h * c4[2]), (T)c4[3]);
+          }
+        } else {
+          double bjm1 = 1.0 / ((2.0 * w0) * (2.0 * w0)), bjm2 = bjm1;
+          double zjm1 = w0, zjm2 = 1.0, dzjm1 = 1.0, dzjm2 = 0.0, d2zjm1 = 0.0, d2zjm2 = 0.0;
+          for (int j = 1; j < m; j++) {
+            const double zj = 2.0 * w0 * zjm1 - zjm2;
+            const double dzj = 2.0 * w0 * dzjm1 - dzjm2 + 2.0 * zjm1;
+            const double d2zj = 2.0 * w0 * d2zjm1 - d2zjm2 + 4.0 * dzjm1;
+            const double bj = d2zj / (dzj * dzj);
+            const double ajm1 = 1.0 - zjm1 * bjm1;
+            const double mu = 2.0 * w0 * bj / bjm1;
+            const double nu = -bj / bjm2;
+            const double musj = mu * w1 / w0;
+            stage(j, (T)mu, (T)nu, (T)(h * musj), (T)ajm1);
+            bjm2 = bjm1; bjm1 = bj;
+            zjm2 = zjm1; zjm1 = zj;
+            dzjm2 = dzjm1; dzjm1 = dzj;
+            d2zjm2 = d2zjm1; d2zjm1 = d2zj;
+          }
         }
         count(2, m + 1);
+        PHASE_ADD(PH_STAGES, tL);
+        PHASE_T0(tE);
         // ---- f(Y_m) (into D2) and the error estimate, solve_rkc.cu:58-73 ----
         // (every weight is positive: atol > 0 is enforced at creation)
         double sq = 0.0;
@@ -550,11 +629,12 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
         err = sqrt(err / (double)C);
         if (err > 1.0) {
           count(1, 1.0);
-          absh = 0.8 * absh / pow(err, 1.0 / 3.0);
+          absh = 0.8 * absh / ctl_pow<T>(err, 1);
           if (absh < hmin) {
             idid = 4;   // falls through to the accept branch, as the reference does
           } else {
             newspc = !jacatt;
+            PHASE_ADD(PH_ERR, tE);
             continue;
           }
         }
@@ -575,11 +655,11 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
         a_valid = false;   // A is rebuilt from y_{n+1}, so rounding does not accumulate
         double fac = 10.0, tp1, tp2;
         if (naccpt == 1) {
-          tp2 = pow(err, 1.0 / 3.0);
+          tp2 = ctl_pow<T>(err, 1);
           if (0.8 < fac * tp2) fac = 0.8 / tp2;
         } else {
-          tp1 = 0.8 * absh * pow(errold, 1.0 / 3.0);
-          tp2 = fabs(hold) * pow(err, 2.0 / 3.0);
+          tp1 = 0.8 * absh * ctl_pow<T>(errold, 1);
+          tp2 = fabs(hold) * ctl_pow<T>(err, 2);
           if (tp1 < fac * tp2) fac = tp1 / tp2;
         }
         absh = fmax(0.1, fac) * absh;
@@ -587,11 +667,12 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
         errold = err;
         hold = h;
         idid = last ? 1 : 2;
+        PHASE_ADD(PH_ERR, tE);
         return;
       }
     };
 
-    if (mode == 1) {
+    if constexpr (INIT) {
       // ---- RKC_vec_initialize (solve_rkc.cu:756-791): b with i_ext = 0 ----
       ion_and_b(0.0, false, false);
       compute_A();
@@ -701,9 +782,25 @@ k_cell(CellArgs<T> a, int mode, int nsteps, double t0, int step0) {
       ctl[C_NEWSPC] = newspc; ctl[C_JACATT] = jacatt; ctl[C_LAST] = last; ctl[C_IDID] = idid;
       for (int q = 0; q < 6; q++) ctl[C_STEPS + q] = s_cnt[q];
       ctl[C_STATUS] = status;
+#ifdef CB_PHASE_CLOCKS
+      for (int q = 0; q < 8; q++) ctl[C_PHASE0 + q] = s_ph[q];
+#endif
     }
     __syncthreads();
   }
+}
+
+// Occupancy target: 4 waves per SIMD for FP32 (4 cells per CU), 2 for FP64.
+template <typename T, int NT, int K>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2, 8)))
+k_cell_run(CellArgs<T> a, int nsteps, double t0, int step0) {
+  cell_body<T, NT, K, false>(a, nsteps, t0, step0);
+}
+
+template <typename T, int NT, int K>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2, 8)))
+k_cell_init(CellArgs<T> a) {
+  cell_body<T, NT, K, true>(a, 0, 0.0, 0);
 }
 
 template <typename T>
@@ -821,6 +918,8 @@ class CellPkj : public Engine {
     a_.refresh_fn = sol.refresh_fn ? 1 : 0;
     a_.rho_floor = nullptr;
     if (!sol.rho_floor.empty()) a_.rho_floor = put(sol.rho_floor);
+    a_.coef_mmax = COEF_MMAX;
+    a_.coef = put(rkc_stage_table(COEF_MMAX));
     a_.topo = put(topo);
     a_.gpar = put(gpar);
     a_.invc = put(invc);
@@ -999,6 +1098,15 @@ class CellPkj : public Engine {
     return variants_[cur_].nt;
   }
 
+  bool select_mapping(int threads) override {
+    for (size_t v = 0; v < variants_.size(); v++)
+      if (variants_[v].nt == threads && (long)variants_[v].nt * variants_[v].k >= C_) {
+        cur_ = (int)v;
+        return true;
+      }
+    return false;
+  }
+
   struct TuneRec {
     int nt, k;
     double sec;
@@ -1070,8 +1178,11 @@ class CellPkj : public Engine {
 
   template <int NT, int K>
   void go(int mode, int nsteps, double t0, int grid) {
-    hipLaunchKernelGGL((k_cell<T, NT, K>), dim3(grid), dim3(NT), lds_bytes(), st_, a_, mode, nsteps, t0,
-                       (int)macro_);
+    if (mode == 1)
+      hipLaunchKernelGGL((k_cell_init<T, NT, K>), dim3(grid), dim3(NT), lds_bytes(), st_, a_);
+    else
+      hipLaunchKernelGGL((k_cell_run<T, NT, K>), dim3(grid), dim3(NT), lds_bytes(), st_, a_, nsteps, t0,
+                         (int)macro_);
   }
 
   void launch(int mode, int nsteps, double t0) {

@@ -121,8 +121,10 @@ int cb_pop_kernel_time(cb_pop *p, double *ms, int64_t *launches, int reset);
 /* Per-cell engines: times every launch mapping (threads per cell x
  * compartments per thread) on a trial of trial_steps macro steps from the
  * current state, keeps the fastest and restores the state; *threads gets the
- * chosen threads per cell (0 for engines with a single mapping).  No
- * counterpart in the reference. */
+ * chosen threads per cell (0 for engines with a single mapping).  When
+ * *threads > 0 on entry, that mapping is selected without a trial
+ * (CB_E_ARG if the engine has no such mapping).  No counterpart in the
+ * reference. */
 int cb_pop_autotune(cb_pop *p, int trial_steps, int *threads);
 
 /* Global-controller scalars: h_others[14] and h_work[8] (param.h:22,

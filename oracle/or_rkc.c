@@ -635,6 +635,52 @@ void or_pop_poisson_step(or_pop *p, uint32_t step, uint32_t k0, uint32_t k1, dou
   }
 }
 
+/* ---------------- test hooks: a population defined by its matrix ----------------
+ * or_generic_create builds a population of nc compartments from a CSR cable
+ * matrix in the RKC sign convention (off-diagonal +g; the diagonal, if
+ * stored, is skipped by dydt exactly as solve_rkc.cu:177-196 skips nothing
+ * but multiplies a zero difference), capacitances and a frozen current b,
+ * with no ion model: only the RKC core (step, err, rkclow, rkcrho, rkcint)
+ * acts on it.  The hooks below expose the static pieces for the pinning
+ * tests. */
+or_pop *or_generic_create(int nc, const int *row, const int *col, const double *val, const double *cm,
+                          const double *b, const double *v0, double atol, double rtol)
+{
+  or_pop *p = (or_pop *)calloc(1, sizeof(or_pop));
+  p->type = -1; p->n = 1; p->nx = 1; p->ny = 1; p->comp = nc; p->nc = nc;
+  p->v = (double *)calloc(nc, sizeof(double)); p->ca = (double *)calloc(nc, sizeof(double));
+  p->cm = (double *)calloc(nc, sizeof(double)); p->area = (double *)calloc(nc, sizeof(double));
+  p->iext = (double *)calloc(nc, sizeof(double)); p->shell = (double *)calloc(nc, sizeof(double));
+  p->rev = (double *)calloc(nc, sizeof(double)); p->compart = (int *)calloc(nc, sizeof(int));
+  p->b = (double *)calloc(nc, sizeof(double)); p->dammy = (double *)calloc(nc, sizeof(double));
+  for (int i = 0; i < OR_NVEC; i++) p->vec[i] = (double *)calloc(nc, sizeof(double));
+  p->nnz = row[nc];
+  p->row = (int *)malloc(sizeof(int) * (nc + 1)); memcpy(p->row, row, sizeof(int) * (nc + 1));
+  p->col = (int *)malloc(sizeof(int) * p->nnz); memcpy(p->col, col, sizeof(int) * p->nnz);
+  p->val = (double *)malloc(sizeof(double) * p->nnz); memcpy(p->val, val, sizeof(double) * p->nnz);
+  memcpy(p->cm, cm, sizeof(double) * nc);
+  memcpy(p->b, b, sizeof(double) * nc);
+  memcpy(p->v, v0, sizeof(double) * nc);
+  p->hmax = H_MAX;
+  p->hmin = pow(2.0, -10.0);
+  p->others[OR_ATOL] = atol;
+  p->others[OR_RTOL] = rtol;
+  if (or_rkc_init(p) != 0) { or_pop_free(p); return NULL; }
+  return p;
+}
+
+/* step(h, m) from y_n = vec[YN], f_n = vec[FN]: leaves Y_m in vec[Y] */
+void or_rkc_step_test(or_pop *p, double h, int m) { step(p, h, m); }
+/* one rkclow() (one accepted step, possibly after rejects) */
+void or_rkclow_test(or_pop *p) { rkclow(p); }
+/* rkcrho(): the spectral-radius estimate lands in others[SPRAD] */
+void or_rkcrho_test(or_pop *p) { rkcrho(p); }
+/* rkcint at time t between the last two accepted steps into dst */
+void or_rkcint_test(or_pop *p, double t, double *dst) { rkcint(p, dst, p->h_work[0], p->h_work[1], t); }
+/* set controller slot i of others[] / h_work[] */
+void or_pop_set_other(or_pop *p, int i, double v) { p->others[i] = v; }
+void or_pop_set_hwork(or_pop *p, int i, double v) { p->h_work[i] = v; }
+
 void or_pop_free(or_pop *p)
 {
   if (!p) return;

@@ -115,6 +115,16 @@ void or_pop_enable_syn(or_pop *p, double e_syn);
 void or_pop_poisson_step(or_pop *p, uint32_t step, uint32_t k0, uint32_t k1, double decay, double prob,
                          double g_event);
 
+/* ---- test hooks (or_rkc.c): the RKC core on a population given by its matrix ---- */
+or_pop *or_generic_create(int nc, const int *row, const int *col, const double *val, const double *cm,
+                          const double *b, const double *v0, double atol, double rtol);
+void or_rkc_step_test(or_pop *p, double h, int m);
+void or_rkclow_test(or_pop *p);
+void or_rkcrho_test(or_pop *p);
+void or_rkcint_test(or_pop *p, double t, double *dst);
+void or_pop_set_other(or_pop *p, int i, double v);
+void or_pop_set_hwork(or_pop *p, int i, double v);
+
 /* ---- Philox4x32-10 (or_philox.c), Salmon et al., SC'11 ---- */
 void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

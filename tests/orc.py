@@ -62,6 +62,17 @@ def lib():
         u32p = ctypes.POINTER(ctypes.c_uint32)
         L.or_philox4x32_10.argtypes = [u32p, u32p, u32p]
         L.srand.argtypes = [ctypes.c_uint]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.or_generic_create.restype = ctypes.c_void_p
+        L.or_generic_create.argtypes = [ctypes.c_int, ip, ip, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double]
+        L.or_rkc_step_test.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+        L.or_rkclow_test.argtypes = [ctypes.c_void_p]
+        L.or_rkcrho_test.argtypes = [ctypes.c_void_p]
+        L.or_rkcint_test.argtypes = [ctypes.c_void_p, ctypes.c_double, dp]
+        L.or_pop_set_other.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.or_pop_set_hwork.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.or_pkj_update_ion.argtypes = [ctypes.c_void_p, dp, ctypes.c_double]
+        L.or_pkj_update_b.argtypes = [ctypes.c_void_p, dp]
         _lib = L
     return _lib
 
@@ -107,6 +118,61 @@ class Pop:
         if not h:
             raise RuntimeError("oracle pkj_create failed")
         return cls(h, nx * ny, 1600)
+
+    @classmethod
+    def generic(cls, L, cm, b, v0, atol=1e-6, rtol=1e-6):
+        """A population given by its cable matrix alone (dense L in the RKC
+        sign convention: off-diagonal +g), no ion model."""
+        L = np.asarray(L, dtype=np.float64)
+        nc = L.shape[0]
+        row, col, val = [0], [], []
+        for i in range(nc):
+            for j in range(nc):
+                if j != i and L[i, j] != 0.0:
+                    col.append(j)
+                    val.append(L[i, j])
+            row.append(len(col))
+        row = np.array(row, dtype=np.int32)
+        col = np.array(col if col else [0], dtype=np.int32)
+        val = np.array(val if val else [0.0])
+        cm = np.ascontiguousarray(cm, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        v0 = np.ascontiguousarray(v0, dtype=np.float64)
+        ip = ctypes.POINTER(ctypes.c_int)
+        h = lib().or_generic_create(nc, row.ctypes.data_as(ip), col.ctypes.data_as(ip), _dp(val), _dp(cm), _dp(b),
+                                    _dp(v0), atol, rtol)
+        if not h:
+            raise RuntimeError("oracle generic_create failed")
+        return cls(h, 1, nc)
+
+    def rkc_step(self, h, m):
+        lib().or_rkc_step_test(self.h, h, m)
+
+    def rkclow(self):
+        lib().or_rkclow_test(self.h)
+
+    def rkcrho(self):
+        lib().or_rkcrho_test(self.h)
+        return self.controller()[0][2]
+
+    def rkcint(self, t):
+        out = np.empty(self.nc)
+        lib().or_rkcint_test(self.h, t, _dp(out))
+        return out
+
+    def set_other(self, i, v):
+        lib().or_pop_set_other(self.h, i, v)
+
+    def set_hwork(self, i, v):
+        lib().or_pop_set_hwork(self.h, i, v)
+
+    def update_ion(self, vnew, dt):
+        vnew = np.ascontiguousarray(vnew, dtype=np.float64)
+        lib().or_pkj_update_ion(self.h, _dp(vnew), dt)
+
+    def update_b(self, vv):
+        vv = np.ascontiguousarray(vv, dtype=np.float64)
+        lib().or_pkj_update_b(self.h, _dp(vv))
 
     def __del__(self):
         if getattr(self, "h", None):
