@@ -1,1 +1,198 @@
-int main(){return 0;}
+// Drop-in for the reference's `main` (main.cu:35-355), on the C-ABI:
+//
+//   ./main S_gr gx gy S_go ox oy S_pkj px py S_io ix iy [options]
+//
+// S_* is "BE", "CN" or "RKC", matched by prefix as main.cu:60-102 does; the
+// cell count of a population is x * y and 0 disables it.  Output files are the
+// reference's: xx_v_by_<S>.csv (t, then the recorded compartment of every
+// cell, "%lf,"), xx_raster_by_<S>.csv ("%lf,%d" per upward threshold
+// crossing), an empty debug_file.csv and one appended comptime.csv row.
+//
+// Options (all optional; the defaults are the reference's behaviour):
+//   --ts MS          simulated time [ms] (default 1500, param.h:7)
+//   --engine E       global (default: FP64, population-global controller, the
+//                    reference's semantics), cell (FP64, one controller per
+//                    cell) or mixed (FP32 stages/gates, FP64 control)
+//   --stable         the safeguards for synthetic trees (ion interval 2^-8 ms,
+//                    fn refresh, spectral-radius floor)
+//   --chunk K        macro steps per device launch between host writes (64)
+//
+// This build provides the Purkinje population with the RKC solver; asking for
+// another population or solver prints an error and exits with status 1.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cbrkc.h"
+
+namespace {
+
+const char *kSolverNames[3] = {"BE", "CN", "RKC"};
+const char *kTypeNames[4] = {"gr", "go", "pkj", "io"};
+const int kSolverArg[4] = {1, 4, 7, 10};
+constexpr double H_MAX = 0.125;   // param.h:12
+
+int solver_of(const char *s) {
+  if (std::strncmp(s, "BE", 2) == 0) return CB_SOLVER_BE;
+  if (std::strncmp(s, "CN", 2) == 0) return CB_SOLVER_CN;
+  if (std::strncmp(s, "RKC", 3) == 0) return CB_SOLVER_RKC;
+  return -1;
+}
+
+int check(int rc, const char *what) {
+  if (rc != CB_OK) {
+    std::fprintf(stderr, "%s failed (%d): %s\n", what, rc, cb_last_error());
+    std::exit(1);
+  }
+  return rc;
+}
+
+}  // namespace
+
+int main(int argc, char *argv[]) {
+  std::printf("argc = %d\n", argc);
+  for (int i = 0; i < argc; i++) std::printf("argv[%d] = %s\n", i, argv[i]);
+  if (argc < 13) {
+    std::printf("usage: %s S_gr gx gy S_go ox oy S_pkj px py S_io ix iy [--ts MS] [--engine global|cell|mixed] "
+                "[--stable] [--chunk K]\n", argv[0]);
+    return 0;
+  }
+  double ts = 1500.0;
+  int engine = CB_ENGINE_GLOBAL_FP64, chunk = 64;
+  bool stable = false;
+  for (int i = 13; i < argc; i++) {
+    std::string o = argv[i];
+    if (o == "--ts" && i + 1 < argc) {
+      ts = std::atof(argv[++i]);
+    } else if (o == "--engine" && i + 1 < argc) {
+      std::string e = argv[++i];
+      engine = e == "cell" ? CB_ENGINE_CELL_FP64 : e == "mixed" ? CB_ENGINE_CELL_MIXED : CB_ENGINE_GLOBAL_FP64;
+    } else if (o == "--stable") {
+      stable = true;
+    } else if (o == "--chunk" && i + 1 < argc) {
+      chunk = std::atoi(argv[++i]);
+      if (chunk < 1) chunk = 1;
+    } else {
+      std::printf("unknown option %s\n", o.c_str());
+      return 1;
+    }
+  }
+
+  int solver[4], nx[4], ny[4];
+  std::printf("GR  solver -> %s\nGO  solver -> %s\nPKJ solver -> %s\nIO  solver -> %s\n\n", argv[1], argv[4],
+              argv[7], argv[10]);
+  std::FILE *fv[4] = {nullptr, nullptr, nullptr, nullptr}, *fr[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (int k = 0; k < 4; k++) {
+    solver[k] = solver_of(argv[kSolverArg[k]]);
+    if (solver[k] < 0) {
+      std::printf("Error!! -> set solver name in argv [ %d ]\n", kSolverArg[k]);
+      return 0;
+    }
+    const std::string s = kSolverNames[solver[k]];
+    fv[k] = std::fopen((std::string(kTypeNames[k]) + "_v_by_" + s + ".csv").c_str(), "w");
+    fr[k] = std::fopen((std::string(kTypeNames[k]) + "_raster_by_" + s + ".csv").c_str(), "w");
+    nx[k] = std::atoi(argv[kSolverArg[k] + 1]);
+    ny[k] = std::atoi(argv[kSolverArg[k] + 2]);
+  }
+  for (int k = 0; k < 4; k++) {
+    if (nx[k] * ny[k] > 0 && (k != CB_PKJ || solver[k] != CB_SOLVER_RKC)) {
+      std::printf("Error!! -> %s population with the %s solver is not provided by this build\n", kTypeNames[k],
+                  kSolverNames[solver[k]]);
+      return 1;
+    }
+  }
+  std::FILE *dbg = std::fopen("debug_file.csv", "w");
+  if (dbg) std::fclose(dbg);
+
+  // the reference seeds libc rand() with 0 before any population draws its
+  // initial voltages (main.cu:37)
+  std::srand(0);
+  const int n = nx[CB_PKJ] * ny[CB_PKJ];
+  cb_pop *pkj = nullptr;
+  int comp = 0;
+  if (n > 0) {
+    cb_pop_config cfg;
+    check(cb_config_default(&cfg, CB_PKJ, CB_SOLVER_RKC, nx[CB_PKJ], ny[CB_PKJ]), "config");
+    cfg.engine = engine;
+    if (stable) {
+      cfg.ion_interval_ms = std::ldexp(1.0, -8);
+      cfg.refresh_fn = 1;
+      cfg.rho_floor = 1;
+    }
+    check(cb_pop_create(&cfg, &pkj), "pkj_initialize");
+    int nn = 0;
+    check(cb_pop_size(pkj, &nn, &comp), "pkj size");
+    comp /= nn;
+  }
+  std::printf("\n");
+
+  // the macro-step loop of main.cu:182-311 for the populations this build
+  // provides; the recorded compartment of every cell comes back once per
+  // chunk of macro steps instead of all of V once per step
+  std::vector<double> rec, raster_prev(n, 100.0);
+  const auto start = std::chrono::steady_clock::now();
+  auto half = start;
+  const long total = (long)std::floor((ts - 0.0001) / H_MAX) + 1;   // while (t < TS - 0.0001)
+  long done = 0;
+  double t = 0.0;
+  while (done < total) {
+    const int k = (int)std::min<long>(chunk, total - done);
+    if (n > 0) {
+      rec.assign((size_t)k * n, 0.0);
+      check(cb_pop_run(pkj, k, t, 1599, rec.data()), "pkj_solve_update_v");
+      // pkj_output_file (pkj.cu:370-412): V row of compartment 1599 per cell,
+      // raster row per upward crossing of -10 mV (previous value starts at 100)
+      for (int s = 0; s < k; s++) {
+        const double ts_ = t + (s + 1) * H_MAX;
+        std::fprintf(fv[CB_PKJ], "%lf,", ts_);
+        for (int c = 0; c < n; c++) {
+          const double v = rec[(size_t)s * n + c];
+          if (std::isnan(v)) {
+            std::printf("isnan in pkj, time=%f\n", ts_ - H_MAX);
+            return 1;
+          }
+          std::fprintf(fv[CB_PKJ], "%lf,", v);
+        }
+        std::fprintf(fv[CB_PKJ], "\n");
+        for (int c = 0; c < n; c++) {
+          const double v = rec[(size_t)s * n + c];
+          if (v > -10.0 && raster_prev[c] <= -10.0) std::fprintf(fr[CB_PKJ], "%lf,%d\n", ts_, c);
+          raster_prev[c] = v;
+        }
+      }
+    }
+    for (int s = 0; s < k; s++) {
+      if ((done + s) % 400 == 0) std::printf("time:%f\n", (done + s) * H_MAX);
+      if (t + (s + 1) * H_MAX >= 500.0 && 500.0 + H_MAX > t + (s + 1) * H_MAX)
+        half = std::chrono::steady_clock::now();
+    }
+    done += k;
+    t = done * H_MAX;
+  }
+  const auto end = std::chrono::steady_clock::now();
+  const double secs = std::chrono::duration<double>(end - start).count();
+  std::printf("time:%f \ncomputational time = %.2f \n", t, secs);
+  for (int k = 0; k < 4; k++) {
+    if (fv[k]) std::fclose(fv[k]);
+    if (fr[k]) std::fclose(fr[k]);
+  }
+
+  std::FILE *tp = std::fopen("comptime.csv", "a");
+  if (tp) {
+    const int sim = (int)ts;
+    std::fprintf(tp, "Gr,%d,%s,Go,%d,%s,Pkj,%d,%s,Io,%d,%s,", nx[0] * ny[0], argv[1], nx[1] * ny[1], argv[4], n,
+                 argv[7], nx[3] * ny[3], argv[10]);
+    if (ts <= 500.0)
+      std::fprintf(tp, "CompTime,%lf,SimTime,%d\n", secs, sim);
+    else
+      std::fprintf(tp, "AllCompTime,%lf,0.5s-endCompTime,%lf,SimTime,%d\n", secs,
+                   std::chrono::duration<double>(end - half).count(), sim);
+    std::fclose(tp);
+  }
+  if (pkj) check(cb_pkj_finalize(pkj), "pkj_finalize");
+  return 0;
+}

@@ -1,0 +1,73 @@
+"""The drop-in `main` (main.cu:35-355): argv handling and output files.
+
+CPU tests cover the argument checks that return before any device work;
+the GPU test runs one Purkinje cell of the reference morphology for 2 ms and
+compares pkj_v_by_RKC.csv with the oracle (libc rand() seeded with 0, as
+main.cu:37 does, gives both the same initial voltage).
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MAIN = os.path.join(ROOT, "cerebellum-neuron-model-simulation-using-rkc_amd", "main")
+
+
+def run_main(args, cwd, timeout=120):
+    return subprocess.run([MAIN] + args, cwd=cwd, capture_output=True, text=True, timeout=timeout)
+
+
+@pytest.fixture(scope="module")
+def built():
+    if not os.path.exists(MAIN):
+        pytest.fail(f"{MAIN} is not built (run __graft_entry__.build())")
+    return MAIN
+
+
+def test_unknown_solver_name(built, tmp_path):
+    r = run_main(["XX", "0", "0", "RKC", "0", "0", "RKC", "1", "1", "RKC", "0", "0"], tmp_path)
+    assert r.returncode == 0
+    assert "Error!! -> set solver name in argv [ 1 ]" in r.stdout
+    r = run_main(["RKC", "0", "0", "RKC", "0", "0", "RKC", "1", "1", "CQ", "0", "0"], tmp_path)
+    assert "Error!! -> set solver name in argv [ 10 ]" in r.stdout
+
+
+def test_prefix_match_and_output_files(built, tmp_path):
+    # "RKCx" matches RKC by prefix; an unprovided population fails loudly
+    r = run_main(["RKCx", "2", "2", "BE", "0", "0", "CN", "0", "0", "RKC", "0", "0", "--ts", "1"], tmp_path)
+    assert r.returncode == 1
+    assert "gr population with the RKC solver is not provided" in r.stdout
+    for name in ("gr_v_by_RKC.csv", "go_raster_by_BE.csv", "pkj_v_by_CN.csv", "io_raster_by_RKC.csv"):
+        assert (tmp_path / name).exists(), name
+
+
+def test_usage_without_arguments(built, tmp_path):
+    r = run_main([], tmp_path)
+    assert r.returncode == 0 and "usage:" in r.stdout
+
+
+@pytest.mark.gpu
+def test_pkj_voltage_rows_match_oracle(built, tmp_path, ref_data):
+    shutil.copy(os.path.join(ref_data, "Pkj.txt"), tmp_path / "Pkj.txt")
+    print("main: 1 PKJ cell, RKC, 2 ms", flush=True)
+    r = run_main(["RKC", "0", "0", "RKC", "0", "0", "RKC", "1", "1", "RKC", "0", "0", "--ts", "2", "--chunk", "5"],
+                 tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = [l.rstrip("\n").rstrip(",").split(",") for l in open(tmp_path / "pkj_v_by_RKC.csv")]
+    got = np.array([[float(x) for x in row] for row in rows])
+    assert got.shape == (16, 2)
+    np.testing.assert_allclose(got[:, 0], 0.125 * np.arange(1, 17))
+    o = orc.Pop.pkj(os.path.join(ref_data, "Pkj.txt"), 1, 1, libc_rand=True, seed=0)
+    ref = []
+    for k in range(16):
+        o.set_current(k * 0.125)
+        o.advance()
+        ref.append(o.get(orc.VNEW)[1599])
+    np.testing.assert_allclose(got[:, 1], ref, rtol=0, atol=2e-6)   # "%lf" keeps 6 decimals
+    assert (tmp_path / "comptime.csv").read_text().startswith("Gr,0,RKC,Go,0,RKC,Pkj,1,RKC,Io,0,RKC,CompTime,")
+    assert (tmp_path / "pkj_raster_by_RKC.csv").read_text() == ""
