@@ -34,11 +34,15 @@ EXPORTS = [
     "cb_synth_pkj_morph", "cb_philox4x32_10", "cb_last_error", "cb_pkj_initialize",
     "cb_pkj_solve_update_v", "cb_pkj_set_current", "cb_pkj_output_file", "cb_pkj_finalize",
     "cb_pop_kernel_time", "cb_pop_autotune", "cb_pkj_cable_spectral_radius",
+    "cb_io_initialize", "cb_io_solve_update_v", "cb_io_set_current", "cb_io_output_file", "cb_io_finalize",
 ]
 
-# The synthetic Purkinje trees (10 um soma on a 3 um trunk) need three
-# safeguards the reference does not have to stay stable; the reference's own
-# morphology does not.  See the README of the tests for the evidence.
+# The synthetic Purkinje trees (10 um soma on a thin tapering trunk) need
+# three safeguards the reference does not have to stay stable: with the
+# reference's ion-update interval 2^-6 ms, 709 of 1024 such cells blow up
+# within 200 ms (115 at 2^-7, none at 2^-8); the stale fn after an ion update
+# and spectral radii underestimated by the power method (up to 2x) each
+# produced blow-ups as well.
 STABLE = dict(ion_interval=2.0 ** -8, refresh_fn=True, rho_floor=True)
 
 
@@ -111,6 +115,11 @@ def lib():
         L.cb_pkj_set_current.argtypes = [vp, ctypes.c_double]
         L.cb_pkj_output_file.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p]
         L.cb_pkj_finalize.argtypes = [vp]
+        L.cb_io_initialize.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.cb_io_solve_update_v.argtypes = [vp]
+        L.cb_io_set_current.argtypes = [vp, ctypes.c_double]
+        L.cb_io_output_file.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p]
+        L.cb_io_finalize.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -282,4 +291,36 @@ def pkj_output_file(p, memv, raster_prev, t, v_csv, raster_csv):
 
 
 def pkj_finalize(p):
+    p.close()
+
+
+# ---- reference-named entry points (inferior olive) ----
+
+def io_initialize(nx, ny, solver="RKC"):
+    """io_initialize + io_solve_initialize + io_gap_create; reads ./IO.txt."""
+    h = ctypes.c_void_p()
+    _check(lib().cb_io_initialize(nx, ny, solver.encode(), ctypes.byref(h)))
+    p = Population.__new__(Population)
+    p.h = h
+    n = ctypes.c_int()
+    nc = ctypes.c_int()
+    _check(lib().cb_pop_size(h, ctypes.byref(n), ctypes.byref(nc)))
+    p.n, p.nc, p.comp, p.engine, p._keep = n.value, nc.value, nc.value // max(n.value, 1), ENGINE_GLOBAL_FP64, []
+    return p
+
+
+def io_set_current(p, t):
+    _check(lib().cb_io_set_current(p.h, t))
+
+
+def io_solve_update_v(p):
+    _check(lib().cb_io_solve_update_v(p.h))
+
+
+def io_output_file(p, memv, raster_prev, t, v_csv, raster_csv):
+    memv = np.ascontiguousarray(memv, dtype=np.float64)
+    _check(lib().cb_io_output_file(p.h, _dp(memv), _dp(raster_prev), t, v_csv.encode(), raster_csv.encode()))
+
+
+def io_finalize(p):
     p.close()

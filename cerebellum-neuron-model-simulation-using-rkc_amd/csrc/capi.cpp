@@ -9,6 +9,7 @@
 
 #include "../../include/cbrkc.h"
 #include "engine.hpp"
+#include "io_model.hpp"
 #include "morph.hpp"
 #include "philox.hpp"
 #include "pkj_model.hpp"
@@ -47,6 +48,60 @@ int guarded(F &&f) {
     g_err = e.what();
     return CB_E_ARG;
   }
+}
+
+PoissonSpec poisson_of(const cb_pop_config &cfg) {
+  PoissonSpec syn;
+  syn.on = cfg.poisson_on != 0;
+  syn.rate_hz = cfg.poisson_rate_hz;
+  syn.g_event = cfg.poisson_g_mS;
+  syn.tau_ms = cfg.poisson_tau_ms;
+  syn.e_syn = cfg.poisson_e_mV;
+  syn.seed = cfg.poisson_seed;
+  return syn;
+}
+
+// io_initialize + io_solve_initialize + io_gap_create (io.cu:186-229,
+// io_solve.cu:192-262, gap.cu:136-147): global FP64 engine only
+int create_io(const cb_pop_config &cfg, cb_pop **out) {
+  const int n = cfg.nx * cfg.ny;
+  if (n == 0) return fail(CB_E_ARG, "empty population");
+  if (cfg.engine != CB_ENGINE_GLOBAL_FP64)
+    return fail(CB_E_UNSUP, "inferior-olive cells are gap-junction coupled: global engine only");
+  IoHost host;
+  std::string err;
+  const std::string path = cfg.morph_path ? cfg.morph_path : "./IO.txt";
+  if (!build_io_host(path, cfg.nx, cfg.ny, cfg.v0, host, err)) return fail(CB_E_IO, err);
+  StimSpec stim;   // io_set_current (io.cu:247-257): the dendrite, [800, 820) ms
+  stim.comp = 0;
+  stim.t0 = io::STIM_T0;
+  stim.t1 = io::STIM_T1;
+  if (cfg.stim_comp >= 0) {
+    if (cfg.stim_comp >= io::COMP) return fail(CB_E_ARG, "stimulus compartment out of range");
+    stim.comp = cfg.stim_comp;
+    stim.t0 = cfg.stim_t0;
+    stim.t1 = cfg.stim_t1;
+  }
+  stim.amp.assign(n, io::STIM);
+  if (cfg.stim_amp)
+    for (int c = 0; c < n; c++) stim.amp[c] = cfg.stim_amp[c];
+  SolverSpec sol;
+  const double tol = 1.0e-6;   // TOL_IO, solve_rkc.cuh:31
+  sol.atol = cfg.atol > 0 ? cfg.atol : tol;
+  sol.rtol = cfg.rtol > 0 ? cfg.rtol : tol;
+  sol.ion_interval = cfg.ion_interval_ms;
+  sol.refresh_fn = cfg.refresh_fn != 0;
+  if (cfg.rho_floor) return fail(CB_E_UNSUP, "the spectral-radius floor is provided for Purkinje cells");
+  if (!(sol.atol > 0.0) || !(sol.rtol >= 0.0)) return fail(CB_E_ARG, "atol must be > 0 and rtol >= 0");
+  std::unique_ptr<cb_pop> p(new cb_pop);
+  p->cell_type = CB_IO;
+  p->solver = cfg.solver;
+  p->engine = cfg.engine;
+  p->comp = io::COMP;
+  p->device = cfg.device;
+  p->eng.reset(make_global_io(host, sol, stim, poisson_of(cfg)));
+  *out = p.release();
+  return CB_OK;
 }
 
 int solver_from_name(const char *s) {
@@ -90,9 +145,11 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     if (!cfg || !out) return fail(CB_E_ARG, "null argument");
     *out = nullptr;
     if (cfg->nx < 0 || cfg->ny < 0) return fail(CB_E_ARG, "negative grid size");
-    if (cfg->cell_type != CB_PKJ) return fail(CB_E_UNSUP, "cell type not provided by this build");
+    if (cfg->cell_type != CB_PKJ && cfg->cell_type != CB_IO)
+      return fail(CB_E_UNSUP, "cell type not provided by this build");
     if (cfg->solver != CB_SOLVER_RKC) return fail(CB_E_UNSUP, "solver not provided by this build");
     CB_HIP(hipSetDevice(cfg->device));
+    if (cfg->cell_type == CB_IO) return create_io(*cfg, out);
     std::vector<MorphRow> rows;
     std::string err;
     const std::string mpath = cfg->morph_path ? cfg->morph_path : "./Pkj.txt";
@@ -300,6 +357,43 @@ int cb_pkj_initialize(int nx, int ny, const char *solver, cb_pop **out) {
   cb_pop_config cfg;
   cb_config_default(&cfg, CB_PKJ, s, nx, ny);
   return cb_pop_create(&cfg, out);
+}
+
+int cb_io_initialize(int nx, int ny, const char *solver, cb_pop **out) {
+  int s = solver_from_name(solver);
+  if (s < 0) return fail(CB_E_ARG, "error in io_initialize");
+  cb_pop_config cfg;
+  cb_config_default(&cfg, CB_IO, s, nx, ny);
+  return cb_pop_create(&cfg, out);
+}
+
+int cb_io_solve_update_v(cb_pop *p) { return cb_pop_advance(p); }
+int cb_io_set_current(cb_pop *p, double t) { return cb_pop_set_current(p, t); }
+int cb_io_finalize(cb_pop *p) { return cb_pop_destroy(p); }
+
+int cb_io_output_file(const cb_pop *p, const double *memv, double *raster_prev, double t, const char *v_csv,
+                      const char *raster_csv) {
+  if (!p || !memv || !raster_prev) return fail(CB_E_ARG, "null argument");
+  if (p->cell_type != CB_IO) return fail(CB_E_ARG, "not an inferior-olive population");
+  const int n = p->eng->ncell(), C = p->comp;
+  if (v_csv) {
+    FILE *f = std::fopen(v_csv, "a");
+    if (!f) return fail(CB_E_IO, std::string("cannot open ") + v_csv);
+    std::fprintf(f, "%lf,", t);
+    for (int j = 0; j < n; j++)
+      for (int k = 0; k < C; k++) std::fprintf(f, "%lf,", memv[k + (size_t)j * C]);
+    std::fprintf(f, "\n");
+    std::fclose(f);
+  }
+  FILE *fr = raster_csv ? std::fopen(raster_csv, "a") : nullptr;
+  if (raster_csv && !fr) return fail(CB_E_IO, std::string("cannot open ") + raster_csv);
+  for (int j = 0; j < n; j++) {
+    const double vj = memv[2 + (size_t)j * C];
+    if (vj > 10.0 && raster_prev[j] <= 10.0 && fr) std::fprintf(fr, "%lf,%d\n", t, j);
+    raster_prev[j] = vj;
+  }
+  if (fr) std::fclose(fr);
+  return CB_OK;
 }
 
 int cb_pkj_solve_update_v(cb_pop *p) { return cb_pop_advance(p); }

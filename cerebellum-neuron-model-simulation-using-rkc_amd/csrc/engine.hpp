@@ -102,6 +102,7 @@ struct SolverSpec {
 std::vector<double> rkc_stage_table(int mtab);
 
 Engine *make_global_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);
+Engine *make_global_io(const IoHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);
 Engine *make_cell_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn,
                       bool mixed);
 

@@ -24,6 +24,7 @@
 #include "detsum.hpp"
 #include "engine.hpp"
 #include "philox.hpp"
+#include "io_model.hpp"
 #include "pkj_model.hpp"
 
 namespace cbrkc {
@@ -235,6 +236,56 @@ __global__ void __launch_bounds__(256) k_pkj_ion_b(PkjState S, const double *__r
   S.b[i] = bb;
 }
 
+struct IoState {
+  double *v, *ca, *iext;
+  const double *cond;   // [NCOND][nc]
+  double *gate;         // [NGATE][nc]
+  const int8_t *compart;
+  const int *nbr;       // [n][4] gap-junction partner cell or -1
+  double *b;
+  const double *syn_g;  // may be null
+  double syn_e;
+  int nc, n;
+};
+
+// io_update_ion_RKC (io_ion.cu:90-127), io_update_matrix (solve_rkc.cu:412-434)
+// and, on each dendrite, the gap currents of io_gap_update / add_io_gap_val
+// (gap.cu:149-160, solve_rkc.cu:400-410), fused.  The gap current uses the
+// partners' voltages after the update (elem[v] = vnew there) and is added in
+// partner order 0..3 (the reference adds with atomics in no fixed order).
+__global__ void __launch_bounds__(256) k_io_ion_b(IoState S, const double *__restrict__ vnew, double DT, int do_ion) {
+  using namespace io;
+  const int nc = S.nc;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nc) return;
+  double g[NGATE], c[NCOND];
+#pragma unroll
+  for (int k = 0; k < NGATE; k++) g[k] = S.gate[(size_t)k * nc + i];
+#pragma unroll
+  for (int k = 0; k < NCOND; k++) c[k] = S.cond[(size_t)k * nc + i];
+  const double vn = vnew[i];
+  if (do_ion) {
+    double ca = S.ca[i];
+    update_ion<double>(S.v[i], vn, ca, c[g_CaH], S.compart[i], DT, g);
+#pragma unroll
+    for (int k = 0; k < NGATE; k++) S.gate[(size_t)k * nc + i] = g[k];
+    S.ca[i] = ca;
+    S.v[i] = vn;
+  }
+  double bb = current<double>(vn, S.iext[i], c, g);
+  if (do_ion && S.n > 1 && i % COMP == 0) {
+    const int cell = i / COMP;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int pre = S.nbr[cell * 4 + s];
+      const double diff_v = (pre > -1) ? (vn - vnew[pre * COMP]) : 0.0;
+      bb += gap_current<double>(diff_v);
+    }
+  }
+  if (S.syn_g) bb += S.syn_g[i] * (vn - S.syn_e);
+  S.b[i] = bb;
+}
+
 // xx_set_current shape: one compartment per cell, a time window (pkj.cu:333-344)
 __global__ void k_set_current(double *__restrict__ iext, const double *__restrict__ amp, int comp, int stim_comp,
                               double t0, double t1, double t, int n) {
@@ -281,51 +332,44 @@ int nint_ori(double x) {
 enum { Y = 0, YN, FN, VT1, VT2, EDE, VNEW, NVEC };
 enum { MMAX = 0, ABSH, SPRAD, NACCPT, ATOL, RTOL, IDID, GT, GT_OLD, LT, TEND, ERROLD, HOLD, NSTSIG, NOTHERS };
 
-class GlobalPkj : public Engine {
+// The RKC machinery shared by every cell type: vectors, the cable CSR, the
+// host controller (rkclow / step / rkcrho in the reference's scalar order)
+// and the macro-step driver.  A cell type supplies the frozen-current and
+// ion-update kernel (model_b) and its ion-update trigger.
+class GlobalRkc : public Engine {
  public:
-  GlobalPkj(const PkjHost &H, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn)
-      : n_(H.n), nc_(H.nc), comp_(H.comp), stim_(stim), syn_(syn) {
-    using namespace pkj;
+  GlobalRkc(int n, int nc, int comp, const std::vector<double> &cm, const std::vector<double> &v,
+            const std::vector<double> &iext, const std::vector<int> &row, const std::vector<int> &col,
+            const std::vector<double> &val, const StimSpec &stim, const PoissonSpec &syn)
+      : n_(n), nc_(nc), comp_(comp), stim_(stim), syn_(syn) {
     CB_HIP(hipStreamCreate(&st_));
-    v_ = dalloc<double>(nc_); h2d(v_, H.v);
-    ca_ = dalloc<double>(nc_); h2d(ca_, H.ca);
-    rev_ = dalloc<double>(nc_); h2d(rev_, H.rev);
-    iext_ = dalloc<double>(nc_); h2d(iext_, H.iext);
-    shell_ = dalloc<double>(nc_); h2d(shell_, H.shell);
-    cm_ = dalloc<double>(nc_); h2d(cm_, H.cm);
-    cond_ = dalloc<double>((size_t)NCOND * nc_); h2d(cond_, H.cond);
-    gate_ = dalloc<double>((size_t)NGATE * nc_); h2d(gate_, H.gate);
+    v_ = dalloc<double>(nc_); h2d(v_, v);
+    iext_ = dalloc<double>(nc_); h2d(iext_, iext);
+    cm_ = dalloc<double>(nc_); h2d(cm_, cm);
     b_ = dalloc<double>(nc_);
     dammy_ = dalloc<double>(nc_);
-    row_ = dalloc<int>(nc_ + 1); h2d(row_, H.row);
-    col_ = dalloc<int>(H.col.size()); h2d(col_, H.col);
-    val_ = dalloc<double>(H.val.size()); h2d(val_, H.val);
+    row_ = dalloc<int>(nc_ + 1); h2d(row_, row);
+    col_ = dalloc<int>(col.size()); h2d(col_, col);
+    val_ = dalloc<double>(val.size()); h2d(val_, val);
     for (int k = 0; k < NVEC; k++) vec_[k] = dalloc<double>(nc_);
     red_a_ = dalloc<double>(nblk(nc_) + 1);
     red_b_ = dalloc<double>(nblk(nc_) + 1);
     bad_ = dalloc<int>(1);
     CB_HIP(hipMemset(bad_, 0, sizeof(int)));
     CB_HIP(hipHostMalloc((void **)&hres_, sizeof(double) * 2, hipHostMallocDefault));
-    if (stim_.amp.size() != (size_t)n_) stim_.amp.assign(n_, pkj::CURRENT);
     amp_ = dalloc<double>(n_); h2d(amp_, stim_.amp);
     if (syn_.on) {
       syn_g_ = dalloc<double>(nc_);
       CB_HIP(hipMemset(syn_g_, 0, sizeof(double) * nc_));
     }
     hmax_ = H_MAX;
-    hmin_ = std::pow(2.0, -10.0);   // H_MIN_PKJ, solve_rkc.cuh:26
-    // one controller for the whole population: the floor is the largest cell's
-    for (double r : sol.rho_floor) rho_floor_ = std::max(rho_floor_, r);
-    init(sol.atol, sol.rtol);
-    if (sol.ion_interval > 0) integ_ = sol.ion_interval;
-    refresh_fn_ = sol.refresh_fn;
   }
 
-  ~GlobalPkj() override {
+  ~GlobalRkc() override {
     (void)hipStreamSynchronize(st_);
-    for (void *p : {(void *)v_, (void *)ca_, (void *)rev_, (void *)iext_, (void *)shell_, (void *)cm_, (void *)cond_,
-                    (void *)gate_, (void *)b_, (void *)dammy_, (void *)row_, (void *)col_, (void *)val_,
-                    (void *)red_a_, (void *)red_b_, (void *)bad_, (void *)amp_, (void *)syn_g_, (void *)zero_})
+    for (void *p : {(void *)v_, (void *)iext_, (void *)cm_, (void *)b_, (void *)dammy_, (void *)row_, (void *)col_,
+                    (void *)val_, (void *)red_a_, (void *)red_b_, (void *)bad_, (void *)amp_, (void *)syn_g_,
+                    (void *)zero_})
       if (p) (void)hipFree(p);
     for (int k = 0; k < NVEC; k++) (void)hipFree(vec_[k]);
     (void)hipHostFree(hres_);
@@ -361,7 +405,7 @@ class GlobalPkj : public Engine {
       rkclow();
       sum_dt_ += o_[GT] - o_[GT_OLD];
       o_[GT_OLD] = o_[GT];
-      if (sum_dt_ >= integ_ && o_[IDID] == 2.0) {   // '>=' for PKJ, solve_rkc.cu:1284
+      if (trigger() && o_[IDID] == 2.0) {
         ion_and_b(sum_dt_);
         sum_dt_ = 0.0;
       }
@@ -387,28 +431,43 @@ class GlobalPkj : public Engine {
     switch (field) {
       case F_VNEW: src = vec_[VNEW]; break;
       case F_V: src = v_; break;
-      case F_CA: src = ca_; break;
-      case F_REV: src = rev_; break;
       case F_B: src = b_; break;
       case F_IEXT: src = iext_; break;
       case F_Y: src = vec_[Y]; break;
       case F_YN: src = vec_[YN]; break;
       case F_FN: src = vec_[FN]; break;
-      default:
-        if (field >= F_GATE0 && field < F_GATE0 + pkj::NGATE) src = gate_ + (size_t)(field - F_GATE0) * nc_;
+      default: src = model_field(field);
     }
     if (!src) throw Error("unknown field " + std::to_string(field));
     CB_HIP(hipMemcpyAsync(dst, src, sizeof(double) * nc_, hipMemcpyDeviceToHost, st_));
     CB_HIP(hipStreamSynchronize(st_));
   }
 
- private:
+ protected:
+  // b from vv; with do_ion, first the ion update over dt with vv as the new
+  // voltage (the reference's xx_update_ion_and_matrix)
+  virtual void model_b(const double *vv, bool do_ion, double dt) = 0;
+  // accumulated time reached the ion-update interval: '>=' for GO/PKJ,
+  // '>' for GR/IO (solve_rkc.cu:1148, 1212, 1284, 1347)
+  virtual bool trigger() const { return sum_dt_ >= integ_; }
+  virtual const double *model_field(int) const { return nullptr; }
+
+  // RKC_vec_initialize and the configuration of the controller; called by the
+  // cell type's constructor once its state is on the device
+  void setup(const SolverSpec &sol, double hmin) {
+    hmin_ = hmin;
+    // one controller for the whole population: the floor is the largest cell's
+    for (double r : sol.rho_floor) rho_floor_ = std::max(rho_floor_, r);
+    init(sol.atol, sol.rtol);
+    if (sol.ion_interval > 0) integ_ = sol.ion_interval;
+    refresh_fn_ = sol.refresh_fn;
+  }
+
   Csr csr() const { return Csr{row_, col_, val_}; }
 
   void ion_and_b(double dt) {
     st.ion_updates++;
-    PkjState S{v_, ca_, rev_, iext_, shell_, cond_, gate_, b_, syn_g_, syn_.e_syn, nc_};
-    k_pkj_ion_b<<<nblk(nc_), 256, 0, st_>>>(S, vec_[Y], dt, 1);
+    model_b(vec_[Y], true, dt);
     if (refresh_fn_) k_stage_rhs(vec_[YN], vec_[FN]);
   }
 
@@ -432,9 +491,8 @@ class GlobalPkj : public Engine {
   }
 
   void init(double atol, double rtol) {
-    // RKC_vec_initialize (solve_rkc.cu:756-791)
-    PkjState S{v_, ca_, rev_, iext_, shell_, cond_, gate_, b_, syn_g_, syn_.e_syn, nc_};
-    k_pkj_ion_b<<<nblk(nc_), 256, 0, st_>>>(S, v_, 0.0, 0);
+    // RKC_vec_initialize (solve_rkc.cu:756-791): b from the initial voltage
+    model_b(v_, false, 0.0);
     // set_initial_vec (solve_rkc.cu:437-487)
     CB_HIP(hipMemcpyAsync(vec_[VNEW], v_, sizeof(double) * nc_, hipMemcpyDeviceToDevice, st_));
     CB_HIP(hipMemcpyAsync(vec_[Y], v_, sizeof(double) * nc_, hipMemcpyDeviceToDevice, st_));
@@ -680,7 +738,7 @@ class GlobalPkj : public Engine {
   StimSpec stim_;
   PoissonSpec syn_;
   hipStream_t st_;
-  double *v_, *ca_, *rev_, *iext_, *shell_, *cm_, *cond_, *gate_, *b_, *dammy_, *val_;
+  double *v_, *iext_, *cm_, *b_, *dammy_, *val_;
   int *row_, *col_;
   double *vec_[NVEC];
   double *red_a_, *red_b_, *hres_, *amp_;
@@ -692,6 +750,83 @@ class GlobalPkj : public Engine {
   double rho_floor_ = 0.0;
   double sum_dt_ = 0.0, integ_ = 0.0, hmin_, hmax_;
   Stats st;
+};
+
+// Purkinje cells (pkj.cu, pkj_solve.cu, pkj_ion.cu)
+class GlobalPkj : public GlobalRkc {
+ public:
+  GlobalPkj(const PkjHost &H, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn)
+      : GlobalRkc(H.n, H.nc, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, with_amp(stim, H.n), syn) {
+    using namespace pkj;
+    ca_ = dalloc<double>(nc_); h2d(ca_, H.ca);
+    rev_ = dalloc<double>(nc_); h2d(rev_, H.rev);
+    shell_ = dalloc<double>(nc_); h2d(shell_, H.shell);
+    cond_ = dalloc<double>((size_t)NCOND * nc_); h2d(cond_, H.cond);
+    gate_ = dalloc<double>((size_t)NGATE * nc_); h2d(gate_, H.gate);
+    setup(sol, std::pow(2.0, -10.0));   // H_MIN_PKJ, solve_rkc.cuh:26
+  }
+  ~GlobalPkj() override {
+    (void)hipStreamSynchronize(st_);
+    for (void *p : {(void *)ca_, (void *)rev_, (void *)shell_, (void *)cond_, (void *)gate_}) (void)hipFree(p);
+  }
+
+ private:
+  static StimSpec with_amp(StimSpec s, int n) {
+    if (s.amp.size() != (size_t)n) s.amp.assign(n, pkj::CURRENT);
+    return s;
+  }
+  void model_b(const double *vv, bool do_ion, double dt) override {
+    PkjState S{v_, ca_, rev_, iext_, shell_, cond_, gate_, b_, syn_g_, syn_.e_syn, nc_};
+    k_pkj_ion_b<<<nblk(nc_), 256, 0, st_>>>(S, vv, dt, do_ion ? 1 : 0);
+  }
+  const double *model_field(int field) const override {
+    switch (field) {
+      case F_CA: return ca_;
+      case F_REV: return rev_;
+    }
+    if (field >= F_GATE0 && field < F_GATE0 + pkj::NGATE) return gate_ + (size_t)(field - F_GATE0) * nc_;
+    return nullptr;
+  }
+  double *ca_, *rev_, *shell_, *cond_, *gate_;
+};
+
+// Inferior-olive cells with their grid gap junctions (io.cu, io_solve.cu,
+// io_ion.cu, gap.cu)
+class GlobalIo : public GlobalRkc {
+ public:
+  GlobalIo(const IoHost &H, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn)
+      : GlobalRkc(H.n, H.nc, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, with_amp(stim, H.n), syn) {
+    using namespace io;
+    ca_ = dalloc<double>(nc_); h2d(ca_, H.ca);
+    cond_ = dalloc<double>((size_t)NCOND * nc_); h2d(cond_, H.cond);
+    gate_ = dalloc<double>((size_t)NGATE * nc_); h2d(gate_, H.gate);
+    compart_ = dalloc<int8_t>(nc_); h2d(compart_, H.compart);
+    nbr_ = dalloc<int>(H.nbr.size()); h2d(nbr_, H.nbr);
+    setup(sol, std::pow(2.0, -10.0));   // H_MIN_IO, solve_rkc.cuh:27
+  }
+  ~GlobalIo() override {
+    (void)hipStreamSynchronize(st_);
+    for (void *p : {(void *)ca_, (void *)cond_, (void *)gate_, (void *)compart_, (void *)nbr_}) (void)hipFree(p);
+  }
+
+ private:
+  static StimSpec with_amp(StimSpec s, int n) {
+    if (s.amp.size() != (size_t)n) s.amp.assign(n, io::STIM);
+    return s;
+  }
+  bool trigger() const override { return sum_dt_ > integ_; }   // strict for IO, solve_rkc.cu:1347
+  void model_b(const double *vv, bool do_ion, double dt) override {
+    IoState S{v_, ca_, iext_, cond_, gate_, compart_, nbr_, b_, syn_g_, syn_.e_syn, nc_, n_};
+    k_io_ion_b<<<nblk(nc_), 256, 0, st_>>>(S, vv, dt, do_ion ? 1 : 0);
+  }
+  const double *model_field(int field) const override {
+    if (field == F_CA) return ca_;
+    if (field >= F_GATE0 && field < F_GATE0 + io::NGATE) return gate_ + (size_t)(field - F_GATE0) * nc_;
+    return nullptr;
+  }
+  double *ca_, *cond_, *gate_;
+  int8_t *compart_;
+  int *nbr_;
 };
 
 }  // namespace
@@ -710,6 +845,10 @@ void Engine::run(int nsteps, double t0, int rec_comp, double *rec_out) {
 
 Engine *make_global_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn) {
   return new GlobalPkj(host, sol, stim, syn);
+}
+
+Engine *make_global_io(const IoHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn) {
+  return new GlobalIo(host, sol, stim, syn);
 }
 
 }  // namespace cbrkc

@@ -17,8 +17,9 @@
 //                    fn refresh, spectral-radius floor)
 //   --chunk K        macro steps per device launch between host writes (64)
 //
-// This build provides the Purkinje population with the RKC solver; asking for
-// another population or solver prints an error and exits with status 1.
+// This build provides the Purkinje and inferior-olive populations with the
+// RKC solver; asking for another population or solver prints an error and
+// exits with status 1.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -99,7 +100,7 @@ int main(int argc, char *argv[]) {
     ny[k] = std::atoi(argv[kSolverArg[k] + 2]);
   }
   for (int k = 0; k < 4; k++) {
-    if (nx[k] * ny[k] > 0 && (k != CB_PKJ || solver[k] != CB_SOLVER_RKC)) {
+    if (nx[k] * ny[k] > 0 && ((k != CB_PKJ && k != CB_IO) || solver[k] != CB_SOLVER_RKC)) {
       std::printf("Error!! -> %s population with the %s solver is not provided by this build\n", kTypeNames[k],
                   kSolverNames[solver[k]]);
       return 1;
@@ -128,6 +129,12 @@ int main(int argc, char *argv[]) {
     check(cb_pop_size(pkj, &nn, &comp), "pkj size");
     comp /= nn;
   }
+  // the inferior olive draws its initial voltages after the Purkinje cells,
+  // as io_initialize follows pkj_initialize in main.cu:125-137
+  const int nio = nx[CB_IO] * ny[CB_IO];
+  cb_pop *iop = nullptr;
+  if (nio > 0) check(cb_io_initialize(nx[CB_IO], ny[CB_IO], "RKC", &iop), "io_initialize");
+  std::vector<double> io_v((size_t)nio * 3), io_prev(nio, 100.0);
   std::printf("\n");
 
   // the macro-step loop of main.cu:182-311 for the populations this build
@@ -165,6 +172,26 @@ int main(int argc, char *argv[]) {
         }
       }
     }
+    // io_output_file (io.cu:259-285): all three compartments of every cell,
+    // raster on compartment 2 through +10 mV
+    for (int s = 0; s < k && nio > 0; s++) {
+      const double ts_ = t + (s + 1) * H_MAX;
+      check(cb_io_set_current(iop, ts_ - H_MAX), "io_set_current");
+      check(cb_io_solve_update_v(iop), "io_solve_update_v");
+      check(cb_pop_read(iop, CB_F_VNEW, io_v.data()), "io read");
+      if (std::isnan(io_v[1])) {
+        std::printf("isnan,time=%f\n", ts_ - H_MAX);
+        return 1;
+      }
+      std::fprintf(fv[CB_IO], "%lf,", ts_);
+      for (double v : io_v) std::fprintf(fv[CB_IO], "%lf,", v);
+      std::fprintf(fv[CB_IO], "\n");
+      for (int c = 0; c < nio; c++) {
+        const double v = io_v[2 + (size_t)c * 3];
+        if (v > 10.0 && io_prev[c] <= 10.0) std::fprintf(fr[CB_IO], "%lf,%d\n", ts_, c);
+        io_prev[c] = v;
+      }
+    }
     for (int s = 0; s < k; s++) {
       if ((done + s) % 400 == 0) std::printf("time:%f\n", (done + s) * H_MAX);
       if (t + (s + 1) * H_MAX >= 500.0 && 500.0 + H_MAX > t + (s + 1) * H_MAX)
@@ -185,7 +212,7 @@ int main(int argc, char *argv[]) {
   if (tp) {
     const int sim = (int)ts;
     std::fprintf(tp, "Gr,%d,%s,Go,%d,%s,Pkj,%d,%s,Io,%d,%s,", nx[0] * ny[0], argv[1], nx[1] * ny[1], argv[4], n,
-                 argv[7], nx[3] * ny[3], argv[10]);
+                 argv[7], nio, argv[10]);
     if (ts <= 500.0)
       std::fprintf(tp, "CompTime,%lf,SimTime,%d\n", secs, sim);
     else
@@ -194,5 +221,6 @@ int main(int argc, char *argv[]) {
     std::fclose(tp);
   }
   if (pkj) check(cb_pkj_finalize(pkj), "pkj_finalize");
+  if (iop) check(cb_io_finalize(iop), "io_finalize");
   return 0;
 }

@@ -55,6 +55,26 @@ struct PkjHost {
 bool build_pkj_host(const std::vector<MorphRow> &rows, int nx, int ny, const std::vector<int> *gj_pre,
                     const std::vector<int> *gj_post, const double *v0, PkjHost &out, std::string &err);
 
+// Inferior-olive population on the host: per-area densities and Cm from
+// IO.txt (io.cu:3-101), the asymmetric three-compartment coupling in CSR with
+// the RKC sign (io_solve.cu:2-150), initial state (io_ion.cu:163-194) and the
+// gap-junction partners of every cell (gap.cu:4-133).
+struct IoHost {
+  int n = 0, nx = 0, ny = 0, comp = 3, nc = 0;
+  std::vector<double> cm, iext, v, ca;
+  std::vector<int8_t> compart;
+  std::vector<double> cond;   // [NCOND][nc]
+  std::vector<double> gate;   // [NGATE][nc]
+  std::vector<int> row, col;
+  std::vector<double> val;
+  std::vector<int> nbr;       // [n][4] partner cell or -1
+};
+
+// Reads IO.txt (idx parent diam len Cm type, 3 rows) and builds nx * ny cells;
+// v0 per cell, or nullptr for -60 + 10 (U - 0.5) from libc rand() per cell in
+// order (io_ion.cu:168-170).
+bool build_io_host(const std::string &param_path, int nx, int ny, const double *v0, IoHost &out, std::string &err);
+
 // Spectral radius of each cell's cable operator C^-1 L (the RKC Jacobian
 // with the ionic current frozen), from the CSR rows of the cell without
 // intercell entries: Lanczos with full reorthogonalisation on the symmetric

@@ -173,6 +173,22 @@ int cb_pkj_output_file(const cb_pop *p, const double *memv, double *raster_prev,
 /* pkj_finalize + pkj_solve_finalize. */
 int cb_pkj_finalize(cb_pop *p);
 
+/* io_initialize + io_solve_initialize + io_gap_create (io.cu:186-229,
+ * io_solve.cu:192-262, gap.cu:136-147): reads ./IO.txt; initial voltages from
+ * libc rand() as io_ion.cu:168-170 draws them. */
+int cb_io_initialize(int nx, int ny, const char *solver, cb_pop **out);
+/* io_solve_update_v (io_solve.cu:296-312) with the grid gap junctions. */
+int cb_io_solve_update_v(cb_pop *p);
+/* io_set_current (io.cu:247-257). */
+int cb_io_set_current(cb_pop *p, double t);
+/* io_output_file (io.cu:259-285): appends "t,V0,V1,V2 of cell 0,...,\n" to
+ * v_csv and "t,cell\n" for every upward crossing of +10 mV by compartment 2
+ * (the axon hillock) to raster_csv; memv holds nc values. */
+int cb_io_output_file(const cb_pop *p, const double *memv, double *raster_prev, double t, const char *v_csv,
+                      const char *raster_csv);
+/* io_finalize + io_solve_finalize + io_gap_finalize. */
+int cb_io_finalize(cb_pop *p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,15 +92,13 @@ static int nint_ori(double x)                                           /* solve
 }
 
 /* ---------------- model dispatch ---------------- */
-extern void or_gr_update_b(or_pop *p, const double *vv);
-extern void or_go_update_b(or_pop *p, const double *vv);
-extern void or_io_update_b(or_pop *p, const double *vv);
 extern void or_model_update_ion_and_b(or_pop *p, double sum_dt);
 
-static void update_b(or_pop *p, const double *vv)
+static void update_b_gap(or_pop *p, const double *vv, int gap)
 {
   switch (p->type) {
     case OR_PKJ: or_pkj_update_b(p, vv); break;
+    case OR_IO: or_io_update_b(p, vv, gap); break;
     default: break;
   }
   if (p->syn_g) {
@@ -108,6 +106,10 @@ static void update_b(or_pop *p, const double *vv)
     for (int i = 0; i < p->nc; i++) p->b[i] += p->syn_g[i] * (vv[i] - p->syn_e);
   }
 }
+
+/* the frozen current at initialisation: no gap currents yet (RKC_vec_initialize,
+   solve_rkc.cu:765-778) */
+static void update_b(or_pop *p, const double *vv) { update_b_gap(p, vv, 0); }
 
 /* ---------------- dydt, solve_rkc.cu:177-196 ---------------- */
 void or_dydt(or_pop *p, const double *y, double *dy)
@@ -504,9 +506,10 @@ void or_model_update_ion_and_b(or_pop *p, double sum_dt)
   p->n_ion_updates++;
   switch (p->type) {
     case OR_PKJ: or_pkj_update_ion(p, p->vec[OR_Y], sum_dt); break;
+    case OR_IO: or_io_update_ion(p, p->vec[OR_Y], sum_dt); break;
     default: break;
   }
-  update_b(p, p->vec[OR_Y]);
+  update_b_gap(p, p->vec[OR_Y], 1);   /* io_update_ion_and_matrix adds the gap currents */
   /* optional (off in the reference): F_0 = F(y_n) with the new b, so the
      next step does not start from a derivative of the old current */
   if (p->refresh_fn) or_dydt(p, p->vec[OR_YN], p->vec[OR_FN]);

@@ -94,6 +94,14 @@ void or_pkj_update_b(or_pop *p, const double *vv);
 void or_pkj_update_ion(or_pop *p, const double *vnew, double dt);
 double or_pkj_rate(int which, double v, double ca);
 
+/* ---- inferior olive (or_io.c) ---- */
+or_pop *or_io_create(const char *param_file, int nx, int ny, const double *v0_per_cell, double atol, double rtol,
+                     int use_libc_rand);
+void or_io_update_b(or_pop *p, const double *vv, int add_gap);
+void or_io_update_ion(or_pop *p, const double *vnew, double dt);
+double or_io_rate(int which, double v, int comp);
+void or_io_neighbours(int id, int nx, int ny, int *out4);
+
 /* ---- stimulus ---- */
 void or_set_stim(or_pop *p, int comp, double t0, double t1, const double *amp_per_cell);
 void or_set_current(or_pop *p, double t);

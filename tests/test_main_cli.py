@@ -71,3 +71,20 @@ def test_pkj_voltage_rows_match_oracle(built, tmp_path, ref_data):
     np.testing.assert_allclose(got[:, 1], ref, rtol=0, atol=2e-6)   # "%lf" keeps 6 decimals
     assert (tmp_path / "comptime.csv").read_text().startswith("Gr,0,RKC,Go,0,RKC,Pkj,1,RKC,Io,0,RKC,CompTime,")
     assert (tmp_path / "pkj_raster_by_RKC.csv").read_text() == ""
+
+
+@pytest.mark.gpu
+def test_io_raster_file_is_the_references(built, tmp_path, ref_data):
+    """./main RKC 0 0 RKC 0 0 RKC 0 0 RKC 8 8 writes io_raster_by_RKC.csv
+    byte for byte as the reference's shipped file (run to 810 ms here; the
+    reference's TS of 1500 ms adds no spikes after the 800-820 ms pulse to
+    the first 810 ms of the file)."""
+    shutil.copy(os.path.join(ref_data, "IO.txt"), tmp_path / "IO.txt")
+    print("main: IO 8x8, RKC, 810 ms", flush=True)
+    r = run_main(["RKC", "0", "0", "RKC", "0", "0", "RKC", "0", "0", "RKC", "8", "8", "--ts", "810"], tmp_path,
+                 timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    got = (tmp_path / "io_raster_by_RKC.csv").read_bytes()
+    assert got == open(os.path.join(ref_data, "io_raster_by_RKC.csv"), "rb").read()
+    rows = (tmp_path / "io_v_by_RKC.csv").read_text().splitlines()
+    assert len(rows) == 6480 and rows[0].count(",") == 1 + 64 * 3
