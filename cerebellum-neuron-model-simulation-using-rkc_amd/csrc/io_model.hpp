@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "exp_acc.hpp"
+
 namespace cbrkc {
 namespace io {
 
@@ -25,45 +27,53 @@ constexpr double CA_PH = 3.0, BETA_CA = 0.075;
 constexpr double STIM = 5.5;              // uA/cm^2 on the dendrite, io.cu:253
 constexpr double STIM_T0 = 800.0, STIM_T1 = 820.0;
 
+// The model's exponential: the host C library's on the host (initial state),
+// the correctly rounded exp_acc on the device.  The gap-coupled grid
+// amplifies last-bit differences of exp into spike-time shifts over hundreds
+// of milliseconds, so the device follows a correctly rounded exp rather than
+// the device library's (faithful, not correctly rounded) one.
+__host__ inline double mexp(double x) { return ::exp(x); }
+__device__ inline double mexp(double x) { return exp_acc(x); }
+
 enum Gate { k_CaL, l_CaL, m_Na, h_Na, n_Kdr, p_Kdr, x_K, r_CaH, s_KCa, q_H, NGATE };
 enum Cond { g_leak, g_CaL, g_Na, g_Kdr, g_K, g_CaH, g_KCa, g_H, NCOND };
 
-template <typename T> __host__ __device__ inline T alpha_r(T v) { return T(1.7) / (T(1) + exp((-(v - T(5))) / T(13.9))); }
+template <typename T> __host__ __device__ inline T alpha_r(T v) { return T(1.7) / (T(1) + mexp((-(v - T(5))) / T(13.9))); }
 template <typename T> __host__ __device__ inline T alpha_s(T x) { return fmin(T(0.00002) * x, T(0.01)); }
-template <typename T> __host__ __device__ inline T alpha_x(T v) { return (T(0.13) * (v + T(25))) / (T(1) - exp(-(v + T(25)) / T(10))); }
-template <typename T> __host__ __device__ inline T beta_r(T v) { return (T(0.02) * (v + T(8.5))) / (exp((v + T(8.5)) / T(5)) - T(1)); }
+template <typename T> __host__ __device__ inline T alpha_x(T v) { return (T(0.13) * (v + T(25))) / (T(1) - mexp(-(v + T(25)) / T(10))); }
+template <typename T> __host__ __device__ inline T beta_r(T v) { return (T(0.02) * (v + T(8.5))) / (mexp((v + T(8.5)) / T(5)) - T(1)); }
 template <typename T> __host__ __device__ inline T beta_s() { return T(0.015); }
-template <typename T> __host__ __device__ inline T beta_x(T v) { return T(1.69) * exp(T(-0.0125) * (v + T(35))); }
+template <typename T> __host__ __device__ inline T beta_x(T v) { return T(1.69) * mexp(T(-0.0125) * (v + T(35))); }
 
 // h_inf and tau_h are defined for the soma (1) and the axon (2) only; the
 // dendrite keeps 1 for both (io_ion.cu:17-21, 37-41)
 template <typename T> __host__ __device__ inline T h_inf(T v, int c) {
   T h = T(1);
-  if (c == 1) h = T(1) / (T(1) + exp((-v - T(70)) / T(-5.8)));
-  if (c == 2) h = T(1) / (T(1) + exp((-v - T(60)) / T(-5.8)));
+  if (c == 1) h = T(1) / (T(1) + mexp((-v - T(70)) / T(-5.8)));
+  if (c == 2) h = T(1) / (T(1) + mexp((-v - T(60)) / T(-5.8)));
   return h;
 }
 template <typename T> __host__ __device__ inline T tau_h(T v, int c) {
   T h = T(1);
-  if (c == 1) h = T(3) * exp((-v - T(40)) / T(33));
-  if (c == 2) h = T(1.5) * exp((-v - T(40)) / T(33));
+  if (c == 1) h = T(3) * mexp((-v - T(40)) / T(33));
+  if (c == 2) h = T(1.5) * mexp((-v - T(40)) / T(33));
   return h;
 }
-template <typename T> __host__ __device__ inline T k_inf(T v) { return T(1) / (T(1) + exp((-v - T(61)) / T(4.2))); }
-template <typename T> __host__ __device__ inline T l_inf(T v) { return T(1) / (T(1) + exp((v + T(85.5)) / T(8.5))); }
-template <typename T> __host__ __device__ inline T m_inf(T v) { return T(1) / (T(1) + exp((-v - T(30)) / T(5.5))); }
-template <typename T> __host__ __device__ inline T n_inf(T v) { return T(1) / (T(1) + exp((-v - T(3)) / T(10))); }
-template <typename T> __host__ __device__ inline T p_inf(T v) { return T(1) / (T(1) + exp((-v - T(51)) / T(-12))); }
-template <typename T> __host__ __device__ inline T q_inf(T v) { return T(1) / (T(1) + exp((v + T(80)) / T(4))); }
+template <typename T> __host__ __device__ inline T k_inf(T v) { return T(1) / (T(1) + mexp((-v - T(61)) / T(4.2))); }
+template <typename T> __host__ __device__ inline T l_inf(T v) { return T(1) / (T(1) + mexp((v + T(85.5)) / T(8.5))); }
+template <typename T> __host__ __device__ inline T m_inf(T v) { return T(1) / (T(1) + mexp((-v - T(30)) / T(5.5))); }
+template <typename T> __host__ __device__ inline T n_inf(T v) { return T(1) / (T(1) + mexp((-v - T(3)) / T(10))); }
+template <typename T> __host__ __device__ inline T p_inf(T v) { return T(1) / (T(1) + mexp((-v - T(51)) / T(-12))); }
+template <typename T> __host__ __device__ inline T q_inf(T v) { return T(1) / (T(1) + mexp((v + T(80)) / T(4))); }
 template <typename T> __host__ __device__ inline T r_inf(T v) { return alpha_r(v) / (alpha_r(v) + beta_r(v)); }
 template <typename T> __host__ __device__ inline T s_inf(T x) { return alpha_s(x) / (alpha_s(x) + beta_s<T>()); }
 template <typename T> __host__ __device__ inline T x_inf(T v) { return alpha_x(v) / (alpha_x(v) + beta_x(v)); }
 template <typename T> __host__ __device__ inline T tau_k(T) { return T(1); }
 template <typename T> __host__ __device__ inline T tau_l(T v) {
-  return ((T(20) * exp((v + T(160)) / T(30))) / (T(1) + exp((v + T(84)) / T(7.3)))) + T(35);
+  return ((T(20) * mexp((v + T(160)) / T(30))) / (T(1) + mexp((v + T(84)) / T(7.3)))) + T(35);
 }
-template <typename T> __host__ __device__ inline T tau_n(T v) { return T(47) * exp((-(-v - T(50))) / T(900)) + T(5); }
-template <typename T> __host__ __device__ inline T tau_q(T v) { return T(1) / (exp(T(-0.086) * v - T(14.6)) + exp(T(0.07) * v - T(1.87))); }
+template <typename T> __host__ __device__ inline T tau_n(T v) { return T(47) * mexp((-(-v - T(50))) / T(900)) + T(5); }
+template <typename T> __host__ __device__ inline T tau_q(T v) { return T(1) / (mexp(T(-0.086) * v - T(14.6)) + mexp(T(0.07) * v - T(1.87))); }
 template <typename T> __host__ __device__ inline T tau_r(T v) { return T(5) / (alpha_r(v) + beta_r(v)); }
 template <typename T> __host__ __device__ inline T tau_s(T x) { return T(1) / (alpha_s(x) + beta_s<T>()); }
 template <typename T> __host__ __device__ inline T tau_x(T v) { return T(1) / (alpha_x(v) + beta_x(v)); }
@@ -114,7 +124,7 @@ __host__ __device__ inline T current(T v, T iext, const T *c, const T *g) {
 
 // gap current into a dendrite, diff = V_post - V_pre (gap.cu:149-160)
 template <typename T> __host__ __device__ inline T gap_current(T diff) {
-  return T(G_C) * (T(0.8) * exp(T(-0.01) * diff * diff) + T(0.2)) * (diff);
+  return T(G_C) * (T(0.8) * mexp(T(-0.01) * diff * diff) + T(0.2)) * (diff);
 }
 
 // the four gap-junction partners of cell id on an nx x ny grid, or -1
