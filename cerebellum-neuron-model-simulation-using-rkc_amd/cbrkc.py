@@ -23,7 +23,7 @@ CB_OK = 0
 GR, GO, PKJ, IO = 0, 1, 2, 3
 BE, CN, RKC = 0, 1, 2
 ENGINE_GLOBAL_FP64, ENGINE_CELL_FP64, ENGINE_CELL_MIXED = 0, 1, 2
-F_VNEW, F_V, F_CA, F_REV, F_B, F_IEXT, F_Y, F_YN, F_FN = range(9)
+F_VNEW, F_V, F_CA, F_REV, F_B, F_IEXT, F_Y, F_YN, F_FN, F_CA2, F_CA_OLD = range(11)
 F_GATE0 = 100
 H_MAX = 0.125
 
@@ -35,6 +35,7 @@ EXPORTS = [
     "cb_pkj_solve_update_v", "cb_pkj_set_current", "cb_pkj_output_file", "cb_pkj_finalize",
     "cb_pop_kernel_time", "cb_pop_autotune", "cb_pkj_cable_spectral_radius",
     "cb_io_initialize", "cb_io_solve_update_v", "cb_io_set_current", "cb_io_output_file", "cb_io_finalize",
+    "cb_go_initialize", "cb_go_solve_update_v", "cb_go_set_current", "cb_go_output_file", "cb_go_finalize",
 ]
 
 # The synthetic Purkinje trees (10 um soma on a thin tapering trunk) need
@@ -120,6 +121,11 @@ def lib():
         L.cb_io_set_current.argtypes = [vp, ctypes.c_double]
         L.cb_io_output_file.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p]
         L.cb_io_finalize.argtypes = [vp]
+        L.cb_go_initialize.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.cb_go_solve_update_v.argtypes = [vp]
+        L.cb_go_set_current.argtypes = [vp, ctypes.c_double]
+        L.cb_go_output_file.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p]
+        L.cb_go_finalize.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -277,6 +283,13 @@ def pkj_initialize(nx, ny, solver="RKC"):
     return p
 
 
+def io_initialize(nx, ny, solver="RKC"):
+    """io_initialize + io_solve_initialize + io_gap_create; reads ./IO.txt."""
+    h = ctypes.c_void_p()
+    _check(lib().cb_io_initialize(nx, ny, solver.encode(), ctypes.byref(h)))
+    return _wrap(h)
+
+
 def pkj_set_current(p, t):
     _check(lib().cb_pkj_set_current(p.h, t))
 
@@ -294,12 +307,9 @@ def pkj_finalize(p):
     p.close()
 
 
-# ---- reference-named entry points (inferior olive) ----
+# ---- reference-named entry points (inferior olive, Golgi cells) ----
 
-def io_initialize(nx, ny, solver="RKC"):
-    """io_initialize + io_solve_initialize + io_gap_create; reads ./IO.txt."""
-    h = ctypes.c_void_p()
-    _check(lib().cb_io_initialize(nx, ny, solver.encode(), ctypes.byref(h)))
+def _wrap(h):
     p = Population.__new__(Population)
     p.h = h
     n = ctypes.c_int()
@@ -307,6 +317,13 @@ def io_initialize(nx, ny, solver="RKC"):
     _check(lib().cb_pop_size(h, ctypes.byref(n), ctypes.byref(nc)))
     p.n, p.nc, p.comp, p.engine, p._keep = n.value, nc.value, nc.value // max(n.value, 1), ENGINE_GLOBAL_FP64, []
     return p
+
+
+def io_initialize(nx, ny, solver="RKC"):
+    """io_initialize + io_solve_initialize + io_gap_create; reads ./IO.txt."""
+    h = ctypes.c_void_p()
+    _check(lib().cb_io_initialize(nx, ny, solver.encode(), ctypes.byref(h)))
+    return _wrap(h)
 
 
 def io_set_current(p, t):
@@ -323,4 +340,28 @@ def io_output_file(p, memv, raster_prev, t, v_csv, raster_csv):
 
 
 def io_finalize(p):
+    p.close()
+
+
+def go_initialize(nx, ny, solver="RKC"):
+    """go_initialize + go_solve_initialize; reads ./Go_131.txt."""
+    h = ctypes.c_void_p()
+    _check(lib().cb_go_initialize(nx, ny, solver.encode(), ctypes.byref(h)))
+    return _wrap(h)
+
+
+def go_set_current(p, t):
+    _check(lib().cb_go_set_current(p.h, t))
+
+
+def go_solve_update_v(p):
+    _check(lib().cb_go_solve_update_v(p.h))
+
+
+def go_output_file(p, memv, raster_prev, t, v_csv, raster_csv):
+    memv = np.ascontiguousarray(memv, dtype=np.float64)
+    _check(lib().cb_go_output_file(p.h, _dp(memv), _dp(raster_prev), t, v_csv.encode(), raster_csv.encode()))
+
+
+def go_finalize(p):
     p.close()

@@ -35,6 +35,8 @@ enum Field {
   F_Y = 6,        // RKC vec[y]
   F_YN = 7,       // RKC vec[yn_r]
   F_FN = 8,       // RKC vec[fn]
+  F_CA2 = 9,      // Golgi cells: the second (CaLVA) Ca pool
+  F_CA_OLD = 10,  // Golgi cells: Ca at the previous ion update (KAHP scheme)
   F_CTL = 50,     // per-cell engines: the per-cell controller records (ncell x 32 doubles)
   F_GATE0 = 100,  // + gate index
 };
@@ -103,6 +105,7 @@ std::vector<double> rkc_stage_table(int mtab);
 
 Engine *make_global_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);
 Engine *make_global_io(const IoHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);
+Engine *make_global_go(const GoHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);
 Engine *make_cell_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn,
                       bool mixed);
 

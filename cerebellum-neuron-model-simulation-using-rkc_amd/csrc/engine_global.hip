@@ -24,6 +24,7 @@
 #include "detsum.hpp"
 #include "engine.hpp"
 #include "philox.hpp"
+#include "go_model.hpp"
 #include "io_model.hpp"
 #include "pkj_model.hpp"
 
@@ -282,6 +283,52 @@ __global__ void __launch_bounds__(256) k_io_ion_b(IoState S, const double *__res
       bb += gap_current<double>(diff_v);
     }
   }
+  if (S.syn_g) bb += S.syn_g[i] * (vn - S.syn_e);
+  S.b[i] = bb;
+}
+
+struct GoState {
+  double *v, *ca, *ca2, *rev, *ca_old, *iext;
+  const double *area;
+  const double *cond;   // [NCOND][nc], density x area
+  double *gate;         // [NGATE][nc]
+  double *b;
+  const double *syn_g;  // may be null
+  double syn_e;
+  int nc;
+};
+
+// go_update_ion_RKC_exp_imp (go_ion.cu:379-437), go_KAHP_update_2order
+// (go_ion.cu:88-213) and go_update_matrix (solve_rkc.cu:288-314), fused: the
+// three reference kernels touch one compartment's state each, in this order
+// (solve_rkc.cu:1174-1179).  The intercell gap junctions are in the cable
+// matrix, not here.
+__global__ void __launch_bounds__(256) k_go_ion_b(GoState S, const double *__restrict__ vnew, double DT, int do_ion) {
+  using namespace go;
+  const int nc = S.nc;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nc) return;
+  double g[NGATE], c[NCOND];
+#pragma unroll
+  for (int k = 0; k < NGATE; k++) g[k] = S.gate[(size_t)k * nc + i];
+#pragma unroll
+  for (int k = 0; k < NCOND; k++) c[k] = S.cond[(size_t)k * nc + i];
+  const double vn = vnew[i];
+  double rev = S.rev[i];
+  if (do_ion) {
+    double ca = S.ca[i], ca2 = S.ca2[i];
+    update_ion(S.v[i], vn, S.area[i], c, ca, ca2, rev, DT, g);
+    const double ca_prev = S.ca_old[i];
+    kahp_step(ca_prev, ca, DT, g);
+#pragma unroll
+    for (int k = 0; k < NGATE; k++) S.gate[(size_t)k * nc + i] = g[k];
+    S.ca[i] = ca;
+    S.ca2[i] = ca2;
+    S.rev[i] = rev;
+    S.ca_old[i] = ca;
+    S.v[i] = vn;
+  }
+  double bb = current(vn, S.iext[i], rev, c, g);
   if (S.syn_g) bb += S.syn_g[i] * (vn - S.syn_e);
   S.b[i] = bb;
 }
@@ -829,6 +876,51 @@ class GlobalIo : public GlobalRkc {
   int *nbr_;
 };
 
+// Golgi cells on a grid, gap junctions in the cable matrix (go.cu,
+// go_solve.cu, go_ion.cu)
+class GlobalGo : public GlobalRkc {
+ public:
+  GlobalGo(const GoHost &H, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn)
+      : GlobalRkc(H.n, H.nc, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, with_amp(stim, H.n), syn) {
+    using namespace go;
+    ca_ = dalloc<double>(nc_); h2d(ca_, H.ca);
+    ca2_ = dalloc<double>(nc_); h2d(ca2_, H.ca2);
+    rev_ = dalloc<double>(nc_); h2d(rev_, H.rev);
+    ca_old_ = dalloc<double>(nc_); h2d(ca_old_, H.ca_old);
+    area_ = dalloc<double>(nc_); h2d(area_, H.area);
+    cond_ = dalloc<double>((size_t)NCOND * nc_); h2d(cond_, H.cond);
+    gate_ = dalloc<double>((size_t)NGATE * nc_); h2d(gate_, H.gate);
+    setup(sol, std::pow(2.0, -10.0));   // H_MIN_GO, solve_rkc.cuh:25
+  }
+  ~GlobalGo() override {
+    (void)hipStreamSynchronize(st_);
+    for (void *p : {(void *)ca_, (void *)ca2_, (void *)rev_, (void *)ca_old_, (void *)area_, (void *)cond_,
+                    (void *)gate_})
+      (void)hipFree(p);
+  }
+
+ private:
+  static StimSpec with_amp(StimSpec s, int n) {
+    if (s.amp.size() != (size_t)n) s.amp.assign(n, go::STIM);
+    return s;
+  }
+  void model_b(const double *vv, bool do_ion, double dt) override {
+    GoState S{v_, ca_, ca2_, rev_, ca_old_, iext_, area_, cond_, gate_, b_, syn_g_, syn_.e_syn, nc_};
+    k_go_ion_b<<<nblk(nc_), 256, 0, st_>>>(S, vv, dt, do_ion ? 1 : 0);
+  }
+  const double *model_field(int field) const override {
+    switch (field) {
+      case F_CA: return ca_;
+      case F_REV: return rev_;
+      case F_CA2: return ca2_;
+      case F_CA_OLD: return ca_old_;
+    }
+    if (field >= F_GATE0 && field < F_GATE0 + go::NGATE) return gate_ + (size_t)(field - F_GATE0) * nc_;
+    return nullptr;
+  }
+  double *ca_, *ca2_, *rev_, *ca_old_, *area_, *cond_, *gate_;
+};
+
 }  // namespace
 
 void Engine::run(int nsteps, double t0, int rec_comp, double *rec_out) {
@@ -849,6 +941,10 @@ Engine *make_global_pkj(const PkjHost &host, const SolverSpec &sol, const StimSp
 
 Engine *make_global_io(const IoHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn) {
   return new GlobalIo(host, sol, stim, syn);
+}
+
+Engine *make_global_go(const GoHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn) {
+  return new GlobalGo(host, sol, stim, syn);
 }
 
 }  // namespace cbrkc

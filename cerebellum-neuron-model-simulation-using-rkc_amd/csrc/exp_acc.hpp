@@ -92,4 +92,13 @@ __host__ __device__ inline double exp_acc(double x) {
   return ldexp(acc.hi + acc.lo, (int)k);
 }
 
+// The models' exponential (inferior olive, Golgi cells): the host C
+// library's on the host (initial state), the correctly rounded exp_acc on the
+// device.  These populations amplify last-bit differences of exp into
+// spike-time shifts over hundreds of milliseconds, so the device follows a
+// correctly rounded exp rather than the device library's (faithful, not
+// correctly rounded) one.
+__host__ inline double mexp(double x) { return ::exp(x); }
+__device__ inline double mexp(double x) { return exp_acc(x); }
+
 }  // namespace cbrkc

@@ -27,14 +27,6 @@ constexpr double CA_PH = 3.0, BETA_CA = 0.075;
 constexpr double STIM = 5.5;              // uA/cm^2 on the dendrite, io.cu:253
 constexpr double STIM_T0 = 800.0, STIM_T1 = 820.0;
 
-// The model's exponential: the host C library's on the host (initial state),
-// the correctly rounded exp_acc on the device.  The gap-coupled grid
-// amplifies last-bit differences of exp into spike-time shifts over hundreds
-// of milliseconds, so the device follows a correctly rounded exp rather than
-// the device library's (faithful, not correctly rounded) one.
-__host__ inline double mexp(double x) { return ::exp(x); }
-__device__ inline double mexp(double x) { return exp_acc(x); }
-
 enum Gate { k_CaL, l_CaL, m_Na, h_Na, n_Kdr, p_Kdr, x_K, r_CaH, s_KCa, q_H, NGATE };
 enum Cond { g_leak, g_CaL, g_Na, g_Kdr, g_K, g_CaH, g_KCa, g_H, NCOND };
 

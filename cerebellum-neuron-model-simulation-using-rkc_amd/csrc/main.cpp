@@ -17,8 +17,8 @@
 //                    fn refresh, spectral-radius floor)
 //   --chunk K        macro steps per device launch between host writes (64)
 //
-// This build provides the Purkinje and inferior-olive populations with the
-// RKC solver; asking for another population or solver prints an error and
+// This build provides the Golgi, Purkinje and inferior-olive populations with
+// the RKC solver; asking for another population or solver prints an error and
 // exits with status 1.
 #include <chrono>
 #include <cmath>
@@ -100,7 +100,7 @@ int main(int argc, char *argv[]) {
     ny[k] = std::atoi(argv[kSolverArg[k] + 2]);
   }
   for (int k = 0; k < 4; k++) {
-    if (nx[k] * ny[k] > 0 && ((k != CB_PKJ && k != CB_IO) || solver[k] != CB_SOLVER_RKC)) {
+    if (nx[k] * ny[k] > 0 && (k == CB_GR || solver[k] != CB_SOLVER_RKC)) {
       std::printf("Error!! -> %s population with the %s solver is not provided by this build\n", kTypeNames[k],
                   kSolverNames[solver[k]]);
       return 1;
@@ -110,8 +110,12 @@ int main(int argc, char *argv[]) {
   if (dbg) std::fclose(dbg);
 
   // the reference seeds libc rand() with 0 before any population draws its
-  // initial voltages (main.cu:37)
+  // initial voltages (main.cu:37), and initialises GR, GO, PKJ, IO in that
+  // order (main.cu:104-137)
   std::srand(0);
+  const int ngo = nx[CB_GO] * ny[CB_GO];
+  cb_pop *gop = nullptr;
+  if (ngo > 0) check(cb_go_initialize(nx[CB_GO], ny[CB_GO], "RKC", &gop), "go_initialize");
   const int n = nx[CB_PKJ] * ny[CB_PKJ];
   cb_pop *pkj = nullptr;
   int comp = 0;
@@ -140,7 +144,7 @@ int main(int argc, char *argv[]) {
   // the macro-step loop of main.cu:182-311 for the populations this build
   // provides; the recorded compartment of every cell comes back once per
   // chunk of macro steps instead of all of V once per step
-  std::vector<double> rec, raster_prev(n, 100.0);
+  std::vector<double> rec, raster_prev(n, 100.0), go_rec, go_prev(ngo, 100.0);
   const auto start = std::chrono::steady_clock::now();
   auto half = start;
   const long total = (long)std::floor((ts - 0.0001) / H_MAX) + 1;   // while (t < TS - 0.0001)
@@ -148,6 +152,30 @@ int main(int argc, char *argv[]) {
   double t = 0.0;
   while (done < total) {
     const int k = (int)std::min<long>(chunk, total - done);
+    // go_output_file (go.cu:270-293): V of compartment 30 per cell, raster on
+    // its upward crossing of +10 mV
+    if (ngo > 0) {
+      go_rec.assign((size_t)k * ngo, 0.0);
+      check(cb_pop_run(gop, k, t, 30, go_rec.data()), "go_solve_update_v");
+      for (int s = 0; s < k; s++) {
+        const double ts_ = t + (s + 1) * H_MAX;
+        std::fprintf(fv[CB_GO], "%lf,", ts_);
+        for (int c = 0; c < ngo; c++) {
+          const double v = go_rec[(size_t)s * ngo + c];
+          if (std::isnan(v)) {
+            std::printf("isnan in go, time=%f\n", ts_ - H_MAX);
+            return 1;
+          }
+          std::fprintf(fv[CB_GO], "%lf,", v);
+        }
+        std::fprintf(fv[CB_GO], "\n");
+        for (int c = 0; c < ngo; c++) {
+          const double v = go_rec[(size_t)s * ngo + c];
+          if (v > 10.0 && go_prev[c] <= 10.0) std::fprintf(fr[CB_GO], "%lf,%d\n", ts_, c);
+          go_prev[c] = v;
+        }
+      }
+    }
     if (n > 0) {
       rec.assign((size_t)k * n, 0.0);
       check(cb_pop_run(pkj, k, t, 1599, rec.data()), "pkj_solve_update_v");
@@ -211,7 +239,7 @@ int main(int argc, char *argv[]) {
   std::FILE *tp = std::fopen("comptime.csv", "a");
   if (tp) {
     const int sim = (int)ts;
-    std::fprintf(tp, "Gr,%d,%s,Go,%d,%s,Pkj,%d,%s,Io,%d,%s,", nx[0] * ny[0], argv[1], nx[1] * ny[1], argv[4], n,
+    std::fprintf(tp, "Gr,%d,%s,Go,%d,%s,Pkj,%d,%s,Io,%d,%s,", nx[0] * ny[0], argv[1], ngo, argv[4], n,
                  argv[7], nio, argv[10]);
     if (ts <= 500.0)
       std::fprintf(tp, "CompTime,%lf,SimTime,%d\n", secs, sim);
@@ -220,6 +248,7 @@ int main(int argc, char *argv[]) {
                    std::chrono::duration<double>(end - half).count(), sim);
     std::fclose(tp);
   }
+  if (gop) check(cb_go_finalize(gop), "go_finalize");
   if (pkj) check(cb_pkj_finalize(pkj), "pkj_finalize");
   if (iop) check(cb_io_finalize(iop), "io_finalize");
   return 0;

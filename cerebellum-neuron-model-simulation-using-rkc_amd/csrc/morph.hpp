@@ -75,6 +75,25 @@ struct IoHost {
 // order (io_ion.cu:168-170).
 bool build_io_host(const std::string &param_path, int nx, int ny, const double *v0, IoHost &out, std::string &err);
 
+// Golgi-cell population on the host: Go_131.txt (idx parent diam len Ra Cm
+// type + 13 channel densities, go.cu:30-63), conductances and Cm as density x
+// cylinder area, the cable CSR with the grid gap junctions between the cells'
+// compartments 0 (go_solve.cu:4-215, RKC sign) and the initial state
+// (go_ion.cu:590-634).
+struct GoHost {
+  int n = 0, nx = 0, ny = 0, comp = 131, nc = 0, n_gj = 0;
+  std::vector<double> cm, area, iext, v, ca, ca2, rev, ca_old;
+  std::vector<double> cond;   // [NCOND][nc]
+  std::vector<double> gate;   // [NGATE][nc]
+  std::vector<int> row, col;
+  std::vector<double> val;
+};
+
+// Reads a Go_131.txt-format file and builds nx * ny cells; v0 per cell, or
+// nullptr for -60 + 2 (U - 0.5) from libc rand() per cell in order
+// (go_ion.cu:597-600).
+bool build_go_host(const std::string &param_path, int nx, int ny, const double *v0, GoHost &out, std::string &err);
+
 // Spectral radius of each cell's cable operator C^-1 L (the RKC Jacobian
 // with the ionic current frozen), from the CSR rows of the cell without
 // intercell entries: Lanczos with full reorthogonalisation on the symmetric

@@ -49,6 +49,8 @@ enum {
   CB_F_Y = 6,
   CB_F_YN = 7,
   CB_F_FN = 8,
+  CB_F_CA2 = 9,    /* Golgi cells: second (CaLVA) Ca pool (go->ca2) */
+  CB_F_CA_OLD = 10, /* Golgi cells: Ca at the previous ion update (go->ca_old) */
   CB_F_GATE0 = 100 /* + gate index in the cell type's gate order */
 };
 
@@ -172,6 +174,22 @@ int cb_pkj_output_file(const cb_pop *p, const double *memv, double *raster_prev,
                        const char *raster_csv);
 /* pkj_finalize + pkj_solve_finalize. */
 int cb_pkj_finalize(cb_pop *p);
+
+/* go_initialize + go_solve_initialize (go.cu:160-230, go_solve.cu:248-333):
+ * reads ./Go_131.txt; the grid gap junctions between the cells' compartments 0
+ * are part of the cable matrix (go_solve.cu:52-200); initial voltages from
+ * libc rand() as go_ion.cu:597-600 draws them. */
+int cb_go_initialize(int nx, int ny, const char *solver, cb_pop **out);
+/* go_solve_update_v (go_solve.cu:361-374) without granule-cell synapses. */
+int cb_go_solve_update_v(cb_pop *p);
+/* go_set_current (go.cu:258-268): 0.2 nA into compartment 30, [800, 1100) ms. */
+int cb_go_set_current(cb_pop *p, double t);
+/* go_output_file (go.cu:270-293): appends "t,V30 of every cell,\n" and a
+ * "t,cell" row per upward crossing of +10 mV. */
+int cb_go_output_file(const cb_pop *p, const double *memv, double *raster_prev, double t, const char *v_csv,
+                      const char *raster_csv);
+/* go_finalize + go_solve_finalize. */
+int cb_go_finalize(cb_pop *p);
 
 /* io_initialize + io_solve_initialize + io_gap_create (io.cu:186-229,
  * io_solve.cu:192-262, gap.cu:136-147): reads ./IO.txt; initial voltages from

@@ -99,6 +99,7 @@ static void update_b_gap(or_pop *p, const double *vv, int gap)
   switch (p->type) {
     case OR_PKJ: or_pkj_update_b(p, vv); break;
     case OR_IO: or_io_update_b(p, vv, gap); break;
+    case OR_GO: or_go_update_b(p, vv); break;
     default: break;
   }
   if (p->syn_g) {
@@ -507,6 +508,7 @@ void or_model_update_ion_and_b(or_pop *p, double sum_dt)
   switch (p->type) {
     case OR_PKJ: or_pkj_update_ion(p, p->vec[OR_Y], sum_dt); break;
     case OR_IO: or_io_update_ion(p, p->vec[OR_Y], sum_dt); break;
+    case OR_GO: or_go_update_ion(p, p->vec[OR_Y], sum_dt); break;
     default: break;
   }
   update_b_gap(p, p->vec[OR_Y], 1);   /* io_update_ion_and_matrix adds the gap currents */
@@ -568,6 +570,8 @@ static double *field(const or_pop *p, int what)
     case 105: return p->iext;
     case 106: return p->shell;
     case 107: return p->area;
+    case 108: return p->ca2;
+    case 109: return p->ca_old;
   }
   if (what >= 200 && what < 200 + p->nion) return p->ion[what - 200];
   if (what >= 300 && what < 300 + p->ncond) return p->cond[what - 300];
@@ -688,7 +692,7 @@ void or_pop_free(or_pop *p)
 {
   if (!p) return;
   free(p->v); free(p->ca); free(p->cm); free(p->area); free(p->iext); free(p->shell);
-  free(p->rev); free(p->compart);
+  free(p->rev); free(p->compart); free(p->ca2); free(p->ca_old);
   for (int i = 0; i < p->ncond; i++) free(p->cond[i]);
   for (int i = 0; i < p->nion; i++) free(p->ion[i]);
   free(p->row); free(p->col); free(p->val); free(p->b); free(p->dammy);

@@ -44,6 +44,7 @@ typedef struct or_pop {
   int ncond, nion;
   /* compartment state, structure of arrays (param.h:27-35 elem/ion/cond) */
   double *v, *ca, *cm, *area, *iext, *shell, *rev;
+  double *ca2, *ca_old; /* GO: second Ca pool and the Ca of the previous ion update */
   int *compart;       /* compartment type */
   double *cond[OR_MAX_COND];
   double *ion[OR_MAX_ION];
@@ -101,6 +102,14 @@ void or_io_update_b(or_pop *p, const double *vv, int add_gap);
 void or_io_update_ion(or_pop *p, const double *vnew, double dt);
 double or_io_rate(int which, double v, int comp);
 void or_io_neighbours(int id, int nx, int ny, int *out4);
+
+/* ---- Golgi cells (or_go.c) ---- */
+or_pop *or_go_create(const char *param_file, int nx, int ny, const double *v0_per_cell, double atol, double rtol,
+                     int use_libc_rand);
+void or_go_update_b(or_pop *p, const double *vv);
+void or_go_update_ion(or_pop *p, const double *vnew, double dt);
+double or_go_rate(int which, double v, double ca);
+int or_go_kahp(double ca_old, double ca, double dt, double *st6);
 
 /* ---- stimulus ---- */
 void or_set_stim(or_pop *p, int comp, double t0, double t1, const double *amp_per_cell);

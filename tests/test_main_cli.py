@@ -95,3 +95,44 @@ def test_io_raster_file_against_the_references(built, tmp_path, ref_data):
     assert max(abs(got[int(c)] - float(t)) for t, c in ref) <= 0.125
     rows = (tmp_path / "io_v_by_RKC.csv").read_text().splitlines()
     assert len(rows) == 6480 and rows[0].count(",") == 1 + 64 * 3
+
+
+@pytest.mark.gpu
+def test_go_rows_and_raster_against_oracle(built, tmp_path, ref_data):
+    """./main RKC 0 0 RKC 2 2 RKC 0 0 RKC 0 0: go_v_by_RKC.csv holds t and the
+    soma (compartment 30) of every cell per macro step, go_raster_by_RKC.csv
+    its upward crossings of +10 mV; libc rand() seeded 0 gives the oracle the
+    same initial voltages.  Spike counts match the oracle, times within two
+    macro steps (the oracle's own spread under a one-ulp change of its
+    inputs, tests/test_oracle_go.py::test_noise_floor_of_the_grid_workload)."""
+    shutil.copy(os.path.join(ref_data, "Go_131.txt"), tmp_path / "Go_131.txt")
+    print("main: GO 2x2, RKC, 250 ms", flush=True)
+    r = run_main(["RKC", "0", "0", "RKC", "2", "2", "RKC", "0", "0", "RKC", "0", "0", "--ts", "250"], tmp_path,
+                 timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    rows = [l.rstrip("\n").rstrip(",").split(",") for l in open(tmp_path / "go_v_by_RKC.csv")]
+    got = np.array([[float(x) for x in row] for row in rows])
+    assert got.shape == (2000, 5)
+    from test_oracle_go import go_pop
+    o = go_pop(ref_data, 2, 2, seed=0)
+    ref = []
+    for k in range(2000):
+        o.set_current(k * 0.125)
+        o.advance()
+        ref.append(o.get(orc.VNEW)[30::131])
+    ref = np.array(ref)
+    np.testing.assert_allclose(got[0, 1:], ref[0], rtol=0, atol=2e-6)
+    lines = (tmp_path / "go_raster_by_RKC.csv").read_text().splitlines()
+    for line in lines:
+        t, c = line.split(",")
+        assert t == "%f" % float(t) and c == str(int(c))
+    spikes = sorted((int(c), float(t)) for t, c in (l.split(",") for l in lines))
+    want = []
+    prev = np.full(4, 100.0)
+    for k in range(2000):
+        for c in np.nonzero((ref[k] > 10.0) & (prev <= 10.0))[0]:
+            want.append((int(c), 0.125 * (k + 1)))
+        prev = ref[k]
+    want.sort()
+    assert len(spikes) == len(want) >= 4
+    assert all(a[0] == b[0] and abs(a[1] - b[1]) <= 0.25 for a, b in zip(spikes, want))
