@@ -36,6 +36,7 @@ EXPORTS = [
     "cb_pop_kernel_time", "cb_pop_autotune", "cb_pkj_cable_spectral_radius",
     "cb_io_initialize", "cb_io_solve_update_v", "cb_io_set_current", "cb_io_output_file", "cb_io_finalize",
     "cb_go_initialize", "cb_go_solve_update_v", "cb_go_set_current", "cb_go_output_file", "cb_go_finalize",
+    "cb_gr_initialize", "cb_gr_solve_update_v", "cb_gr_set_current", "cb_gr_output_file", "cb_gr_finalize",
 ]
 
 # The synthetic Purkinje trees (10 um soma on a thin tapering trunk) need
@@ -126,6 +127,11 @@ def lib():
         L.cb_go_set_current.argtypes = [vp, ctypes.c_double]
         L.cb_go_output_file.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p]
         L.cb_go_finalize.argtypes = [vp]
+        L.cb_gr_initialize.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.cb_gr_solve_update_v.argtypes = [vp]
+        L.cb_gr_set_current.argtypes = [vp, ctypes.c_double]
+        L.cb_gr_output_file.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p]
+        L.cb_gr_finalize.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -364,4 +370,28 @@ def go_output_file(p, memv, raster_prev, t, v_csv, raster_csv):
 
 
 def go_finalize(p):
+    p.close()
+
+
+def gr_initialize(nx, ny, solver="RKC"):
+    """gr_initialize + gr_solve_initialize; reads ./Gr.txt."""
+    h = ctypes.c_void_p()
+    _check(lib().cb_gr_initialize(nx, ny, solver.encode(), ctypes.byref(h)))
+    return _wrap(h)
+
+
+def gr_set_current(p, t):
+    _check(lib().cb_gr_set_current(p.h, t))
+
+
+def gr_solve_update_v(p):
+    _check(lib().cb_gr_solve_update_v(p.h))
+
+
+def gr_output_file(p, memv, raster_prev, t, v_csv, raster_csv):
+    memv = np.ascontiguousarray(memv, dtype=np.float64)
+    _check(lib().cb_gr_output_file(p.h, _dp(memv), _dp(raster_prev), t, v_csv.encode(), raster_csv.encode()))
+
+
+def gr_finalize(p):
     p.close()

@@ -10,6 +10,7 @@
 #include "../../include/cbrkc.h"
 #include "engine.hpp"
 #include "go_model.hpp"
+#include "gr_model.hpp"
 #include "io_model.hpp"
 #include "morph.hpp"
 #include "philox.hpp"
@@ -148,6 +149,48 @@ int create_go(const cb_pop_config &cfg, cb_pop **out) {
   return CB_OK;
 }
 
+// gr_initialize + gr_solve_initialize (gr.cu:151-218, gr_solve.cu:170-259):
+// global FP64 engine (the reference's semantics)
+int create_gr(const cb_pop_config &cfg, cb_pop **out) {
+  const int n = cfg.nx * cfg.ny;
+  if (n == 0) return fail(CB_E_ARG, "empty population");
+  if (cfg.engine != CB_ENGINE_GLOBAL_FP64) return fail(CB_E_UNSUP, "granule cells: global engine only");
+  GrHost host;
+  std::string err;
+  const std::string path = cfg.morph_path ? cfg.morph_path : "./Gr.txt";
+  if (!build_gr_host(path, cfg.nx, cfg.ny, cfg.v0, host, err)) return fail(CB_E_IO, err);
+  StimSpec stim;   // gr_set_current (gr.cu:245-254): compartment 16, [50, 100) ms
+  stim.comp = gr::REC;
+  stim.t0 = gr::STIM_T0;
+  stim.t1 = gr::STIM_T1;
+  if (cfg.stim_comp >= 0) {
+    if (cfg.stim_comp >= gr::COMP) return fail(CB_E_ARG, "stimulus compartment out of range");
+    stim.comp = cfg.stim_comp;
+    stim.t0 = cfg.stim_t0;
+    stim.t1 = cfg.stim_t1;
+  }
+  stim.amp.assign(n, gr::STIM);
+  if (cfg.stim_amp)
+    for (int c = 0; c < n; c++) stim.amp[c] = cfg.stim_amp[c];
+  SolverSpec sol;
+  const double tol = 1.0e-9;   // TOL_GR, solve_rkc.cuh:28
+  sol.atol = cfg.atol > 0 ? cfg.atol : tol;
+  sol.rtol = cfg.rtol > 0 ? cfg.rtol : tol;
+  sol.ion_interval = cfg.ion_interval_ms;
+  sol.refresh_fn = cfg.refresh_fn != 0;
+  if (cfg.rho_floor) return fail(CB_E_UNSUP, "the spectral-radius floor is provided for Purkinje cells");
+  if (!(sol.atol > 0.0) || !(sol.rtol >= 0.0)) return fail(CB_E_ARG, "atol must be > 0 and rtol >= 0");
+  std::unique_ptr<cb_pop> p(new cb_pop);
+  p->cell_type = CB_GR;
+  p->solver = cfg.solver;
+  p->engine = cfg.engine;
+  p->comp = gr::COMP;
+  p->device = cfg.device;
+  p->eng.reset(make_global_gr(host, sol, stim, poisson_of(cfg)));
+  *out = p.release();
+  return CB_OK;
+}
+
 int solver_from_name(const char *s) {
   if (!s) return -1;
   if (std::strncmp(s, "BE", 2) == 0) return CB_SOLVER_BE;
@@ -189,12 +232,12 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     if (!cfg || !out) return fail(CB_E_ARG, "null argument");
     *out = nullptr;
     if (cfg->nx < 0 || cfg->ny < 0) return fail(CB_E_ARG, "negative grid size");
-    if (cfg->cell_type != CB_PKJ && cfg->cell_type != CB_IO && cfg->cell_type != CB_GO)
-      return fail(CB_E_UNSUP, "cell type not provided by this build");
+    if (cfg->cell_type < CB_GR || cfg->cell_type > CB_IO) return fail(CB_E_ARG, "unknown cell type");
     if (cfg->solver != CB_SOLVER_RKC) return fail(CB_E_UNSUP, "solver not provided by this build");
     CB_HIP(hipSetDevice(cfg->device));
     if (cfg->cell_type == CB_IO) return create_io(*cfg, out);
     if (cfg->cell_type == CB_GO) return create_go(*cfg, out);
+    if (cfg->cell_type == CB_GR) return create_gr(*cfg, out);
     std::vector<MorphRow> rows;
     std::string err;
     const std::string mpath = cfg->morph_path ? cfg->morph_path : "./Pkj.txt";
@@ -434,6 +477,42 @@ int cb_io_output_file(const cb_pop *p, const double *memv, double *raster_prev, 
   if (raster_csv && !fr) return fail(CB_E_IO, std::string("cannot open ") + raster_csv);
   for (int j = 0; j < n; j++) {
     const double vj = memv[2 + (size_t)j * C];
+    if (vj > 10.0 && raster_prev[j] <= 10.0 && fr) std::fprintf(fr, "%lf,%d\n", t, j);
+    raster_prev[j] = vj;
+  }
+  if (fr) std::fclose(fr);
+  return CB_OK;
+}
+
+int cb_gr_initialize(int nx, int ny, const char *solver, cb_pop **out) {
+  int s = solver_from_name(solver);
+  if (s < 0) return fail(CB_E_ARG, "error in gr_initialize");
+  cb_pop_config cfg;
+  cb_config_default(&cfg, CB_GR, s, nx, ny);
+  return cb_pop_create(&cfg, out);
+}
+
+int cb_gr_solve_update_v(cb_pop *p) { return cb_pop_advance(p); }
+int cb_gr_set_current(cb_pop *p, double t) { return cb_pop_set_current(p, t); }
+int cb_gr_finalize(cb_pop *p) { return cb_pop_destroy(p); }
+
+int cb_gr_output_file(const cb_pop *p, const double *memv, double *raster_prev, double t, const char *v_csv,
+                      const char *raster_csv) {
+  if (!p || !memv || !raster_prev) return fail(CB_E_ARG, "null argument");
+  if (p->cell_type != CB_GR) return fail(CB_E_ARG, "not a granule-cell population");
+  const int n = p->eng->ncell(), C = p->comp;
+  if (v_csv) {
+    FILE *f = std::fopen(v_csv, "a");
+    if (!f) return fail(CB_E_IO, std::string("cannot open ") + v_csv);
+    std::fprintf(f, "%lf,", t);
+    for (int j = 0; j < n; j++) std::fprintf(f, "%lf,", memv[gr::REC + (size_t)j * C]);
+    std::fprintf(f, "\n");
+    std::fclose(f);
+  }
+  FILE *fr = raster_csv ? std::fopen(raster_csv, "a") : nullptr;
+  if (raster_csv && !fr) return fail(CB_E_IO, std::string("cannot open ") + raster_csv);
+  for (int j = 0; j < n; j++) {
+    const double vj = memv[gr::RASTER + (size_t)j * C];
     if (vj > 10.0 && raster_prev[j] <= 10.0 && fr) std::fprintf(fr, "%lf,%d\n", t, j);
     raster_prev[j] = vj;
   }

@@ -106,6 +106,7 @@ std::vector<double> rkc_stage_table(int mtab);
 Engine *make_global_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);
 Engine *make_global_io(const IoHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);
 Engine *make_global_go(const GoHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);
+Engine *make_global_gr(const GrHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);
 Engine *make_cell_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn,
                       bool mixed);
 

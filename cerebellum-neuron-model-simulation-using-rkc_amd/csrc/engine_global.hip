@@ -25,6 +25,7 @@
 #include "engine.hpp"
 #include "philox.hpp"
 #include "go_model.hpp"
+#include "gr_model.hpp"
 #include "io_model.hpp"
 #include "pkj_model.hpp"
 
@@ -329,6 +330,54 @@ __global__ void __launch_bounds__(256) k_go_ion_b(GoState S, const double *__res
     S.v[i] = vn;
   }
   double bb = current(vn, S.iext[i], rev, c, g);
+  if (S.syn_g) bb += S.syn_g[i] * (vn - S.syn_e);
+  S.b[i] = bb;
+}
+
+struct GrState {
+  double *v, *iext;
+  const double *ca;
+  const int8_t *compart;
+  const double *cond;   // [NCOND][nc], density x area
+  double *gate;         // [NGATE][nc]
+  const gr::NaType *na; // per compartment type
+  double *b;
+  const double *syn_g;  // may be null
+  double syn_e;
+  int nc;
+};
+
+// gr_Na_update_2order (gr_ion.cu:45-207), gr_update_ion_RKC_exp_imp
+// (gr_ion.cu:461-480) and gr_update_matrix (solve_rkc.cu:226-249), fused in
+// the reference's order (solve_rkc.cu:1111-1116): the Na scheme sees the
+// voltage of the previous ion update and the new one, then the gates move and
+// the stored voltage becomes the new one.
+__global__ void __launch_bounds__(256) k_gr_ion_b(GrState S, const double *__restrict__ vnew, double DT, int do_ion) {
+  using namespace gr;
+  const int nc = S.nc;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nc) return;
+  double g[NGATE], c[NCOND];
+#pragma unroll
+  for (int k = 0; k < NGATE; k++) g[k] = S.gate[(size_t)k * nc + i];
+#pragma unroll
+  for (int k = 0; k < NCOND; k++) c[k] = S.cond[(size_t)k * nc + i];
+  const double vn = vnew[i];
+  if (do_ion) {
+    const double vo = S.v[i];
+    const int t = S.compart[i];
+    if (t >= HILL) {
+      const NaType T = S.na[t];
+      double x[12];
+      na_gather(g, x);
+      if (na_step(T, vo, vn, DT, x)) na_scatter(x, g);
+    }
+    update_gates(vo, vn, S.ca[i], DT, g);
+#pragma unroll
+    for (int k = 0; k < NGATE; k++) S.gate[(size_t)k * nc + i] = g[k];
+    S.v[i] = vn;
+  }
+  double bb = current(vn, S.iext[i], c, g);
   if (S.syn_g) bb += S.syn_g[i] * (vn - S.syn_e);
   S.b[i] = bb;
 }
@@ -921,6 +970,46 @@ class GlobalGo : public GlobalRkc {
   double *ca_, *ca2_, *rev_, *ca_old_, *area_, *cond_, *gate_;
 };
 
+// Granule cells (gr.cu, gr_solve.cu, gr_ion.cu)
+class GlobalGr : public GlobalRkc {
+ public:
+  GlobalGr(const GrHost &H, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn)
+      : GlobalRkc(H.n, H.nc, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, with_amp(stim, H.n), syn) {
+    using namespace gr;
+    ca_ = dalloc<double>(nc_); h2d(ca_, H.ca);
+    compart_ = dalloc<int8_t>(nc_); h2d(compart_, H.compart);
+    cond_ = dalloc<double>((size_t)NCOND * nc_); h2d(cond_, H.cond);
+    gate_ = dalloc<double>((size_t)NGATE * nc_); h2d(gate_, H.gate);
+    const NaTable T = na_table();
+    na_ = dalloc<NaType>(NTYPE);
+    CB_HIP(hipMemcpy(na_, T.t, sizeof(T.t), hipMemcpyHostToDevice));
+    setup(sol, std::pow(2.0, -12.0));   // H_MIN_GR, solve_rkc.cuh:24
+  }
+  ~GlobalGr() override {
+    (void)hipStreamSynchronize(st_);
+    for (void *p : {(void *)ca_, (void *)compart_, (void *)cond_, (void *)gate_, (void *)na_}) (void)hipFree(p);
+  }
+
+ private:
+  static StimSpec with_amp(StimSpec s, int n) {
+    if (s.amp.size() != (size_t)n) s.amp.assign(n, gr::STIM);
+    return s;
+  }
+  bool trigger() const override { return sum_dt_ > integ_; }   // strict for GR, solve_rkc.cu:1148
+  void model_b(const double *vv, bool do_ion, double dt) override {
+    GrState S{v_, iext_, ca_, compart_, cond_, gate_, na_, b_, syn_g_, syn_.e_syn, nc_};
+    k_gr_ion_b<<<nblk(nc_), 256, 0, st_>>>(S, vv, dt, do_ion ? 1 : 0);
+  }
+  const double *model_field(int field) const override {
+    if (field == F_CA) return ca_;
+    if (field >= F_GATE0 && field < F_GATE0 + gr::NGATE) return gate_ + (size_t)(field - F_GATE0) * nc_;
+    return nullptr;
+  }
+  double *ca_, *cond_, *gate_;
+  int8_t *compart_;
+  gr::NaType *na_;
+};
+
 }  // namespace
 
 void Engine::run(int nsteps, double t0, int rec_comp, double *rec_out) {
@@ -945,6 +1034,10 @@ Engine *make_global_io(const IoHost &host, const SolverSpec &sol, const StimSpec
 
 Engine *make_global_go(const GoHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn) {
   return new GlobalGo(host, sol, stim, syn);
+}
+
+Engine *make_global_gr(const GrHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn) {
+  return new GlobalGr(host, sol, stim, syn);
 }
 
 }  // namespace cbrkc

@@ -17,9 +17,8 @@
 //                    fn refresh, spectral-radius floor)
 //   --chunk K        macro steps per device launch between host writes (64)
 //
-// This build provides the Golgi, Purkinje and inferior-olive populations with
-// the RKC solver; asking for another population or solver prints an error and
-// exits with status 1.
+// This build provides all four populations with the RKC solver; asking for
+// another solver prints an error and exits with status 1.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -100,7 +99,7 @@ int main(int argc, char *argv[]) {
     ny[k] = std::atoi(argv[kSolverArg[k] + 2]);
   }
   for (int k = 0; k < 4; k++) {
-    if (nx[k] * ny[k] > 0 && (k == CB_GR || solver[k] != CB_SOLVER_RKC)) {
+    if (nx[k] * ny[k] > 0 && solver[k] != CB_SOLVER_RKC) {
       std::printf("Error!! -> %s population with the %s solver is not provided by this build\n", kTypeNames[k],
                   kSolverNames[solver[k]]);
       return 1;
@@ -113,6 +112,10 @@ int main(int argc, char *argv[]) {
   // initial voltages (main.cu:37), and initialises GR, GO, PKJ, IO in that
   // order (main.cu:104-137)
   std::srand(0);
+  const int ngr = nx[CB_GR] * ny[CB_GR];
+  cb_pop *grp = nullptr;
+  if (ngr > 0) check(cb_gr_initialize(nx[CB_GR], ny[CB_GR], "RKC", &grp), "gr_initialize");
+  std::vector<double> gr_v(ngr > 0 ? (size_t)ngr * 578 : 0), gr_prev(ngr, 100.0);
   const int ngo = nx[CB_GO] * ny[CB_GO];
   cb_pop *gop = nullptr;
   if (ngo > 0) check(cb_go_initialize(nx[CB_GO], ny[CB_GO], "RKC", &gop), "go_initialize");
@@ -152,6 +155,26 @@ int main(int argc, char *argv[]) {
   double t = 0.0;
   while (done < total) {
     const int k = (int)std::min<long>(chunk, total - done);
+    // gr_output_file (gr.cu:256-277): V of compartment 16 per cell, raster on
+    // the upward crossing of +10 mV at compartment 77
+    for (int s = 0; s < k && ngr > 0; s++) {
+      const double ts_ = t + (s + 1) * H_MAX;
+      check(cb_gr_set_current(grp, ts_ - H_MAX), "gr_set_current");
+      check(cb_gr_solve_update_v(grp), "gr_solve_update_v");
+      check(cb_pop_read(grp, CB_F_VNEW, gr_v.data()), "gr read");
+      if (std::isnan(gr_v[1])) {
+        std::printf("isnan in gr, time=%f\n", ts_ - H_MAX);
+        return 1;
+      }
+      std::fprintf(fv[CB_GR], "%lf,", ts_);
+      for (int c = 0; c < ngr; c++) std::fprintf(fv[CB_GR], "%lf,", gr_v[16 + (size_t)c * 578]);
+      std::fprintf(fv[CB_GR], "\n");
+      for (int c = 0; c < ngr; c++) {
+        const double v = gr_v[77 + (size_t)c * 578];
+        if (v > 10.0 && gr_prev[c] <= 10.0) std::fprintf(fr[CB_GR], "%lf,%d\n", ts_, c);
+        gr_prev[c] = v;
+      }
+    }
     // go_output_file (go.cu:270-293): V of compartment 30 per cell, raster on
     // its upward crossing of +10 mV
     if (ngo > 0) {
@@ -239,7 +262,7 @@ int main(int argc, char *argv[]) {
   std::FILE *tp = std::fopen("comptime.csv", "a");
   if (tp) {
     const int sim = (int)ts;
-    std::fprintf(tp, "Gr,%d,%s,Go,%d,%s,Pkj,%d,%s,Io,%d,%s,", nx[0] * ny[0], argv[1], ngo, argv[4], n,
+    std::fprintf(tp, "Gr,%d,%s,Go,%d,%s,Pkj,%d,%s,Io,%d,%s,", ngr, argv[1], ngo, argv[4], n,
                  argv[7], nio, argv[10]);
     if (ts <= 500.0)
       std::fprintf(tp, "CompTime,%lf,SimTime,%d\n", secs, sim);
@@ -248,6 +271,7 @@ int main(int argc, char *argv[]) {
                    std::chrono::duration<double>(end - half).count(), sim);
     std::fclose(tp);
   }
+  if (grp) check(cb_gr_finalize(grp), "gr_finalize");
   if (gop) check(cb_go_finalize(gop), "go_finalize");
   if (pkj) check(cb_pkj_finalize(pkj), "pkj_finalize");
   if (iop) check(cb_io_finalize(iop), "io_finalize");

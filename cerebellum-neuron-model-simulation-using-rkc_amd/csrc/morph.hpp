@@ -94,6 +94,25 @@ struct GoHost {
 // (go_ion.cu:597-600).
 bool build_go_host(const std::string &param_path, int nx, int ny, const double *v0, GoHost &out, std::string &err);
 
+// Granule-cell population on the host: Gr.txt (idx parent diam len Ra Cm type
+// + 10 channel densities, gr.cu:29-58), conductances and Cm as density x
+// cylinder area, the cable CSR (gr_solve.cu:2-139, RKC sign) and the initial
+// state (gr_ion.cu:635-672).
+struct GrHost {
+  int n = 0, nx = 0, ny = 0, comp = 578, nc = 0;
+  std::vector<double> cm, iext, v, ca;
+  std::vector<int8_t> compart;
+  std::vector<double> cond;   // [NCOND][nc]
+  std::vector<double> gate;   // [NGATE][nc]
+  std::vector<int> row, col;
+  std::vector<double> val;
+};
+
+// Reads a Gr.txt-format file and builds nx * ny cells; v0 per cell, or
+// nullptr for -80 + 2 ((rand() + 1) / (RAND_MAX + 2) - 0.5) from libc rand()
+// per cell in order (gr_ion.cu:642-644).
+bool build_gr_host(const std::string &param_path, int nx, int ny, const double *v0, GrHost &out, std::string &err);
+
 // Spectral radius of each cell's cable operator C^-1 L (the RKC Jacobian
 // with the ionic current frozen), from the CSR rows of the cell without
 // intercell entries: Lanczos with full reorthogonalisation on the symmetric

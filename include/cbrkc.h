@@ -175,6 +175,21 @@ int cb_pkj_output_file(const cb_pop *p, const double *memv, double *raster_prev,
 /* pkj_finalize + pkj_solve_finalize. */
 int cb_pkj_finalize(cb_pop *p);
 
+/* gr_initialize + gr_solve_initialize (gr.cu:151-218, gr_solve.cu:170-259):
+ * reads ./Gr.txt; initial voltages from libc rand() as gr_ion.cu:642-644
+ * draws them. */
+int cb_gr_initialize(int nx, int ny, const char *solver, cb_pop **out);
+/* gr_solve_update_v (gr_solve.cu:292-307) without mossy-fibre / Golgi synapses. */
+int cb_gr_solve_update_v(cb_pop *p);
+/* gr_set_current (gr.cu:245-254): 0.01 nA into compartment 16, [50, 100) ms. */
+int cb_gr_set_current(cb_pop *p, double t);
+/* gr_output_file (gr.cu:256-277): appends "t,V16 of every cell,\n" and a
+ * "t,cell" row per upward crossing of +10 mV at compartment 77. */
+int cb_gr_output_file(const cb_pop *p, const double *memv, double *raster_prev, double t, const char *v_csv,
+                      const char *raster_csv);
+/* gr_finalize + gr_solve_finalize. */
+int cb_gr_finalize(cb_pop *p);
+
 /* go_initialize + go_solve_initialize (go.cu:160-230, go_solve.cu:248-333):
  * reads ./Go_131.txt; the grid gap junctions between the cells' compartments 0
  * are part of the cable matrix (go_solve.cu:52-200); initial voltages from

@@ -100,6 +100,7 @@ static void update_b_gap(or_pop *p, const double *vv, int gap)
     case OR_PKJ: or_pkj_update_b(p, vv); break;
     case OR_IO: or_io_update_b(p, vv, gap); break;
     case OR_GO: or_go_update_b(p, vv); break;
+    case OR_GR: or_gr_update_b(p, vv); break;
     default: break;
   }
   if (p->syn_g) {
@@ -509,6 +510,7 @@ void or_model_update_ion_and_b(or_pop *p, double sum_dt)
     case OR_PKJ: or_pkj_update_ion(p, p->vec[OR_Y], sum_dt); break;
     case OR_IO: or_io_update_ion(p, p->vec[OR_Y], sum_dt); break;
     case OR_GO: or_go_update_ion(p, p->vec[OR_Y], sum_dt); break;
+    case OR_GR: or_gr_update_ion(p, p->vec[OR_Y], sum_dt); break;
     default: break;
   }
   update_b_gap(p, p->vec[OR_Y], 1);   /* io_update_ion_and_matrix adds the gap currents */

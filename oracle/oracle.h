@@ -111,6 +111,14 @@ void or_go_update_ion(or_pop *p, const double *vnew, double dt);
 double or_go_rate(int which, double v, double ca);
 int or_go_kahp(double ca_old, double ca, double dt, double *st6);
 
+/* ---- granule cells (or_gr.c) ---- */
+or_pop *or_gr_create(const char *param_file, int nx, int ny, const double *v0_per_cell, double atol, double rtol,
+                     int use_libc_rand);
+void or_gr_update_b(or_pop *p, const double *vv);
+void or_gr_update_ion(or_pop *p, const double *vnew, double dt);
+double or_gr_rate(int which, double v, double ca);
+int or_gr_na_step(int type, double v_old, double v_new, double dt, double *x12);
+
 /* ---- stimulus ---- */
 void or_set_stim(or_pop *p, int comp, double t0, double t1, const double *amp_per_cell);
 void or_set_current(or_pop *p, double t);

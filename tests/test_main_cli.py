@@ -39,9 +39,9 @@ def test_unknown_solver_name(built, tmp_path):
 
 def test_prefix_match_and_output_files(built, tmp_path):
     # "RKCx" matches RKC by prefix; an unprovided population fails loudly
-    r = run_main(["RKCx", "2", "2", "BE", "0", "0", "CN", "0", "0", "RKC", "0", "0", "--ts", "1"], tmp_path)
+    r = run_main(["RKCx", "0", "0", "BE", "2", "2", "CN", "0", "0", "RKC", "0", "0", "--ts", "1"], tmp_path)
     assert r.returncode == 1
-    assert "gr population with the RKC solver is not provided" in r.stdout
+    assert "go population with the BE solver is not provided" in r.stdout
     for name in ("gr_v_by_RKC.csv", "go_raster_by_BE.csv", "pkj_v_by_CN.csv", "io_raster_by_RKC.csv"):
         assert (tmp_path / name).exists(), name
 
@@ -136,3 +136,27 @@ def test_go_rows_and_raster_against_oracle(built, tmp_path, ref_data):
     want.sort()
     assert len(spikes) == len(want) >= 4
     assert all(a[0] == b[0] and abs(a[1] - b[1]) <= 0.25 for a, b in zip(spikes, want))
+
+
+@pytest.mark.gpu
+def test_gr_rows_against_oracle(built, tmp_path, ref_data):
+    """./main RKC 1 1 RKC 0 0 RKC 0 0 RKC 0 0: gr_v_by_RKC.csv holds t and
+    compartment 16 per macro step; libc rand() seeded 0 gives the oracle the
+    same initial voltage (first 2 ms, before the 50 ms pulse)."""
+    shutil.copy(os.path.join(ref_data, "Gr.txt"), tmp_path / "Gr.txt")
+    print("main: GR 1x1, RKC, 2 ms", flush=True)
+    r = run_main(["RKC", "1", "1", "RKC", "0", "0", "RKC", "0", "0", "RKC", "0", "0", "--ts", "2"], tmp_path,
+                 timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    rows = [l.rstrip("\n").rstrip(",").split(",") for l in open(tmp_path / "gr_v_by_RKC.csv")]
+    got = np.array([[float(x) for x in row] for row in rows])
+    assert got.shape == (16, 2)
+    from test_oracle_gr import gr_pop
+    o = gr_pop(ref_data, 1, 1, seed=0)
+    ref = []
+    for k in range(16):
+        o.set_current(k * 0.125)
+        o.advance()
+        ref.append(o.get(orc.VNEW)[16])
+    np.testing.assert_allclose(got[:, 1], ref, rtol=0, atol=2e-6)
+    assert (tmp_path / "comptime.csv").read_text().startswith("Gr,1,RKC,Go,0,RKC,Pkj,0,RKC,Io,0,RKC,CompTime,")
