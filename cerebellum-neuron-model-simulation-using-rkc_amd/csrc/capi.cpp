@@ -101,7 +101,10 @@ int create_io(const cb_pop_config &cfg, cb_pop **out) {
   p->engine = cfg.engine;
   p->comp = io::COMP;
   p->device = cfg.device;
-  p->eng.reset(make_global_io(host, sol, stim, poisson_of(cfg)));
+  if (cfg.solver == CB_SOLVER_CN)
+    p->eng.reset(make_cn_io(host, stim));
+  else
+    p->eng.reset(make_global_io(host, sol, stim, poisson_of(cfg)));
   *out = p.release();
   return CB_OK;
 }
@@ -144,7 +147,10 @@ int create_go(const cb_pop_config &cfg, cb_pop **out) {
   p->engine = cfg.engine;
   p->comp = go::COMP;
   p->device = cfg.device;
-  p->eng.reset(make_global_go(host, sol, stim, poisson_of(cfg)));
+  if (cfg.solver == CB_SOLVER_CN)
+    p->eng.reset(make_cn_go(host, stim));
+  else
+    p->eng.reset(make_global_go(host, sol, stim, poisson_of(cfg)));
   *out = p.release();
   return CB_OK;
 }
@@ -186,7 +192,10 @@ int create_gr(const cb_pop_config &cfg, cb_pop **out) {
   p->engine = cfg.engine;
   p->comp = gr::COMP;
   p->device = cfg.device;
-  p->eng.reset(make_global_gr(host, sol, stim, poisson_of(cfg)));
+  if (cfg.solver == CB_SOLVER_CN)
+    p->eng.reset(make_cn_gr(host, stim));
+  else
+    p->eng.reset(make_global_gr(host, sol, stim, poisson_of(cfg)));
   *out = p.release();
   return CB_OK;
 }
@@ -233,7 +242,12 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     *out = nullptr;
     if (cfg->nx < 0 || cfg->ny < 0) return fail(CB_E_ARG, "negative grid size");
     if (cfg->cell_type < CB_GR || cfg->cell_type > CB_IO) return fail(CB_E_ARG, "unknown cell type");
-    if (cfg->solver != CB_SOLVER_RKC) return fail(CB_E_UNSUP, "solver not provided by this build");
+    if (cfg->solver != CB_SOLVER_RKC && cfg->solver != CB_SOLVER_CN)
+      return fail(CB_E_UNSUP, "solver not provided by this build");
+    if (cfg->solver == CB_SOLVER_CN && cfg->engine != CB_ENGINE_GLOBAL_FP64)
+      return fail(CB_E_UNSUP, "the CN solver runs on the global FP64 engine");
+    if (cfg->solver == CB_SOLVER_CN && cfg->poisson_on)
+      return fail(CB_E_UNSUP, "the Poisson conductance drive is provided for the RKC solver");
     CB_HIP(hipSetDevice(cfg->device));
     if (cfg->cell_type == CB_IO) return create_io(*cfg, out);
     if (cfg->cell_type == CB_GO) return create_go(*cfg, out);
@@ -295,6 +309,11 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     p->engine = cfg->engine;
     p->comp = pkj::COMP;
     p->device = cfg->device;
+    if (cfg->solver == CB_SOLVER_CN) {
+      p->eng.reset(make_cn_pkj(host, stim));
+      *out = p.release();
+      return CB_OK;
+    }
     switch (cfg->engine) {
       case CB_ENGINE_GLOBAL_FP64:
         p->eng.reset(make_global_pkj(host, sol, stim, syn));

@@ -37,6 +37,7 @@ enum Field {
   F_FN = 8,       // RKC vec[fn]
   F_CA2 = 9,      // Golgi cells: the second (CaLVA) Ca pool
   F_CA_OLD = 10,  // Golgi cells: Ca at the previous ion update (KAHP scheme)
+  F_VOLD = 11,    // CN solver: V of the previous step (vec[cn_v_old])
   F_CTL = 50,     // per-cell engines: the per-cell controller records (ncell x 32 doubles)
   F_GATE0 = 100,  // + gate index
 };
@@ -107,6 +108,11 @@ Engine *make_global_pkj(const PkjHost &host, const SolverSpec &sol, const StimSp
 Engine *make_global_io(const IoHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);
 Engine *make_global_go(const GoHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);
 Engine *make_global_gr(const GrHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);
+// the Crank-Nicolson engines (solve_cnm.cu), FP64, population-wide CG / BiCG
+Engine *make_cn_pkj(const PkjHost &host, const StimSpec &stim);
+Engine *make_cn_io(const IoHost &host, const StimSpec &stim);
+Engine *make_cn_go(const GoHost &host, const StimSpec &stim);
+Engine *make_cn_gr(const GrHost &host, const StimSpec &stim);
 Engine *make_cell_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn,
                       bool mixed);
 

@@ -143,13 +143,16 @@ HD void cn(double &x, double inf, double tau, double DT) { x = (2.0 * DT * inf +
 // 23 C, then every voltage-gated state by Crank-Nicolson at the mid-step
 // voltage (c_KC at the mid-step Ca of the first pool).  cond holds density x
 // area, so the Ca current densities divide by the area again.
-HD void update_ion(double vold, double vnew, double area, const double *c, double &ca, double &ca2, double &rev,
-                   double DT, double *g) {
-  const double v_val = (vold + vnew) / 2.0;
+// The Ca currents are taken at v_ca and the gates move at (v_a + v_b) / 2:
+// the RKC path has v_ca = v_a = V before the update, v_b = the new V; the CN
+// path (go_update_ion_exp_imp, go_ion.cu:319-377) v_ca = v_b = V_old, v_a = V.
+HD void update_ion_at(double v_ca, double v_a, double v_b, double area, const double *c, double &ca, double &ca2,
+                      double &rev, double DT, double *g) {
+  const double v_val = (v_a + v_b) / 2.0;
   double Ca_val = ca;
   const double Ca2_val = ca2;
-  const double I_Ca1 = 1e-3 * c[g_CaHVA] / area * g[ch_CaHVA] * g[ch_CaHVA] * g[ci_CaHVA] * (vold - V_CA);
-  const double I_Ca2 = 1e-3 * c[g_CaLVA] / area * g[cl_CaLVA] * g[cl_CaLVA] * g[cm_CaLVA] * (vold - (rev));
+  const double I_Ca1 = 1e-3 * c[g_CaHVA] / area * g[ch_CaHVA] * g[ch_CaHVA] * g[ci_CaHVA] * (v_ca - V_CA);
+  const double I_Ca2 = 1e-3 * c[g_CaLVA] / area * g[cl_CaLVA] * g[cl_CaLVA] * g[cm_CaLVA] * (v_ca - (rev));
   const double den = 2.0 * FARADAY * SHELL1_D;
   double k1 = DT * (-I_Ca1 / den - B_CA1 * (Ca_val - CA1_0));
   double k2 = DT * (-I_Ca1 / den - B_CA1 * (Ca_val + k1 / 2.0 - CA1_0));
@@ -181,6 +184,11 @@ HD void update_ion(double vold, double vnew, double area, const double *c, doubl
   cn(g[hf_HCN2], inf_hf_HCN2(v_val), tau_hf_HCN2(v_val), DT);
   cn(g[hs_HCN1], inf_hs_HCN1(v_val), tau_hs_HCN1(v_val), DT);
   cn(g[hs_HCN2], inf_hs_HCN2(v_val), tau_hs_HCN2(v_val), DT);
+}
+
+HD void update_ion(double vold, double vnew, double area, const double *c, double &ca, double &ca2, double &rev,
+                   double DT, double *g) {
+  update_ion_at(vold, vold, vnew, area, c, ca, ca2, rev, DT, g);
 }
 
 // One Crank-Nicolson step of the KAHP scheme (go_ion.cu:88-213) on
@@ -278,6 +286,36 @@ HD double current(double v, double iext, double rev, const double *c, const doub
           + c[g_Kslow] * (v - V_K) * g[sl_Kslow]
           + c[g_HCN1] * (v - V_H) * (g[hf_HCN1] + g[hs_HCN1])
           + c[g_HCN2] * (v - V_H) * (g[hf_HCN2] + g[hs_HCN2]));
+}
+// CN path sums before halving (go_cnm_vec_initialize / go_update_matrix,
+// solve_cnm.cu:269-300, 320-351)
+HD void cn_ge(const double *c, const double *g, double rev, double iext, double &gs, double &es) {
+  gs = (c[g_leak]
+        + c[g_NaT] * g[m_NaT] * g[m_NaT] * g[m_NaT] * g[h_NaT]
+        + c[g_NaR] * g[r_NaR] * g[s_NaR]
+        + c[g_NaP] * g[p_NaP]
+        + c[g_CaHVA] * g[ch_CaHVA] * g[ch_CaHVA] * g[ci_CaHVA]
+        + c[g_CaLVA] * g[cl_CaLVA] * g[cl_CaLVA] * g[cm_CaLVA]
+        + c[g_KAHP] * (g[o1_KAHP] + g[o2_KAHP])
+        + c[g_KV] * g[n_KV] * g[n_KV] * g[n_KV] * g[n_KV]
+        + c[g_KA] * g[a_KA] * g[a_KA] * g[a_KA] * g[b_KA]
+        + c[g_KC] * g[c_KC]
+        + c[g_Kslow] * g[sl_Kslow]
+        + c[g_HCN1] * (g[hf_HCN1] + g[hs_HCN1])
+        + c[g_HCN2] * (g[hf_HCN2] + g[hs_HCN2]));
+  es = (c[g_leak] * V_LEAK + iext
+        + c[g_NaT] * V_NA * g[m_NaT] * g[m_NaT] * g[m_NaT] * g[h_NaT]
+        + c[g_NaR] * V_NA * g[r_NaR] * g[s_NaR]
+        + c[g_NaP] * V_NA * g[p_NaP]
+        + c[g_CaHVA] * V_CA * g[ch_CaHVA] * g[ch_CaHVA] * g[ci_CaHVA]
+        + c[g_CaLVA] * (rev) * g[cl_CaLVA] * g[cl_CaLVA] * g[cm_CaLVA]
+        + c[g_KAHP] * V_K * (g[o1_KAHP] + g[o2_KAHP])
+        + c[g_KV] * V_K * g[n_KV] * g[n_KV] * g[n_KV] * g[n_KV]
+        + c[g_KA] * V_K * g[a_KA] * g[a_KA] * g[a_KA] * g[b_KA]
+        + c[g_KC] * V_K * g[c_KC]
+        + c[g_Kslow] * V_K * g[sl_Kslow]
+        + c[g_HCN1] * V_H * (g[hf_HCN1] + g[hs_HCN1])
+        + c[g_HCN2] * V_H * (g[hf_HCN2] + g[hs_HCN2]));
 }
 #undef HD
 

@@ -244,6 +244,26 @@ HD double current(double v, double iext, const double *c, const double *g) {
           + c[g_KCa] * (v - V_K) * g[c_KCa]
           + c[g_KM] * (v - V_K) * g[s_KM]);
 }
+// CN path sums before halving (gr_cnm_vec_initialize / gr_update_matrix,
+// solve_cnm.cu:138-161, 179-200)
+HD void cn_ge(const double *c, const double *g, double iext, double &gs, double &es) {
+  gs = (c[g_leak1] + c[g_leak2] + c[g_leak3]
+        + c[g_Na] * g[o_Na]
+        + c[g_Ca] * g[ch_Ca] * g[ch_Ca] * g[ci_Ca]
+        + c[g_KV] * g[n_KV] * g[n_KV] * g[n_KV] * g[n_KV]
+        + c[g_KIR] * g[ir_KIR]
+        + c[g_KA] * g[a_KA] * g[a_KA] * g[a_KA] * g[b_KA]
+        + c[g_KCa] * g[c_KCa]
+        + c[g_KM] * g[s_KM]);
+  es = (c[g_leak1] * V_LEAK1 + c[g_leak2] * V_LEAK2 + c[g_leak3] * V_LEAK3 + iext
+        + c[g_Na] * V_NA * g[o_Na]
+        + c[g_Ca] * V_CA * g[ch_Ca] * g[ch_Ca] * g[ci_Ca]
+        + c[g_KV] * V_K * g[n_KV] * g[n_KV] * g[n_KV] * g[n_KV]
+        + c[g_KIR] * V_K * g[ir_KIR]
+        + c[g_KA] * V_K * g[a_KA] * g[a_KA] * g[a_KA] * g[b_KA]
+        + c[g_KCa] * V_K * g[c_KCa]
+        + c[g_KM] * V_K * g[s_KM]);
+}
 #undef HD
 
 }  // namespace gr

@@ -74,14 +74,15 @@ template <typename T> __host__ __device__ inline void cn(T &x, T inf, T tau, T d
   x = (T(2) * dt * inf + (T(2) * tau - dt) * x) / (T(2) * tau + dt);
 }
 
-// io_update_ion_RKC (io_ion.cu:90-127): Ca RK4 with the Ca current at the old
-// voltage and gate, then every gate by Crank-Nicolson at the mid-step voltage
-// (m_Na instantaneous, s_KCa at the mid-step Ca).
+// Ca RK4 with the Ca current at v_ca and the present gate, then every gate by
+// Crank-Nicolson at v_val (m_Na instantaneous, s_KCa at the mid-step Ca).
+// io_update_ion_RKC (io_ion.cu:90-127) uses v_ca = V before the update and
+// v_val = the mid-step voltage; io_update_ion_2nd (:53-89, the CN path)
+// v_ca = V and v_val = 1.5 V - 0.5 V_old.
 template <typename T>
-__host__ __device__ inline void update_ion(T vold, T vnew, T &ca, T gCaH, int comp, T DT, T *g) {
-  const T v_val = T(0.5) * (vnew + vold);
+__host__ __device__ inline void update_ion_at(T v_ca, T v_val, T &ca, T gCaH, int comp, T DT, T *g) {
   T Ca_val = ca;
-  const T I_Ca1 = gCaH * g[r_CaH] * g[r_CaH] * (vold - T(V_CA));
+  const T I_Ca1 = gCaH * g[r_CaH] * g[r_CaH] * (v_ca - T(V_CA));
   const T k1 = DT * (-I_Ca1 * T(CA_PH) - T(BETA_CA) * (Ca_val));
   const T k2 = DT * (-I_Ca1 * T(CA_PH) - T(BETA_CA) * (Ca_val + k1 / T(2)));
   const T k3 = DT * (-I_Ca1 * T(CA_PH) - T(BETA_CA) * (Ca_val + k2 / T(2)));
@@ -98,6 +99,32 @@ __host__ __device__ inline void update_ion(T vold, T vnew, T &ca, T gCaH, int co
   cn(g[s_KCa], s_inf(Ca_val), tau_s(Ca_val), DT);
   g[m_Na] = m_inf(v_val);
   cn(g[h_Na], h_inf(v_val, comp), tau_h(v_val, comp), DT);
+}
+
+template <typename T>
+__host__ __device__ inline void update_ion(T vold, T vnew, T &ca, T gCaH, int comp, T DT, T *g) {
+  update_ion_at(vold, T(0.5) * (vnew + vold), ca, gCaH, comp, DT, g);
+}
+
+// CN path sums before halving (io_cnm_vec_initialize / io_update_matrix,
+// solve_cnm.cu:751-768, 793-810)
+__host__ __device__ inline void cn_ge(const double *c, const double *g, double iext, double &gs, double &es) {
+  gs = (c[g_leak]
+        + c[g_CaL] * g[k_CaL] * g[k_CaL] * g[k_CaL] * g[l_CaL]
+        + c[g_Na] * g[m_Na] * g[m_Na] * g[m_Na] * g[h_Na]
+        + c[g_Kdr] * g[n_Kdr] * g[p_Kdr]
+        + c[g_K] * g[x_K] * g[x_K] * g[x_K] * g[x_K]
+        + c[g_CaH] * g[r_CaH] * g[r_CaH]
+        + c[g_KCa] * g[s_KCa]
+        + c[g_H] * g[q_H]);
+  es = (c[g_leak] * V_LEAK + iext
+        + c[g_CaL] * V_CA * g[k_CaL] * g[k_CaL] * g[k_CaL] * g[l_CaL]
+        + c[g_Na] * V_NA * g[m_Na] * g[m_Na] * g[m_Na] * g[h_Na]
+        + c[g_Kdr] * V_K * g[n_Kdr] * g[p_Kdr]
+        + c[g_K] * V_K * g[x_K] * g[x_K] * g[x_K] * g[x_K]
+        + c[g_CaH] * V_CA * g[r_CaH] * g[r_CaH]
+        + c[g_KCa] * V_K * g[s_KCa]
+        + c[g_H] * V_H * g[q_H]);
 }
 
 // io_update_matrix (solve_rkc.cu:412-434), terms in the reference's order

@@ -17,8 +17,10 @@
 //                    fn refresh, spectral-radius floor)
 //   --chunk K        macro steps per device launch between host writes (64)
 //
-// This build provides all four populations with the RKC solver; asking for
-// another solver prints an error and exits with status 1.
+// This build provides all four populations with the RKC and CN solvers;
+// asking for BE prints an error and exits with status 1.  Each population
+// advances one H_MAX = 0.125 ms output interval per row, in sub-steps of its
+// solver's DT (RKC: one step of H_MAX; CN: 0.025 ms, main.cu:193-200).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -42,6 +44,9 @@ int solver_of(const char *s) {
   if (std::strncmp(s, "RKC", 3) == 0) return CB_SOLVER_RKC;
   return -1;
 }
+
+// a solver's step: one RKC call covers H_MAX, CN and BE step 0.025 ms (param.h:10-12)
+double dt_of(int solver) { return solver == CB_SOLVER_RKC ? H_MAX : 0.025; }
 
 int check(int rc, const char *what) {
   if (rc != CB_OK) {
@@ -99,7 +104,7 @@ int main(int argc, char *argv[]) {
     ny[k] = std::atoi(argv[kSolverArg[k] + 2]);
   }
   for (int k = 0; k < 4; k++) {
-    if (nx[k] * ny[k] > 0 && solver[k] != CB_SOLVER_RKC) {
+    if (nx[k] * ny[k] > 0 && solver[k] == CB_SOLVER_BE) {
       std::printf("Error!! -> %s population with the %s solver is not provided by this build\n", kTypeNames[k],
                   kSolverNames[solver[k]]);
       return 1;
@@ -114,19 +119,19 @@ int main(int argc, char *argv[]) {
   std::srand(0);
   const int ngr = nx[CB_GR] * ny[CB_GR];
   cb_pop *grp = nullptr;
-  if (ngr > 0) check(cb_gr_initialize(nx[CB_GR], ny[CB_GR], "RKC", &grp), "gr_initialize");
+  if (ngr > 0) check(cb_gr_initialize(nx[CB_GR], ny[CB_GR], argv[1], &grp), "gr_initialize");
   std::vector<double> gr_v(ngr > 0 ? (size_t)ngr * 578 : 0), gr_prev(ngr, 100.0);
   const int ngo = nx[CB_GO] * ny[CB_GO];
   cb_pop *gop = nullptr;
-  if (ngo > 0) check(cb_go_initialize(nx[CB_GO], ny[CB_GO], "RKC", &gop), "go_initialize");
+  if (ngo > 0) check(cb_go_initialize(nx[CB_GO], ny[CB_GO], argv[4], &gop), "go_initialize");
   const int n = nx[CB_PKJ] * ny[CB_PKJ];
   cb_pop *pkj = nullptr;
   int comp = 0;
   if (n > 0) {
     cb_pop_config cfg;
-    check(cb_config_default(&cfg, CB_PKJ, CB_SOLVER_RKC, nx[CB_PKJ], ny[CB_PKJ]), "config");
-    cfg.engine = engine;
-    if (stable) {
+    check(cb_config_default(&cfg, CB_PKJ, solver[CB_PKJ], nx[CB_PKJ], ny[CB_PKJ]), "config");
+    if (solver[CB_PKJ] == CB_SOLVER_RKC) cfg.engine = engine;
+    if (stable && solver[CB_PKJ] == CB_SOLVER_RKC) {
       cfg.ion_interval_ms = std::ldexp(1.0, -8);
       cfg.refresh_fn = 1;
       cfg.rho_floor = 1;
@@ -140,7 +145,7 @@ int main(int argc, char *argv[]) {
   // as io_initialize follows pkj_initialize in main.cu:125-137
   const int nio = nx[CB_IO] * ny[CB_IO];
   cb_pop *iop = nullptr;
-  if (nio > 0) check(cb_io_initialize(nx[CB_IO], ny[CB_IO], "RKC", &iop), "io_initialize");
+  if (nio > 0) check(cb_io_initialize(nx[CB_IO], ny[CB_IO], argv[10], &iop), "io_initialize");
   std::vector<double> io_v((size_t)nio * 3), io_prev(nio, 100.0);
   std::printf("\n");
 
@@ -159,8 +164,10 @@ int main(int argc, char *argv[]) {
     // the upward crossing of +10 mV at compartment 77
     for (int s = 0; s < k && ngr > 0; s++) {
       const double ts_ = t + (s + 1) * H_MAX;
-      check(cb_gr_set_current(grp, ts_ - H_MAX), "gr_set_current");
-      check(cb_gr_solve_update_v(grp), "gr_solve_update_v");
+      for (double l_dt = 0.0; l_dt < H_MAX; l_dt += dt_of(solver[CB_GR])) {
+        check(cb_gr_set_current(grp, ts_ - H_MAX + l_dt), "gr_set_current");
+        check(cb_gr_solve_update_v(grp), "gr_solve_update_v");
+      }
       check(cb_pop_read(grp, CB_F_VNEW, gr_v.data()), "gr read");
       if (std::isnan(gr_v[1])) {
         std::printf("isnan in gr, time=%f\n", ts_ - H_MAX);
@@ -227,8 +234,10 @@ int main(int argc, char *argv[]) {
     // raster on compartment 2 through +10 mV
     for (int s = 0; s < k && nio > 0; s++) {
       const double ts_ = t + (s + 1) * H_MAX;
-      check(cb_io_set_current(iop, ts_ - H_MAX), "io_set_current");
-      check(cb_io_solve_update_v(iop), "io_solve_update_v");
+      for (double l_dt = 0.0; l_dt < H_MAX; l_dt += dt_of(solver[CB_IO])) {
+        check(cb_io_set_current(iop, ts_ - H_MAX + l_dt), "io_set_current");
+        check(cb_io_solve_update_v(iop), "io_solve_update_v");
+      }
       check(cb_pop_read(iop, CB_F_VNEW, io_v.data()), "io read");
       if (std::isnan(io_v[1])) {
         std::printf("isnan,time=%f\n", ts_ - H_MAX);

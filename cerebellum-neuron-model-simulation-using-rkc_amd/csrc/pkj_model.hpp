@@ -293,5 +293,37 @@ __host__ __device__ inline T current(T v, T iext, T rev, const T *c, const T *g)
           + c[g_K2] * g[m_K2] * g[z_K2] * g[z_K2] * (v - T(V_K)));
 }
 
+// CN path: the ionic conductance sum and the conductance-weighted reversal
+// sum plus the injected current, before halving, in the reference's term
+// order (pkj_cnm_vec_initialize / pkj_update_matrix, solve_cnm.cu:429-456,
+// 480-509)
+__host__ __device__ inline void cn_ge(const double *c, const double *g, double rev, double iext, double &gs,
+                                      double &es) {
+  gs = (+c[g_leak]
+        + c[g_NaF] * g[m_NaF] * g[m_NaF] * g[m_NaF] * g[h_NaF]
+        + c[g_NaP] * g[m_NaP] * g[m_NaP] * g[m_NaP]
+        + c[g_CaP] * g[m_CaP] * g[h_CaP]
+        + c[g_CaT] * g[m_CaT] * g[h_CaT]
+        + c[g_Kh] * g[m_Kh1]
+        + c[g_Kh] * g[m_Kh2]
+        + c[g_Kdr] * g[m_Kdr] * g[m_Kdr] * g[h_Kdr]
+        + c[g_KM] * g[m_KM]
+        + c[g_KA] * g[m_KA] * g[m_KA] * g[m_KA] * g[m_KA] * g[h_KA]
+        + c[g_KC] * g[m_KC] * g[z_KC] * g[z_KC]
+        + c[g_K2] * g[m_K2] * g[z_K2] * g[z_K2]);
+  es = (+c[g_leak] * (V_LEAK) + iext
+        + c[g_NaF] * g[m_NaF] * g[m_NaF] * g[m_NaF] * g[h_NaF] * (V_NA)
+        + c[g_NaP] * g[m_NaP] * g[m_NaP] * g[m_NaP] * (V_NA)
+        + c[g_CaP] * g[m_CaP] * g[h_CaP] * (rev)
+        + c[g_CaT] * g[m_CaT] * g[h_CaT] * (rev)
+        + c[g_Kh] * g[m_Kh1] * (V_KH)
+        + c[g_Kh] * g[m_Kh2] * (V_KH)
+        + c[g_Kdr] * g[m_Kdr] * g[m_Kdr] * g[h_Kdr] * (V_K)
+        + c[g_KM] * g[m_KM] * (V_K)
+        + c[g_KA] * g[m_KA] * g[m_KA] * g[m_KA] * g[m_KA] * g[h_KA] * (V_K)
+        + c[g_KC] * g[m_KC] * g[z_KC] * g[z_KC] * (V_K)
+        + c[g_K2] * g[m_K2] * g[z_K2] * g[z_K2] * (V_K));
+}
+
 }  // namespace pkj
 }  // namespace cbrkc

@@ -51,6 +51,7 @@ enum {
   CB_F_FN = 8,
   CB_F_CA2 = 9,    /* Golgi cells: second (CaLVA) Ca pool (go->ca2) */
   CB_F_CA_OLD = 10, /* Golgi cells: Ca at the previous ion update (go->ca_old) */
+  CB_F_VOLD = 11,  /* CN solver: V of the previous step (vec[cn_v_old]) */
   CB_F_GATE0 = 100 /* + gate index in the cell type's gate order */
 };
 

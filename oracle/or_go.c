@@ -14,6 +14,8 @@
  *   go_update_matrix          solve_rkc.cu:288-314
  *   go_update_ion_and_matrix  solve_rkc.cu:1163-1182 (ion update, KAHP, then b)
  *   go_set_current            go.cu:258-268
+ *   go_update_ion_exp_imp     go_ion.cu:319-377 (CN path)
+ *   CN conductance sums       solve_cnm.cu:259-362
  *
  * Reference quirks kept: the Ca currents of the update divide the stored
  * conductance (density x area) by the area again; the second pool's current
@@ -454,51 +456,56 @@ void or_go_update_b(or_pop *p, const double *vv)
 #define CNU(gt, INF, TAU) \
   g[gt][id] = (2.0 * DT * (INF) + (2.0 * (TAU) - DT) * g[gt][id]) / (2.0 * (TAU) + DT)
 
-/* go_update_ion_RKC_exp_imp (go_ion.cu:379-437) then go_KAHP_update_2order
-   (go_ion.cu:88-213), as go_update_ion_and_matrix runs them (solve_rkc.cu:1174-1177) */
-void or_go_update_ion(or_pop *p, const double *vnew, double DT)
+/* both Ca pools, the reversal and the gates of one compartment: Ca currents
+   at v_ca, gates at v_val = (p->v + v_other) / 2 (go_update_ion_RKC_exp_imp,
+   go_ion.cu:379-437, with v_ca = elem[v] and v_other = vnew;
+   go_update_ion_exp_imp, :319-377, with v_ca = v_other = v_old) */
+static void go_ion_at(or_pop *p, int id, double v_ca, double v_other, double DT)
 {
   double **g = p->ion, **c = p->cond;
-  for (int id = 0; id < p->nc; id++) {
-    const double v_val = (p->v[id] + vnew[id]) / 2.0;
-    double Ca_val = p->ca[id];
-    const double Ca2_val = p->ca2[id];
-    const double I_Ca1 = 1e-3 * c[g_CaHVA][id] / p->area[id] * g[ch_CaHVA][id] * g[ch_CaHVA][id] * g[ci_CaHVA][id] *
-                         (p->v[id] - V_Ca_GO);
-    const double I_Ca2 = 1e-3 * c[g_CaLVA][id] / p->area[id] * g[cl_CaLVA][id] * g[cl_CaLVA][id] * g[cm_CaLVA][id] *
-                         (p->v[id] - (p->rev[id]));
-    double k1 = DT * (-I_Ca1 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca_val - Ca1_0_GO));
-    double k2 = DT * (-I_Ca1 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca_val + k1 / 2.0 - Ca1_0_GO));
-    double k3 = DT * (-I_Ca1 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca_val + k2 / 2.0 - Ca1_0_GO));
-    double k4 = DT * (-I_Ca1 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca_val + k3 - Ca1_0_GO));
-    p->ca[id] += (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
-    k1 = DT * (-I_Ca2 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca2_val - Ca1_0_GO));
-    k2 = DT * (-I_Ca2 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca2_val + k1 / 2.0 - Ca1_0_GO));
-    k3 = DT * (-I_Ca2 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca2_val + k2 / 2.0 - Ca1_0_GO));
-    k4 = DT * (-I_Ca2 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca2_val + k3 - Ca1_0_GO));
-    p->ca2[id] += (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
-    p->rev[id] = (1e3) * (8.313424 * (23.0 + 273.15)) / (2 * F_GO) * log(Ca1OUT_GO / p->ca2[id]);
-    Ca_val = (Ca_val + p->ca[id]) / 2.0;
-    CNU(m_NaT, inf_m_NaT(v_val), tau_m_NaT(v_val));
-    CNU(h_NaT, inf_h_NaT(v_val), tau_h_NaT(v_val));
-    CNU(r_NaR, inf_r_NaR(v_val), tau_r_NaR(v_val));
-    CNU(s_NaR, inf_s_NaR(v_val), tau_s_NaR(v_val));
-    CNU(p_NaP, inf_p_NaP(v_val), tau_p_NaP(v_val));
-    CNU(n_KV, inf_n_KV(v_val), tau_n_KV(v_val));
-    CNU(a_KA, inf_a_KA(v_val), tau_a_KA(v_val));
-    CNU(b_KA, inf_b_KA(v_val), tau_b_KA(v_val));
-    CNU(sl_Kslow, inf_sl_Kslow(v_val), tau_sl_Kslow(v_val));
-    CNU(c_KC, inf_c_KC(v_val, Ca_val), tau_c_KC(v_val, Ca_val));
-    CNU(ch_CaHVA, inf_ch_CaHVA(v_val), tau_ch_CaHVA(v_val));
-    CNU(ci_CaHVA, inf_ci_CaHVA(v_val), tau_ci_CaHVA(v_val));
-    CNU(cl_CaLVA, inf_cl_CaLVA(v_val), tau_cl_CaLVA(v_val));
-    CNU(cm_CaLVA, inf_cm_CaLVA(v_val), tau_cm_CaLVA(v_val));
-    CNU(hf_HCN1, inf_hf_HCN1(v_val), tau_hf_HCN1(v_val));
-    CNU(hf_HCN2, inf_hf_HCN2(v_val), tau_hf_HCN2(v_val));
-    CNU(hs_HCN1, inf_hs_HCN1(v_val), tau_hs_HCN1(v_val));
-    CNU(hs_HCN2, inf_hs_HCN2(v_val), tau_hs_HCN2(v_val));
-    p->v[id] = vnew[id];
-  }
+  const double v_val = (p->v[id] + v_other) / 2.0;
+  double Ca_val = p->ca[id];
+  const double Ca2_val = p->ca2[id];
+  const double I_Ca1 = 1e-3 * c[g_CaHVA][id] / p->area[id] * g[ch_CaHVA][id] * g[ch_CaHVA][id] * g[ci_CaHVA][id] *
+                       (v_ca - V_Ca_GO);
+  const double I_Ca2 = 1e-3 * c[g_CaLVA][id] / p->area[id] * g[cl_CaLVA][id] * g[cl_CaLVA][id] * g[cm_CaLVA][id] *
+                       (v_ca - (p->rev[id]));
+  double k1 = DT * (-I_Ca1 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca_val - Ca1_0_GO));
+  double k2 = DT * (-I_Ca1 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca_val + k1 / 2.0 - Ca1_0_GO));
+  double k3 = DT * (-I_Ca1 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca_val + k2 / 2.0 - Ca1_0_GO));
+  double k4 = DT * (-I_Ca1 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca_val + k3 - Ca1_0_GO));
+  p->ca[id] += (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
+  k1 = DT * (-I_Ca2 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca2_val - Ca1_0_GO));
+  k2 = DT * (-I_Ca2 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca2_val + k1 / 2.0 - Ca1_0_GO));
+  k3 = DT * (-I_Ca2 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca2_val + k2 / 2.0 - Ca1_0_GO));
+  k4 = DT * (-I_Ca2 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca2_val + k3 - Ca1_0_GO));
+  p->ca2[id] += (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
+  p->rev[id] = (1e3) * (8.313424 * (23.0 + 273.15)) / (2 * F_GO) * log(Ca1OUT_GO / p->ca2[id]);
+  Ca_val = (Ca_val + p->ca[id]) / 2.0;
+  CNU(m_NaT, inf_m_NaT(v_val), tau_m_NaT(v_val));
+  CNU(h_NaT, inf_h_NaT(v_val), tau_h_NaT(v_val));
+  CNU(r_NaR, inf_r_NaR(v_val), tau_r_NaR(v_val));
+  CNU(s_NaR, inf_s_NaR(v_val), tau_s_NaR(v_val));
+  CNU(p_NaP, inf_p_NaP(v_val), tau_p_NaP(v_val));
+  CNU(n_KV, inf_n_KV(v_val), tau_n_KV(v_val));
+  CNU(a_KA, inf_a_KA(v_val), tau_a_KA(v_val));
+  CNU(b_KA, inf_b_KA(v_val), tau_b_KA(v_val));
+  CNU(sl_Kslow, inf_sl_Kslow(v_val), tau_sl_Kslow(v_val));
+  CNU(c_KC, inf_c_KC(v_val, Ca_val), tau_c_KC(v_val, Ca_val));
+  CNU(ch_CaHVA, inf_ch_CaHVA(v_val), tau_ch_CaHVA(v_val));
+  CNU(ci_CaHVA, inf_ci_CaHVA(v_val), tau_ci_CaHVA(v_val));
+  CNU(cl_CaLVA, inf_cl_CaLVA(v_val), tau_cl_CaLVA(v_val));
+  CNU(cm_CaLVA, inf_cm_CaLVA(v_val), tau_cm_CaLVA(v_val));
+  CNU(hf_HCN1, inf_hf_HCN1(v_val), tau_hf_HCN1(v_val));
+  CNU(hf_HCN2, inf_hf_HCN2(v_val), tau_hf_HCN2(v_val));
+  CNU(hs_HCN1, inf_hs_HCN1(v_val), tau_hs_HCN1(v_val));
+  CNU(hs_HCN2, inf_hs_HCN2(v_val), tau_hs_HCN2(v_val));
+}
+
+/* go_KAHP_update_2order over every compartment (go_ion.cu:88-213) */
+static void go_kahp_all(or_pop *p, double DT)
+{
+  double **g = p->ion;
   for (int id = 0; id < p->nc; id++) {
     const double l_ca_old = p->ca_old[id];
     p->ca_old[id] = p->ca[id];
@@ -507,4 +514,59 @@ void or_go_update_ion(or_pop *p, const double *vnew, double DT)
     g[c1_KAHP][id] = st[0]; g[c2_KAHP][id] = st[1]; g[c3_KAHP][id] = st[2];
     g[c4_KAHP][id] = st[3]; g[o1_KAHP][id] = st[4]; g[o2_KAHP][id] = st[5];
   }
+}
+
+/* go_update_ion_RKC_exp_imp (go_ion.cu:379-437) then go_KAHP_update_2order
+   (go_ion.cu:88-213), as go_update_ion_and_matrix runs them (solve_rkc.cu:1174-1177) */
+void or_go_update_ion(or_pop *p, const double *vnew, double DT)
+{
+  for (int id = 0; id < p->nc; id++) {
+    go_ion_at(p, id, p->v[id], vnew[id], DT);
+    p->v[id] = vnew[id];
+  }
+  go_kahp_all(p, DT);
+}
+
+/* go_update_ion_exp_imp (go_ion.cu:319-377) then the KAHP step, the CN path
+   (go_solve_by_cnm, solve_cnm.cu:371-375) */
+void or_go_cn_ion(or_pop *p, double DT)
+{
+  for (int id = 0; id < p->nc; id++) {
+    go_ion_at(p, id, p->cn_vold[id], p->cn_vold[id], DT);
+    p->cn_vold[id] = p->v[id];
+  }
+  go_kahp_all(p, DT);
+}
+
+/* conductance and reversal sums of the CN path before halving
+   (go_cnm_vec_initialize / go_update_matrix, solve_cnm.cu:269-300, 320-351) */
+void or_go_cn_ge(const or_pop *p, int id, double *gsum, double *esum)
+{
+  double *const *c = p->cond, *const *g = p->ion;
+  *gsum = (c[g_leak][id]
+           + c[g_NaT][id] * g[m_NaT][id] * g[m_NaT][id] * g[m_NaT][id] * g[h_NaT][id]
+           + c[g_NaR][id] * g[r_NaR][id] * g[s_NaR][id]
+           + c[g_NaP][id] * g[p_NaP][id]
+           + c[g_CaHVA][id] * g[ch_CaHVA][id] * g[ch_CaHVA][id] * g[ci_CaHVA][id]
+           + c[g_CaLVA][id] * g[cl_CaLVA][id] * g[cl_CaLVA][id] * g[cm_CaLVA][id]
+           + c[g_KAHP][id] * (g[o1_KAHP][id] + g[o2_KAHP][id])
+           + c[g_KV][id] * g[n_KV][id] * g[n_KV][id] * g[n_KV][id] * g[n_KV][id]
+           + c[g_KA][id] * g[a_KA][id] * g[a_KA][id] * g[a_KA][id] * g[b_KA][id]
+           + c[g_KC][id] * g[c_KC][id]
+           + c[g_Kslow][id] * g[sl_Kslow][id]
+           + c[g_HCN1][id] * (g[hf_HCN1][id] + g[hs_HCN1][id])
+           + c[g_HCN2][id] * (g[hf_HCN2][id] + g[hs_HCN2][id]));
+  *esum = (c[g_leak][id] * V_LEAK_GO + p->iext[id]
+           + c[g_NaT][id] * V_Na_GO * g[m_NaT][id] * g[m_NaT][id] * g[m_NaT][id] * g[h_NaT][id]
+           + c[g_NaR][id] * V_Na_GO * g[r_NaR][id] * g[s_NaR][id]
+           + c[g_NaP][id] * V_Na_GO * g[p_NaP][id]
+           + c[g_CaHVA][id] * V_Ca_GO * g[ch_CaHVA][id] * g[ch_CaHVA][id] * g[ci_CaHVA][id]
+           + c[g_CaLVA][id] * (p->rev[id]) * g[cl_CaLVA][id] * g[cl_CaLVA][id] * g[cm_CaLVA][id]
+           + c[g_KAHP][id] * V_K_GO * (g[o1_KAHP][id] + g[o2_KAHP][id])
+           + c[g_KV][id] * V_K_GO * g[n_KV][id] * g[n_KV][id] * g[n_KV][id] * g[n_KV][id]
+           + c[g_KA][id] * V_K_GO * g[a_KA][id] * g[a_KA][id] * g[a_KA][id] * g[b_KA][id]
+           + c[g_KC][id] * V_K_GO * g[c_KC][id]
+           + c[g_Kslow][id] * V_K_GO * g[sl_Kslow][id]
+           + c[g_HCN1][id] * V_H_GO * (g[hf_HCN1][id] + g[hs_HCN1][id])
+           + c[g_HCN2][id] * V_H_GO * (g[hf_HCN2][id] + g[hs_HCN2][id]));
 }

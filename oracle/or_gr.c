@@ -15,6 +15,8 @@
  *   gr_update_matrix           solve_rkc.cu:226-249
  *   gr_update_ion_and_matrix   solve_rkc.cu:1100-1119 (Na scheme, gates, then b)
  *   gr_set_current             gr.cu:245-254
+ *   gr_update_ion_exp_imp      gr_ion.cu:438-459 (CN path)
+ *   CN conductance sums        solve_cnm.cu:128-208
  *
  * Reference quirks kept: the Na scheme's explicit half uses the voltage of
  * the previous ion update and its implicit half the new one; Ca stays at its
@@ -358,10 +360,11 @@ void or_gr_update_b(or_pop *p, const double *vv)
 #define CNU(gt, INF, TAU) \
   g[gt][id] = (2.0 * DT * (INF) + (2.0 * (TAU) - DT) * g[gt][id]) / (2.0 * (TAU) + DT)
 
-/* gr_Na_update_2order (gr_ion.cu:45-207) then gr_update_ion_RKC_exp_imp
-   (gr_ion.cu:461-480), as gr_update_ion_and_matrix runs them
-   (solve_rkc.cu:1111-1115) */
-void or_gr_update_ion(or_pop *p, const double *vnew, double DT)
+/* Na scheme from v_na_old to v_na_new, then the gates at (p->v + v_other) / 2
+   (gr_update_ion_and_matrix, solve_rkc.cu:1111-1115, with v_na_old = elem[v],
+   v_na_new = v_other = vnew; gr_solve_by_cnm, solve_cnm.cu:221-226, with
+   v_na_old = v_other = v_old, v_na_new = elem[v]) */
+static void gr_ion_all(or_pop *p, const double *v_na_old, const double *v_na_new, const double *v_other, double DT)
 {
   double **g = p->ion;
   for (int id = 0; id < p->nc; id++) {
@@ -369,13 +372,13 @@ void or_gr_update_ion(or_pop *p, const double *vnew, double DT)
     if (t < GR_hill) continue;
     double x[12] = {g[c1_Na][id], g[c2_Na][id], g[c3_Na][id], g[c4_Na][id], g[c5_Na][id], g[i1_Na][id],
                     g[i2_Na][id], g[i3_Na][id], g[i4_Na][id], g[i5_Na][id], g[i6_Na][id], g[o_Na][id]};
-    if (na_step(t, p->v[id], vnew[id], DT, x) != 0) continue;
+    if (na_step(t, v_na_old[id], v_na_new[id], DT, x) != 0) continue;
     g[c1_Na][id] = x[0]; g[c2_Na][id] = x[1]; g[c3_Na][id] = x[2]; g[c4_Na][id] = x[3]; g[c5_Na][id] = x[4];
     g[i1_Na][id] = x[5]; g[i2_Na][id] = x[6]; g[i3_Na][id] = x[7]; g[i4_Na][id] = x[8]; g[i5_Na][id] = x[9];
     g[i6_Na][id] = x[10]; g[o_Na][id] = x[11];
   }
   for (int id = 0; id < p->nc; id++) {
-    const double v_val = (p->v[id] + vnew[id]) / 2.0;
+    const double v_val = (p->v[id] + v_other[id]) / 2.0;
     const double Ca_val = p->ca[id];
     CNU(n_KV, inf_n_KV(v_val), tau_n_KV(v_val));
     CNU(a_KA, inf_a_KA(v_val), tau_a_KA(v_val));
@@ -383,6 +386,45 @@ void or_gr_update_ion(or_pop *p, const double *vnew, double DT)
     CNU(ir_KIR, inf_ir_KIR(v_val), tau_ir_KIR(v_val));
     CNU(s_KM, inf_s_KM(v_val), tau_s_KM(v_val));
     CNU(c_KCa, inf_c_KCa(v_val, Ca_val), tau_c_KCa(v_val, Ca_val));
-    p->v[id] = vnew[id];
   }
+}
+
+/* gr_Na_update_2order (gr_ion.cu:45-207) then gr_update_ion_RKC_exp_imp
+   (gr_ion.cu:461-480), as gr_update_ion_and_matrix runs them
+   (solve_rkc.cu:1111-1115) */
+void or_gr_update_ion(or_pop *p, const double *vnew, double DT)
+{
+  gr_ion_all(p, p->v, vnew, vnew, DT);
+  for (int id = 0; id < p->nc; id++) p->v[id] = vnew[id];
+}
+
+/* the CN path: gr_Na_update_2order from v_old to v, then gr_update_ion_exp_imp
+   (gr_ion.cu:438-459) */
+void or_gr_cn_ion(or_pop *p, double DT)
+{
+  gr_ion_all(p, p->cn_vold, p->v, p->cn_vold, DT);
+  for (int id = 0; id < p->nc; id++) p->cn_vold[id] = p->v[id];
+}
+
+/* conductance and reversal sums of the CN path before halving
+   (gr_cnm_vec_initialize / gr_update_matrix, solve_cnm.cu:138-161, 179-200) */
+void or_gr_cn_ge(const or_pop *p, int id, double *gsum, double *esum)
+{
+  double *const *c = p->cond, *const *g = p->ion;
+  *gsum = (c[g_leak1][id] + c[g_leak2][id] + c[g_leak3][id]
+           + c[g_Na][id] * g[o_Na][id]
+           + c[g_Ca][id] * g[ch_Ca][id] * g[ch_Ca][id] * g[ci_Ca][id]
+           + c[g_KV][id] * g[n_KV][id] * g[n_KV][id] * g[n_KV][id] * g[n_KV][id]
+           + c[g_KIR][id] * g[ir_KIR][id]
+           + c[g_KA][id] * g[a_KA][id] * g[a_KA][id] * g[a_KA][id] * g[b_KA][id]
+           + c[g_KCa][id] * g[c_KCa][id]
+           + c[g_KM][id] * g[s_KM][id]);
+  *esum = (c[g_leak1][id] * V_LEAK1_GR + c[g_leak2][id] * V_LEAK2_GR + c[g_leak3][id] * V_LEAK3_GR + p->iext[id]
+           + c[g_Na][id] * V_Na_GR * g[o_Na][id]
+           + c[g_Ca][id] * V_Ca_GR * g[ch_Ca][id] * g[ch_Ca][id] * g[ci_Ca][id]
+           + c[g_KV][id] * V_K_GR * g[n_KV][id] * g[n_KV][id] * g[n_KV][id] * g[n_KV][id]
+           + c[g_KIR][id] * V_K_GR * g[ir_KIR][id]
+           + c[g_KA][id] * V_K_GR * g[a_KA][id] * g[a_KA][id] * g[a_KA][id] * g[b_KA][id]
+           + c[g_KCa][id] * V_K_GR * g[c_KCa][id]
+           + c[g_KM][id] * V_K_GR * g[s_KM][id]);
 }

@@ -13,6 +13,8 @@
  *   io_gap_initialize       gap.cu:4-133  (4-neighbour grid on compartment 0)
  *   io_gap_update           gap.cu:149-160, add_io_gap_val solve_rkc.cu:400-410
  *   io_set_current          io.cu:247-257
+ *   io_update_ion_2nd       io_ion.cu:53-89 (CN path)
+ *   CN conductance sums     solve_cnm.cu:741-821, gap term :729-739
  *
  * Reference quirks kept: the grid row of a cell is id / ny (gap.cu:11), so
  * non-square grids get the reference's neighbour sets; the dendrite's Na
@@ -305,31 +307,88 @@ void or_io_update_b(or_pop *p, const double *vv, int add_gap)
 #define CNU(gt, INF, TAU) \
   g[gt][id] = (2.0 * DT * (INF) + (2.0 * (TAU) - DT) * g[gt][id]) / (2.0 * (TAU) + DT)
 
-/* io_update_ion_RKC, io_ion.cu:90-127 */
-void or_io_update_ion(or_pop *p, const double *vnew, double DT)
+/* Ca and gates of one compartment at the gate voltage v_val; the Ca current
+   uses the stored voltage p->v (io_update_ion_RKC, io_ion.cu:90-127, and
+   io_update_ion_2nd, :53-89, differ only in v_val) */
+static void io_ion_at(or_pop *p, int id, double v_val, double DT)
 {
   double **g = p->ion;
+  int l_comp = p->compart[id];
+  double Ca_val = p->ca[id];
+  double I_Ca1 = p->cond[g_CaH][id] * g[r_CaH][id] * g[r_CaH][id] * (p->v[id] - V_Ca_IO);
+  double k1 = DT * (-I_Ca1 * CA_PH_IO - BETA_CA_IO * (Ca_val));
+  double k2 = DT * (-I_Ca1 * CA_PH_IO - BETA_CA_IO * (Ca_val + k1 / 2.0));
+  double k3 = DT * (-I_Ca1 * CA_PH_IO - BETA_CA_IO * (Ca_val + k2 / 2.0));
+  double k4 = DT * (-I_Ca1 * CA_PH_IO - BETA_CA_IO * (Ca_val + k3));
+  p->ca[id] += (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
+  Ca_val = (Ca_val + p->ca[id]) / 2.0;
+  CNU(k_CaL, k_infty(v_val), tau_k(v_val));
+  CNU(l_CaL, l_infty(v_val), tau_l(v_val));
+  CNU(n_Kdr, n_infty(v_val), tau_n(v_val));
+  CNU(p_Kdr, p_infty(v_val), tau_p(v_val));
+  CNU(x_K, x_infty(v_val), tau_x(v_val));
+  CNU(r_CaH, r_infty(v_val), tau_r(v_val));
+  CNU(q_H, q_infty(v_val), tau_q(v_val));
+  CNU(s_KCa, s_infty(Ca_val), tau_s(Ca_val));
+  g[m_Na][id] = m_infty(v_val);
+  CNU(h_Na, h_infty(v_val, l_comp), tau_h(v_val, l_comp));
+}
+
+/* io_update_ion_RKC, io_ion.cu:90-127: gates at the mid-step voltage */
+void or_io_update_ion(or_pop *p, const double *vnew, double DT)
+{
   for (int id = 0; id < p->nc; id++) {
-    int l_comp = p->compart[id];
-    double v_val = 0.5 * (vnew[id] + p->v[id]);
-    double Ca_val = p->ca[id];
-    double I_Ca1 = p->cond[g_CaH][id] * g[r_CaH][id] * g[r_CaH][id] * (p->v[id] - V_Ca_IO);
-    double k1 = DT * (-I_Ca1 * CA_PH_IO - BETA_CA_IO * (Ca_val));
-    double k2 = DT * (-I_Ca1 * CA_PH_IO - BETA_CA_IO * (Ca_val + k1 / 2.0));
-    double k3 = DT * (-I_Ca1 * CA_PH_IO - BETA_CA_IO * (Ca_val + k2 / 2.0));
-    double k4 = DT * (-I_Ca1 * CA_PH_IO - BETA_CA_IO * (Ca_val + k3));
-    p->ca[id] += (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
-    Ca_val = (Ca_val + p->ca[id]) / 2.0;
-    CNU(k_CaL, k_infty(v_val), tau_k(v_val));
-    CNU(l_CaL, l_infty(v_val), tau_l(v_val));
-    CNU(n_Kdr, n_infty(v_val), tau_n(v_val));
-    CNU(p_Kdr, p_infty(v_val), tau_p(v_val));
-    CNU(x_K, x_infty(v_val), tau_x(v_val));
-    CNU(r_CaH, r_infty(v_val), tau_r(v_val));
-    CNU(q_H, q_infty(v_val), tau_q(v_val));
-    CNU(s_KCa, s_infty(Ca_val), tau_s(Ca_val));
-    g[m_Na][id] = m_infty(v_val);
-    CNU(h_Na, h_infty(v_val, l_comp), tau_h(v_val, l_comp));
+    io_ion_at(p, id, 0.5 * (vnew[id] + p->v[id]), DT);
     p->v[id] = vnew[id];
   }
+}
+
+/* io_update_ion_2nd, io_ion.cu:53-89 (CN path): gates at the voltage
+   extrapolated from the last two steps */
+void or_io_cn_ion(or_pop *p, double DT)
+{
+  for (int id = 0; id < p->nc; id++) {
+    io_ion_at(p, id, 1.5 * p->v[id] - 0.5 * p->cn_vold[id], DT);
+    p->cn_vold[id] = p->v[id];
+  }
+}
+
+/* conductance and reversal sums of the CN path before halving
+   (io_cnm_vec_initialize / io_update_matrix, solve_cnm.cu:751-768, 793-810) */
+void or_io_cn_ge(const or_pop *p, int id, double *gsum, double *esum)
+{
+  double *const *c = p->cond, *const *g = p->ion;
+  *gsum = (c[g_leak][id]
+           + c[g_CaL][id] * g[k_CaL][id] * g[k_CaL][id] * g[k_CaL][id] * g[l_CaL][id]
+           + c[g_Na][id] * g[m_Na][id] * g[m_Na][id] * g[m_Na][id] * g[h_Na][id]
+           + c[g_Kdr][id] * g[n_Kdr][id] * g[p_Kdr][id]
+           + c[g_K][id] * g[x_K][id] * g[x_K][id] * g[x_K][id] * g[x_K][id]
+           + c[g_CaH][id] * g[r_CaH][id] * g[r_CaH][id]
+           + c[g_KCa][id] * g[s_KCa][id]
+           + c[g_H][id] * g[q_H][id]);
+  *esum = (c[g_leak][id] * V_LEAK_IO + p->iext[id]
+           + c[g_CaL][id] * V_Ca_IO * g[k_CaL][id] * g[k_CaL][id] * g[k_CaL][id] * g[l_CaL][id]
+           + c[g_Na][id] * V_Na_IO * g[m_Na][id] * g[m_Na][id] * g[m_Na][id] * g[h_Na][id]
+           + c[g_Kdr][id] * V_K_IO * g[n_Kdr][id] * g[p_Kdr][id]
+           + c[g_K][id] * V_K_IO * g[x_K][id] * g[x_K][id] * g[x_K][id] * g[x_K][id]
+           + c[g_CaH][id] * V_Ca_IO * g[r_CaH][id] * g[r_CaH][id]
+           + c[g_KCa][id] * V_K_IO * g[s_KCa][id]
+           + c[g_H][id] * V_H_IO * g[q_H][id]);
+}
+
+/* the gap current into each cell's dendrite from the stored voltages
+   (io_gap_update, gap.cu:142-160); the CN path subtracts it on the
+   right-hand side (add_io_gap_val, solve_cnm.cu:729-739) */
+double or_io_gap_into(const or_pop *p, int cell)
+{
+  int nb[4];
+  double s = 0.0;
+  io_neighbours(cell, p->nx, p->ny, nb);
+  const int post = cell * IO_COMP;
+  for (int k = 0; k < 4; k++) {
+    const int pre = (nb[k] > -1 && nb[k] < p->n) ? nb[k] : -1;
+    double diff_v = (pre > -1) ? (p->v[post] - p->v[pre * IO_COMP]) : 0.0;
+    s += -1.0 * (G_C_IO * (0.8 * exp(-0.01 * diff_v * diff_v) + 0.2) * (diff_v));
+  }
+  return s;
 }

@@ -9,6 +9,8 @@
  *   pkj_initialize_ion      pkj_ion.cu:360-395
  *   pkj_update_ion_RKC      pkj_ion.cu:304-358
  *   pkj_update_matrix       solve_rkc.cu:369-395
+ *   pkj_update_ion_2nd      pkj_ion.cu:246-302 (CN path)
+ *   CN conductance sums     solve_cnm.cu:420-512
  *
  * Reference quirks kept: the fixed somatic Ca reversal is applied where the
  * GLOBAL compartment index is a multiple of 1599 (pkj_ion.cu:329-331), not on
@@ -400,45 +402,97 @@ void or_pkj_update_b(or_pop *p, const double *vv)
 #define CNU(g, INF, TAU) \
   ion[g][id] = (2.0 * DT * (INF) + (2.0 * (TAU) - DT) * ion[g][id]) / (2.0 * (TAU) + DT)
 
-void or_pkj_update_ion(or_pop *p, const double *elem_vnew, double DT)
+/* Ca shell and gates of one compartment at the gate voltage v_val; the Ca
+   current uses the stored voltage p->v (pkj_ion.cu:255-297 and :312-353
+   differ only in v_val) */
+static void pkj_ion_at(or_pop *p, int id, double v_val, double DT)
 {
   double **ion = p->ion;
+  double Ca_val = p->ca[id];
+  double I_Ca1 = 1e-3 * (p->v[id] - p->rev[id])
+                 * (p->cond[g_CaP][id] * ion[m_CaP][id] * ion[h_CaP][id]
+                    + p->cond[g_CaT][id] * ion[m_CaT][id] * ion[h_CaT][id]);
+  (I_Ca1 > 0.0) ? (I_Ca1 = 0.0) : (I_Ca1 = -I_Ca1 / (2.0 * F_PKJ * p->shell[id]));
+  double k1 = DT * (I_Ca1 - B_Ca1_PKJ * (Ca_val - Ca1_0_PKJ));
+  double k2 = DT * (I_Ca1 - B_Ca1_PKJ * (Ca_val + k1 / 2.0 - Ca1_0_PKJ));
+  double k3 = DT * (I_Ca1 - B_Ca1_PKJ * (Ca_val + k2 / 2.0 - Ca1_0_PKJ));
+  double k4 = DT * (I_Ca1 - B_Ca1_PKJ * (Ca_val + k3 - Ca1_0_PKJ));
+  k1 = Ca_val + (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
+  p->ca[id] = k1;
+  Ca_val = 0.5 * (Ca_val + k1);
+  ((id + p->gid_offset) % 1599 == 0) ? (p->rev[id] = 12.5 * log(Ca1OUT_PKJ / Ca1_0_PKJ))
+                   : (p->rev[id] = 13.361624877 * log(Ca1OUT_PKJ / Ca_val));
+  Ca_val = floor((Ca_val * 1000 - 0.04) / 0.099986667) * 0.099986667 + 0.04;
+  CNU(m_NaF, inf_m_NaF(v_val), tau_m_NaF(v_val));
+  CNU(h_NaF, inf_h_NaF(v_val), tau_h_NaF(v_val));
+  CNU(m_NaP, inf_m_NaP(v_val), tau_m_NaP(v_val));
+  CNU(m_CaP, inf_m_CaP(v_val), tau_m_CaP(v_val));
+  CNU(h_CaP, inf_h_CaP(v_val), tau_h_CaP(v_val));
+  CNU(m_CaT, inf_m_CaT(v_val), tau_m_CaT(v_val));
+  CNU(h_CaT, inf_h_CaT(v_val), tau_h_CaT(v_val));
+  CNU(m_Kh1, inf_m_Kh1(v_val), tau_m_Kh1(v_val));
+  CNU(m_Kh2, inf_m_Kh2(v_val), tau_m_Kh2(v_val));
+  CNU(m_Kdr, inf_m_Kdr(v_val), tau_m_Kdr(v_val));
+  CNU(h_Kdr, inf_h_Kdr(v_val), tau_h_Kdr(v_val));
+  CNU(m_KM, inf_m_KM(v_val), tau_m_KM(v_val));
+  CNU(m_KA, inf_m_KA(v_val), tau_m_KA(v_val));
+  CNU(h_KA, inf_h_KA(v_val), tau_h_KA(v_val));
+  CNU(m_KC, inf_m_KC(v_val), tau_m_KC(v_val));
+  CNU(z_KC, inf_z_KC(v_val, Ca_val), tau_z_KC(v_val));
+  CNU(m_K2, inf_m_K2(v_val), tau_m_K2(v_val));
+  CNU(z_K2, inf_z_K2(v_val, Ca_val), tau_z_K2(v_val));
+}
+
+/* pkj_update_ion_RKC, pkj_ion.cu:304-358: gates at the mid-step voltage */
+void or_pkj_update_ion(or_pop *p, const double *elem_vnew, double DT)
+{
 #pragma omp parallel for schedule(static)
   for (int id = 0; id < p->nc; id++) {
-    double v_val = 0.5 * (elem_vnew[id] + p->v[id]);
-    double Ca_val = p->ca[id];
-    double I_Ca1 = 1e-3 * (p->v[id] - p->rev[id])
-                   * (p->cond[g_CaP][id] * ion[m_CaP][id] * ion[h_CaP][id]
-                      + p->cond[g_CaT][id] * ion[m_CaT][id] * ion[h_CaT][id]);
-    (I_Ca1 > 0.0) ? (I_Ca1 = 0.0) : (I_Ca1 = -I_Ca1 / (2.0 * F_PKJ * p->shell[id]));
-    double k1 = DT * (I_Ca1 - B_Ca1_PKJ * (Ca_val - Ca1_0_PKJ));
-    double k2 = DT * (I_Ca1 - B_Ca1_PKJ * (Ca_val + k1 / 2.0 - Ca1_0_PKJ));
-    double k3 = DT * (I_Ca1 - B_Ca1_PKJ * (Ca_val + k2 / 2.0 - Ca1_0_PKJ));
-    double k4 = DT * (I_Ca1 - B_Ca1_PKJ * (Ca_val + k3 - Ca1_0_PKJ));
-    k1 = Ca_val + (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
-    p->ca[id] = k1;
-    Ca_val = 0.5 * (Ca_val + k1);
-    ((id + p->gid_offset) % 1599 == 0) ? (p->rev[id] = 12.5 * log(Ca1OUT_PKJ / Ca1_0_PKJ))
-                     : (p->rev[id] = 13.361624877 * log(Ca1OUT_PKJ / Ca_val));
-    Ca_val = floor((Ca_val * 1000 - 0.04) / 0.099986667) * 0.099986667 + 0.04;
-    CNU(m_NaF, inf_m_NaF(v_val), tau_m_NaF(v_val));
-    CNU(h_NaF, inf_h_NaF(v_val), tau_h_NaF(v_val));
-    CNU(m_NaP, inf_m_NaP(v_val), tau_m_NaP(v_val));
-    CNU(m_CaP, inf_m_CaP(v_val), tau_m_CaP(v_val));
-    CNU(h_CaP, inf_h_CaP(v_val), tau_h_CaP(v_val));
-    CNU(m_CaT, inf_m_CaT(v_val), tau_m_CaT(v_val));
-    CNU(h_CaT, inf_h_CaT(v_val), tau_h_CaT(v_val));
-    CNU(m_Kh1, inf_m_Kh1(v_val), tau_m_Kh1(v_val));
-    CNU(m_Kh2, inf_m_Kh2(v_val), tau_m_Kh2(v_val));
-    CNU(m_Kdr, inf_m_Kdr(v_val), tau_m_Kdr(v_val));
-    CNU(h_Kdr, inf_h_Kdr(v_val), tau_h_Kdr(v_val));
-    CNU(m_KM, inf_m_KM(v_val), tau_m_KM(v_val));
-    CNU(m_KA, inf_m_KA(v_val), tau_m_KA(v_val));
-    CNU(h_KA, inf_h_KA(v_val), tau_h_KA(v_val));
-    CNU(m_KC, inf_m_KC(v_val), tau_m_KC(v_val));
-    CNU(z_KC, inf_z_KC(v_val, Ca_val), tau_z_KC(v_val));
-    CNU(m_K2, inf_m_K2(v_val), tau_m_K2(v_val));
-    CNU(z_K2, inf_z_K2(v_val, Ca_val), tau_z_K2(v_val));
+    pkj_ion_at(p, id, 0.5 * (elem_vnew[id] + p->v[id]), DT);
     p->v[id] = elem_vnew[id];
   }
+}
+
+/* pkj_update_ion_2nd, pkj_ion.cu:246-302 (CN path): gates at the voltage
+   extrapolated from the last two steps */
+void or_pkj_cn_ion(or_pop *p, double DT)
+{
+#pragma omp parallel for schedule(static)
+  for (int id = 0; id < p->nc; id++) {
+    pkj_ion_at(p, id, (1.5 * p->v[id] - 0.5 * p->cn_vold[id]), DT);
+    p->cn_vold[id] = p->v[id];
+  }
+}
+
+/* the ionic conductance sum and the conductance-weighted reversal sum plus
+   the injected current, before halving (pkj_cnm_vec_initialize /
+   pkj_update_matrix, solve_cnm.cu:429-456, 480-509) */
+void or_pkj_cn_ge(const or_pop *p, int id, double *gsum, double *esum)
+{
+  double *const *c = p->cond, *const *g = p->ion;
+  const double l_v_Ca = p->rev[id];
+  *gsum = (+c[g_leak][id]
+           + c[g_NaF][id] * g[m_NaF][id] * g[m_NaF][id] * g[m_NaF][id] * g[h_NaF][id]
+           + c[g_NaP][id] * g[m_NaP][id] * g[m_NaP][id] * g[m_NaP][id]
+           + c[g_CaP][id] * g[m_CaP][id] * g[h_CaP][id]
+           + c[g_CaT][id] * g[m_CaT][id] * g[h_CaT][id]
+           + c[g_Kh][id] * g[m_Kh1][id]
+           + c[g_Kh][id] * g[m_Kh2][id]
+           + c[g_Kdr][id] * g[m_Kdr][id] * g[m_Kdr][id] * g[h_Kdr][id]
+           + c[g_KM][id] * g[m_KM][id]
+           + c[g_KA][id] * g[m_KA][id] * g[m_KA][id] * g[m_KA][id] * g[m_KA][id] * g[h_KA][id]
+           + c[g_KC][id] * g[m_KC][id] * g[z_KC][id] * g[z_KC][id]
+           + c[g_K2][id] * g[m_K2][id] * g[z_K2][id] * g[z_K2][id]);
+  *esum = (+c[g_leak][id] * (V_LEAK_PKJ) + p->iext[id]
+           + c[g_NaF][id] * g[m_NaF][id] * g[m_NaF][id] * g[m_NaF][id] * g[h_NaF][id] * (V_Na_PKJ)
+           + c[g_NaP][id] * g[m_NaP][id] * g[m_NaP][id] * g[m_NaP][id] * (V_Na_PKJ)
+           + c[g_CaP][id] * g[m_CaP][id] * g[h_CaP][id] * (l_v_Ca)
+           + c[g_CaT][id] * g[m_CaT][id] * g[h_CaT][id] * (l_v_Ca)
+           + c[g_Kh][id] * g[m_Kh1][id] * (V_KH_PKJ)
+           + c[g_Kh][id] * g[m_Kh2][id] * (V_KH_PKJ)
+           + c[g_Kdr][id] * g[m_Kdr][id] * g[m_Kdr][id] * g[h_Kdr][id] * (V_K_PKJ)
+           + c[g_KM][id] * g[m_KM][id] * (V_K_PKJ)
+           + c[g_KA][id] * g[m_KA][id] * g[m_KA][id] * g[m_KA][id] * g[m_KA][id] * g[h_KA][id] * (V_K_PKJ)
+           + c[g_KC][id] * g[m_KC][id] * g[z_KC][id] * g[z_KC][id] * (V_K_PKJ)
+           + c[g_K2][id] * g[m_K2][id] * g[z_K2][id] * g[z_K2][id] * (V_K_PKJ));
 }

@@ -574,6 +574,9 @@ static double *field(const or_pop *p, int what)
     case 107: return p->area;
     case 108: return p->ca2;
     case 109: return p->ca_old;
+    case 110: return p->cn_vold;
+    case 111: return p->cn_gamma1;
+    case 112: return p->cn_omega1;
   }
   if (what >= 200 && what < 200 + p->nion) return p->ion[what - 200];
   if (what >= 300 && what < 300 + p->ncond) return p->cond[what - 300];
@@ -695,6 +698,8 @@ void or_pop_free(or_pop *p)
   if (!p) return;
   free(p->v); free(p->ca); free(p->cm); free(p->area); free(p->iext); free(p->shell);
   free(p->rev); free(p->compart); free(p->ca2); free(p->ca_old);
+  free(p->cn_vold); free(p->cn_gamma1); free(p->cn_omega1); free(p->cn_val); free(p->cn_dpos);
+  free(p->cn_trow); free(p->cn_tcol); free(p->cn_tsrc);
   for (int i = 0; i < p->ncond; i++) free(p->cond[i]);
   for (int i = 0; i < p->nion; i++) free(p->ion[i]);
   free(p->row); free(p->col); free(p->val); free(p->b); free(p->dammy);

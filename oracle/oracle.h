@@ -45,6 +45,11 @@ typedef struct or_pop {
   /* compartment state, structure of arrays (param.h:27-35 elem/ion/cond) */
   double *v, *ca, *cm, *area, *iext, *shell, *rev;
   double *ca2, *ca_old; /* GO: second Ca pool and the Ca of the previous ion update */
+  /* CN path (or_cn.c): previous voltage, half conductance / reversal sums of
+     the last step, the halved cable matrix (reference sign), the position of
+     each row's diagonal and the transpose's structure */
+  double *cn_vold, *cn_gamma1, *cn_omega1, *cn_val;
+  int *cn_dpos, *cn_trow, *cn_tcol, *cn_tsrc;
   int *compart;       /* compartment type */
   double *cond[OR_MAX_COND];
   double *ion[OR_MAX_ION];
@@ -118,6 +123,21 @@ void or_gr_update_b(or_pop *p, const double *vv);
 void or_gr_update_ion(or_pop *p, const double *vnew, double dt);
 double or_gr_rate(int which, double v, double ca);
 int or_gr_na_step(int type, double v_old, double v_new, double dt, double *x12);
+
+/* ---- CN path (or_cn.c) and the models' CN pieces ---- */
+int or_cn_init(or_pop *p);
+int or_cn_step(or_pop *p);
+int or_cn_advance(or_pop *p, double t);
+int or_cn_solve_test(or_pop *p, const double *diag, const double *b, double *x, int sym);
+void or_pkj_cn_ion(or_pop *p, double dt);
+void or_pkj_cn_ge(const or_pop *p, int id, double *gsum, double *esum);
+void or_io_cn_ion(or_pop *p, double dt);
+void or_io_cn_ge(const or_pop *p, int id, double *gsum, double *esum);
+double or_io_gap_into(const or_pop *p, int cell);
+void or_go_cn_ion(or_pop *p, double dt);
+void or_go_cn_ge(const or_pop *p, int id, double *gsum, double *esum);
+void or_gr_cn_ion(or_pop *p, double dt);
+void or_gr_cn_ge(const or_pop *p, int id, double *gsum, double *esum);
 
 /* ---- stimulus ---- */
 void or_set_stim(or_pop *p, int comp, double t0, double t1, const double *amp_per_cell);

@@ -160,3 +160,26 @@ def test_gr_rows_against_oracle(built, tmp_path, ref_data):
         ref.append(o.get(orc.VNEW)[16])
     np.testing.assert_allclose(got[:, 1], ref, rtol=0, atol=2e-6)
     assert (tmp_path / "comptime.csv").read_text().startswith("Gr,1,RKC,Go,0,RKC,Pkj,0,RKC,Io,0,RKC,CompTime,")
+
+
+@pytest.mark.gpu
+def test_cn_pkj_rows_against_oracle(built, tmp_path, ref_data):
+    """./main CN 0 0 CN 0 0 CN 1 1 CN 0 0 (pkj1.sh's CN runs): pkj_v_by_CN.csv
+    from the CN engine, five 0.025 ms CN steps per row, against the oracle's CN
+    path from the same libc rand() initial voltage."""
+    shutil.copy(os.path.join(ref_data, "Pkj.txt"), tmp_path / "Pkj.txt")
+    print("main: PKJ 1x1, CN, 2 ms", flush=True)
+    r = run_main(["CN", "0", "0", "CN", "0", "0", "CN", "1", "1", "CN", "0", "0", "--ts", "2"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = [l.rstrip("\n").rstrip(",").split(",") for l in open(tmp_path / "pkj_v_by_CN.csv")]
+    got = np.array([[float(x) for x in row] for row in rows])
+    assert got.shape == (16, 2)
+    from test_oracle_cn import L as LC
+    o = orc.Pop.pkj(os.path.join(ref_data, "Pkj.txt"), 1, 1, libc_rand=True, seed=0)
+    assert LC().or_cn_init(o.h) == 0
+    ref = []
+    for k in range(16):
+        LC().or_cn_advance(o.h, k * 0.125)
+        ref.append(o.get(orc.V)[1599])
+    np.testing.assert_allclose(got[:, 1], ref, rtol=0, atol=2e-6)
+    assert (tmp_path / "comptime.csv").read_text().startswith("Gr,0,CN,Go,0,CN,Pkj,1,CN,Io,0,CN,CompTime,")
