@@ -103,6 +103,8 @@ int create_io(const cb_pop_config &cfg, cb_pop **out) {
   p->device = cfg.device;
   if (cfg.solver == CB_SOLVER_CN)
     p->eng.reset(make_cn_io(host, stim));
+  else if (cfg.solver == CB_SOLVER_BE)
+    p->eng.reset(make_be_io(host, stim));
   else
     p->eng.reset(make_global_io(host, sol, stim, poisson_of(cfg)));
   *out = p.release();
@@ -149,6 +151,8 @@ int create_go(const cb_pop_config &cfg, cb_pop **out) {
   p->device = cfg.device;
   if (cfg.solver == CB_SOLVER_CN)
     p->eng.reset(make_cn_go(host, stim));
+  else if (cfg.solver == CB_SOLVER_BE)
+    p->eng.reset(make_be_go(host, stim));
   else
     p->eng.reset(make_global_go(host, sol, stim, poisson_of(cfg)));
   *out = p.release();
@@ -194,6 +198,8 @@ int create_gr(const cb_pop_config &cfg, cb_pop **out) {
   p->device = cfg.device;
   if (cfg.solver == CB_SOLVER_CN)
     p->eng.reset(make_cn_gr(host, stim));
+  else if (cfg.solver == CB_SOLVER_BE)
+    p->eng.reset(make_be_gr(host, stim));
   else
     p->eng.reset(make_global_gr(host, sol, stim, poisson_of(cfg)));
   *out = p.release();
@@ -242,11 +248,11 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     *out = nullptr;
     if (cfg->nx < 0 || cfg->ny < 0) return fail(CB_E_ARG, "negative grid size");
     if (cfg->cell_type < CB_GR || cfg->cell_type > CB_IO) return fail(CB_E_ARG, "unknown cell type");
-    if (cfg->solver != CB_SOLVER_RKC && cfg->solver != CB_SOLVER_CN)
-      return fail(CB_E_UNSUP, "solver not provided by this build");
-    if (cfg->solver == CB_SOLVER_CN && cfg->engine != CB_ENGINE_GLOBAL_FP64)
-      return fail(CB_E_UNSUP, "the CN solver runs on the global FP64 engine");
-    if (cfg->solver == CB_SOLVER_CN && cfg->poisson_on)
+    if (cfg->solver != CB_SOLVER_RKC && cfg->solver != CB_SOLVER_CN && cfg->solver != CB_SOLVER_BE)
+      return fail(CB_E_ARG, "unknown solver");
+    if (cfg->solver != CB_SOLVER_RKC && cfg->engine != CB_ENGINE_GLOBAL_FP64)
+      return fail(CB_E_UNSUP, "the CN and BE solvers run on the global FP64 engine");
+    if (cfg->solver != CB_SOLVER_RKC && cfg->poisson_on)
       return fail(CB_E_UNSUP, "the Poisson conductance drive is provided for the RKC solver");
     CB_HIP(hipSetDevice(cfg->device));
     if (cfg->cell_type == CB_IO) return create_io(*cfg, out);
@@ -309,8 +315,8 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     p->engine = cfg->engine;
     p->comp = pkj::COMP;
     p->device = cfg->device;
-    if (cfg->solver == CB_SOLVER_CN) {
-      p->eng.reset(make_cn_pkj(host, stim));
+    if (cfg->solver != CB_SOLVER_RKC) {
+      p->eng.reset(cfg->solver == CB_SOLVER_CN ? make_cn_pkj(host, stim) : make_be_pkj(host, stim));
       *out = p.release();
       return CB_OK;
     }

@@ -113,6 +113,11 @@ Engine *make_cn_pkj(const PkjHost &host, const StimSpec &stim);
 Engine *make_cn_io(const IoHost &host, const StimSpec &stim);
 Engine *make_cn_go(const GoHost &host, const StimSpec &stim);
 Engine *make_cn_gr(const GrHost &host, const StimSpec &stim);
+// the backward-Euler solver of solve_bem.cu on the same Krylov core
+Engine *make_be_pkj(const PkjHost &h, const StimSpec &s);
+Engine *make_be_io(const IoHost &h, const StimSpec &s);
+Engine *make_be_go(const GoHost &h, const StimSpec &s);
+Engine *make_be_gr(const GrHost &h, const StimSpec &s);
 Engine *make_cell_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn,
                       bool mixed);
 

@@ -16,6 +16,15 @@
 //      that tolerance, so the result is the truncated iterate -- the solve
 //      must be the same global iteration, not a per-cell direct solve.
 //
+// The backward-Euler (BE) solver of solve_bem.cu shares the Krylov core: one
+// fused kernel runs the type's BE ion update (pkj_update_ion, io_update_ion,
+// go_KAHP_update + go_update_ion, gr_Na_update + gr_update_ion: exponential
+// Euler gates and implicit Markov steps at the step's starting voltage) and
+// builds the BE system on the reference's unhalved matrix, diagonal
+// val_ori + Cm/DT + conductances, right-hand side Cm/DT V + reversal-weighted
+// conductances + I_ext (minus the inferior olive's gap currents); the solve is
+// the same CG / BiCG.
+//
 // The Krylov loop runs on the device without host round trips: each
 // iteration is two launches, every dot product is reduced deterministically
 // (256-wide stride-halving trees, the partials reduced again with the same
@@ -42,6 +51,7 @@ namespace cbrkc {
 namespace {
 
 constexpr double CN_DT = 0.025;   // param.h:11
+constexpr double BE_DT = 0.025;   // param.h:10
 constexpr double H_MAX = 0.125;   // param.h:12, the output interval
 constexpr int NB = 256;
 
@@ -101,10 +111,21 @@ __device__ bool block_sum_last(const CnSys &S, double x, double *lds, double *ou
   return true;
 }
 
+// b = r = p = bb (and the shadow vectors of BiCG); returns bb
+__device__ inline double set_rhs(const CnSys &S, int i, double bb) {
+  S.b[i] = bb;
+  S.r[i] = bb;
+  S.p[0][i] = bb;
+  if (S.bicg) {
+    S.rs[i] = bb;
+    S.ps[0][i] = bb;
+  }
+  return bb;
+}
+
 // the generic part of xx_update_matrix for compartment i given the step's
 // conductance and reversal sums (gs, es) and the initial omega2 (the IO gap
-// term, else 0); writes the diagonal, b, r = p = b (and the shadow vectors of
-// BiCG); returns b_i
+// term, else 0); writes the diagonal and the right-hand side; returns b_i
 __device__ inline double cn_row(const CnSys &S, int i, double gs, double es, double om0) {
   const double DT = CN_DT;
   const double gamma2 = 0.0 + gs / 2.0;
@@ -116,14 +137,13 @@ __device__ inline double cn_row(const CnSys &S, int i, double gs, double es, dou
   bb += (S.cm[i] / DT - S.gamma1[i]) * vi + S.omega1[i] + omega2;
   S.gamma1[i] = gamma2;
   S.omega1[i] = omega2;
-  S.b[i] = bb;
-  S.r[i] = bb;
-  S.p[0][i] = bb;
-  if (S.bicg) {
-    S.rs[i] = bb;
-    S.ps[0][i] = bb;
-  }
-  return bb;
+  return set_rhs(S, i, bb);
+}
+
+// the BE row: diagonal val_ori[d] + gs (gs carries Cm/DT first), b = bb
+__device__ inline double be_row(const CnSys &S, int i, double gs, double bb) {
+  S.diag[i] = S.val[S.dpos[i]] + gs;
+  return set_rhs(S, i, bb);
 }
 
 // end of the prep kernel: |b|^2 (= the first r.r) and the solve's flags
@@ -141,8 +161,9 @@ __device__ inline void cn_prep_finish(const CnSys &S, double bb, double *lds) {
 
 struct PkjCn {
   double *ca, *rev;
-  const double *shell, *cond;
+  const double *shell, *area, *cond;
   double *gate;
+  const int8_t *compart;
 };
 
 __global__ void __launch_bounds__(NB) k_cn_prep_pkj(CnSys S, PkjCn M) {
@@ -299,6 +320,134 @@ __global__ void __launch_bounds__(NB) k_cn_prep_gr(CnSys S, GrCn M) {
   cn_prep_finish(S, bb, lds);
 }
 
+// ---- the cell types' BE ion updates (fused with the matrix row) ----
+
+__global__ void __launch_bounds__(NB) k_be_prep_pkj(CnSys S, PkjCn M) {
+  using namespace pkj;
+  __shared__ double lds[NB];
+  const int nc = S.nc, i = blockIdx.x * NB + threadIdx.x;
+  double bb = 0.0;
+  if (i < nc) {
+    double g[NGATE], c[NCOND];
+#pragma unroll
+    for (int k = 0; k < NGATE; k++) g[k] = M.gate[(size_t)k * nc + i];
+#pragma unroll
+    for (int k = 0; k < NCOND; k++) c[k] = M.cond[(size_t)k * nc + i];
+    const double v = S.v[i];
+    // pkj_update_ion (pkj_ion.cu:94-151) then pkj_update_matrix
+    // (solve_bem.cu:407-445)
+    double ca = M.ca[i], rev = M.rev[i];
+    update_ion_be(v, ca, rev, M.shell[i], M.area[i], M.compart[i] == SOMA, c[g_CaP] * g[m_CaP] * g[h_CaP],
+                  c[g_CaT] * g[m_CaT] * g[h_CaT], BE_DT, g);
+#pragma unroll
+    for (int k = 0; k < NGATE; k++) M.gate[(size_t)k * nc + i] = g[k];
+    M.ca[i] = ca;
+    M.rev[i] = rev;
+    double gs, es;
+    be_ge(c, g, rev, S.iext[i], S.cm[i] / BE_DT, v, gs, es);
+    bb = be_row(S, i, gs, 0.0 + es);
+  }
+  cn_prep_finish(S, bb, lds);
+}
+
+__global__ void __launch_bounds__(NB) k_be_prep_io(CnSys S, IoCn M) {
+  using namespace io;
+  __shared__ double lds[NB];
+  const int nc = S.nc, i = blockIdx.x * NB + threadIdx.x;
+  double bb = 0.0;
+  if (i < nc) {
+    double g[NGATE], c[NCOND];
+#pragma unroll
+    for (int k = 0; k < NGATE; k++) g[k] = M.gate[(size_t)k * nc + i];
+#pragma unroll
+    for (int k = 0; k < NCOND; k++) c[k] = M.cond[(size_t)k * nc + i];
+    const double v = S.v[i];
+    // io_update_ion (io_ion.cu:128-161) then io_update_matrix
+    // (solve_bem.cu:476-508)
+    double ca = M.ca[i];
+    update_ion_be<double>(v, ca, c[g_CaH], M.compart[i], BE_DT, g);
+#pragma unroll
+    for (int k = 0; k < NGATE; k++) M.gate[(size_t)k * nc + i] = g[k];
+    M.ca[i] = ca;
+    double gs, es;
+    be_ge(c, g, S.iext[i], S.cm[i] / BE_DT, v, gs, es);
+    bb = 0.0 + es;
+    // io_gap_update + add_io_gap_val (gap.cu:142-160, solve_bem.cu:463-473):
+    // minus each gap current into the dendrite, partners in slot order
+    if (M.n > 1 && i % COMP == 0) {
+      const int cell = i / COMP;
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        const int pre = M.nbr[cell * 4 + s];
+        const double diff_v = (pre > -1) ? (v - S.v[pre * COMP]) : 0.0;
+        bb += -1.0 * gap_current<double>(diff_v);
+      }
+    }
+    bb = be_row(S, i, gs, bb);
+  }
+  cn_prep_finish(S, bb, lds);
+}
+
+__global__ void __launch_bounds__(NB) k_be_prep_go(CnSys S, GoCn M) {
+  using namespace go;
+  __shared__ double lds[NB];
+  const int nc = S.nc, i = blockIdx.x * NB + threadIdx.x;
+  double bb = 0.0;
+  if (i < nc) {
+    double g[NGATE], c[NCOND];
+#pragma unroll
+    for (int k = 0; k < NGATE; k++) g[k] = M.gate[(size_t)k * nc + i];
+#pragma unroll
+    for (int k = 0; k < NCOND; k++) c[k] = M.cond[(size_t)k * nc + i];
+    const double v = S.v[i];
+    // go_KAHP_update at the present Ca (go_ion.cu:214-317), go_update_ion
+    // (go_ion.cu:438-511), then go_update_matrix (solve_bem.cu:323-347)
+    double ca = M.ca[i], ca2 = M.ca2[i], rev = M.rev[i];
+    kahp_be(ca, BE_DT, g);
+    update_ion_be(v, M.area[i], c, ca, ca2, rev, BE_DT, g);
+#pragma unroll
+    for (int k = 0; k < NGATE; k++) M.gate[(size_t)k * nc + i] = g[k];
+    M.ca[i] = ca;
+    M.ca2[i] = ca2;
+    M.rev[i] = rev;
+    double gs, es;
+    be_ge(c, g, rev, S.iext[i], S.cm[i] / BE_DT, v, gs, es);
+    bb = be_row(S, i, gs, 0.0 + es);
+  }
+  cn_prep_finish(S, bb, lds);
+}
+
+__global__ void __launch_bounds__(NB) k_be_prep_gr(CnSys S, GrCn M) {
+  using namespace gr;
+  __shared__ double lds[NB];
+  const int nc = S.nc, i = blockIdx.x * NB + threadIdx.x;
+  double bb = 0.0;
+  if (i < nc) {
+    double g[NGATE], c[NCOND];
+#pragma unroll
+    for (int k = 0; k < NGATE; k++) g[k] = M.gate[(size_t)k * nc + i];
+#pragma unroll
+    for (int k = 0; k < NCOND; k++) c[k] = M.cond[(size_t)k * nc + i];
+    const double v = S.v[i];
+    // gr_Na_update and gr_update_ion at V (gr_ion.cu:209-357, 579-600), then
+    // gr_update_matrix (solve_bem.cu:258-291)
+    const int t = M.compart[i];
+    if (t >= HILL) {
+      const NaType T = M.na[t];
+      double x[12];
+      na_gather(g, x);
+      if (na_be(T, v, BE_DT, x)) na_scatter(x, g);
+    }
+    update_gates_be(v, M.ca[i], BE_DT, g);
+#pragma unroll
+    for (int k = 0; k < NGATE; k++) M.gate[(size_t)k * nc + i] = g[k];
+    double gs, es;
+    be_ge(c, g, S.iext[i], S.cm[i] / BE_DT, v, gs, es);
+    bb = be_row(S, i, gs, 0.0 + es);
+  }
+  cn_prep_finish(S, bb, lds);
+}
+
 // ---- Krylov iteration k: A (or cg) half ----
 // p_k = beta p_{k-1} + r (k > 0) on the fly, Ap = A p_k (BiCG: also A^T ps_k),
 // alpha = rr / (p_k . Ap) (BiCG: ps_k . Ap)
@@ -412,19 +561,20 @@ class CnEngine : public Engine {
  public:
   CnEngine(int n, int comp, const std::vector<double> &cm, const std::vector<double> &v,
            const std::vector<double> &iext, const std::vector<int> &row, const std::vector<int> &col,
-           const std::vector<double> &rkc_val, const StimSpec &stim, bool bicg)
-      : n_(n), comp_(comp), nc_((int)cm.size()), stim_(stim) {
+           const std::vector<double> &rkc_val, const StimSpec &stim, bool bicg, bool be)
+      : n_(n), comp_(comp), nc_((int)cm.size()), be_(be), dt_(be ? BE_DT : CN_DT), stim_(stim) {
     CB_HIP(hipStreamCreate(&st_));
     const int nnz = (int)col.size();
-    // CN matrix: the reference's unflipped matrix halved = -(RKC-signed) / 2
+    // the reference's unflipped matrix = -(RKC-signed); halved for CN
+    // (pkj_solve.cu:238-245 and the other xx_solve.cu files)
     std::vector<double> val(nnz);
-    for (int j = 0; j < nnz; j++) val[j] = -rkc_val[j] / 2.0;
+    for (int j = 0; j < nnz; j++) val[j] = be ? -rkc_val[j] : -rkc_val[j] / 2.0;
     std::vector<int> dpos(nc_, -1);
     for (int i = 0; i < nc_; i++)
       for (int j = row[i]; j < row[i + 1]; j++)
         if (col[j] == i) dpos[i] = j;
     for (int i = 0; i < nc_; i++)
-      if (dpos[i] < 0) throw Error("the CN path needs a stored diagonal in every row");
+      if (dpos[i] < 0) throw Error("the CN and BE paths need a stored diagonal in every row");
     // transpose for BiCG: rows of A in order, so transposed rows are in column order
     std::vector<int> trow(nc_ + 1, 0), tcol(nnz), tsrc(nnz), fill(nc_, 0);
     for (int j = 0; j < nnz; j++) trow[col[j] + 1]++;
@@ -488,7 +638,7 @@ class CnEngine : public Engine {
   void set_current(double t) override {
     k_cn_stim<<<(n_ + 127) / 128, 128, 0, st_>>>(S_.iext, amp_, comp_, stim_.comp, stim_.t0, stim_.t1, t, n_);
   }
-  // one CN step of CN_DT (xx_solve_update_v with the CN solver)
+  // one CN (BE) step of CN_DT (BE_DT): xx_solve_update_v with that solver
   void advance() override {
     prep();
     for (int k = 0; k < 100; k++) {
@@ -510,7 +660,7 @@ class CnEngine : public Engine {
       while (l_dt < H_MAX) {
         set_current(t + l_dt);
         advance();
-        l_dt += CN_DT;
+        l_dt += dt_;
       }
       st.macro_steps++;
       if (drec) k_cn_record<<<(n_ + 127) / 128, 128, 0, st_>>>(S_.v, drec + (size_t)k * n_, comp_, rec_comp, n_);
@@ -552,6 +702,8 @@ class CnEngine : public Engine {
     up(S_.omega1, o1);
   }
   int n_, comp_, nc_, nb_ = 1;
+  bool be_;
+  double dt_;
   StimSpec stim_;
   CnSys S_{};
   double *amp_ = nullptr, *rec_ = nullptr;
@@ -561,14 +713,17 @@ class CnEngine : public Engine {
 
 class CnPkj : public CnEngine {
  public:
-  CnPkj(const PkjHost &H, const StimSpec &stim)
-      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false) {
+  CnPkj(const PkjHost &H, const StimSpec &stim, bool be)
+      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false, be) {
     using namespace pkj;
     M_.ca = dup(H.ca);
     M_.rev = dup(H.rev);
     M_.shell = dup(H.shell);
+    M_.area = dup(H.area);
     M_.cond = dup(H.cond);
     M_.gate = dup(H.gate);
+    M_.compart = dup(H.compart);
+    if (be) return;
     std::vector<double> gs(nc_), es(nc_);
     double c[NCOND], g[NGATE];
     for (int i = 0; i < nc_; i++) {
@@ -580,13 +735,16 @@ class CnPkj : public CnEngine {
   }
   ~CnPkj() override {
     (void)hipStreamSynchronize(st_);
-    for (const void *p : {(const void *)M_.ca, (const void *)M_.rev, (const void *)M_.shell, (const void *)M_.cond,
-                          (const void *)M_.gate})
+    for (const void *p : {(const void *)M_.ca, (const void *)M_.rev, (const void *)M_.shell, (const void *)M_.area,
+                          (const void *)M_.cond, (const void *)M_.gate, (const void *)M_.compart})
       (void)hipFree(const_cast<void *>(p));
   }
 
  private:
-  void prep() override { k_cn_prep_pkj<<<nb_, NB, 0, st_>>>(S_, M_); }
+  void prep() override {
+    if (be_) k_be_prep_pkj<<<nb_, NB, 0, st_>>>(S_, M_);
+    else k_cn_prep_pkj<<<nb_, NB, 0, st_>>>(S_, M_);
+  }
   const double *model_field(int f) const override {
     if (f == F_CA) return M_.ca;
     if (f == F_REV) return M_.rev;
@@ -598,8 +756,8 @@ class CnPkj : public CnEngine {
 
 class CnIo : public CnEngine {
  public:
-  CnIo(const IoHost &H, const StimSpec &stim)
-      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, true) {
+  CnIo(const IoHost &H, const StimSpec &stim, bool be)
+      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, true, be) {
     using namespace io;
     M_.ca = dup(H.ca);
     M_.cond = dup(H.cond);
@@ -607,6 +765,7 @@ class CnIo : public CnEngine {
     M_.compart = dup(H.compart);
     M_.nbr = dup(H.nbr);
     M_.n = H.n;
+    if (be) return;
     std::vector<double> gs(nc_), es(nc_);
     double c[NCOND], g[NGATE];
     for (int i = 0; i < nc_; i++) {
@@ -624,7 +783,10 @@ class CnIo : public CnEngine {
   }
 
  private:
-  void prep() override { k_cn_prep_io<<<nb_, NB, 0, st_>>>(S_, M_); }
+  void prep() override {
+    if (be_) k_be_prep_io<<<nb_, NB, 0, st_>>>(S_, M_);
+    else k_cn_prep_io<<<nb_, NB, 0, st_>>>(S_, M_);
+  }
   const double *model_field(int f) const override {
     if (f == F_CA) return M_.ca;
     if (f >= F_GATE0 && f < F_GATE0 + io::NGATE) return M_.gate + (size_t)(f - F_GATE0) * nc_;
@@ -635,8 +797,8 @@ class CnIo : public CnEngine {
 
 class CnGo : public CnEngine {
  public:
-  CnGo(const GoHost &H, const StimSpec &stim)
-      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false) {
+  CnGo(const GoHost &H, const StimSpec &stim, bool be)
+      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false, be) {
     using namespace go;
     M_.ca = dup(H.ca);
     M_.ca2 = dup(H.ca2);
@@ -645,6 +807,7 @@ class CnGo : public CnEngine {
     M_.area = dup(H.area);
     M_.cond = dup(H.cond);
     M_.gate = dup(H.gate);
+    if (be) return;
     std::vector<double> gs(nc_), es(nc_);
     double c[NCOND], g[NGATE];
     for (int i = 0; i < nc_; i++) {
@@ -662,7 +825,10 @@ class CnGo : public CnEngine {
   }
 
  private:
-  void prep() override { k_cn_prep_go<<<nb_, NB, 0, st_>>>(S_, M_); }
+  void prep() override {
+    if (be_) k_be_prep_go<<<nb_, NB, 0, st_>>>(S_, M_);
+    else k_cn_prep_go<<<nb_, NB, 0, st_>>>(S_, M_);
+  }
   const double *model_field(int f) const override {
     switch (f) {
       case F_CA: return M_.ca;
@@ -678,8 +844,8 @@ class CnGo : public CnEngine {
 
 class CnGr : public CnEngine {
  public:
-  CnGr(const GrHost &H, const StimSpec &stim)
-      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false) {
+  CnGr(const GrHost &H, const StimSpec &stim, bool be)
+      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false, be) {
     using namespace gr;
     ca_ = dup(H.ca);
     M_.ca = ca_;
@@ -692,6 +858,7 @@ class CnGr : public CnEngine {
     na_ = dal<NaType>(NTYPE);
     CB_HIP(hipMemcpy(na_, T.t, sizeof(T.t), hipMemcpyHostToDevice));
     M_.na = na_;
+    if (be) return;
     std::vector<double> gs(nc_), es(nc_);
     double c[NCOND], g[NGATE];
     for (int i = 0; i < nc_; i++) {
@@ -707,7 +874,10 @@ class CnGr : public CnEngine {
   }
 
  private:
-  void prep() override { k_cn_prep_gr<<<nb_, NB, 0, st_>>>(S_, M_); }
+  void prep() override {
+    if (be_) k_be_prep_gr<<<nb_, NB, 0, st_>>>(S_, M_);
+    else k_cn_prep_gr<<<nb_, NB, 0, st_>>>(S_, M_);
+  }
   const double *model_field(int f) const override {
     if (f == F_CA) return ca_;
     if (f >= F_GATE0 && f < F_GATE0 + gr::NGATE) return M_.gate + (size_t)(f - F_GATE0) * nc_;
@@ -721,9 +891,13 @@ class CnGr : public CnEngine {
 
 }  // namespace
 
-Engine *make_cn_pkj(const PkjHost &h, const StimSpec &s) { return new CnPkj(h, s); }
-Engine *make_cn_io(const IoHost &h, const StimSpec &s) { return new CnIo(h, s); }
-Engine *make_cn_go(const GoHost &h, const StimSpec &s) { return new CnGo(h, s); }
-Engine *make_cn_gr(const GrHost &h, const StimSpec &s) { return new CnGr(h, s); }
+Engine *make_cn_pkj(const PkjHost &h, const StimSpec &s) { return new CnPkj(h, s, false); }
+Engine *make_cn_io(const IoHost &h, const StimSpec &s) { return new CnIo(h, s, false); }
+Engine *make_cn_go(const GoHost &h, const StimSpec &s) { return new CnGo(h, s, false); }
+Engine *make_cn_gr(const GrHost &h, const StimSpec &s) { return new CnGr(h, s, false); }
+Engine *make_be_pkj(const PkjHost &h, const StimSpec &s) { return new CnPkj(h, s, true); }
+Engine *make_be_io(const IoHost &h, const StimSpec &s) { return new CnIo(h, s, true); }
+Engine *make_be_go(const GoHost &h, const StimSpec &s) { return new CnGo(h, s, true); }
+Engine *make_be_gr(const GrHost &h, const StimSpec &s) { return new CnGr(h, s, true); }
 
 }  // namespace cbrkc

@@ -136,6 +136,8 @@ HD void steady_state(double v, double ca, double *g) {
 }
 
 HD void cn(double &x, double inf, double tau, double DT) { x = (2.0 * DT * inf + (2.0 * tau - DT) * x) / (2.0 * tau + DT); }
+// exponential Euler, the BE path's gate update: x = inf + (x - inf) exp(-DT / tau)
+HD void ee(double &x, double inf, double tau, double DT) { x = inf + (x - inf) * mexp(-DT / tau); }
 
 // go_update_ion_RKC_exp_imp (go_ion.cu:379-437): both Ca pools by RK4 with
 // their currents at the old voltage and gates (the second pool against the
@@ -191,37 +193,73 @@ HD void update_ion(double vold, double vnew, double area, const double *c, doubl
   update_ion_at(vold, vold, vnew, area, c, ca, ca2, rev, DT, g);
 }
 
-// One Crank-Nicolson step of the KAHP scheme (go_ion.cu:88-213) on
-// x = (c1, c2, c3, c4, o1, o2): (A(ca) - 2/DT) x' = -(A(ca_old) + 2/DT) x,
-// Gaussian elimination with partial pivoting.  Every loop is unrolled so the
-// 6 x 6 system stays in registers; a row exchange is a conditional swap with
-// constant indices.  Returns false, leaving the state alone, on a pivot below
-// 1e-12 (the reference returns early there).
-HD bool kahp_step(double ca_old, double ca, double DT, double *g) {
+// One step of the KAHP scheme on x = (c1, c2, c3, c4, o1, o2), the system
+// solved by Gaussian elimination with partial pivoting as go_ion.cu:146-198 /
+// :252-304 write it (strictly larger |pivot| wins, rows swapped from the
+// pivot column on).  BE = false: the Crank-Nicolson step of
+// go_KAHP_update_2order (go_ion.cu:88-213),
+// (A(ca) - 2/DT) x' = -(A(ca_old) + 2/DT) x.  BE = true: the implicit-Euler
+// step of go_KAHP_update (go_ion.cu:214-317), (I - DT A(ca)) x' = x with the
+// matrix formed as the reference forms it (every entry times -DT, then 1
+// added to the diagonal); ca_old is unused.  Every loop is unrolled so the
+// system stays in registers (the elimination sits in the same body as the
+// matrix build for that); a row exchange is a conditional swap with constant
+// indices.  Returns false, leaving the state alone, on a pivot below 1e-12
+// (the reference returns early there).
+template <bool BE>
+HD bool kahp_solve(double ca_old, double ca, double DT, double *g) {
   const double i1 = 80e-3, i2 = 80e-3, i3 = 200e-3, a1 = 1.0, a2 = 100e-3, b1 = 160e-3, b2 = 1.2;
-  const double l_dt = 2.0 / DT;
-  double d1 = 200 * ca_old / 3.0, d2 = 160 * ca_old / 3.0, d3 = 80 * ca_old / 3.0;
-  const double An[6][6] = {{-d1 + l_dt, i1, 0.0, 0.0, 0.0, 0.0},
-                           {d1, -d2 - i1 + l_dt, i2, 0.0, 0.0, 0.0},
-                           {0.0, d2, -(d3 + i2 + b1) + l_dt, i3, a1, 0.0},
-                           {0.0, 0.0, d3, -(i3 + b2) + l_dt, 0.0, a2},
-                           {0.0, 0.0, b1, 0.0, -a1 + l_dt, 0.0},
-                           {0.0, 0.0, 0.0, b2, 0.0, -a2 + l_dt}};
-  const double x[6] = {g[c1_KAHP], g[c2_KAHP], g[c3_KAHP], g[c4_KAHP], g[o1_KAHP], g[o2_KAHP]};
-  double r[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double A[6][6], r[6];
+  if constexpr (BE) {
+    const double d1 = 200 * ca / 3.0, d2 = 160 * ca / 3.0, d3 = 80 * ca / 3.0;
+    const double A0[6][6] = {{-d1, i1, 0.0, 0.0, 0.0, 0.0},
+                             {d1, -d2 - i1, i2, 0.0, 0.0, 0.0},
+                             {0.0, d2, -(d3 + i2 + b1), i3, a1, 0.0},
+                             {0.0, 0.0, d3, -(i3 + b2), 0.0, a2},
+                             {0.0, 0.0, b1, 0.0, -a1, 0.0},
+                             {0.0, 0.0, 0.0, b2, 0.0, -a2}};
 #pragma unroll
-  for (int i = 0; i < 6; i++)
+    for (int i = 0; i < 6; i++)
 #pragma unroll
-    for (int j = 0; j < 6; j++) r[i] -= x[j] * An[i][j];
-  d1 = 200 * ca / 3.0;
-  d2 = 160 * ca / 3.0;
-  d3 = 80 * ca / 3.0;
-  double A[6][6] = {{-d1 - l_dt, i1, 0.0, 0.0, 0.0, 0.0},
-                    {d1, -d2 - i1 - l_dt, i2, 0.0, 0.0, 0.0},
-                    {0.0, d2, -(d3 + i2 + b1) - l_dt, i3, a1, 0.0},
-                    {0.0, 0.0, d3, -(i3 + b2) - l_dt, 0.0, a2},
-                    {0.0, 0.0, b1, 0.0, -a1 - l_dt, 0.0},
-                    {0.0, 0.0, 0.0, b2, 0.0, -a2 - l_dt}};
+      for (int j = 0; j < 6; j++) A[i][j] = A0[i][j] * -DT;
+#pragma unroll
+    for (int i = 0; i < 6; i++) A[i][i] += 1.0;
+    r[0] = g[c1_KAHP];
+    r[1] = g[c2_KAHP];
+    r[2] = g[c3_KAHP];
+    r[3] = g[c4_KAHP];
+    r[4] = g[o1_KAHP];
+    r[5] = g[o2_KAHP];
+  } else {
+    const double l_dt = 2.0 / DT;
+    double d1 = 200 * ca_old / 3.0, d2 = 160 * ca_old / 3.0, d3 = 80 * ca_old / 3.0;
+    const double An[6][6] = {{-d1 + l_dt, i1, 0.0, 0.0, 0.0, 0.0},
+                             {d1, -d2 - i1 + l_dt, i2, 0.0, 0.0, 0.0},
+                             {0.0, d2, -(d3 + i2 + b1) + l_dt, i3, a1, 0.0},
+                             {0.0, 0.0, d3, -(i3 + b2) + l_dt, 0.0, a2},
+                             {0.0, 0.0, b1, 0.0, -a1 + l_dt, 0.0},
+                             {0.0, 0.0, 0.0, b2, 0.0, -a2 + l_dt}};
+    const double x[6] = {g[c1_KAHP], g[c2_KAHP], g[c3_KAHP], g[c4_KAHP], g[o1_KAHP], g[o2_KAHP]};
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      r[i] = 0.0;
+#pragma unroll
+      for (int j = 0; j < 6; j++) r[i] -= x[j] * An[i][j];
+    }
+    d1 = 200 * ca / 3.0;
+    d2 = 160 * ca / 3.0;
+    d3 = 80 * ca / 3.0;
+    const double A1[6][6] = {{-d1 - l_dt, i1, 0.0, 0.0, 0.0, 0.0},
+                             {d1, -d2 - i1 - l_dt, i2, 0.0, 0.0, 0.0},
+                             {0.0, d2, -(d3 + i2 + b1) - l_dt, i3, a1, 0.0},
+                             {0.0, 0.0, d3, -(i3 + b2) - l_dt, 0.0, a2},
+                             {0.0, 0.0, b1, 0.0, -a1 - l_dt, 0.0},
+                             {0.0, 0.0, 0.0, b2, 0.0, -a2 - l_dt}};
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+      for (int j = 0; j < 6; j++) A[i][j] = A1[i][j];
+  }
 #pragma unroll
   for (int i = 0; i < 5; i++) {
     int piv = i;
@@ -270,6 +308,50 @@ HD bool kahp_step(double ca_old, double ca, double DT, double *g) {
   return true;
 }
 
+HD bool kahp_step(double ca_old, double ca, double DT, double *g) { return kahp_solve<false>(ca_old, ca, DT, g); }
+HD bool kahp_be(double ca, double DT, double *g) { return kahp_solve<true>(ca, ca, DT, g); }
+
+// The BE ion update after the KAHP step (go_update_ion, go_ion.cu:438-511):
+// Ca currents at V with the present gates and reversal, every gate by
+// exponential Euler at V (c_KC at the present Ca), both Ca pools by RK4 and
+// the reversal from the new second pool.
+HD void update_ion_be(double v, double area, const double *c, double &ca, double &ca2, double &rev, double DT,
+                      double *g) {
+  const double Ca_val = ca, Ca2_val = ca2;
+  const double I_Ca1 = 1e-3 * c[g_CaHVA] / area * g[ch_CaHVA] * g[ch_CaHVA] * g[ci_CaHVA] * (v - V_CA);
+  const double I_Ca2 = 1e-3 * c[g_CaLVA] / area * g[cl_CaLVA] * g[cl_CaLVA] * g[cm_CaLVA] * (v - (rev));
+  ee(g[m_NaT], inf_m_NaT(v), tau_m_NaT(v), DT);
+  ee(g[h_NaT], inf_h_NaT(v), tau_h_NaT(v), DT);
+  ee(g[r_NaR], inf_r_NaR(v), tau_r_NaR(v), DT);
+  ee(g[s_NaR], inf_s_NaR(v), tau_s_NaR(v), DT);
+  ee(g[p_NaP], inf_p_NaP(v), tau_p_NaP(v), DT);
+  ee(g[n_KV], inf_n_KV(v), tau_n_KV(v), DT);
+  ee(g[a_KA], inf_a_KA(v), tau_a_KA(v), DT);
+  ee(g[b_KA], inf_b_KA(v), tau_b_KA(v), DT);
+  ee(g[c_KC], inf_c_KC(v, Ca_val), tau_c_KC(v, Ca_val), DT);
+  ee(g[sl_Kslow], inf_sl_Kslow(v), tau_sl_Kslow(v), DT);
+  ee(g[ch_CaHVA], inf_ch_CaHVA(v), tau_ch_CaHVA(v), DT);
+  ee(g[ci_CaHVA], inf_ci_CaHVA(v), tau_ci_CaHVA(v), DT);
+  ee(g[cl_CaLVA], inf_cl_CaLVA(v), tau_cl_CaLVA(v), DT);
+  ee(g[cm_CaLVA], inf_cm_CaLVA(v), tau_cm_CaLVA(v), DT);
+  ee(g[hf_HCN1], inf_hf_HCN1(v), tau_hf_HCN1(v), DT);
+  ee(g[hf_HCN2], inf_hf_HCN2(v), tau_hf_HCN2(v), DT);
+  ee(g[hs_HCN1], inf_hs_HCN1(v), tau_hs_HCN1(v), DT);
+  ee(g[hs_HCN2], inf_hs_HCN2(v), tau_hs_HCN2(v), DT);
+  const double den = 2.0 * FARADAY * SHELL1_D;
+  double k1 = DT * (-I_Ca1 / den - B_CA1 * (Ca_val - CA1_0));
+  double k2 = DT * (-I_Ca1 / den - B_CA1 * (Ca_val + k1 / 2.0 - CA1_0));
+  double k3 = DT * (-I_Ca1 / den - B_CA1 * (Ca_val + k2 / 2.0 - CA1_0));
+  double k4 = DT * (-I_Ca1 / den - B_CA1 * (Ca_val + k3 - CA1_0));
+  ca += (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
+  k1 = DT * (-I_Ca2 / den - B_CA1 * (Ca2_val - CA1_0));
+  k2 = DT * (-I_Ca2 / den - B_CA1 * (Ca2_val + k1 / 2.0 - CA1_0));
+  k3 = DT * (-I_Ca2 / den - B_CA1 * (Ca2_val + k2 / 2.0 - CA1_0));
+  k4 = DT * (-I_Ca2 / den - B_CA1 * (Ca2_val + k3 - CA1_0));
+  ca2 += (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
+  rev = (1e3) * (8.313424 * (23.0 + 273.15)) / (2 * FARADAY) * log(CA1_OUT / ca2);
+}
+
 // go_update_matrix (solve_rkc.cu:288-314), terms in the reference's order
 HD double current(double v, double iext, double rev, const double *c, const double *g) {
   return (-iext
@@ -304,6 +386,39 @@ HD void cn_ge(const double *c, const double *g, double rev, double iext, double 
         + c[g_HCN1] * (g[hf_HCN1] + g[hs_HCN1])
         + c[g_HCN2] * (g[hf_HCN2] + g[hs_HCN2]));
   es = (c[g_leak] * V_LEAK + iext
+        + c[g_NaT] * V_NA * g[m_NaT] * g[m_NaT] * g[m_NaT] * g[h_NaT]
+        + c[g_NaR] * V_NA * g[r_NaR] * g[s_NaR]
+        + c[g_NaP] * V_NA * g[p_NaP]
+        + c[g_CaHVA] * V_CA * g[ch_CaHVA] * g[ch_CaHVA] * g[ci_CaHVA]
+        + c[g_CaLVA] * (rev) * g[cl_CaLVA] * g[cl_CaLVA] * g[cm_CaLVA]
+        + c[g_KAHP] * V_K * (g[o1_KAHP] + g[o2_KAHP])
+        + c[g_KV] * V_K * g[n_KV] * g[n_KV] * g[n_KV] * g[n_KV]
+        + c[g_KA] * V_K * g[a_KA] * g[a_KA] * g[a_KA] * g[b_KA]
+        + c[g_KC] * V_K * g[c_KC]
+        + c[g_Kslow] * V_K * g[sl_Kslow]
+        + c[g_HCN1] * V_H * (g[hf_HCN1] + g[hs_HCN1])
+        + c[g_HCN2] * V_H * (g[hf_HCN2] + g[hs_HCN2]));
+}
+// BE path: the system diagonal addend and right-hand side with the
+// capacitive part first (go_update_matrix, solve_bem.cu:323-347)
+HD void be_ge(const double *c, const double *g, double rev, double iext, double cmdt, double v, double &gs,
+              double &es) {
+  gs = (cmdt
+        + c[g_leak]
+        + c[g_NaT] * g[m_NaT] * g[m_NaT] * g[m_NaT] * g[h_NaT]
+        + c[g_NaR] * g[r_NaR] * g[s_NaR]
+        + c[g_NaP] * g[p_NaP]
+        + c[g_CaHVA] * g[ch_CaHVA] * g[ch_CaHVA] * g[ci_CaHVA]
+        + c[g_CaLVA] * g[cl_CaLVA] * g[cl_CaLVA] * g[cm_CaLVA]
+        + c[g_KAHP] * (g[o1_KAHP] + g[o2_KAHP])
+        + c[g_KV] * g[n_KV] * g[n_KV] * g[n_KV] * g[n_KV]
+        + c[g_KA] * g[a_KA] * g[a_KA] * g[a_KA] * g[b_KA]
+        + c[g_KC] * g[c_KC]
+        + c[g_Kslow] * g[sl_Kslow]
+        + c[g_HCN1] * (g[hf_HCN1] + g[hs_HCN1])
+        + c[g_HCN2] * (g[hf_HCN2] + g[hs_HCN2]));
+  es = (cmdt * v
+        + c[g_leak] * V_LEAK + iext
         + c[g_NaT] * V_NA * g[m_NaT] * g[m_NaT] * g[m_NaT] * g[h_NaT]
         + c[g_NaR] * V_NA * g[r_NaR] * g[s_NaR]
         + c[g_NaP] * V_NA * g[p_NaP]

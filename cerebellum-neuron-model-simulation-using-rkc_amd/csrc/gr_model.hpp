@@ -71,6 +71,8 @@ HD double inf_ch_Ca(double v) { return (a_ch_Ca(v) / (a_ch_Ca(v) + b_ch_Ca(v)));
 HD double inf_ci_Ca(double v) { return (a_ci_Ca(v) / (a_ci_Ca(v) + b_ci_Ca(v))); }
 
 HD void cn(double &x, double inf, double tau, double DT) { x = (2.0 * DT * inf + (2.0 * tau - DT) * x) / (2.0 * tau + DT); }
+// exponential Euler, the BE path's gate update: x = inf + (x - inf) exp(-DT / tau)
+HD void ee(double &x, double inf, double tau, double DT) { x = inf + (x - inf) * mexp(-DT / tau); }
 
 // Na scheme constants of one compartment type (gr_ion.cu:57-69, 88-89), with
 // a = (Oon / Con)^(1/4), b = (Ooff / Coff)^(1/4) and their fourth powers
@@ -141,26 +143,48 @@ HD void na_matrix(double M[12][12], const NaType &T, double alpha, double beta, 
   M[11][11] = -(69.3 + Oon + s2dt);
 }
 
-// One Crank-Nicolson step of the Na scheme (gr_ion.cu:86-202) on
-// x = (c1..c5, i1..i6, o): (A(v_new) - 2/DT) x' = -(A(v_old) + 2/DT) x by
-// Gaussian elimination with partial pivoting.  Every loop is unrolled so
-// the system stays in registers; a row exchange is a conditional swap with
-// constant indices.  Returns false, leaving x alone, on a pivot below
-// 1e-12 (the reference returns early there).
-HD bool na_step(const NaType &T, double v_old, double v_new, double DT, double *x) {
+// One step of the Na scheme on x = (c1..c5, i1..i6, o), the system solved by
+// Gaussian elimination with partial pivoting as gr_ion.cu:150-199 / :276-330
+// write it (strictly larger |pivot| wins, rows swapped from the pivot column
+// on).  BE = false: the Crank-Nicolson step of gr_Na_update_2order
+// (gr_ion.cu:86-202), (A(v_new) - 2/DT) x' = -(A(v_old) + 2/DT) x.
+// BE = true: the implicit-Euler step of gr_Na_update (gr_ion.cu:209-357),
+// (I - DT A(v_new)) x' = x with the matrix formed as the reference forms it
+// (every entry times -DT, then 1 added to the diagonal); v_old is unused.
+// Every loop is unrolled so the system stays in registers (the elimination
+// sits in the same body as the matrix build for that); a row exchange is a
+// conditional swap with constant indices.  Returns false, leaving x alone,
+// on a pivot below 1e-12 (the reference returns early there).
+template <bool BE>
+HD bool na_solve(const NaType &T, double v_old, double v_new, double DT, double *x) {
   double M[12][12], r[12];
-  double alpha = T.Aalpha * mexp((v_old - T.Vshift) / 10.0);
-  double beta = T.Abeta * mexp(-(v_old - T.Vshift) / 10.0);
-  na_matrix(M, T, alpha, beta, -2.0 / DT);
+  if constexpr (BE) {
+    const double alpha = T.Aalpha * mexp((v_new - T.Vshift) / 10.0);
+    const double beta = T.Abeta * mexp(-(v_new - T.Vshift) / 10.0);
+    na_matrix(M, T, alpha, beta, 0.0);
 #pragma unroll
-  for (int i = 0; i < 12; i++) {
-    r[i] = 0.0;
+    for (int i = 0; i < 12; i++)
 #pragma unroll
-    for (int j = 0; j < 12; j++) r[i] -= x[j] * M[i][j];
+      for (int j = 0; j < 12; j++) M[i][j] *= -DT;
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+      M[i][i] += 1.0;
+      r[i] = x[i];
+    }
+  } else {
+    double alpha = T.Aalpha * mexp((v_old - T.Vshift) / 10.0);
+    double beta = T.Abeta * mexp(-(v_old - T.Vshift) / 10.0);
+    na_matrix(M, T, alpha, beta, -2.0 / DT);
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+      r[i] = 0.0;
+#pragma unroll
+      for (int j = 0; j < 12; j++) r[i] -= x[j] * M[i][j];
+    }
+    alpha = T.Aalpha * mexp((v_new - T.Vshift) / 10.0);
+    beta = T.Abeta * mexp(-(v_new - T.Vshift) / 10.0);
+    na_matrix(M, T, alpha, beta, 2.0 / DT);
   }
-  alpha = T.Aalpha * mexp((v_new - T.Vshift) / 10.0);
-  beta = T.Abeta * mexp(-(v_new - T.Vshift) / 10.0);
-  na_matrix(M, T, alpha, beta, 2.0 / DT);
 #pragma unroll
   for (int i = 0; i < 11; i++) {
     int piv = i;
@@ -205,6 +229,11 @@ HD bool na_step(const NaType &T, double v_old, double v_new, double DT, double *
   return true;
 }
 
+HD bool na_step(const NaType &T, double v_old, double v_new, double DT, double *x) {
+  return na_solve<false>(T, v_old, v_new, DT, x);
+}
+HD bool na_be(const NaType &T, double v, double DT, double *x) { return na_solve<true>(T, v, v, DT, x); }
+
 // the Na states in the scheme's order, gathered from / scattered to the gate
 // vector (ion order: o_Na first, then c1..c5, i1..i6)
 HD void na_gather(const double *g, double *x) {
@@ -228,6 +257,17 @@ HD void update_gates(double vold, double vnew, double ca, double DT, double *g) 
   cn(g[ir_KIR], inf_ir_KIR(v_val), tau_ir_KIR(v_val), DT);
   cn(g[s_KM], inf_s_KM(v_val), tau_s_KM(v_val), DT);
   cn(g[c_KCa], inf_c_KCa(v_val, ca), tau_c_KCa(v_val, ca), DT);
+}
+
+// gr_update_ion (gr_ion.cu:579-600, the BE path): exponential-Euler gates of
+// KV, KA, KIR, KM and KCa at V; Ca and the Ca gates stay
+HD void update_gates_be(double v, double ca, double DT, double *g) {
+  ee(g[n_KV], inf_n_KV(v), tau_n_KV(v), DT);
+  ee(g[a_KA], inf_a_KA(v), tau_a_KA(v), DT);
+  ee(g[b_KA], inf_b_KA(v), tau_b_KA(v), DT);
+  ee(g[ir_KIR], inf_ir_KIR(v), tau_ir_KIR(v), DT);
+  ee(g[s_KM], inf_s_KM(v), tau_s_KM(v), DT);
+  ee(g[c_KCa], inf_c_KCa(v, ca), tau_c_KCa(v, ca), DT);
 }
 
 // gr_update_matrix (solve_rkc.cu:226-249), terms in the reference's order
@@ -256,6 +296,28 @@ HD void cn_ge(const double *c, const double *g, double iext, double &gs, double 
         + c[g_KCa] * g[c_KCa]
         + c[g_KM] * g[s_KM]);
   es = (c[g_leak1] * V_LEAK1 + c[g_leak2] * V_LEAK2 + c[g_leak3] * V_LEAK3 + iext
+        + c[g_Na] * V_NA * g[o_Na]
+        + c[g_Ca] * V_CA * g[ch_Ca] * g[ch_Ca] * g[ci_Ca]
+        + c[g_KV] * V_K * g[n_KV] * g[n_KV] * g[n_KV] * g[n_KV]
+        + c[g_KIR] * V_K * g[ir_KIR]
+        + c[g_KA] * V_K * g[a_KA] * g[a_KA] * g[a_KA] * g[b_KA]
+        + c[g_KCa] * V_K * g[c_KCa]
+        + c[g_KM] * V_K * g[s_KM]);
+}
+// BE path: the system diagonal addend and right-hand side with the
+// capacitive part first (gr_update_matrix, solve_bem.cu:258-291)
+HD void be_ge(const double *c, const double *g, double iext, double cmdt, double v, double &gs, double &es) {
+  gs = (cmdt
+        + c[g_leak1] + c[g_leak2] + c[g_leak3]
+        + c[g_Na] * g[o_Na]
+        + c[g_Ca] * g[ch_Ca] * g[ch_Ca] * g[ci_Ca]
+        + c[g_KV] * g[n_KV] * g[n_KV] * g[n_KV] * g[n_KV]
+        + c[g_KIR] * g[ir_KIR]
+        + c[g_KA] * g[a_KA] * g[a_KA] * g[a_KA] * g[b_KA]
+        + c[g_KCa] * g[c_KCa]
+        + c[g_KM] * g[s_KM]);
+  es = (cmdt * v
+        + c[g_leak1] * V_LEAK1 + c[g_leak2] * V_LEAK2 + c[g_leak3] * V_LEAK3 + iext
         + c[g_Na] * V_NA * g[o_Na]
         + c[g_Ca] * V_CA * g[ch_Ca] * g[ch_Ca] * g[ci_Ca]
         + c[g_KV] * V_K * g[n_KV] * g[n_KV] * g[n_KV] * g[n_KV]

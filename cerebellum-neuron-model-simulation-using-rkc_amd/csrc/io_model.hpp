@@ -106,6 +106,33 @@ __host__ __device__ inline void update_ion(T vold, T vnew, T &ca, T gCaH, int co
   update_ion_at(vold, T(0.5) * (vnew + vold), ca, gCaH, comp, DT, g);
 }
 
+// exponential Euler, the BE path's gate update: x = inf + (x - inf) exp(-dt / tau)
+template <typename T> __host__ __device__ inline void ee(T &x, T inf, T tau, T dt) { x = inf + (x - inf) * mexp(-dt / tau); }
+
+// io_update_ion (io_ion.cu:128-161), the BE path: every gate by exponential
+// Euler at V (s_KCa at the present Ca, m_Na instantaneous), then the Ca RK4
+// with the Ca current of the already-updated r_CaH gate.
+template <typename T>
+__host__ __device__ inline void update_ion_be(T v, T &ca, T gCaH, int comp, T DT, T *g) {
+  const T Ca_val = ca;
+  ee(g[k_CaL], k_inf(v), tau_k(v), DT);
+  ee(g[l_CaL], l_inf(v), tau_l(v), DT);
+  ee(g[n_Kdr], n_inf(v), tau_n(v), DT);
+  ee(g[p_Kdr], p_inf(v), tau_n(v), DT);
+  ee(g[x_K], x_inf(v), tau_x(v), DT);
+  ee(g[r_CaH], r_inf(v), tau_r(v), DT);
+  ee(g[q_H], q_inf(v), tau_q(v), DT);
+  ee(g[s_KCa], s_inf(Ca_val), tau_s(Ca_val), DT);
+  g[m_Na] = m_inf(v);
+  ee(g[h_Na], h_inf(v, comp), tau_h(v, comp), DT);
+  const T I_Ca1 = gCaH * g[r_CaH] * g[r_CaH] * (v - T(V_CA));
+  const T k1 = DT * (-I_Ca1 * T(CA_PH) - T(BETA_CA) * (Ca_val));
+  const T k2 = DT * (-I_Ca1 * T(CA_PH) - T(BETA_CA) * (Ca_val + k1 / T(2)));
+  const T k3 = DT * (-I_Ca1 * T(CA_PH) - T(BETA_CA) * (Ca_val + k2 / T(2)));
+  const T k4 = DT * (-I_Ca1 * T(CA_PH) - T(BETA_CA) * (Ca_val + k3));
+  ca += (k1 + k2 * T(2) + k3 * T(2) + k4) / T(6);
+}
+
 // CN path sums before halving (io_cnm_vec_initialize / io_update_matrix,
 // solve_cnm.cu:751-768, 793-810)
 __host__ __device__ inline void cn_ge(const double *c, const double *g, double iext, double &gs, double &es) {
@@ -118,6 +145,30 @@ __host__ __device__ inline void cn_ge(const double *c, const double *g, double i
         + c[g_KCa] * g[s_KCa]
         + c[g_H] * g[q_H]);
   es = (c[g_leak] * V_LEAK + iext
+        + c[g_CaL] * V_CA * g[k_CaL] * g[k_CaL] * g[k_CaL] * g[l_CaL]
+        + c[g_Na] * V_NA * g[m_Na] * g[m_Na] * g[m_Na] * g[h_Na]
+        + c[g_Kdr] * V_K * g[n_Kdr] * g[p_Kdr]
+        + c[g_K] * V_K * g[x_K] * g[x_K] * g[x_K] * g[x_K]
+        + c[g_CaH] * V_CA * g[r_CaH] * g[r_CaH]
+        + c[g_KCa] * V_K * g[s_KCa]
+        + c[g_H] * V_H * g[q_H]);
+}
+
+// BE path: the system diagonal addend and right-hand side with the
+// capacitive part first (io_update_matrix, solve_bem.cu:478-508)
+__host__ __device__ inline void be_ge(const double *c, const double *g, double iext, double cmdt, double v,
+                                      double &gs, double &es) {
+  gs = (cmdt
+        + c[g_leak]
+        + c[g_CaL] * g[k_CaL] * g[k_CaL] * g[k_CaL] * g[l_CaL]
+        + c[g_Na] * g[m_Na] * g[m_Na] * g[m_Na] * g[h_Na]
+        + c[g_Kdr] * g[n_Kdr] * g[p_Kdr]
+        + c[g_K] * g[x_K] * g[x_K] * g[x_K] * g[x_K]
+        + c[g_CaH] * g[r_CaH] * g[r_CaH]
+        + c[g_KCa] * g[s_KCa]
+        + c[g_H] * g[q_H]);
+  es = (cmdt * v
+        + c[g_leak] * V_LEAK + iext
         + c[g_CaL] * V_CA * g[k_CaL] * g[k_CaL] * g[k_CaL] * g[l_CaL]
         + c[g_Na] * V_NA * g[m_Na] * g[m_Na] * g[m_Na] * g[h_Na]
         + c[g_Kdr] * V_K * g[n_Kdr] * g[p_Kdr]

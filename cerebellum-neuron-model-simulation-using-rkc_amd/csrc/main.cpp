@@ -17,10 +17,10 @@
 //                    fn refresh, spectral-radius floor)
 //   --chunk K        macro steps per device launch between host writes (64)
 //
-// This build provides all four populations with the RKC and CN solvers;
-// asking for BE prints an error and exits with status 1.  Each population
-// advances one H_MAX = 0.125 ms output interval per row, in sub-steps of its
-// solver's DT (RKC: one step of H_MAX; CN: 0.025 ms, main.cu:193-200).
+// This build provides all four populations with all three solvers.  Each
+// population advances one H_MAX = 0.125 ms output interval per row, in
+// sub-steps of its solver's DT (RKC: one step of H_MAX; CN and BE: 0.025 ms,
+// main.cu:193-200).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -102,13 +102,6 @@ int main(int argc, char *argv[]) {
     fr[k] = std::fopen((std::string(kTypeNames[k]) + "_raster_by_" + s + ".csv").c_str(), "w");
     nx[k] = std::atoi(argv[kSolverArg[k] + 1]);
     ny[k] = std::atoi(argv[kSolverArg[k] + 2]);
-  }
-  for (int k = 0; k < 4; k++) {
-    if (nx[k] * ny[k] > 0 && solver[k] == CB_SOLVER_BE) {
-      std::printf("Error!! -> %s population with the %s solver is not provided by this build\n", kTypeNames[k],
-                  kSolverNames[solver[k]]);
-      return 1;
-    }
   }
   std::FILE *dbg = std::fopen("debug_file.csv", "w");
   if (dbg) std::fclose(dbg);

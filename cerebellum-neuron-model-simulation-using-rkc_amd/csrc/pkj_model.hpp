@@ -79,11 +79,25 @@ __host__ __device__ inline void cn_gate(T &x, T inf, T tau, T DT) {
   x = (T(2.0) * DT * inf + (T(2.0) * tau - DT) * x) / (T(2.0) * tau + DT);
 }
 
+// exponential Euler, the BE path's gate update (pkj_ion.cu:112-129):
+// x = inf + (x - inf) exp(-dt / tau)
 template <typename T>
+__host__ __device__ inline void ee_gate(T &x, T inf, T tau, T DT) {
+  x = inf + (x - inf) * ex<T>(-DT / tau);
+}
+
+// one gate step by Crank-Nicolson (EE = false) or exponential Euler (EE = true)
+template <typename T, bool EE>
+__host__ __device__ inline void step_gate(T &x, T inf, T tau, T DT) {
+  if constexpr (EE) ee_gate<T>(x, inf, tau, DT);
+  else cn_gate<T>(x, inf, tau, DT);
+}
+
+template <typename T, bool EE = false>
 __host__ __device__ inline void ab_gate(T &x, T a, T b, T DT) {
   T tau = T(1.0) / (a + b);
   T inf = a / (a + b);
-  cn_gate<T>(x, inf, tau, DT);
+  step_gate<T, EE>(x, inf, tau, DT);
 }
 
 template <typename T> __host__ __device__ inline T kdr_alpha(T u) {
@@ -119,21 +133,22 @@ __host__ __device__ inline void steady_state(T v, T ca, T *g) {
   g[z_K2] = T(1.0) / (T(1.0) + T(0.20) / ca);
 }
 
-// Crank-Nicolson update of all 18 gates at the interval midpoint voltage
-// vm over dt, with the quantised Ca caq (pkj_ion.cu:337-354).
-template <typename T>
-__host__ __device__ inline void gates_cn(T vm, T caq, T DT, T *g) {
-  ab_gate<T>(g[m_NaF], lf<T>(vm, 35.0, 0.0, 5.0, -10.0), lf<T>(vm, 7.0, 0.0, 65.0, 20.0), DT);
-  ab_gate<T>(g[h_NaF], lf<T>(vm, 0.225, 1.0, 80.0, 10.0), lf<T>(vm, 7.5, 0.0, -3.0, -18.0), DT);
-  ab_gate<T>(g[m_NaP], lf<T>(vm, 200.0, 1.0, -18.0, -16.0), lf<T>(vm, 25.0, 1.0, 58.0, 8.0), DT);
-  ab_gate<T>(g[m_CaP], lf<T>(vm, 8.5, 1.0, -8.0, -12.5), lf<T>(vm, 35.0, 1.0, 74.0, 14.5), DT);
-  ab_gate<T>(g[h_CaP], lf<T>(vm, 0.0015, 1.0, 29.0, 8.0), lf<T>(vm, 0.0055, 1.0, 23.0, -8.0), DT);
-  ab_gate<T>(g[m_CaT], lf<T>(vm, 2.60, 1.0, 21.0, -8.0), lf<T>(vm, 0.180, 1.0, 40.0, 4.0), DT);
-  ab_gate<T>(g[h_CaT], lf<T>(vm, 0.0025, 1.0, 40.0, 8.0), lf<T>(vm, 0.190, 1.0, 50.0, -10.0), DT);
+// Update of all 18 gates at voltage vm over dt with the quantised Ca caq:
+// Crank-Nicolson (EE = false, pkj_ion.cu:337-354, vm the interval midpoint)
+// or exponential Euler (EE = true, pkj_ion.cu:112-129, the BE path).
+template <typename T, bool EE>
+__host__ __device__ inline void gates_upd(T vm, T caq, T DT, T *g) {
+  ab_gate<T, EE>(g[m_NaF], lf<T>(vm, 35.0, 0.0, 5.0, -10.0), lf<T>(vm, 7.0, 0.0, 65.0, 20.0), DT);
+  ab_gate<T, EE>(g[h_NaF], lf<T>(vm, 0.225, 1.0, 80.0, 10.0), lf<T>(vm, 7.5, 0.0, -3.0, -18.0), DT);
+  ab_gate<T, EE>(g[m_NaP], lf<T>(vm, 200.0, 1.0, -18.0, -16.0), lf<T>(vm, 25.0, 1.0, 58.0, 8.0), DT);
+  ab_gate<T, EE>(g[m_CaP], lf<T>(vm, 8.5, 1.0, -8.0, -12.5), lf<T>(vm, 35.0, 1.0, 74.0, 14.5), DT);
+  ab_gate<T, EE>(g[h_CaP], lf<T>(vm, 0.0015, 1.0, 29.0, 8.0), lf<T>(vm, 0.0055, 1.0, 23.0, -8.0), DT);
+  ab_gate<T, EE>(g[m_CaT], lf<T>(vm, 2.60, 1.0, 21.0, -8.0), lf<T>(vm, 0.180, 1.0, 40.0, 4.0), DT);
+  ab_gate<T, EE>(g[h_CaT], lf<T>(vm, 0.0025, 1.0, 40.0, 8.0), lf<T>(vm, 0.190, 1.0, 50.0, -10.0), DT);
   {
     T e = ex<T>((vm + T(82.0)) / T(7.0));
-    cn_gate<T>(g[m_Kh1], T(0.8) / (T(1.0) + e), T(7.6), DT);
-    cn_gate<T>(g[m_Kh2], T(0.2) / (T(1.0) + e), T(36.8), DT);
+    step_gate<T, EE>(g[m_Kh1], T(0.8) / (T(1.0) + e), T(7.6), DT);
+    step_gate<T, EE>(g[m_Kh2], T(0.2) / (T(1.0) + e), T(36.8), DT);
   }
   {
     T a = kdr_alpha<T>(vm - T(8.0));
@@ -142,18 +157,23 @@ __host__ __device__ inline void gates_cn(T vm, T caq, T DT, T *g) {
     T a2 = kdr_alpha<T>(vm + T(12.0));
     T b2 = T(5.0) * ex<T>(-(vm + T(147.0)) / T(30.0));
     T tau = T(1.0) / (a2 + b2);
-    cn_gate<T>(g[m_Kdr], inf, tau, DT);
+    step_gate<T, EE>(g[m_Kdr], inf, tau, DT);
   }
-  cn_gate<T>(g[h_Kdr], T(1.0) / (T(1.0) + ex<T>((vm + T(25.0)) / T(4.0))),
+  step_gate<T, EE>(g[h_Kdr], T(1.0) / (T(1.0) + ex<T>((vm + T(25.0)) / T(4.0))),
              (vm < T(-25.0)) ? T(1200.0) : T(10.0), DT);
-  cn_gate<T>(g[m_KM], T(1.0) / (T(1.0) + ex<T>(-(vm + T(35.0)) / T(10.0))),
+  step_gate<T, EE>(g[m_KM], T(1.0) / (T(1.0) + ex<T>(-(vm + T(35.0)) / T(10.0))),
              T(200.0) / (T(3.3) * (ex<T>((vm + T(35.0)) / T(20.0))) + ex<T>(-(vm + T(35.0)) / T(20.0))), DT);
-  ab_gate<T>(g[m_KA], lf<T>(vm, 1.40, 1.0, 27.0, -12.0), lf<T>(vm, 0.490, 1.0, 30.0, 4.0), DT);
-  ab_gate<T>(g[h_KA], lf<T>(vm, 0.0175, 1.0, 50.0, 8.0), lf<T>(vm, 1.30, 1.0, 13.0, -10.0), DT);
-  ab_gate<T>(g[m_KC], T(7.5), lf<T>(vm, 0.110, 0.0, -35.0, 14.9), DT);
-  cn_gate<T>(g[z_KC], T(1.0) / (T(1.0) + T(4.00) / caq), T(10.0), DT);
-  ab_gate<T>(g[m_K2], T(25.0), lf<T>(vm, 0.075, 0.0, 25.0, 6.0), DT);
-  cn_gate<T>(g[z_K2], T(1.0) / (T(1.0) + T(0.20) / caq), T(10.0), DT);
+  ab_gate<T, EE>(g[m_KA], lf<T>(vm, 1.40, 1.0, 27.0, -12.0), lf<T>(vm, 0.490, 1.0, 30.0, 4.0), DT);
+  ab_gate<T, EE>(g[h_KA], lf<T>(vm, 0.0175, 1.0, 50.0, 8.0), lf<T>(vm, 1.30, 1.0, 13.0, -10.0), DT);
+  ab_gate<T, EE>(g[m_KC], T(7.5), lf<T>(vm, 0.110, 0.0, -35.0, 14.9), DT);
+  step_gate<T, EE>(g[z_KC], T(1.0) / (T(1.0) + T(4.00) / caq), T(10.0), DT);
+  ab_gate<T, EE>(g[m_K2], T(25.0), lf<T>(vm, 0.075, 0.0, 25.0, 6.0), DT);
+  step_gate<T, EE>(g[z_K2], T(1.0) / (T(1.0) + T(0.20) / caq), T(10.0), DT);
+}
+
+template <typename T>
+__host__ __device__ inline void gates_cn(T vm, T caq, T DT, T *g) {
+  gates_upd<T, false>(vm, caq, DT, g);
 }
 
 // Ca shell update (RK4 over dt of dCa/dt = I - B (Ca - Ca0)), the reversal
@@ -176,6 +196,29 @@ __host__ __device__ inline double ca_rk4(double v_old, double &ca, double &rev, 
   Ca_val = 0.5 * (Ca_val + k1);
   rev = fixed_rev ? 12.5 * log(CA_OUT / CA_0) : 13.361624877 * log(CA_OUT / Ca_val);
   return floor((Ca_val * 1000 - 0.04) / 0.099986667) * 0.099986667 + 0.04;
+}
+
+// pkj_update_ion (pkj_ion.cu:94-151), the BE path: every gate by
+// exponential Euler at V (KC / K2 at the quantised present Ca), then the Ca
+// shell by RK4 with the present gates' Ca current clamped at zero from above
+// and divided by shell x area -- the stored shell volume already carries the
+// area, so the area enters twice, as in the reference -- and the reversal
+// from the present Ca, fixed on compartments of the soma type.
+__host__ __device__ inline void update_ion_be(double v, double &ca, double &rev, double shell, double area,
+                                              bool soma, double gCaP_open, double gCaT_open, double DT,
+                                              double *g) {
+  const double Ca_val = ca;
+  const double cafloor = floor((Ca_val * 1000 - 0.04) / 0.099986667) * 0.099986667 + 0.04;
+  double I_Ca1 = 1e-3 * (v - rev) * (gCaP_open + gCaT_open);
+  if (I_Ca1 > 0.0) I_Ca1 = 0.0;
+  gates_upd<double, true>(v, cafloor, DT, g);
+  const double l_shell = shell * area;
+  const double k1 = DT * (-I_Ca1 / (2.0 * FARADAY * l_shell) - B_CA * (Ca_val - CA_0));
+  const double k2 = DT * (-I_Ca1 / (2.0 * FARADAY * l_shell) - B_CA * (Ca_val + k1 / 2.0 - CA_0));
+  const double k3 = DT * (-I_Ca1 / (2.0 * FARADAY * l_shell) - B_CA * (Ca_val + k2 / 2.0 - CA_0));
+  const double k4 = DT * (-I_Ca1 / (2.0 * FARADAY * l_shell) - B_CA * (Ca_val + k3 - CA_0));
+  ca += (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
+  rev = soma ? 12.5 * log(CA_OUT / CA_0) : 13.361624877 * log(CA_OUT / Ca_val);
 }
 
 // ---------------------------------------------------------------------------
@@ -312,6 +355,38 @@ __host__ __device__ inline void cn_ge(const double *c, const double *g, double r
         + c[g_KC] * g[m_KC] * g[z_KC] * g[z_KC]
         + c[g_K2] * g[m_K2] * g[z_K2] * g[z_K2]);
   es = (+c[g_leak] * (V_LEAK) + iext
+        + c[g_NaF] * g[m_NaF] * g[m_NaF] * g[m_NaF] * g[h_NaF] * (V_NA)
+        + c[g_NaP] * g[m_NaP] * g[m_NaP] * g[m_NaP] * (V_NA)
+        + c[g_CaP] * g[m_CaP] * g[h_CaP] * (rev)
+        + c[g_CaT] * g[m_CaT] * g[h_CaT] * (rev)
+        + c[g_Kh] * g[m_Kh1] * (V_KH)
+        + c[g_Kh] * g[m_Kh2] * (V_KH)
+        + c[g_Kdr] * g[m_Kdr] * g[m_Kdr] * g[h_Kdr] * (V_K)
+        + c[g_KM] * g[m_KM] * (V_K)
+        + c[g_KA] * g[m_KA] * g[m_KA] * g[m_KA] * g[m_KA] * g[h_KA] * (V_K)
+        + c[g_KC] * g[m_KC] * g[z_KC] * g[z_KC] * (V_K)
+        + c[g_K2] * g[m_K2] * g[z_K2] * g[z_K2] * (V_K));
+}
+
+// BE path: the system diagonal addend and right-hand side with the
+// capacitive part first (pkj_update_matrix, solve_bem.cu:407-445)
+__host__ __device__ inline void be_ge(const double *c, const double *g, double rev, double iext, double cmdt,
+                                      double v, double &gs, double &es) {
+  gs = (cmdt
+        + c[g_leak]
+        + c[g_NaF] * g[m_NaF] * g[m_NaF] * g[m_NaF] * g[h_NaF]
+        + c[g_NaP] * g[m_NaP] * g[m_NaP] * g[m_NaP]
+        + c[g_CaP] * g[m_CaP] * g[h_CaP]
+        + c[g_CaT] * g[m_CaT] * g[h_CaT]
+        + c[g_Kh] * g[m_Kh1]
+        + c[g_Kh] * g[m_Kh2]
+        + c[g_Kdr] * g[m_Kdr] * g[m_Kdr] * g[h_Kdr]
+        + c[g_KM] * g[m_KM]
+        + c[g_KA] * g[m_KA] * g[m_KA] * g[m_KA] * g[m_KA] * g[h_KA]
+        + c[g_KC] * g[m_KC] * g[z_KC] * g[z_KC]
+        + c[g_K2] * g[m_K2] * g[z_K2] * g[z_K2]);
+  es = (cmdt * v + iext
+        + c[g_leak] * (V_LEAK)
         + c[g_NaF] * g[m_NaF] * g[m_NaF] * g[m_NaF] * g[h_NaF] * (V_NA)
         + c[g_NaP] * g[m_NaP] * g[m_NaP] * g[m_NaP] * (V_NA)
         + c[g_CaP] * g[m_CaP] * g[h_CaP] * (rev)

@@ -60,7 +60,7 @@ typedef struct cb_pop cb_pop;
 typedef struct cb_pop_config {
   int cell_type;            /* CB_PKJ, ... */
   int solver;               /* CB_SOLVER_* */
-  int engine;               /* CB_ENGINE_* (RKC only) */
+  int engine;               /* CB_ENGINE_* (RKC only; CN and BE run on CB_ENGINE_GLOBAL_FP64) */
   int nx, ny;               /* cell grid; nx * ny cells */
   const char *morph_path;   /* morphology file; NULL: "./Pkj.txt" */
   const char *gj_path;      /* gap-junction pair list; NULL: "./Pkj_GJ.csv" (grid with nx > 1 only) */
@@ -85,7 +85,10 @@ int cb_config_default(cb_pop_config *cfg, int cell_type, int solver, int nx, int
 
 /* Creates a population and initialises its solver: xx_initialize +
  * xx_solve_initialize + RKC_vec_initialize (pkj.cu:240-306,
- * pkj_solve.cu:286-366, solve_rkc.cu:756-791). */
+ * pkj_solve.cu:286-366, solve_rkc.cu:756-791); for CN the halved matrix and
+ * xx_cnm_vec_initialize (solve_cnm.cu:420-465), for BE the unhalved matrix
+ * (pkj_solve.cu:238-245).  CB_E_ARG for an unknown solver, CB_E_UNSUP for
+ * CN / BE with a per-cell engine or the Poisson drive. */
 int cb_pop_create(const cb_pop_config *cfg, cb_pop **out);
 
 /* Releases every resource of the population: xx_finalize + xx_solve_finalize
@@ -95,14 +98,18 @@ int cb_pop_destroy(cb_pop *p);
 /* Sets the external current for time t [ms]: xx_set_current (pkj.cu:333-344). */
 int cb_pop_set_current(cb_pop *p, double t);
 
-/* Advances one macro step (H_MAX = 0.125 ms for RKC): xx_solve_update_v
- * (pkj_solve.cu:395-409 -> solve_rkc.cu:1252-1299). */
+/* Advances one step of the population's solver: xx_solve_update_v
+ * (pkj_solve.cu:395-409): RKC one macro step of H_MAX = 0.125 ms
+ * (solve_rkc.cu:1252-1299), CN one 0.025 ms step (xx_solve_by_cnm,
+ * solve_cnm.cu), BE one 0.025 ms step (xx_solve_by_bem, solve_bem.cu). */
 int cb_pop_advance(cb_pop *p);
 
-/* nsteps x (set_current(t0 + k H_MAX); advance), recording compartment
- * rec_comp of every cell after each macro step into rec_out[k * ncell + c]
- * (rec_out may be NULL).  The main loop of main.cu:182-311 for one
- * population, without a host round trip per step where the engine allows. */
+/* nsteps output intervals of H_MAX from t0 (RKC: set_current(t0 + k H_MAX);
+ * advance; CN / BE: the five 0.025 ms sub-steps of main.cu:193-200, each with
+ * set_current(t + l_dt)), recording compartment rec_comp of every cell after
+ * each interval into rec_out[k * ncell + c] (rec_out may be NULL).  The main
+ * loop of main.cu:182-311 for one population, without a host round trip per
+ * step where the engine allows. */
 int cb_pop_run(cb_pop *p, int nsteps, double t0, int rec_comp, double *rec_out);
 
 /* Copies nc values of a state field to host memory. */

@@ -14,6 +14,16 @@
  *   bicg_cusparse_crs       solve_cnm.cu:617-727 (BiCG with A^T, same start and stop; IO)
  *   the CN sub-steps        main.cu:193-200 etc. (l_dt += CN_DT while l_dt < H_MAX)
  *
+ * and the backward-Euler (BE) path of solve_bem.cu, which shares the solvers:
+ *   xx_solve_by_bem         solve_bem.cu:293-305, 350-360, 448-461, 510-531
+ *                           (ion update at the present voltage, val = val_ori,
+ *                           b = 0, xx_update_matrix, the IO gap term, solve)
+ *   xx_update_matrix        solve_bem.cu:258-291, 323-347, 407-445, 476-508
+ *                           (diagonal += Cm/DT + conductances, b += Cm/DT v +
+ *                           reversal-weighted conductances + i_ext)
+ *   cg_cusparse_crs /       solve_bem.cu:7-77, 122-227 (the same CG and BiCG)
+ *   bicg_cusparse_crs
+ *
  * The CN cable matrix is the reference's unflipped matrix halved
  * (pkj_solve.cu:243 and the other xx_solve.cu files): off-diagonal -g/2,
  * diagonal +sum(g)/2; here it is -val/2 of the RKC-signed matrix, which is the
@@ -24,9 +34,14 @@
  * column order.  Without network synapses the reference's synaptic terms add
  * zero and are left out.
  *
- * Pinning (tests/test_oracle_cn.py): on a passive cable the CN step equals the
- * dense solve of (C/DT + L/2 + G/2) v' = (C/DT - L/2 - G/2) v + I, to the
- * CG tolerance when CG converges; CG and BiCG against numpy's dense solve on
+ * The BE matrix is the unflipped, unhalved one (-val of the RKC-signed
+ * matrix, pkj_solve.cu:238-245 with neither the RKC flip nor the CN halving).
+ *
+ * Pinning (tests/test_oracle_cn.py, tests/test_oracle_be.py): on a passive
+ * cable the CN step equals the dense solve of
+ * (C/DT + L/2 + G/2) v' = (C/DT - L/2 - G/2) v + I and the BE step the dense
+ * solve of (C/DT + L + G) v' = (C/DT) v + G E + I, to the CG tolerance when CG
+ * converges; CG and BiCG against numpy's dense solve on
  * symmetric and unsymmetric systems and their 100-iteration truncation on an
  * ill-conditioned one; the time step's second-order convergence on a
  * linear cable with a known solution.
@@ -38,6 +53,7 @@
 #include <string.h>
 
 #define CN_DT_ (0.025)
+#define BE_DT_ (0.025)
 #define H_MAX_ (0.125)
 
 static void cn_ge(const or_pop *p, int id, double *g, double *e)
@@ -61,6 +77,28 @@ static void cn_ion(or_pop *p, double DT)
     default:
       for (int i = 0; i < p->nc; i++) p->cn_vold[i] = p->v[i];
       break;
+  }
+}
+
+static void be_ge(const or_pop *p, int id, double DT, double *g, double *e)
+{
+  switch (p->type) {
+    case OR_PKJ: or_pkj_be_ge(p, id, DT, g, e); break;
+    case OR_IO: or_io_be_ge(p, id, DT, g, e); break;
+    case OR_GO: or_go_be_ge(p, id, DT, g, e); break;
+    case OR_GR: or_gr_be_ge(p, id, DT, g, e); break;
+    default: *g = p->cm[id] / DT; *e = (p->cm[id] / DT) * p->v[id] + p->iext[id]; break;
+  }
+}
+
+static void be_ion(or_pop *p, double DT)
+{
+  switch (p->type) {
+    case OR_PKJ: or_pkj_be_ion(p, DT); break;
+    case OR_IO: or_io_be_ion(p, DT); break;
+    case OR_GO: or_go_be_ion(p, DT); break;
+    case OR_GR: or_gr_be_ion(p, DT); break;
+    default: break;
   }
 }
 
@@ -152,7 +190,9 @@ static int bicg(or_pop *p, const double *diag, double *x, const double *b)
   return k;
 }
 
-int or_cn_init(or_pop *p)
+/* the matrix (CN: the unflipped matrix halved, BE: unhalved), the diagonal
+   positions and the transpose; 1 when a row lacks its stored diagonal */
+static int sys_init(or_pop *p, int be)
 {
   const int n = p->nc;
   free(p->cn_vold); free(p->cn_gamma1); free(p->cn_omega1); free(p->cn_val); free(p->cn_dpos);
@@ -160,12 +200,12 @@ int or_cn_init(or_pop *p)
   p->cn_vold = zal(n); p->cn_gamma1 = zal(n); p->cn_omega1 = zal(n);
   p->cn_val = zal(p->nnz);
   p->cn_dpos = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
-  for (int j = 0; j < p->nnz; j++) p->cn_val[j] = -p->val[j] / 2.0;
+  for (int j = 0; j < p->nnz; j++) p->cn_val[j] = be ? -p->val[j] : -p->val[j] / 2.0;
   for (int i = 0; i < n; i++) {
     p->cn_dpos[i] = -1;
     for (int j = p->row[i]; j < p->row[i + 1]; j++)
       if (p->col[j] == i) p->cn_dpos[i] = j;
-    if (p->cn_dpos[i] < 0) return 1;   /* the CN path needs the stored diagonal */
+    if (p->cn_dpos[i] < 0) return 1;   /* both paths need the stored diagonal */
   }
   /* transpose (structure, and where each entry comes from) for BiCG */
   p->cn_trow = (int *)calloc(n + 1, sizeof(int));
@@ -181,15 +221,21 @@ int or_cn_init(or_pop *p)
       p->cn_tsrc[t] = j;
     }
   free(fill);
+  p->n_steps = p->n_stages = 0;
+  return 0;
+}
+
+int or_cn_init(or_pop *p)
+{
+  if (sys_init(p, 0) != 0) return 1;
   /* xx_cnm_vec_initialize */
-  for (int i = 0; i < n; i++) {
+  for (int i = 0; i < p->nc; i++) {
     double g, e;
     cn_ge(p, i, &g, &e);
     p->cn_gamma1[i] = g / 2.0;
     p->cn_omega1[i] = e / 2.0;
     p->cn_vold[i] = p->v[i];
   }
-  p->n_steps = p->n_stages = 0;
   return 0;
 }
 
@@ -239,4 +285,56 @@ int or_cn_advance(or_pop *p, double t)
 int or_cn_solve_test(or_pop *p, const double *diag, const double *b, double *x, int sym)
 {
   return sym ? cg(p, diag, x, b) : bicg(p, diag, x, b);
+}
+
+/* ---------------- the backward-Euler path ---------------- */
+
+int or_be_init(or_pop *p) { return sys_init(p, 1); }
+
+/* the BE system of the present state (xx_update_matrix after val = val_ori,
+   b = 0, then the IO gap term): the diagonal and the right-hand side */
+static void be_assemble(const or_pop *p, double DT, double *diag, double *b)
+{
+  for (int i = 0; i < p->nc; i++) {
+    double g, e;
+    be_ge(p, i, DT, &g, &e);
+    diag[i] = p->cn_val[p->cn_dpos[i]] + g;
+    b[i] = 0.0 + e;
+  }
+  if (p->type == OR_IO && p->n > 1)
+    for (int c = 0; c < p->n; c++) b[c * 3] = or_io_gap_add(p, c, b[c * 3]);
+}
+
+/* test hook: the BE system of the present state, without the ion update */
+int or_be_assemble_test(or_pop *p, double *diag, double *b)
+{
+  be_assemble(p, BE_DT_, diag, b);
+  return 0;
+}
+
+/* one BE step of BE_DT (xx_solve_by_bem) */
+int or_be_step(or_pop *p)
+{
+  const int n = p->nc;
+  const double DT = BE_DT_;
+  be_ion(p, DT);
+  double *diag = zal(n), *b = zal(n);
+  be_assemble(p, DT, diag, b);
+  const int it = (p->type == OR_IO) ? bicg(p, diag, p->v, b) : cg(p, diag, p->v, b);
+  p->n_steps++;
+  p->n_stages += it;
+  free(diag); free(b);
+  return 0;
+}
+
+/* the sub-step loop for one H_MAX output interval from time t */
+int or_be_advance(or_pop *p, double t)
+{
+  double l_dt = 0.0;
+  while (l_dt < H_MAX_) {
+    or_set_current(p, t + l_dt);
+    or_be_step(p);
+    l_dt += BE_DT_;
+  }
+  return 0;
 }

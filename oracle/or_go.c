@@ -190,6 +190,39 @@ double or_go_rate(int which, double v, double ca)
   return NAN;
 }
 
+/* Gaussian elimination with partial pivoting on the 6 x 6 system A x = rhs,
+   as go_ion.cu:146-198 and :252-304 write it (strictly larger |pivot| wins,
+   rows swapped from the pivot column on); returns 1 and leaves x alone on a
+   pivot below 1e-12 (the reference returns there). */
+static int solve6(double A[6][6], double *x, const double *rhs_in)
+{
+  double rhs[6];
+  memcpy(rhs, rhs_in, sizeof(rhs));
+  for (int i = 0; i < 5; i++) {
+    int piv = i;
+    double pmax = fabs(A[i][i]);
+    for (int j = i + 1; j < 6; j++)
+      if (fabs(A[j][i]) > pmax) { piv = j; pmax = fabs(A[j][i]); }
+    if (fabs(pmax) < 1.0e-12) return 1;
+    if (piv != i) {
+      for (int j = i; j < 6; j++) { double t = A[i][j]; A[i][j] = A[piv][j]; A[piv][j] = t; }
+      double t = rhs[i]; rhs[i] = rhs[piv]; rhs[piv] = t;
+    }
+    for (int j = i + 1; j < 6; j++) {
+      double w = A[j][i] / A[i][i];
+      A[j][i] = 0.0;
+      for (int k = i + 1; k < 6; k++) A[j][k] = A[j][k] - A[i][k] * w;
+      rhs[j] = rhs[j] - rhs[i] * w;
+    }
+  }
+  for (int i = 5; i >= 0; i--) {
+    for (int j = i + 1; j < 6; j++) rhs[i] = rhs[i] - A[i][j] * rhs[j];
+    rhs[i] = rhs[i] / A[i][i];
+  }
+  memcpy(x, rhs, sizeof(rhs));
+  return 0;
+}
+
 /* One KAHP step, go_ion.cu:88-213: Crank-Nicolson on the 6-state scheme
    x = (c1, c2, c3, c4, o1, o2) with Ca-dependent forward rates,
    (A(ca) - 2/DT) x' = -(A(ca_old) + 2/DT) x, solved by Gaussian elimination
@@ -222,30 +255,31 @@ static int kahp_step(double ca_old, double ca, double DT, double *st /* c1 c2 c3
     {0.0, 0.0, d3, -(i3 + b2) - l_dt, 0.0, a2},
     {0.0, 0.0, b1, 0.0, -a1 - l_dt, 0.0},
     {0.0, 0.0, 0.0, b2, 0.0, -a2 - l_dt}};
-  for (int i = 0; i < 5; i++) {
-    int piv = i;
-    double pmax = fabs(A[i][i]);
-    for (int j = i + 1; j < 6; j++)
-      if (fabs(A[j][i]) > pmax) { piv = j; pmax = fabs(A[j][i]); }
-    if (fabs(pmax) < 1.0e-12) return 1;
-    if (piv != i) {
-      for (int j = i; j < 6; j++) { double t = A[i][j]; A[i][j] = A[piv][j]; A[piv][j] = t; }
-      double t = rhs[i]; rhs[i] = rhs[piv]; rhs[piv] = t;
-    }
-    for (int j = i + 1; j < 6; j++) {
-      double w = A[j][i] / A[i][i];
-      A[j][i] = 0.0;
-      for (int k = i + 1; k < 6; k++) A[j][k] = A[j][k] - A[i][k] * w;
-      rhs[j] = rhs[j] - rhs[i] * w;
-    }
-  }
-  for (int i = 5; i >= 0; i--) {
-    for (int j = i + 1; j < 6; j++) rhs[i] = rhs[i] - A[i][j] * rhs[j];
-    rhs[i] = rhs[i] / A[i][i];
-  }
-  memcpy(st, rhs, sizeof(rhs));
-  return 0;
+  return solve6(A, st, rhs);
 }
+
+/* One implicit-Euler KAHP step at ca (go_KAHP_update, go_ion.cu:214-317, the
+   BE path): (I - DT A(ca)) x' = x, the matrix formed as the reference forms
+   it (every entry times -DT, then 1 added to the diagonal). */
+static int kahp_be(double ca, double DT, double *st)
+{
+  const double i1 = 80e-3, i2 = 80e-3, i3 = 200e-3, a1 = 1.0, a2 = 100e-3, b1 = 160e-3, b2 = 1.2;
+  const double d1 = 200 * ca / 3.0, d2 = 160 * ca / 3.0, d3 = 80 * ca / 3.0;
+  double A[6][6] = {
+    {-d1, i1, 0.0, 0.0, 0.0, 0.0},
+    {d1, -d2 - i1, i2, 0.0, 0.0, 0.0},
+    {0.0, d2, -(d3 + i2 + b1), i3, a1, 0.0},
+    {0.0, 0.0, d3, -(i3 + b2), 0.0, a2},
+    {0.0, 0.0, b1, 0.0, -a1, 0.0},
+    {0.0, 0.0, 0.0, b2, 0.0, -a2}};
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 6; j++) A[i][j] *= -DT;
+  for (int i = 0; i < 6; i++) A[i][i] += 1.0;
+  return solve6(A, st, st);
+}
+
+/* test hook: one implicit-Euler KAHP step, st6 in and out */
+int or_go_kahp_be(double ca, double DT, double *st6) { return kahp_be(ca, DT, st6); }
 
 /* test hook: st6 = (c1, c2, c3, c4, o1, o2) in and out */
 int or_go_kahp(double ca_old, double ca, double DT, double *st6) { return kahp_step(ca_old, ca, DT, st6); }
@@ -557,6 +591,97 @@ void or_go_cn_ge(const or_pop *p, int id, double *gsum, double *esum)
            + c[g_HCN1][id] * (g[hf_HCN1][id] + g[hs_HCN1][id])
            + c[g_HCN2][id] * (g[hf_HCN2][id] + g[hs_HCN2][id]));
   *esum = (c[g_leak][id] * V_LEAK_GO + p->iext[id]
+           + c[g_NaT][id] * V_Na_GO * g[m_NaT][id] * g[m_NaT][id] * g[m_NaT][id] * g[h_NaT][id]
+           + c[g_NaR][id] * V_Na_GO * g[r_NaR][id] * g[s_NaR][id]
+           + c[g_NaP][id] * V_Na_GO * g[p_NaP][id]
+           + c[g_CaHVA][id] * V_Ca_GO * g[ch_CaHVA][id] * g[ch_CaHVA][id] * g[ci_CaHVA][id]
+           + c[g_CaLVA][id] * (p->rev[id]) * g[cl_CaLVA][id] * g[cl_CaLVA][id] * g[cm_CaLVA][id]
+           + c[g_KAHP][id] * V_K_GO * (g[o1_KAHP][id] + g[o2_KAHP][id])
+           + c[g_KV][id] * V_K_GO * g[n_KV][id] * g[n_KV][id] * g[n_KV][id] * g[n_KV][id]
+           + c[g_KA][id] * V_K_GO * g[a_KA][id] * g[a_KA][id] * g[a_KA][id] * g[b_KA][id]
+           + c[g_KC][id] * V_K_GO * g[c_KC][id]
+           + c[g_Kslow][id] * V_K_GO * g[sl_Kslow][id]
+           + c[g_HCN1][id] * V_H_GO * (g[hf_HCN1][id] + g[hs_HCN1][id])
+           + c[g_HCN2][id] * V_H_GO * (g[hf_HCN2][id] + g[hs_HCN2][id]));
+}
+
+/* ---------------- the backward-Euler (BE) path ---------------- */
+#define EEU(gt, INF, TAU) \
+  do { const double inf_ = (INF); g[gt][id] = inf_ + (g[gt][id] - inf_) * exp(-DT / (TAU)); } while (0)
+
+/* go_KAHP_update at the present Ca (go_ion.cu:214-317), then go_update_ion
+   (go_ion.cu:438-511): Ca currents with the present gates and reversal,
+   every gate by exponential Euler at V (c_KC at the present Ca), both Ca
+   pools by RK4 and the reversal from the new second pool; the order of
+   go_solve_by_bem (solve_bem.cu:350-354) */
+void or_go_be_ion(or_pop *p, double DT)
+{
+  double **g = p->ion, **c = p->cond;
+  for (int id = 0; id < p->nc; id++) {
+    double st[6] = {g[c1_KAHP][id], g[c2_KAHP][id], g[c3_KAHP][id], g[c4_KAHP][id], g[o1_KAHP][id], g[o2_KAHP][id]};
+    if (kahp_be(p->ca[id], DT, st) != 0) continue;
+    g[c1_KAHP][id] = st[0]; g[c2_KAHP][id] = st[1]; g[c3_KAHP][id] = st[2];
+    g[c4_KAHP][id] = st[3]; g[o1_KAHP][id] = st[4]; g[o2_KAHP][id] = st[5];
+  }
+  for (int id = 0; id < p->nc; id++) {
+    const double v_val = p->v[id], Ca_val = p->ca[id], Ca2_val = p->ca2[id];
+    const double I_Ca1 = 1e-3 * c[g_CaHVA][id] / p->area[id] * g[ch_CaHVA][id] * g[ch_CaHVA][id] * g[ci_CaHVA][id] *
+                         (p->v[id] - V_Ca_GO);
+    const double I_Ca2 = 1e-3 * c[g_CaLVA][id] / p->area[id] * g[cl_CaLVA][id] * g[cl_CaLVA][id] * g[cm_CaLVA][id] *
+                         (p->v[id] - (p->rev[id]));
+    EEU(m_NaT, inf_m_NaT(v_val), tau_m_NaT(v_val));
+    EEU(h_NaT, inf_h_NaT(v_val), tau_h_NaT(v_val));
+    EEU(r_NaR, inf_r_NaR(v_val), tau_r_NaR(v_val));
+    EEU(s_NaR, inf_s_NaR(v_val), tau_s_NaR(v_val));
+    EEU(p_NaP, inf_p_NaP(v_val), tau_p_NaP(v_val));
+    EEU(n_KV, inf_n_KV(v_val), tau_n_KV(v_val));
+    EEU(a_KA, inf_a_KA(v_val), tau_a_KA(v_val));
+    EEU(b_KA, inf_b_KA(v_val), tau_b_KA(v_val));
+    EEU(c_KC, inf_c_KC(v_val, Ca_val), tau_c_KC(v_val, Ca_val));
+    EEU(sl_Kslow, inf_sl_Kslow(v_val), tau_sl_Kslow(v_val));
+    EEU(ch_CaHVA, inf_ch_CaHVA(v_val), tau_ch_CaHVA(v_val));
+    EEU(ci_CaHVA, inf_ci_CaHVA(v_val), tau_ci_CaHVA(v_val));
+    EEU(cl_CaLVA, inf_cl_CaLVA(v_val), tau_cl_CaLVA(v_val));
+    EEU(cm_CaLVA, inf_cm_CaLVA(v_val), tau_cm_CaLVA(v_val));
+    EEU(hf_HCN1, inf_hf_HCN1(v_val), tau_hf_HCN1(v_val));
+    EEU(hf_HCN2, inf_hf_HCN2(v_val), tau_hf_HCN2(v_val));
+    EEU(hs_HCN1, inf_hs_HCN1(v_val), tau_hs_HCN1(v_val));
+    EEU(hs_HCN2, inf_hs_HCN2(v_val), tau_hs_HCN2(v_val));
+    double k1 = DT * (-I_Ca1 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca_val - Ca1_0_GO));
+    double k2 = DT * (-I_Ca1 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca_val + k1 / 2.0 - Ca1_0_GO));
+    double k3 = DT * (-I_Ca1 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca_val + k2 / 2.0 - Ca1_0_GO));
+    double k4 = DT * (-I_Ca1 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca_val + k3 - Ca1_0_GO));
+    p->ca[id] += (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
+    k1 = DT * (-I_Ca2 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca2_val - Ca1_0_GO));
+    k2 = DT * (-I_Ca2 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca2_val + k1 / 2.0 - Ca1_0_GO));
+    k3 = DT * (-I_Ca2 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca2_val + k2 / 2.0 - Ca1_0_GO));
+    k4 = DT * (-I_Ca2 / (2.0 * F_GO * SHELL1_D_GO) - B_Ca1_GO * (Ca2_val + k3 - Ca1_0_GO));
+    p->ca2[id] += (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
+    p->rev[id] = (1e3) * (8.313424 * (23.0 + 273.15)) / (2 * F_GO) * log(Ca1OUT_GO / p->ca2[id]);
+  }
+}
+
+/* the BE system's diagonal addend and right-hand side, Cm/DT first, in the
+   reference's term order (go_update_matrix, solve_bem.cu:323-347) */
+void or_go_be_ge(const or_pop *p, int id, double DT, double *gsum, double *esum)
+{
+  double *const *c = p->cond, *const *g = p->ion;
+  *gsum = ((p->cm[id] / DT)
+           + c[g_leak][id]
+           + c[g_NaT][id] * g[m_NaT][id] * g[m_NaT][id] * g[m_NaT][id] * g[h_NaT][id]
+           + c[g_NaR][id] * g[r_NaR][id] * g[s_NaR][id]
+           + c[g_NaP][id] * g[p_NaP][id]
+           + c[g_CaHVA][id] * g[ch_CaHVA][id] * g[ch_CaHVA][id] * g[ci_CaHVA][id]
+           + c[g_CaLVA][id] * g[cl_CaLVA][id] * g[cl_CaLVA][id] * g[cm_CaLVA][id]
+           + c[g_KAHP][id] * (g[o1_KAHP][id] + g[o2_KAHP][id])
+           + c[g_KV][id] * g[n_KV][id] * g[n_KV][id] * g[n_KV][id] * g[n_KV][id]
+           + c[g_KA][id] * g[a_KA][id] * g[a_KA][id] * g[a_KA][id] * g[b_KA][id]
+           + c[g_KC][id] * g[c_KC][id]
+           + c[g_Kslow][id] * g[sl_Kslow][id]
+           + c[g_HCN1][id] * (g[hf_HCN1][id] + g[hs_HCN1][id])
+           + c[g_HCN2][id] * (g[hf_HCN2][id] + g[hs_HCN2][id]));
+  *esum = ((p->cm[id] / DT) * p->v[id]
+           + c[g_leak][id] * V_LEAK_GO + p->iext[id]
            + c[g_NaT][id] * V_Na_GO * g[m_NaT][id] * g[m_NaT][id] * g[m_NaT][id] * g[h_NaT][id]
            + c[g_NaR][id] * V_Na_GO * g[r_NaR][id] * g[s_NaR][id]
            + c[g_NaP][id] * V_Na_GO * g[p_NaP][id]

@@ -172,21 +172,14 @@ static void na_matrix(double M[12][12], int t, double alpha, double beta, double
   M[11][4] = 259.8; M[11][10] = Ooff; M[11][11] = -(69.3 + Oon + s2dt);
 }
 
-/* One Na step, gr_ion.cu:86-202.  x = (c1..c5, i1..i6, o); returns 1 and
-   leaves x alone on a pivot below 1e-12 (the reference returns there). */
-static int na_step(int t, double v_old, double v_new, double DT, double *x)
+/* Gaussian elimination with partial pivoting on the 12 x 12 system M r' = r,
+   as gr_ion.cu:150-199 and :276-330 write it (strictly larger |pivot|
+   wins, rows swapped from the pivot column on); returns 1 and leaves r
+   alone on a pivot below 1e-12 (the reference returns there). */
+static int solve12(double M[12][12], double *x, const double *r_in)
 {
-  double M[12][12], r[12];
-  double alpha = Na_Aalpha[t] * exp((v_old - Na_Vshift[t]) / 10.0);
-  double beta = Na_Abeta[t] * exp(-(v_old - Na_Vshift[t]) / 10.0);
-  na_matrix(M, t, alpha, beta, -2.0 / DT);
-  for (int i = 0; i < 12; i++) {
-    r[i] = 0.0;
-    for (int j = 0; j < 12; j++) r[i] -= x[j] * M[i][j];
-  }
-  alpha = Na_Aalpha[t] * exp((v_new - Na_Vshift[t]) / 10.0);
-  beta = Na_Abeta[t] * exp(-(v_new - Na_Vshift[t]) / 10.0);
-  na_matrix(M, t, alpha, beta, 2.0 / DT);
+  double r[12];
+  memcpy(r, r_in, sizeof(r));
   for (int i = 0; i < 11; i++) {
     int piv = i;
     double pmax = fabs(M[i][i]);
@@ -212,11 +205,51 @@ static int na_step(int t, double v_old, double v_new, double DT, double *x)
   return 0;
 }
 
+/* One Na step, gr_ion.cu:86-202.  x = (c1..c5, i1..i6, o); returns 1 and
+   leaves x alone on a pivot below 1e-12 (the reference returns there). */
+static int na_step(int t, double v_old, double v_new, double DT, double *x)
+{
+  double M[12][12], r[12];
+  double alpha = Na_Aalpha[t] * exp((v_old - Na_Vshift[t]) / 10.0);
+  double beta = Na_Abeta[t] * exp(-(v_old - Na_Vshift[t]) / 10.0);
+  na_matrix(M, t, alpha, beta, -2.0 / DT);
+  for (int i = 0; i < 12; i++) {
+    r[i] = 0.0;
+    for (int j = 0; j < 12; j++) r[i] -= x[j] * M[i][j];
+  }
+  alpha = Na_Aalpha[t] * exp((v_new - Na_Vshift[t]) / 10.0);
+  beta = Na_Abeta[t] * exp(-(v_new - Na_Vshift[t]) / 10.0);
+  na_matrix(M, t, alpha, beta, 2.0 / DT);
+  return solve12(M, x, r);
+}
+
+/* One implicit-Euler Na step at voltage v (gr_Na_update, gr_ion.cu:209-357,
+   the BE path): (I - DT A(v)) x' = x, the matrix formed as the reference
+   forms it (every entry times -DT, then 1 added to the diagonal). */
+static int na_be(int t, double v, double DT, double *x)
+{
+  double M[12][12];
+  const double alpha = Na_Aalpha[t] * exp((v - Na_Vshift[t]) / 10.0);
+  const double beta = Na_Abeta[t] * exp(-(v - Na_Vshift[t]) / 10.0);
+  na_matrix(M, t, alpha, beta, 0.0);
+  for (int i = 0; i < 12; i++)
+    for (int j = 0; j < 12; j++) M[i][j] *= -DT;
+  for (int i = 0; i < 12; i++) M[i][i] += 1.0;
+  return solve12(M, x, x);
+}
+
 /* test hook: x12 = (c1..c5, i1..i6, o) in and out */
 int or_gr_na_step(int type, double v_old, double v_new, double dt, double *x12)
 {
   if (type < GR_hill || type > 8) return 2;
   return na_step(type, v_old, v_new, dt, x12);
+}
+
+/* test hook: one implicit-Euler Na step at v */
+int or_gr_na_be(int type, double v, double dt, double *x12)
+{
+  if (type < GR_hill || type > 8) return 2;
+  return na_be(type, v, dt, x12);
 }
 
 static double *dal(int n) { return (double *)calloc(n > 0 ? n : 1, sizeof(double)); }
@@ -420,6 +453,62 @@ void or_gr_cn_ge(const or_pop *p, int id, double *gsum, double *esum)
            + c[g_KCa][id] * g[c_KCa][id]
            + c[g_KM][id] * g[s_KM][id]);
   *esum = (c[g_leak1][id] * V_LEAK1_GR + c[g_leak2][id] * V_LEAK2_GR + c[g_leak3][id] * V_LEAK3_GR + p->iext[id]
+           + c[g_Na][id] * V_Na_GR * g[o_Na][id]
+           + c[g_Ca][id] * V_Ca_GR * g[ch_Ca][id] * g[ch_Ca][id] * g[ci_Ca][id]
+           + c[g_KV][id] * V_K_GR * g[n_KV][id] * g[n_KV][id] * g[n_KV][id] * g[n_KV][id]
+           + c[g_KIR][id] * V_K_GR * g[ir_KIR][id]
+           + c[g_KA][id] * V_K_GR * g[a_KA][id] * g[a_KA][id] * g[a_KA][id] * g[b_KA][id]
+           + c[g_KCa][id] * V_K_GR * g[c_KCa][id]
+           + c[g_KM][id] * V_K_GR * g[s_KM][id]);
+}
+
+/* ---------------- the backward-Euler (BE) path ---------------- */
+#define EEU(gt, INF, TAU) \
+  do { const double inf_ = (INF); g[gt][id] = inf_ + (g[gt][id] - inf_) * exp(-DT / (TAU)); } while (0)
+
+/* gr_Na_update (gr_ion.cu:209-357) then gr_update_ion (gr_ion.cu:579-600),
+   both at the step's starting voltage, as gr_solve_by_bem runs them
+   (solve_bem.cu:298-304); the Ca gates and Ca stay as they are */
+void or_gr_be_ion(or_pop *p, double DT)
+{
+  double **g = p->ion;
+  for (int id = 0; id < p->nc; id++) {
+    const int t = p->compart[id];
+    if (t < GR_hill) continue;
+    double x[12] = {g[c1_Na][id], g[c2_Na][id], g[c3_Na][id], g[c4_Na][id], g[c5_Na][id], g[i1_Na][id],
+                    g[i2_Na][id], g[i3_Na][id], g[i4_Na][id], g[i5_Na][id], g[i6_Na][id], g[o_Na][id]};
+    if (na_be(t, p->v[id], DT, x) != 0) continue;
+    g[c1_Na][id] = x[0]; g[c2_Na][id] = x[1]; g[c3_Na][id] = x[2]; g[c4_Na][id] = x[3]; g[c5_Na][id] = x[4];
+    g[i1_Na][id] = x[5]; g[i2_Na][id] = x[6]; g[i3_Na][id] = x[7]; g[i4_Na][id] = x[8]; g[i5_Na][id] = x[9];
+    g[i6_Na][id] = x[10]; g[o_Na][id] = x[11];
+  }
+  for (int id = 0; id < p->nc; id++) {
+    const double v_val = p->v[id], Ca_val = p->ca[id];
+    EEU(n_KV, inf_n_KV(v_val), tau_n_KV(v_val));
+    EEU(a_KA, inf_a_KA(v_val), tau_a_KA(v_val));
+    EEU(b_KA, inf_b_KA(v_val), tau_b_KA(v_val));
+    EEU(ir_KIR, inf_ir_KIR(v_val), tau_ir_KIR(v_val));
+    EEU(s_KM, inf_s_KM(v_val), tau_s_KM(v_val));
+    EEU(c_KCa, inf_c_KCa(v_val, Ca_val), tau_c_KCa(v_val, Ca_val));
+  }
+}
+
+/* the BE system's diagonal addend and right-hand side, Cm/DT first, in the
+   reference's term order (gr_update_matrix, solve_bem.cu:258-291) */
+void or_gr_be_ge(const or_pop *p, int id, double DT, double *gsum, double *esum)
+{
+  double *const *c = p->cond, *const *g = p->ion;
+  *gsum = ((p->cm[id] / DT)
+           + c[g_leak1][id] + c[g_leak2][id] + c[g_leak3][id]
+           + c[g_Na][id] * g[o_Na][id]
+           + c[g_Ca][id] * g[ch_Ca][id] * g[ch_Ca][id] * g[ci_Ca][id]
+           + c[g_KV][id] * g[n_KV][id] * g[n_KV][id] * g[n_KV][id] * g[n_KV][id]
+           + c[g_KIR][id] * g[ir_KIR][id]
+           + c[g_KA][id] * g[a_KA][id] * g[a_KA][id] * g[a_KA][id] * g[b_KA][id]
+           + c[g_KCa][id] * g[c_KCa][id]
+           + c[g_KM][id] * g[s_KM][id]);
+  *esum = ((p->cm[id] / DT) * p->v[id]
+           + c[g_leak1][id] * V_LEAK1_GR + c[g_leak2][id] * V_LEAK2_GR + c[g_leak3][id] * V_LEAK3_GR + p->iext[id]
            + c[g_Na][id] * V_Na_GR * g[o_Na][id]
            + c[g_Ca][id] * V_Ca_GR * g[ch_Ca][id] * g[ch_Ca][id] * g[ci_Ca][id]
            + c[g_KV][id] * V_K_GR * g[n_KV][id] * g[n_KV][id] * g[n_KV][id] * g[n_KV][id]

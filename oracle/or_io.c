@@ -376,13 +376,14 @@ void or_io_cn_ge(const or_pop *p, int id, double *gsum, double *esum)
            + c[g_H][id] * V_H_IO * g[q_H][id]);
 }
 
-/* the gap current into each cell's dendrite from the stored voltages
-   (io_gap_update, gap.cu:142-160); the CN path subtracts it on the
-   right-hand side (add_io_gap_val, solve_cnm.cu:729-739) */
-double or_io_gap_into(const or_pop *p, int cell)
+/* base minus each gap current into the cell's dendrite from the stored
+   voltages (io_gap_update, gap.cu:142-160), partners in slot order; the CN
+   and BE paths subtract the currents on the right-hand side (add_io_gap_val,
+   solve_cnm.cu:729-739, solve_bem.cu:463-473) */
+double or_io_gap_add(const or_pop *p, int cell, double base)
 {
   int nb[4];
-  double s = 0.0;
+  double s = base;
   io_neighbours(cell, p->nx, p->ny, nb);
   const int post = cell * IO_COMP;
   for (int k = 0; k < 4; k++) {
@@ -391,4 +392,63 @@ double or_io_gap_into(const or_pop *p, int cell)
     s += -1.0 * (G_C_IO * (0.8 * exp(-0.01 * diff_v * diff_v) + 0.2) * (diff_v));
   }
   return s;
+}
+
+double or_io_gap_into(const or_pop *p, int cell) { return or_io_gap_add(p, cell, 0.0); }
+
+/* ---------------- the backward-Euler (BE) path ---------------- */
+#define EEU(gt, INF, TAU) \
+  do { const double inf_ = (INF); g[gt][id] = inf_ + (g[gt][id] - inf_) * exp(-DT / (TAU)); } while (0)
+
+/* io_update_ion, io_ion.cu:128-161 (the BE path): every gate by exponential
+   Euler at V (s_KCa at the present Ca, m_Na instantaneous), then the Ca RK4
+   with the Ca current of the updated r_CaH gate */
+void or_io_be_ion(or_pop *p, double DT)
+{
+  double **g = p->ion;
+  for (int id = 0; id < p->nc; id++) {
+    const int l_comp = p->compart[id];
+    const double v_val = p->v[id], Ca_val = p->ca[id];
+    EEU(k_CaL, k_infty(v_val), tau_k(v_val));
+    EEU(l_CaL, l_infty(v_val), tau_l(v_val));
+    EEU(n_Kdr, n_infty(v_val), tau_n(v_val));
+    EEU(p_Kdr, p_infty(v_val), tau_p(v_val));
+    EEU(x_K, x_infty(v_val), tau_x(v_val));
+    EEU(r_CaH, r_infty(v_val), tau_r(v_val));
+    EEU(q_H, q_infty(v_val), tau_q(v_val));
+    EEU(s_KCa, s_infty(Ca_val), tau_s(Ca_val));
+    g[m_Na][id] = m_infty(v_val);
+    EEU(h_Na, h_infty(v_val, l_comp), tau_h(v_val, l_comp));
+    const double I_Ca1 = p->cond[g_CaH][id] * g[r_CaH][id] * g[r_CaH][id] * (p->v[id] - V_Ca_IO);
+    const double k1 = DT * (-I_Ca1 * CA_PH_IO - BETA_CA_IO * (Ca_val));
+    const double k2 = DT * (-I_Ca1 * CA_PH_IO - BETA_CA_IO * (Ca_val + k1 / 2.0));
+    const double k3 = DT * (-I_Ca1 * CA_PH_IO - BETA_CA_IO * (Ca_val + k2 / 2.0));
+    const double k4 = DT * (-I_Ca1 * CA_PH_IO - BETA_CA_IO * (Ca_val + k3));
+    p->ca[id] += (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
+  }
+}
+
+/* the BE system's diagonal addend and right-hand side, Cm/DT first, in the
+   reference's term order (io_update_matrix, solve_bem.cu:478-508) */
+void or_io_be_ge(const or_pop *p, int id, double DT, double *gsum, double *esum)
+{
+  double *const *c = p->cond, *const *g = p->ion;
+  *gsum = ((p->cm[id] / DT)
+           + c[g_leak][id]
+           + c[g_CaL][id] * g[k_CaL][id] * g[k_CaL][id] * g[k_CaL][id] * g[l_CaL][id]
+           + c[g_Na][id] * g[m_Na][id] * g[m_Na][id] * g[m_Na][id] * g[h_Na][id]
+           + c[g_Kdr][id] * g[n_Kdr][id] * g[p_Kdr][id]
+           + c[g_K][id] * g[x_K][id] * g[x_K][id] * g[x_K][id] * g[x_K][id]
+           + c[g_CaH][id] * g[r_CaH][id] * g[r_CaH][id]
+           + c[g_KCa][id] * g[s_KCa][id]
+           + c[g_H][id] * g[q_H][id]);
+  *esum = ((p->cm[id] / DT) * p->v[id]
+           + c[g_leak][id] * V_LEAK_IO + p->iext[id]
+           + c[g_CaL][id] * V_Ca_IO * g[k_CaL][id] * g[k_CaL][id] * g[k_CaL][id] * g[l_CaL][id]
+           + c[g_Na][id] * V_Na_IO * g[m_Na][id] * g[m_Na][id] * g[m_Na][id] * g[h_Na][id]
+           + c[g_Kdr][id] * V_K_IO * g[n_Kdr][id] * g[p_Kdr][id]
+           + c[g_K][id] * V_K_IO * g[x_K][id] * g[x_K][id] * g[x_K][id] * g[x_K][id]
+           + c[g_CaH][id] * V_Ca_IO * g[r_CaH][id] * g[r_CaH][id]
+           + c[g_KCa][id] * V_K_IO * g[s_KCa][id]
+           + c[g_H][id] * V_H_IO * g[q_H][id]);
 }

@@ -496,3 +496,89 @@ void or_pkj_cn_ge(const or_pop *p, int id, double *gsum, double *esum)
            + c[g_KC][id] * g[m_KC][id] * g[z_KC][id] * g[z_KC][id] * (V_K_PKJ)
            + c[g_K2][id] * g[m_K2][id] * g[z_K2][id] * g[z_K2][id] * (V_K_PKJ));
 }
+
+/* ---------------- the backward-Euler (BE) path ---------------- */
+#define EEU(gt, INF, TAU) \
+  do { const double inf_ = (INF); ion[gt][id] = inf_ + (ion[gt][id] - inf_) * exp(-DT / (TAU)); } while (0)
+
+/* pkj_update_ion, pkj_ion.cu:94-151 (the BE path): every gate by exponential
+   Euler at V, KC / K2 at the quantised present Ca; the Ca shell by RK4 with
+   the present gates' Ca current, clamped at zero from above and divided by
+   shell x area (the stored shell volume already carries the area, so the
+   area enters twice, as in the reference); the reversal from the present Ca,
+   fixed on compartments of the soma type. */
+void or_pkj_be_ion(or_pop *p, double DT)
+{
+#pragma omp parallel for schedule(static)
+  for (int id = 0; id < p->nc; id++) {
+    double **ion = p->ion, **cond = p->cond;
+    const double v_val = p->v[id];
+    const double Ca_val = p->ca[id];
+    const double l_v_Ca = p->rev[id];
+    const double cafloor = floor((Ca_val * 1000 - 0.04) / 0.099986667) * 0.099986667 + 0.04;
+    double I_Ca1 = 1e-3 * (v_val - l_v_Ca)
+                   * (cond[g_CaP][id] * ion[m_CaP][id] * ion[h_CaP][id]
+                      + cond[g_CaT][id] * ion[m_CaT][id] * ion[h_CaT][id]);
+    if (I_Ca1 > 0.0) I_Ca1 = 0.0;
+    EEU(m_NaF, inf_m_NaF(v_val), tau_m_NaF(v_val));
+    EEU(h_NaF, inf_h_NaF(v_val), tau_h_NaF(v_val));
+    EEU(m_NaP, inf_m_NaP(v_val), tau_m_NaP(v_val));
+    EEU(m_CaP, inf_m_CaP(v_val), tau_m_CaP(v_val));
+    EEU(h_CaP, inf_h_CaP(v_val), tau_h_CaP(v_val));
+    EEU(m_CaT, inf_m_CaT(v_val), tau_m_CaT(v_val));
+    EEU(h_CaT, inf_h_CaT(v_val), tau_h_CaT(v_val));
+    EEU(m_Kh1, inf_m_Kh1(v_val), tau_m_Kh1(v_val));
+    EEU(m_Kh2, inf_m_Kh2(v_val), tau_m_Kh2(v_val));
+    EEU(m_Kdr, inf_m_Kdr(v_val), tau_m_Kdr(v_val));
+    EEU(h_Kdr, inf_h_Kdr(v_val), tau_h_Kdr(v_val));
+    EEU(m_KM, inf_m_KM(v_val), tau_m_KM(v_val));
+    EEU(m_KA, inf_m_KA(v_val), tau_m_KA(v_val));
+    EEU(h_KA, inf_h_KA(v_val), tau_h_KA(v_val));
+    EEU(m_KC, inf_m_KC(v_val), tau_m_KC(v_val));
+    EEU(z_KC, inf_z_KC(v_val, cafloor), tau_z_KC(v_val));
+    EEU(m_K2, inf_m_K2(v_val), tau_m_K2(v_val));
+    EEU(z_K2, inf_z_K2(v_val, cafloor), tau_z_K2(v_val));
+    const double l_shell = p->shell[id] * p->area[id];
+    const double k1 = DT * (-I_Ca1 / (2.0 * F_PKJ * l_shell) - B_Ca1_PKJ * (Ca_val - Ca1_0_PKJ));
+    const double k2 = DT * (-I_Ca1 / (2.0 * F_PKJ * l_shell) - B_Ca1_PKJ * (Ca_val + k1 / 2.0 - Ca1_0_PKJ));
+    const double k3 = DT * (-I_Ca1 / (2.0 * F_PKJ * l_shell) - B_Ca1_PKJ * (Ca_val + k2 / 2.0 - Ca1_0_PKJ));
+    const double k4 = DT * (-I_Ca1 / (2.0 * F_PKJ * l_shell) - B_Ca1_PKJ * (Ca_val + k3 - Ca1_0_PKJ));
+    p->ca[id] += (k1 + k2 * 2.0 + k3 * 2.0 + k4) / 6.0;
+    p->rev[id] = (p->compart[id] == PKJ_SOMA) ? 12.5 * log(Ca1OUT_PKJ / Ca1_0_PKJ)
+                                              : 13.361624877 * log(Ca1OUT_PKJ / Ca_val);
+  }
+}
+
+/* the BE system's diagonal addend and right-hand side, Cm/DT first, in the
+   reference's term order (pkj_update_matrix, solve_bem.cu:407-445) */
+void or_pkj_be_ge(const or_pop *p, int id, double DT, double *gsum, double *esum)
+{
+  double *const *c = p->cond, *const *g = p->ion;
+  const double l_v_Ca = p->rev[id];
+  *gsum = ((p->cm[id] / DT)
+           + c[g_leak][id]
+           + c[g_NaF][id] * g[m_NaF][id] * g[m_NaF][id] * g[m_NaF][id] * g[h_NaF][id]
+           + c[g_NaP][id] * g[m_NaP][id] * g[m_NaP][id] * g[m_NaP][id]
+           + c[g_CaP][id] * g[m_CaP][id] * g[h_CaP][id]
+           + c[g_CaT][id] * g[m_CaT][id] * g[h_CaT][id]
+           + c[g_Kh][id] * g[m_Kh1][id]
+           + c[g_Kh][id] * g[m_Kh2][id]
+           + c[g_Kdr][id] * g[m_Kdr][id] * g[m_Kdr][id] * g[h_Kdr][id]
+           + c[g_KM][id] * g[m_KM][id]
+           + c[g_KA][id] * g[m_KA][id] * g[m_KA][id] * g[m_KA][id] * g[m_KA][id] * g[h_KA][id]
+           + c[g_KC][id] * g[m_KC][id] * g[z_KC][id] * g[z_KC][id]
+           + c[g_K2][id] * g[m_K2][id] * g[z_K2][id] * g[z_K2][id]);
+  *esum = ((p->cm[id] / DT) * p->v[id] + p->iext[id]
+           + c[g_leak][id] * (V_LEAK_PKJ)
+           + c[g_NaF][id] * g[m_NaF][id] * g[m_NaF][id] * g[m_NaF][id] * g[h_NaF][id] * (V_Na_PKJ)
+           + c[g_NaP][id] * g[m_NaP][id] * g[m_NaP][id] * g[m_NaP][id] * (V_Na_PKJ)
+           + c[g_CaP][id] * g[m_CaP][id] * g[h_CaP][id] * (l_v_Ca)
+           + c[g_CaT][id] * g[m_CaT][id] * g[h_CaT][id] * (l_v_Ca)
+           + c[g_Kh][id] * g[m_Kh1][id] * (V_KH_PKJ)
+           + c[g_Kh][id] * g[m_Kh2][id] * (V_KH_PKJ)
+           + c[g_Kdr][id] * g[m_Kdr][id] * g[m_Kdr][id] * g[h_Kdr][id] * (V_K_PKJ)
+           + c[g_KM][id] * g[m_KM][id] * (V_K_PKJ)
+           + c[g_KA][id] * g[m_KA][id] * g[m_KA][id] * g[m_KA][id] * g[m_KA][id] * g[h_KA][id] * (V_K_PKJ)
+           + c[g_KC][id] * g[m_KC][id] * g[z_KC][id] * g[z_KC][id] * (V_K_PKJ)
+           + c[g_K2][id] * g[m_K2][id] * g[z_K2][id] * g[z_K2][id] * (V_K_PKJ));
+}

@@ -139,6 +139,23 @@ void or_go_cn_ge(const or_pop *p, int id, double *gsum, double *esum);
 void or_gr_cn_ion(or_pop *p, double dt);
 void or_gr_cn_ge(const or_pop *p, int id, double *gsum, double *esum);
 
+/* backward-Euler path (or_cn.c and the cell-type files) */
+int or_be_init(or_pop *p);
+int or_be_step(or_pop *p);
+int or_be_advance(or_pop *p, double t);
+int or_be_assemble_test(or_pop *p, double *diag, double *b);
+void or_pkj_be_ion(or_pop *p, double dt);
+void or_pkj_be_ge(const or_pop *p, int id, double dt, double *gsum, double *esum);
+void or_io_be_ion(or_pop *p, double dt);
+void or_io_be_ge(const or_pop *p, int id, double dt, double *gsum, double *esum);
+double or_io_gap_add(const or_pop *p, int cell, double base);
+void or_go_be_ion(or_pop *p, double dt);
+void or_go_be_ge(const or_pop *p, int id, double dt, double *gsum, double *esum);
+int or_go_kahp_be(double ca, double dt, double *st6);
+void or_gr_be_ion(or_pop *p, double dt);
+void or_gr_be_ge(const or_pop *p, int id, double dt, double *gsum, double *esum);
+int or_gr_na_be(int type, double v, double dt, double *x12);
+
 /* ---- stimulus ---- */
 void or_set_stim(or_pop *p, int comp, double t0, double t1, const double *amp_per_cell);
 void or_set_current(or_pop *p, double t);

@@ -1,0 +1,136 @@
+"""The backward-Euler engine on the GPU (BE ion updates fused with the system
+build, then the same device CG / BiCG as the CN engine) through the C-ABI,
+against the oracle's BE path.
+
+Both sides run the same algorithm with the same reduction order, so they
+differ only by the device's and the host's exp / log.  How far that carries
+depends on the solve:
+
+* the inferior olive (BiCG converges in ~8 iterations) and the Golgi grid
+  (CG all but converged at 100 iterations) follow the oracle closely: first
+  output interval within 1e-9 mV, then equal spike counts per cell with
+  times within one output interval;
+* Purkinje and granule cells hit the 100-iteration CG cap far from
+  convergence on the unhalved BE system, and the truncated iterate moves by
+  1e-5 .. 1e-3 mV under a one-ulp change of its input (the oracle's own
+  floor, pinned in tests/test_oracle_be.py::test_truncated_cg_noise_floor):
+  Purkinje within 1e-3 mV of the oracle after every output interval and
+  every solve the full 100 iterations; granule cells within 2e-2 mV after
+  the first interval, then equal spike counts with times within one output
+  interval.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cbrkc
+import orc
+from test_oracle_be import L as LB
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_be(pop):
+    assert LB().or_be_init(pop.h) == 0
+    return pop
+
+
+def spikes(trace, thr):
+    prev = np.full(trace.shape[1], 100.0)
+    out = []
+    for k in range(trace.shape[0]):
+        v = trace[k]
+        for c in np.nonzero((v > thr) & (prev <= thr))[0]:
+            out.append((0.125 * (k + 1), int(c)))
+        prev = v
+    return out
+
+
+def run_pair(p, o, steps, rec, comp, first_tol, every_tol=None):
+    tg, to = [], []
+    worst = 0.0
+    for k in range(steps):
+        t = k * 0.125
+        # one output interval: the reference's five BE sub-steps
+        l_dt = 0.0
+        while l_dt < 0.125:
+            p.set_current(t + l_dt)
+            p.advance()
+            l_dt += 0.025
+        LB().or_be_advance(o.h, t)
+        vg, vo = p.read(cbrkc.F_V), o.get(orc.V)
+        d = np.max(np.abs(vg - vo))
+        worst = max(worst, d)
+        if k == 0:
+            print(f"first interval: max |GPU - oracle| = {d:.3e} mV", flush=True)
+            assert d <= first_tol
+        if every_tol is not None:
+            assert d <= every_tol, (k, d)
+        tg.append(vg[rec::comp])
+        to.append(vo[rec::comp])
+    print(f"worst interval: max |GPU - oracle| = {worst:.3e} mV", flush=True)
+    return np.array(tg), np.array(to)
+
+
+def same_spikes(sg, so):
+    key = lambda x: (x[1], x[0])
+    return len(sg) == len(so) and all(a[1] == b[1] and abs(a[0] - b[0]) <= 0.125
+                                      for a, b in zip(sorted(sg, key=key), sorted(so, key=key)))
+
+
+def test_be_pkj_reference_cell(ref_data):
+    v0 = np.array([-65.0])
+    stim = (1599, 0.5, 1e9, 2e-3)
+    p = cbrkc.Population(cbrkc.PKJ, 1, 1, solver=cbrkc.BE, morph=os.path.join(ref_data, "Pkj.txt"), v0=v0,
+                         stim=stim)
+    o = oracle_be(orc.Pop.pkj(os.path.join(ref_data, "Pkj.txt"), 1, 1, v0=v0, libc_rand=False))
+    o.set_stim(*stim)
+    print("BE PKJ 1 cell, 16 output intervals", flush=True)
+    run_pair(p, o, 16, 1599, 1600, 1e-3, every_tol=1e-3)
+    st = p.stats()
+    assert st["steps"] == 80 and st["stages"] == 8000, st   # 100 CG iterations per step, as the oracle
+
+
+def test_be_io_grid(ref_data):
+    from test_oracle_io import io_pop
+    n = 9
+    v0 = np.linspace(-64.0, -56.0, n)
+    stim = (0, 2.0, 12.0, 5.5)
+    p = cbrkc.Population(cbrkc.IO, 3, 3, solver=cbrkc.BE, morph=os.path.join(ref_data, "IO.txt"), v0=v0, stim=stim)
+    o = oracle_be(io_pop(ref_data, 3, 3, v0=v0))
+    orc.lib().or_set_stim(o.h, 0, 2.0, 12.0, orc._dp(np.full(n, 5.5)))
+    print("BE IO 3x3, 240 output intervals", flush=True)
+    tg, to = run_pair(p, o, 240, 2, 3, 1e-9)
+    sg, so = spikes(tg, 10.0), spikes(to, 10.0)
+    print(f"BE IO: spikes GPU {sg} oracle {so}", flush=True)
+    assert len(so) >= 3 and same_spikes(sg, so)
+    st = p.stats()
+    assert st["steps"] == 1200 and 0 < st["stages"] < 100 * 1200, st
+
+
+def test_be_go_grid(ref_data):
+    from test_oracle_go import go_pop
+    v0 = np.array([-61.0, -60.5, -60.0, -59.5])
+    p = cbrkc.Population(cbrkc.GO, 2, 2, solver=cbrkc.BE, morph=os.path.join(ref_data, "Go_131.txt"), v0=v0)
+    o = oracle_be(go_pop(ref_data, 2, 2, v0=v0))
+    print("BE GO 2x2, 480 output intervals", flush=True)
+    tg, to = run_pair(p, o, 480, 30, 131, 1e-9)
+    sg, so = spikes(tg, 10.0), spikes(to, 10.0)
+    print(f"BE GO: spikes GPU {sg} oracle {so}", flush=True)
+    assert len(so) >= 4 and same_spikes(sg, so)
+
+
+def test_be_gr_cells(ref_data):
+    from test_oracle_gr import gr_pop
+    v0 = np.array([-70.0, -72.0])
+    amp = np.array([0.012e-3, 0.02e-3])
+    p = cbrkc.Population(cbrkc.GR, 2, 1, solver=cbrkc.BE, morph=os.path.join(ref_data, "Gr.txt"), v0=v0,
+                         stim=(16, 2.0, 1e9, amp))
+    o = oracle_be(gr_pop(ref_data, 2, 1, v0=v0))
+    o.set_stim(16, 2.0, 1e9, amp)
+    print("BE GR 2 cells, 96 output intervals", flush=True)
+    tg, to = run_pair(p, o, 96, 77, 578, 2e-2)
+    sg, so = spikes(tg, 10.0), spikes(to, 10.0)
+    print(f"BE GR: spikes GPU {sg} oracle {so}", flush=True)
+    assert len(so) >= 2 and same_spikes(sg, so)
