@@ -375,8 +375,9 @@ def test_truncated_cg_noise_floor(ref_data):
     100-iteration cap far from convergence (Purkinje, granule cells): a
     1e-12 mV change of the initial voltage moves the result by 1e-7 .. 2e-4
     mV (Purkinje, 16 output intervals) and 1e-5 .. 5e-3 mV (granule cells,
-    first output interval).  These bound the GPU parity bars (1e-3 and 2e-2
-    mV), which sit a factor of 5 above them."""
+    first output interval).  The GPU parity bar for Purkinje cells (1e-3 mV
+    every interval) sits a factor of 5 above the first; granule cells are
+    compared by spike times after the first interval."""
     from test_oracle_gr import gr_pop
     pk = []
     for eps in (0.0, 1e-12):
