@@ -248,6 +248,7 @@ int or_cn_step(or_pop *p)
   double *gamma2 = zal(n), *omega2 = zal(n), *diag = zal(n), *b = zal(n);
   if (p->type == OR_IO && p->n > 1)
     for (int c = 0; c < p->n; c++) omega2[c * 3] = or_io_gap_into(p, c);
+  if (p->n_syn_in) or_syn_cn(p, gamma2, omega2);
   for (int i = 0; i < n; i++) {
     double g, e;
     cn_ge(p, i, &g, &e);
@@ -296,10 +297,15 @@ int or_be_init(or_pop *p) { return sys_init(p, 1); }
 static void be_assemble(const or_pop *p, double DT, double *diag, double *b)
 {
   for (int i = 0; i < p->nc; i++) {
+    diag[i] = p->cn_val[p->cn_dpos[i]];
+    b[i] = 0.0;
+  }
+  if (p->n_syn_in) or_syn_be(p, diag, b);   /* add_*_val before xx_update_matrix */
+  for (int i = 0; i < p->nc; i++) {
     double g, e;
     be_ge(p, i, DT, &g, &e);
-    diag[i] = p->cn_val[p->cn_dpos[i]] + g;
-    b[i] = 0.0 + e;
+    diag[i] = diag[i] + g;
+    b[i] = b[i] + e;
   }
   if (p->type == OR_IO && p->n > 1)
     for (int c = 0; c < p->n; c++) b[c * 3] = or_io_gap_add(p, c, b[c * 3]);

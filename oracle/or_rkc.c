@@ -481,6 +481,7 @@ static int trigger(const or_pop *p)
 int or_rkc_advance(or_pop *p)
 {
   double **vec = p->vec;
+  or_syn_macro_start(p);   /* Golgi / Purkinje synaptic current of this macro step */
   if (p->sum_dt != 0.0) {
     or_model_update_ion_and_b(p, p->sum_dt);
     p->sum_dt = 0.0;
@@ -514,6 +515,7 @@ void or_model_update_ion_and_b(or_pop *p, double sum_dt)
     default: break;
   }
   update_b_gap(p, p->vec[OR_Y], 1);   /* io_update_ion_and_matrix adds the gap currents */
+  if (p->n_syn_in) or_syn_ion_b(p, p->vec[OR_Y], sum_dt);
   /* optional (off in the reference): F_0 = F(y_n) with the new b, so the
      next step does not start from a derivative of the old current */
   if (p->refresh_fn) or_dydt(p, p->vec[OR_YN], p->vec[OR_FN]);
@@ -706,5 +708,6 @@ void or_pop_free(or_pop *p)
   for (int i = 0; i < OR_NVEC; i++) free(p->vec[i]);
   free(p->stim_amp);
   free(p->syn_g);
+  free(p->syn_s);
   free(p);
 }

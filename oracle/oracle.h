@@ -36,6 +36,24 @@ enum { OR_MMAX = 0, OR_ABSH, OR_SPRAD, OR_NACCPT, OR_ATOL, OR_RTOL, OR_IDID,
 #define OR_MAX_COND 16
 #define OR_MAX_ION 24
 
+/* synapse kinds (syn.cu), in the order the reference creates them */
+enum { OR_SYN_MFGR = 0, OR_SYN_GOGR, OR_SYN_GRGO, OR_SYN_GRPKJ, OR_SYN_MLIPKJ };
+
+struct or_pop;
+/* one synapse set (the reference's synapse_t, syn.cuh:76-82): global pre and
+   post compartments, the decaying traces, weight, conductance val and the
+   last presynaptic voltage */
+typedef struct or_syn {
+  int kind, n;
+  int *pre, *post;
+  double *x[3];
+  double *weight, *val, *old_v;
+  double E;                      /* reversal potential [mV] */
+  const struct or_pop *pre_pop;  /* spike source (GoGr: Golgi; GrGo, GrPkj: granule) */
+  uint32_t key[2];               /* Philox key of the random draws */
+  long step;                     /* updates done (Philox counter) */
+} or_syn;
+
 typedef struct or_pop {
   int type;           /* OR_PKJ, ... */
   int n, nx, ny;      /* cells */
@@ -81,6 +99,11 @@ typedef struct or_pop {
   long gid_offset;
   int refresh_fn;     /* 1: recompute fn = F(y_n) after every ion update */
   double rho_floor;   /* lower bound applied to every spectral-radius estimate (0: none) */
+  /* incoming synapse sets (or_syn.c), in the reference's launch order, and
+     the macro step's synaptic current of the Golgi / Purkinje RKC paths */
+  int n_syn_in;
+  or_syn *syn_in[2];
+  double *syn_s;
 } or_pop;
 
 /* ---- deterministic sums (or_rkc.c) ---- */
@@ -186,6 +209,26 @@ void or_rkcrho_test(or_pop *p);
 void or_rkcint_test(or_pop *p, double t, double *dst);
 void or_pop_set_other(or_pop *p, int i, double v);
 void or_pop_set_hwork(or_pop *p, int i, double v);
+
+/* ---- synapses (or_syn.c), syn.cu ---- */
+or_syn *or_mfgr_create(int n_gr, uint64_t seed);
+or_syn *or_grgo_create(const or_pop *gr, const or_pop *go);
+or_syn *or_gogr_create(const or_pop *go, const or_pop *gr);
+or_syn *or_grpkj_create(const or_pop *gr, const or_pop *pkj, const double *x1600, uint64_t seed);
+or_syn *or_mlipkj_create(int n_pkj, uint64_t seed);
+void or_syn_free(or_syn *s);
+void or_syn_update(or_syn *s, double t);
+void or_net_update(or_syn *mfgr, or_syn *gogr, or_syn *grgo, or_syn *grpkj, double t);
+int or_pop_attach_syn(or_pop *p, or_syn *s);
+void or_syn_add_b(or_pop *p, const double *vv);
+void or_syn_macro_start(or_pop *p);
+void or_syn_ion_b(or_pop *p, const double *vv, double sum_dt);
+void or_syn_cn(const or_pop *p, double *gamma2, double *omega2);
+void or_syn_be(const or_pop *p, double *diag, double *b);
+int or_syn_n(const or_syn *s);
+void or_syn_get(const or_syn *s, int field, double *dst);
+void or_syn_set(or_syn *s, int field, const double *src);
+void or_syn_comp(const or_syn *s, int *pre, int *post);
 
 /* ---- Philox4x32-10 (or_philox.c), Salmon et al., SC'11 ---- */
 void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
