@@ -37,7 +37,14 @@ EXPORTS = [
     "cb_io_initialize", "cb_io_solve_update_v", "cb_io_set_current", "cb_io_output_file", "cb_io_finalize",
     "cb_go_initialize", "cb_go_solve_update_v", "cb_go_set_current", "cb_go_output_file", "cb_go_finalize",
     "cb_gr_initialize", "cb_gr_solve_update_v", "cb_gr_set_current", "cb_gr_output_file", "cb_gr_finalize",
+    "cb_mfgr_create", "cb_gogr_create", "cb_grgo_create", "cb_grpkj_create", "cb_mlipkj_create",
+    "cb_syn_finalize", "cb_gr_synapse_update", "cb_go_synapse_update", "cb_pkj_synapse_update",
+    "cb_syn_size", "cb_syn_comp", "cb_syn_read", "cb_syn_build",
 ]
+
+# synapse kinds and fields (cb_syn_build, cb_syn_read)
+SYN_MFGR, SYN_GOGR, SYN_GRGO, SYN_GRPKJ, SYN_MLIPKJ = range(5)
+SYN_TRACE0, SYN_TRACE1, SYN_TRACE2, SYN_WEIGHT, SYN_VAL, SYN_OLD_V = range(6)
 
 # The synthetic Purkinje trees (10 um soma on a thin tapering trunk) need
 # three safeguards the reference does not have to stay stable: with the
@@ -132,6 +139,21 @@ def lib():
         L.cb_gr_set_current.argtypes = [vp, ctypes.c_double]
         L.cb_gr_output_file.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p]
         L.cb_gr_finalize.argtypes = [vp]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.cb_mfgr_create.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(vp)]
+        L.cb_gogr_create.argtypes = [vp, vp, ctypes.POINTER(vp)]
+        L.cb_grgo_create.argtypes = [vp, vp, ctypes.POINTER(vp)]
+        L.cb_grpkj_create.argtypes = [vp, vp, ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(vp)]
+        L.cb_mlipkj_create.argtypes = [vp, vp, ctypes.c_uint64, ctypes.POINTER(vp)]
+        L.cb_syn_finalize.argtypes = [vp]
+        L.cb_gr_synapse_update.argtypes = [ctypes.c_double, vp, vp]
+        L.cb_go_synapse_update.argtypes = [ctypes.c_double, vp]
+        L.cb_pkj_synapse_update.argtypes = [ctypes.c_double, vp, vp]
+        L.cb_syn_size.argtypes = [vp, ip]
+        L.cb_syn_comp.argtypes = [vp, ip, ip]
+        L.cb_syn_read.argtypes = [vp, ctypes.c_int, dp]
+        L.cb_syn_build.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp,
+                                   ctypes.c_uint64, ip, ip, ip, dp]
         _lib = L
     return _lib
 
@@ -395,3 +417,105 @@ def gr_output_file(p, memv, raster_prev, t, v_csv, raster_csv):
 
 def gr_finalize(p):
     p.close()
+
+
+# ---- synapses (syn.cu) ----
+
+class Synapses:
+    """One synapse set (the reference's synapse_t); coupled into its
+    postsynaptic population on creation, uncoupled by close()."""
+
+    def __init__(self, h, keep=()):
+        self.h = h
+        self._keep = list(keep)
+        n = ctypes.c_int()
+        _check(lib().cb_syn_size(self.h, ctypes.byref(n)))
+        self.n = n.value
+
+    def comp(self):
+        pre, post = np.zeros(self.n, dtype=np.int32), np.zeros(self.n, dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        if self.n:
+            _check(lib().cb_syn_comp(self.h, pre.ctypes.data_as(ip), post.ctypes.data_as(ip)))
+        return pre, post
+
+    def read(self, field):
+        out = np.zeros(self.n)
+        if self.n:
+            _check(lib().cb_syn_read(self.h, field, _dp(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            _check(lib().cb_syn_finalize(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _ph(p):
+    return p.h if p is not None else None
+
+
+def _syn(fn, *args, keep=()):
+    h = ctypes.c_void_p()
+    _check(fn(*args, ctypes.byref(h)))
+    return Synapses(h, keep)
+
+
+def mfgr_create(gr, seed):
+    """mfgr_create (syn.cu:104-128)."""
+    return _syn(lib().cb_mfgr_create, _ph(gr), seed, keep=(gr,))
+
+
+def gogr_create(go, gr):
+    """gogr_create (syn.cu:339-410)."""
+    return _syn(lib().cb_gogr_create, _ph(go), _ph(gr), keep=(go, gr))
+
+
+def grgo_create(gr, go):
+    """grgo_create (syn.cu:181-261)."""
+    return _syn(lib().cb_grgo_create, _ph(gr), _ph(go), keep=(gr, go))
+
+
+def grpkj_create(gr, pkj, location_csv, seed):
+    """grpkj_create (syn.cu:519-644)."""
+    return _syn(lib().cb_grpkj_create, _ph(gr), _ph(pkj), location_csv.encode() if location_csv else None, seed,
+                keep=(gr, pkj))
+
+
+def mlipkj_create(pkj, gr, seed):
+    """mlipkj_create (syn.cu:690-738)."""
+    return _syn(lib().cb_mlipkj_create, _ph(pkj), _ph(gr), seed, keep=(pkj, gr))
+
+
+def gr_synapse_update(t, mfgr, gogr):
+    _check(lib().cb_gr_synapse_update(t, _ph(mfgr), _ph(gogr)))
+
+
+def go_synapse_update(t, grgo):
+    _check(lib().cb_go_synapse_update(t, _ph(grgo)))
+
+
+def pkj_synapse_update(t, grpkj, mlipkj=None):
+    _check(lib().cb_pkj_synapse_update(t, _ph(grpkj), _ph(mlipkj)))
+
+
+def syn_build(kind, nx_a, ny_a, nx_b=0, ny_b=0, x1600=None, seed=0):
+    """The connection rules on the host (no device): (pre, post, weight)."""
+    n = ctypes.c_int()
+    xp = None
+    if x1600 is not None:
+        x1600 = np.ascontiguousarray(x1600, dtype=np.float64)
+        xp = _dp(x1600)
+    _check(lib().cb_syn_build(kind, nx_a, ny_a, nx_b, ny_b, xp, seed, ctypes.byref(n), None, None, None))
+    pre, post = np.zeros(n.value, dtype=np.int32), np.zeros(n.value, dtype=np.int32)
+    w = np.zeros(n.value)
+    ip = ctypes.POINTER(ctypes.c_int)
+    _check(lib().cb_syn_build(kind, nx_a, ny_a, nx_b, ny_b, xp, seed, ctypes.byref(n), pre.ctypes.data_as(ip),
+                              post.ctypes.data_as(ip), _dp(w)))
+    return pre, post, w

@@ -18,6 +18,8 @@
 
 using namespace cbrkc;
 
+struct cb_syn;
+
 struct cb_pop {
   std::unique_ptr<Engine> eng;
   int cell_type = CB_PKJ;
@@ -25,6 +27,33 @@ struct cb_pop {
   int engine = CB_ENGINE_GLOBAL_FP64;
   int comp = 0;
   int device = 0;
+  int nx = 0, ny = 0;
+  // incoming synapse sets (at most two: granule cells take mossy-fibre and
+  // Golgi input) and their CSR by postsynaptic compartment on the device
+  std::vector<cb_syn *> syn_in;
+  int *syn_row = nullptr, *syn_slot = nullptr;
+  ~cb_pop() {
+    eng.reset();
+    if (syn_row) (void)hipFree(syn_row);
+    if (syn_slot) (void)hipFree(syn_slot);
+  }
+};
+
+// one synapse set (the reference's synapse_t, syn.cuh:76-82) on the device
+struct cb_syn {
+  SynHost host;
+  cb_pop *pre = nullptr, *post = nullptr;
+  int *pre_d = nullptr;
+  double *x_d[3] = {nullptr, nullptr, nullptr};
+  double *w_d = nullptr, *val_d = nullptr, *old_d = nullptr;
+  uint64_t seed = 0;
+  uint32_t step = 0;
+  int device = 0;
+  ~cb_syn() {
+    for (void *p : {(void *)pre_d, (void *)x_d[0], (void *)x_d[1], (void *)x_d[2], (void *)w_d, (void *)val_d,
+                    (void *)old_d})
+      if (p) (void)hipFree(p);
+  }
 };
 
 
@@ -45,6 +74,7 @@ int guarded(F &&f) {
     g_err = e.msg;
     if (e.msg.rfind("HIP error", 0) == 0) return CB_E_HIP;
     if (e.code > 0) return CB_E_SOLVER;
+    if (e.code == CB_E_UNSUP) return CB_E_UNSUP;
     return CB_E_ARG;
   } catch (const std::exception &e) {
     g_err = e.what();
@@ -206,6 +236,127 @@ int create_gr(const cb_pop_config &cfg, cb_pop **out) {
   return CB_OK;
 }
 
+template <typename T>
+T *dev_copy(const std::vector<T> &h) {
+  void *p = nullptr;
+  CB_HIP(hipMalloc(&p, sizeof(T) * (h.empty() ? 1 : h.size())));
+  if (!h.empty()) CB_HIP(hipMemcpy(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  return (T *)p;
+}
+
+// rebuilds the population's incoming-synapse CSR (entries of a compartment
+// in set order, then synapse order) and hands it to the engine; the previous
+// CSR is released only once the engine has taken the new one
+void recouple(cb_pop *p) {
+  CB_HIP(hipSetDevice(p->device));
+  const int nn = p->eng->nc();
+  SynIn in;
+  int *row_d = nullptr, *slot_d = nullptr;
+  if (!p->syn_in.empty()) {
+    std::vector<int> row(nn + 1, 0);
+    for (const cb_syn *s : p->syn_in)
+      for (int k = 0; k < s->host.n; k++) row[s->host.post[k] + 1]++;
+    for (int i = 0; i < nn; i++) row[i + 1] += row[i];
+    std::vector<int> fill(row.begin(), row.end() - 1), slot(row[nn]);
+    for (size_t t = 0; t < p->syn_in.size(); t++) {
+      const cb_syn *s = p->syn_in[t];
+      for (int k = 0; k < s->host.n; k++) slot[fill[s->host.post[k]]++] = (int)(t << 30) | k;
+      in.val[t] = s->val_d;
+      in.E[t] = s->host.E;
+    }
+    row_d = dev_copy(row);
+    slot_d = dev_copy(slot);
+    in.row = row_d;
+    in.slot = slot_d;
+  }
+  p->eng->sync();
+  try {
+    p->eng->set_synapses(in);
+  } catch (...) {
+    if (row_d) (void)hipFree(row_d);
+    if (slot_d) (void)hipFree(slot_d);
+    throw;
+  }
+  if (p->syn_row) (void)hipFree(p->syn_row);
+  if (p->syn_slot) (void)hipFree(p->syn_slot);
+  p->syn_row = row_d;
+  p->syn_slot = slot_d;
+}
+
+// creates a set from the host rules, uploads it and couples it into post
+int make_syn(int kind, cb_pop *a, cb_pop *b, cb_pop *pre, cb_pop *post, const double *x1600, uint64_t seed,
+             cb_syn **out) {
+  std::unique_ptr<cb_syn> s(new cb_syn);
+  const int nx_a = a ? a->nx : 0, ny_a = a ? a->ny : 0, nx_b = b ? b->nx : 0, ny_b = b ? b->ny : 0;
+  build_syn(kind, nx_a, ny_a, nx_b, ny_b, x1600, seed, s->host);
+  s->seed = seed;
+  s->pre = pre;
+  s->post = post;
+  const int n = s->host.n;
+  if (n > 0) {
+    s->device = post->device;
+    CB_HIP(hipSetDevice(s->device));
+    s->pre_d = dev_copy(s->host.pre);
+    s->w_d = dev_copy(s->host.weight);
+    std::vector<double> zero(n, 0.0), thousand(n, 1000.0);
+    for (int k = 0; k < 3; k++) s->x_d[k] = dev_copy(zero);
+    s->val_d = dev_copy(zero);
+    s->old_d = dev_copy(thousand);   // old_v = 1000: no spike on the first update
+    // the reference couples every set except the inhibitory Purkinje input
+    if (kind != SYN_MLIPKJ) {
+      if (post->syn_in.size() >= 2) return fail(CB_E_UNSUP, "a population takes at most two synapse sets");
+      post->syn_in.push_back(s.get());
+      try {
+        recouple(post);
+      } catch (...) {
+        post->syn_in.pop_back();
+        throw;
+      }
+    }
+  }
+  *out = s.release();
+  return CB_OK;
+}
+
+void syn_step(cb_syn *s, double t) {
+  if (!s || s->host.n == 0) return;
+  CB_HIP(hipSetDevice(s->device));
+  const double *pv = nullptr;
+  if (s->pre) {
+    s->pre->eng->sync();
+    pv = s->pre->eng->device_v();
+    if (!pv) throw Error("the presynaptic population's engine exposes no voltage");
+  }
+  s->post->eng->sync();
+  SynDev d{s->host.kind, s->host.n, s->pre_d, s->x_d[0], s->x_d[1], s->x_d[2], s->val_d, s->old_d, s->w_d,
+           s->host.decay[0], s->host.decay[1], s->host.decay[2], (uint32_t)s->seed, (uint32_t)(s->seed >> 32)};
+  hipStream_t st = s->post->eng->stream();
+  syn_update(d, pv, t, s->step, st);
+  CB_HIP(hipStreamSynchronize(st));
+  s->step++;
+}
+
+bool read_location_x(const char *path, std::vector<double> &x, std::string &err) {
+  std::FILE *f = std::fopen(path, "r");
+  if (!f) {
+    err = std::string("no such file ") + path;
+    return false;
+  }
+  x.assign(1600, 0.0);
+  for (int i = 0; i < 1600; i++) {
+    int i1, i5;
+    double i2, i3, i4;
+    if (std::fscanf(f, "%d,%lf,%lf,%lf,%d", &i1, &i2, &i3, &i4, &i5) != 5) {
+      std::fclose(f);
+      err = std::string("PARAM_FILE_READING_ERROR in ") + path;
+      return false;
+    }
+    x[i] = i2;
+  }
+  std::fclose(f);
+  return true;
+}
+
 int solver_from_name(const char *s) {
   if (!s) return -1;
   if (std::strncmp(s, "BE", 2) == 0) return CB_SOLVER_BE;
@@ -243,7 +394,7 @@ int cb_config_default(cb_pop_config *cfg, int cell_type, int solver, int nx, int
 }
 
 int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
-  return guarded([&]() -> int {
+  const int rc = guarded([&]() -> int {
     if (!cfg || !out) return fail(CB_E_ARG, "null argument");
     *out = nullptr;
     if (cfg->nx < 0 || cfg->ny < 0) return fail(CB_E_ARG, "negative grid size");
@@ -335,6 +486,11 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     *out = p.release();
     return CB_OK;
   });
+  if (rc == CB_OK && out && *out) {
+    (*out)->nx = cfg->nx;
+    (*out)->ny = cfg->ny;
+  }
+  return rc;
 }
 
 int cb_pop_destroy(cb_pop *p) {
@@ -606,6 +762,150 @@ int cb_pkj_output_file(const cb_pop *p, const double *memv, double *raster_prev,
   }
   if (fr) std::fclose(fr);
   return CB_OK;
+}
+
+
+int cb_mfgr_create(cb_pop *gr, uint64_t seed, cb_syn **out) {
+  return guarded([&]() -> int {
+    if (!out) return fail(CB_E_ARG, "null argument");
+    if (gr && gr->cell_type != CB_GR) return fail(CB_E_ARG, "mfgr_create takes a granule population");
+    return make_syn(SYN_MFGR, gr, nullptr, nullptr, gr, nullptr, seed, out);
+  });
+}
+
+int cb_gogr_create(cb_pop *go, cb_pop *gr, cb_syn **out) {
+  return guarded([&]() -> int {
+    if (!out) return fail(CB_E_ARG, "null argument");
+    if ((go && go->cell_type != CB_GO) || (gr && gr->cell_type != CB_GR))
+      return fail(CB_E_ARG, "gogr_create takes a Golgi and a granule population");
+    return make_syn(SYN_GOGR, go, gr, go, gr, nullptr, 0, out);
+  });
+}
+
+int cb_grgo_create(cb_pop *gr, cb_pop *go, cb_syn **out) {
+  return guarded([&]() -> int {
+    if (!out) return fail(CB_E_ARG, "null argument");
+    if ((go && go->cell_type != CB_GO) || (gr && gr->cell_type != CB_GR))
+      return fail(CB_E_ARG, "grgo_create takes a granule and a Golgi population");
+    return make_syn(SYN_GRGO, gr, go, gr, go, nullptr, 0, out);
+  });
+}
+
+int cb_grpkj_create(cb_pop *gr, cb_pop *pkj, const char *location_csv, uint64_t seed, cb_syn **out) {
+  return guarded([&]() -> int {
+    if (!out) return fail(CB_E_ARG, "null argument");
+    if ((gr && gr->cell_type != CB_GR) || (pkj && pkj->cell_type != CB_PKJ))
+      return fail(CB_E_ARG, "grpkj_create takes a granule and a Purkinje population");
+    std::vector<double> x;
+    if (gr && pkj && gr->nx * gr->ny > 0 && pkj->nx * pkj->ny > 0) {
+      std::string err;
+      if (!read_location_x(location_csv ? location_csv : "./Pkj_location_info.csv", x, err))
+        return fail(CB_E_IO, err);
+    }
+    return make_syn(SYN_GRPKJ, gr, pkj, gr, pkj, x.empty() ? nullptr : x.data(), seed, out);
+  });
+}
+
+int cb_mlipkj_create(cb_pop *pkj, cb_pop *gr, uint64_t seed, cb_syn **out) {
+  return guarded([&]() -> int {
+    if (!out) return fail(CB_E_ARG, "null argument");
+    if ((gr && gr->cell_type != CB_GR) || (pkj && pkj->cell_type != CB_PKJ))
+      return fail(CB_E_ARG, "mlipkj_create takes a Purkinje and a granule population");
+    return make_syn(SYN_MLIPKJ, pkj, gr, nullptr, pkj, nullptr, seed, out);
+  });
+}
+
+int cb_syn_finalize(cb_syn *s) {
+  return guarded([&]() -> int {
+    if (!s) return CB_OK;
+    std::unique_ptr<cb_syn> own(s);
+    if (s->post) {
+      auto &v = s->post->syn_in;
+      for (size_t t = 0; t < v.size(); t++)
+        if (v[t] == s) {
+          v.erase(v.begin() + t);
+          recouple(s->post);
+          break;
+        }
+    }
+    if (s->host.n > 0) CB_HIP(hipSetDevice(s->device));
+    return CB_OK;
+  });
+}
+
+int cb_gr_synapse_update(double t, cb_syn *mfgr, cb_syn *gogr) {
+  return guarded([&]() -> int {
+    syn_step(mfgr, t);
+    syn_step(gogr, t);
+    return CB_OK;
+  });
+}
+
+int cb_go_synapse_update(double t, cb_syn *grgo) {
+  return guarded([&]() -> int {
+    syn_step(grgo, t);
+    return CB_OK;
+  });
+}
+
+int cb_pkj_synapse_update(double t, cb_syn *grpkj, cb_syn *mlipkj) {
+  (void)mlipkj;   // mlipkj_update is commented out in every branch of pkj_synapse_update
+  return guarded([&]() -> int {
+    syn_step(grpkj, t);
+    return CB_OK;
+  });
+}
+
+int cb_syn_size(const cb_syn *s, int *n) {
+  if (!s || !n) return fail(CB_E_ARG, "null argument");
+  *n = s->host.n;
+  return CB_OK;
+}
+
+int cb_syn_comp(const cb_syn *s, int *pre, int *post) {
+  if (!s || !pre || !post) return fail(CB_E_ARG, "null argument");
+  std::memcpy(pre, s->host.pre.data(), sizeof(int) * s->host.n);
+  std::memcpy(post, s->host.post.data(), sizeof(int) * s->host.n);
+  return CB_OK;
+}
+
+int cb_syn_read(const cb_syn *s, int field, double *dst) {
+  return guarded([&]() -> int {
+    if (!s || !dst) return fail(CB_E_ARG, "null argument");
+    if (s->host.n == 0) return CB_OK;
+    const double *src = nullptr;
+    switch (field) {
+      case CB_SYN_TRACE0: case CB_SYN_TRACE1: case CB_SYN_TRACE2: src = s->x_d[field]; break;
+      case CB_SYN_WEIGHT: src = s->w_d; break;
+      case CB_SYN_VAL: src = s->val_d; break;
+      case CB_SYN_OLD_V: src = s->old_d; break;
+      default: return fail(CB_E_ARG, "unknown synapse field");
+    }
+    CB_HIP(hipSetDevice(s->device));
+    if (s->post) s->post->eng->sync();
+    CB_HIP(hipMemcpy(dst, src, sizeof(double) * s->host.n, hipMemcpyDeviceToHost));
+    return CB_OK;
+  });
+}
+
+int cb_syn_build(int kind, int nx_a, int ny_a, int nx_b, int ny_b, const double *x1600, uint64_t seed, int *n,
+                 int *pre, int *post, double *weight) {
+  return guarded([&]() -> int {
+    if (!n) return fail(CB_E_ARG, "null argument");
+    if (kind < CB_SYN_MFGR || kind > CB_SYN_MLIPKJ) return fail(CB_E_ARG, "unknown synapse kind");
+    if (nx_a < 0 || ny_a < 0 || nx_b < 0 || ny_b < 0) return fail(CB_E_ARG, "negative grid size");
+    if (kind == CB_SYN_GRPKJ && nx_a * ny_a > 0 && nx_b * ny_b > 0 && !x1600)
+      return fail(CB_E_ARG, "the granule-Purkinje rule needs the compartment x positions");
+    SynHost h;
+    build_syn(kind, nx_a, ny_a, nx_b, ny_b, x1600, seed, h);
+    *n = h.n;
+    if (pre && post && weight) {
+      std::memcpy(pre, h.pre.data(), sizeof(int) * h.n);
+      std::memcpy(post, h.post.data(), sizeof(int) * h.n);
+      std::memcpy(weight, h.weight.data(), sizeof(double) * h.n);
+    }
+    return CB_OK;
+  });
 }
 
 }  // extern "C"

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "morph.hpp"
+#include "synapse.hpp"
 
 #define CB_HIP(call)                                                                      \
   do {                                                                                    \
@@ -89,7 +90,25 @@ class Engine {
   // many macro steps without host round trips; soma V and spikes are
   // buffered on the device.  Default: loop over advance().
   virtual void run(int nsteps, double t0, int rec_comp, double *rec_out) ;
+  // couples the population's incoming synapses (in.on() false: none); the
+  // device arrays stay owned by the caller and must outlive the coupling
+  virtual void set_synapses(const SynIn &in) {
+    if (in.on()) throw Error("synaptic input is not provided for this population's engine", -5);
+  }
+  // V at the last ion update / solve on the device (elem[v]), the
+  // presynaptic voltage read by synapse updates, and the stream the engine
+  // works on
+  virtual const double *device_v() const { return nullptr; }
+  virtual hipStream_t stream() const { return nullptr; }
 };
+
+// hipStreamCreate re-seeds the C library's rand() state.  The reference
+// draws every population's initial voltages from one rand() sequence seeded
+// once (main.cu:37) and creates no streams, so a stream created between two
+// populations' draws would change the later draws.  This creates the stream
+// with rand()'s state parked on a private buffer and restores the caller's
+// state afterwards (glibc rand() and random() share that state).
+void create_stream(hipStream_t *st);
 
 // Tolerances and the operator-splitting interval of one population.
 struct SolverSpec {

@@ -825,7 +825,7 @@ class CellPkj : public Engine {
   CellPkj(const PkjHost &H, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn)
       : n_(H.n), C_(H.comp), syn_(syn) {
     using namespace pkj;
-    CB_HIP(hipStreamCreate(&st_));
+    create_stream(&st_);
     CB_HIP(hipEventCreate(&ev0_));
     CB_HIP(hipEventCreate(&ev1_));
     const int n = n_, C = C_;

@@ -70,6 +70,7 @@ struct CnSys {
   double *part, *scal;
   int *flag;
   int nc, bicg;
+  SynIn syn;                         // incoming synapses (none: syn.on() false)
 };
 
 // Deterministic sum of the block's values x (one per thread) into
@@ -128,8 +129,18 @@ __device__ inline double set_rhs(const CnSys &S, int i, double bb) {
 // term, else 0); writes the diagonal and the right-hand side; returns b_i
 __device__ inline double cn_row(const CnSys &S, int i, double gs, double es, double om0) {
   const double DT = CN_DT;
-  const double gamma2 = 0.0 + gs / 2.0;
-  const double omega2 = om0 + es / 2.0;
+  // add_*_val (solve_cnm.cu:100-128, 245-257, 406-418): gamma2 += val / 2,
+  // omega2 += val E / 2 per synapse, before the ion sums
+  double g0 = 0.0, o0 = om0;
+  if (S.syn.on())
+    for (int j = S.syn.row[i]; j < S.syn.row[i + 1]; j++) {
+      double E;
+      const double val = syn_entry(S.syn, j, E);
+      g0 += 0.5 * val;
+      o0 += 0.5 * val * E;
+    }
+  const double gamma2 = g0 + gs / 2.0;
+  const double omega2 = o0 + es / 2.0;
   S.diag[i] = S.val[S.dpos[i]] + ((S.cm[i] / DT) + gamma2);
   double bb = 0.0;
   for (int j = S.row[i]; j < S.row[i + 1]; j++) bb -= S.v[S.col[j]] * S.val[j];
@@ -140,8 +151,24 @@ __device__ inline double cn_row(const CnSys &S, int i, double gs, double es, dou
   return set_rhs(S, i, bb);
 }
 
-// the BE row: diagonal val_ori[d] + gs (gs carries Cm/DT first), b = bb
-__device__ inline double be_row(const CnSys &S, int i, double gs, double bb) {
+// the BE row: diagonal val_ori[d] (+ each synapse's val, add_*_val,
+// solve_bem.cu:229-255, 306-320) + gs (gs carries Cm/DT first); b = the
+// synapses' val E from 0, + es
+__device__ inline double be_row(const CnSys &S, int i, double gs, double es) {
+  double d0 = S.val[S.dpos[i]], b0 = 0.0;
+  if (S.syn.on())
+    for (int j = S.syn.row[i]; j < S.syn.row[i + 1]; j++) {
+      double E;
+      const double val = syn_entry(S.syn, j, E);
+      d0 += val;
+      b0 += val * E;
+    }
+  S.diag[i] = d0 + gs;
+  return set_rhs(S, i, b0 + es);
+}
+
+// the BE row without synapses and with b given (IO: the gap currents follow es)
+__device__ inline double be_row_b(const CnSys &S, int i, double gs, double bb) {
   S.diag[i] = S.val[S.dpos[i]] + gs;
   return set_rhs(S, i, bb);
 }
@@ -345,7 +372,7 @@ __global__ void __launch_bounds__(NB) k_be_prep_pkj(CnSys S, PkjCn M) {
     M.rev[i] = rev;
     double gs, es;
     be_ge(c, g, rev, S.iext[i], S.cm[i] / BE_DT, v, gs, es);
-    bb = be_row(S, i, gs, 0.0 + es);
+    bb = be_row_b(S, i, gs, 0.0 + es);   // pkj_solve_by_bem couples no synapses
   }
   cn_prep_finish(S, bb, lds);
 }
@@ -383,7 +410,7 @@ __global__ void __launch_bounds__(NB) k_be_prep_io(CnSys S, IoCn M) {
         bb += -1.0 * gap_current<double>(diff_v);
       }
     }
-    bb = be_row(S, i, gs, bb);
+    bb = be_row_b(S, i, gs, bb);
   }
   cn_prep_finish(S, bb, lds);
 }
@@ -412,7 +439,7 @@ __global__ void __launch_bounds__(NB) k_be_prep_go(CnSys S, GoCn M) {
     M.rev[i] = rev;
     double gs, es;
     be_ge(c, g, rev, S.iext[i], S.cm[i] / BE_DT, v, gs, es);
-    bb = be_row(S, i, gs, 0.0 + es);
+    bb = be_row(S, i, gs, es);
   }
   cn_prep_finish(S, bb, lds);
 }
@@ -443,7 +470,7 @@ __global__ void __launch_bounds__(NB) k_be_prep_gr(CnSys S, GrCn M) {
     for (int k = 0; k < NGATE; k++) M.gate[(size_t)k * nc + i] = g[k];
     double gs, es;
     be_ge(c, g, S.iext[i], S.cm[i] / BE_DT, v, gs, es);
-    bb = be_row(S, i, gs, 0.0 + es);
+    bb = be_row(S, i, gs, es);
   }
   cn_prep_finish(S, bb, lds);
 }
@@ -563,7 +590,7 @@ class CnEngine : public Engine {
            const std::vector<double> &iext, const std::vector<int> &row, const std::vector<int> &col,
            const std::vector<double> &rkc_val, const StimSpec &stim, bool bicg, bool be)
       : n_(n), comp_(comp), nc_((int)cm.size()), be_(be), dt_(be ? BE_DT : CN_DT), stim_(stim) {
-    CB_HIP(hipStreamCreate(&st_));
+    create_stream(&st_);
     const int nnz = (int)col.size();
     // the reference's unflipped matrix = -(RKC-signed); halved for CN
     // (pkj_solve.cu:238-245 and the other xx_solve.cu files)
@@ -672,6 +699,15 @@ class CnEngine : public Engine {
       std::memcpy(rec_out, host.data(), sizeof(double) * host.size());
     }
   }
+  const double *device_v() const override { return S_.v; }
+  hipStream_t stream() const override { return st_; }
+  // add_*_val of the CN and BE paths (GR, GO, PKJ; the BE Purkinje path
+  // couples none); the inferior olive has no synapses
+  void set_synapses(const SynIn &in) override {
+    if (in.on() && !takes_synapses()) throw Error("synaptic input is not provided for this population", -5);
+    S_.syn = in;
+  }
+
   void read(int field, double *dst) override {
     const double *src = nullptr;
     switch (field) {
@@ -690,6 +726,7 @@ class CnEngine : public Engine {
  protected:
   virtual void prep() = 0;
   virtual const double *model_field(int) const { return nullptr; }
+  virtual bool takes_synapses() const { return true; }
   // the half conductance / reversal sums of the initial state
   // (xx_cnm_vec_initialize): gamma1 = gs / 2, omega1 = es / 2
   void init_half_sums(const std::vector<double> &gs, const std::vector<double> &es) {
@@ -787,6 +824,7 @@ class CnIo : public CnEngine {
     if (be_) k_be_prep_io<<<nb_, NB, 0, st_>>>(S_, M_);
     else k_cn_prep_io<<<nb_, NB, 0, st_>>>(S_, M_);
   }
+  bool takes_synapses() const override { return false; }
   const double *model_field(int f) const override {
     if (f == F_CA) return M_.ca;
     if (f >= F_GATE0 && f < F_GATE0 + io::NGATE) return M_.gate + (size_t)(f - F_GATE0) * nc_;

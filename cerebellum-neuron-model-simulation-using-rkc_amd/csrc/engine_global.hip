@@ -17,6 +17,8 @@
 // header for the list).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -203,6 +205,8 @@ struct PkjState {
   const double *syn_g;  // may be null
   double syn_e;
   int nc;
+  const double *net_s;  // synaptic current of the macro step (granule input), may be null
+  double net_decay;     // exp(-dt / 0.6), dt the ion interval
 };
 
 // pkj_update_ion_RKC (pkj_ion.cu:304-358) followed by pkj_update_matrix
@@ -235,6 +239,8 @@ __global__ void __launch_bounds__(256) k_pkj_ion_b(PkjState S, const double *__r
   }
   double bb = current<double>(vn, S.iext[i], rev, c, g);
   if (S.syn_g) bb += S.syn_g[i] * (vn - S.syn_e);
+  // pkj_update_matrix_new (solve_rkc.cu:358-367, 1245-1249)
+  if (S.net_s) bb += S.net_s[i] * S.net_decay;
   S.b[i] = bb;
 }
 
@@ -297,6 +303,7 @@ struct GoState {
   const double *syn_g;  // may be null
   double syn_e;
   int nc;
+  const double *net_s;  // synaptic current of the macro step (granule input), may be null
 };
 
 // go_update_ion_RKC_exp_imp (go_ion.cu:379-437), go_KAHP_update_2order
@@ -331,6 +338,8 @@ __global__ void __launch_bounds__(256) k_go_ion_b(GoState S, const double *__res
   }
   double bb = current(vn, S.iext[i], rev, c, g);
   if (S.syn_g) bb += S.syn_g[i] * (vn - S.syn_e);
+  // go_update_matrix_new (solve_rkc.cu:279-286, 1181)
+  if (S.net_s) bb += S.net_s[i];
   S.b[i] = bb;
 }
 
@@ -345,6 +354,7 @@ struct GrState {
   const double *syn_g;  // may be null
   double syn_e;
   int nc;
+  SynIn net;            // mossy-fibre and Golgi input, per compartment
 };
 
 // gr_Na_update_2order (gr_ion.cu:45-207), gr_update_ion_RKC_exp_imp
@@ -379,6 +389,14 @@ __global__ void __launch_bounds__(256) k_gr_ion_b(GrState S, const double *__res
   }
   double bb = current(vn, S.iext[i], c, g);
   if (S.syn_g) bb += S.syn_g[i] * (vn - S.syn_e);
+  // add_mfgr_val / add_gogr_val (solve_rkc.cu:200-224, 1117-1118), one
+  // synapse after the other
+  if (S.net.on())
+    for (int j = S.net.row[i]; j < S.net.row[i + 1]; j++) {
+      double E;
+      const double gs = syn_entry(S.net, j, E);
+      bb += gs * (vn - E);
+    }
   S.b[i] = bb;
 }
 
@@ -438,7 +456,7 @@ class GlobalRkc : public Engine {
             const std::vector<double> &iext, const std::vector<int> &row, const std::vector<int> &col,
             const std::vector<double> &val, const StimSpec &stim, const PoissonSpec &syn)
       : n_(n), nc_(nc), comp_(comp), stim_(stim), syn_(syn) {
-    CB_HIP(hipStreamCreate(&st_));
+    create_stream(&st_);
     v_ = dalloc<double>(nc_); h2d(v_, v);
     iext_ = dalloc<double>(nc_); h2d(iext_, iext);
     cm_ = dalloc<double>(nc_); h2d(cm_, cm);
@@ -465,7 +483,7 @@ class GlobalRkc : public Engine {
     (void)hipStreamSynchronize(st_);
     for (void *p : {(void *)v_, (void *)iext_, (void *)cm_, (void *)b_, (void *)dammy_, (void *)row_, (void *)col_,
                     (void *)val_, (void *)red_a_, (void *)red_b_, (void *)bad_, (void *)amp_, (void *)syn_g_,
-                    (void *)zero_})
+                    (void *)zero_, (void *)net_s_})
       if (p) (void)hipFree(p);
     for (int k = 0; k < NVEC; k++) (void)hipFree(vec_[k]);
     (void)hipHostFree(hres_);
@@ -493,6 +511,9 @@ class GlobalRkc : public Engine {
                                             (uint32_t)syn_.seed, (uint32_t)(syn_.seed >> 32),
                                             (uint32_t)st.macro_steps, nc_);
     }
+    // the synaptic current of this macro step from its starting y (Golgi and
+    // Purkinje cells: reset_b + add_*_val_new, solve_rkc.cu:1199-1200, 1267-1269)
+    if (net_.on() && net_s_) syn_macro_current(net_, vec_[Y], net_s_, nc_, st_);
     if (sum_dt_ != 0.0) {
       ion_and_b(sum_dt_);
       sum_dt_ = 0.0;
@@ -522,6 +543,9 @@ class GlobalRkc : public Engine {
     st.macro_steps++;
   }
 
+  const double *device_v() const override { return v_; }
+  hipStream_t stream() const override { return st_; }
+
   void read(int field, double *dst) override {
     const double *src = nullptr;
     switch (field) {
@@ -547,6 +571,16 @@ class GlobalRkc : public Engine {
   // '>' for GR/IO (solve_rkc.cu:1148, 1212, 1284, 1347)
   virtual bool trigger() const { return sum_dt_ >= integ_; }
   virtual const double *model_field(int) const { return nullptr; }
+
+  // Golgi / Purkinje coupling: keep the synapse view and a buffer for the
+  // macro step's synaptic current
+  void attach_net(const SynIn &in) {
+    net_ = in;
+    if (in.on() && !net_s_) {
+      net_s_ = dalloc<double>(nc_);
+      CB_HIP(hipMemset(net_s_, 0, sizeof(double) * nc_));
+    }
+  }
 
   // RKC_vec_initialize and the configuration of the controller; called by the
   // cell type's constructor once its state is on the device
@@ -839,6 +873,8 @@ class GlobalRkc : public Engine {
   double *vec_[NVEC];
   double *red_a_, *red_b_, *hres_, *amp_;
   double *syn_g_ = nullptr;
+  SynIn net_;               // incoming synapses (none: net_.on() false)
+  double *net_s_ = nullptr; // Golgi / Purkinje: the macro step's synaptic current
   double *zero_ = nullptr;
   int *bad_;
   double hw_[8], o_[NOTHERS];
@@ -872,9 +908,16 @@ class GlobalPkj : public GlobalRkc {
     return s;
   }
   void model_b(const double *vv, bool do_ion, double dt) override {
-    PkjState S{v_, ca_, rev_, iext_, shell_, cond_, gate_, b_, syn_g_, syn_.e_syn, nc_};
+    // the granule input enters at ion updates only (pkj_update_ion_and_matrix)
+    const bool net = do_ion && net_.on();
+    PkjState S{v_, ca_, rev_, iext_, shell_, cond_, gate_, b_, syn_g_, syn_.e_syn, nc_,
+               net ? net_s_ : nullptr, net ? std::exp(-dt / 0.6) : 0.0};
     k_pkj_ion_b<<<nblk(nc_), 256, 0, st_>>>(S, vv, dt, do_ion ? 1 : 0);
   }
+ public:
+  void set_synapses(const SynIn &in) override { attach_net(in); }
+
+ private:
   const double *model_field(int field) const override {
     switch (field) {
       case F_CA: return ca_;
@@ -954,9 +997,15 @@ class GlobalGo : public GlobalRkc {
     return s;
   }
   void model_b(const double *vv, bool do_ion, double dt) override {
-    GoState S{v_, ca_, ca2_, rev_, ca_old_, iext_, area_, cond_, gate_, b_, syn_g_, syn_.e_syn, nc_};
+    const bool net = do_ion && net_.on();
+    GoState S{v_, ca_, ca2_, rev_, ca_old_, iext_, area_, cond_, gate_, b_, syn_g_, syn_.e_syn, nc_,
+              net ? net_s_ : nullptr};
     k_go_ion_b<<<nblk(nc_), 256, 0, st_>>>(S, vv, dt, do_ion ? 1 : 0);
   }
+ public:
+  void set_synapses(const SynIn &in) override { attach_net(in); }
+
+ private:
   const double *model_field(int field) const override {
     switch (field) {
       case F_CA: return ca_;
@@ -997,9 +1046,14 @@ class GlobalGr : public GlobalRkc {
   }
   bool trigger() const override { return sum_dt_ > integ_; }   // strict for GR, solve_rkc.cu:1148
   void model_b(const double *vv, bool do_ion, double dt) override {
-    GrState S{v_, iext_, ca_, compart_, cond_, gate_, na_, b_, syn_g_, syn_.e_syn, nc_};
+    GrState S{v_, iext_, ca_, compart_, cond_, gate_, na_, b_, syn_g_, syn_.e_syn, nc_,
+              do_ion ? net_ : SynIn()};
     k_gr_ion_b<<<nblk(nc_), 256, 0, st_>>>(S, vv, dt, do_ion ? 1 : 0);
   }
+ public:
+  void set_synapses(const SynIn &in) override { net_ = in; }
+
+ private:
   const double *model_field(int field) const override {
     if (field == F_CA) return ca_;
     if (field >= F_GATE0 && field < F_GATE0 + gr::NGATE) return gate_ + (size_t)(field - F_GATE0) * nc_;
@@ -1011,6 +1065,14 @@ class GlobalGr : public GlobalRkc {
 };
 
 }  // namespace
+
+void create_stream(hipStream_t *st) {
+  static char park[256];
+  char *caller = initstate(1u, park, sizeof(park));
+  const hipError_t e = hipStreamCreate(st);
+  setstate(caller);
+  CB_HIP(e);
+}
 
 void Engine::run(int nsteps, double t0, int rec_comp, double *rec_out) {
   std::vector<double> buf(nc());

@@ -16,6 +16,10 @@
 //   --stable         the safeguards for synthetic trees (ion interval 2^-8 ms,
 //                    fn refresh, spectral-radius floor)
 //   --chunk K        macro steps per device launch between host writes (64)
+//   --network        update the synapses after every macro step (the
+//                    reference's network_sim, main.cu:38, false there); the
+//                    synapse sets are created and coupled either way, as
+//                    main.cu:139-145 does, so without it they stay silent
 //
 // This build provides all four populations with all three solvers.  Each
 // population advances one H_MAX = 0.125 ms output interval per row, in
@@ -63,12 +67,12 @@ int main(int argc, char *argv[]) {
   for (int i = 0; i < argc; i++) std::printf("argv[%d] = %s\n", i, argv[i]);
   if (argc < 13) {
     std::printf("usage: %s S_gr gx gy S_go ox oy S_pkj px py S_io ix iy [--ts MS] [--engine global|cell|mixed] "
-                "[--stable] [--chunk K]\n", argv[0]);
+                "[--stable] [--chunk K] [--network]\n", argv[0]);
     return 0;
   }
   double ts = 1500.0;
   int engine = CB_ENGINE_GLOBAL_FP64, chunk = 64;
-  bool stable = false;
+  bool stable = false, network = false;
   for (int i = 13; i < argc; i++) {
     std::string o = argv[i];
     if (o == "--ts" && i + 1 < argc) {
@@ -81,6 +85,8 @@ int main(int argc, char *argv[]) {
     } else if (o == "--chunk" && i + 1 < argc) {
       chunk = std::atoi(argv[++i]);
       if (chunk < 1) chunk = 1;
+    } else if (o == "--network") {
+      network = true;
     } else {
       std::printf("unknown option %s\n", o.c_str());
       return 1;
@@ -141,6 +147,39 @@ int main(int argc, char *argv[]) {
   if (nio > 0) check(cb_io_initialize(nx[CB_IO], ny[CB_IO], argv[10], &iop), "io_initialize");
   std::vector<double> io_v((size_t)nio * 3), io_prev(nio, 100.0);
   std::printf("\n");
+
+  // create synapse (main.cu:139-145): each Poisson / jittered set draws its
+  // seed from rand() when it is not empty, in this order, as the reference
+  // seeds its cuRAND states
+  cb_syn *mfgr = nullptr, *gogr = nullptr, *grgo = nullptr, *grpkj = nullptr, *mlipkj = nullptr;
+  {
+    const uint64_t mf_seed = ngr > 0 ? (uint64_t)std::rand() : 0;
+    check(cb_mfgr_create(grp, mf_seed, &mfgr), "mfgr_create");
+    check(cb_gogr_create(gop, grp, &gogr), "gogr_create");
+    check(cb_grgo_create(grp, gop, &grgo), "grgo_create");
+    const uint64_t pk_seed = (ngr > 0 && n > 0) ? (uint64_t)std::rand() : 0;
+    check(cb_grpkj_create(grp, pkj, "./Pkj_location_info.csv", pk_seed, &grpkj), "grpkj_create");
+    const uint64_t mli_seed = (ngr > 0 && n > 0) ? (uint64_t)std::rand() : 0;
+    check(cb_mlipkj_create(pkj, grp, mli_seed, &mlipkj), "mlipkj_create");
+    int nsyn[5] = {0, 0, 0, 0, 0};
+    const cb_syn *sets[5] = {mfgr, gogr, grgo, grpkj, mlipkj};
+    const char *names[5] = {"mfgr", "gogr", "grgo", "grpkj", "mlipkj"};
+    for (int k = 0; k < 5; k++) {
+      check(cb_syn_size(sets[k], &nsyn[k]), "syn_size");
+      std::printf("# of %s = %d\n", names[k], nsyn[k]);
+    }
+    // mfgr_create / mlipkj_create open these files, which stay empty
+    // (mf_output_file and mli_output_file are not called, main.cu:215)
+    if (nsyn[0] > 0) {
+      std::FILE *f = std::fopen("MF_RASTER.csv", "w");
+      if (f) std::fclose(f);
+    }
+    if (nsyn[4] > 0) {
+      std::FILE *f = std::fopen("MLI_RASTER.csv", "w");
+      if (f) std::fclose(f);
+    }
+  }
+  if (network) chunk = 1;   // the synapses are updated between macro steps
 
   // the macro-step loop of main.cu:182-311 for the populations this build
   // provides; the recorded compartment of every cell comes back once per
@@ -245,6 +284,12 @@ int main(int argc, char *argv[]) {
         io_prev[c] = v;
       }
     }
+    // Update synapses (main.cu:302-306), after every population's macro step
+    if (network) {
+      check(cb_gr_synapse_update(t, mfgr, gogr), "gr_synapse_update");
+      check(cb_go_synapse_update(t, grgo), "go_synapse_update");
+      check(cb_pkj_synapse_update(t, grpkj, mlipkj), "pkj_synapse_update");
+    }
     for (int s = 0; s < k; s++) {
       if ((done + s) % 400 == 0) std::printf("time:%f\n", (done + s) * H_MAX);
       if (t + (s + 1) * H_MAX >= 500.0 && 500.0 + H_MAX > t + (s + 1) * H_MAX)
@@ -273,6 +318,8 @@ int main(int argc, char *argv[]) {
                    std::chrono::duration<double>(end - half).count(), sim);
     std::fclose(tp);
   }
+  // synapse_finalize (main.cu:325-331), before the populations
+  for (cb_syn *x : {mfgr, gogr, grgo, grpkj, mlipkj}) check(cb_syn_finalize(x), "syn_finalize");
   if (grp) check(cb_gr_finalize(grp), "gr_finalize");
   if (gop) check(cb_go_finalize(gop), "go_finalize");
   if (pkj) check(cb_pkj_finalize(pkj), "pkj_finalize");

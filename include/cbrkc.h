@@ -230,6 +230,60 @@ int cb_io_output_file(const cb_pop *p, const double *memv, double *raster_prev, 
 /* io_finalize + io_solve_finalize + io_gap_finalize. */
 int cb_io_finalize(cb_pop *p);
 
+/* ---- synapses between the populations (syn.cu) ----
+ * A set is created between existing populations (a NULL population gives an
+ * empty set) and couples itself into its postsynaptic population's solver
+ * as the reference's xx_solve_update_v does with the synapse_t it is passed:
+ * RKC granule cells b += val (y - E) at each ion update, Golgi / Purkinje
+ * cells the macro step's val (y - E) from its start at each ion update
+ * (Purkinje: x exp(-dt / 0.6)); CN gamma2 / omega2 += val / 2, val E / 2; BE
+ * diagonal / b += val, val E (granule and Golgi cells only).  Finalize every
+ * set before its populations.  The random draws (mossy-fibre and inhibitory
+ * Poisson inputs, contact jitter) come from Philox4x32-10 keyed by seed; the
+ * reference's cuRAND streams are not reproduced. */
+typedef struct cb_syn cb_syn;
+
+/* set fields for cb_syn_read: decaying traces (MF: AMPA, NMDA; GoGr: GABAA,
+ * GABAB; GrGo: AMPA, NMDA1, NMDA2; GrPkj: AMPA; MliPkj: GABA), weight,
+ * conductance val [mS], last presynaptic V (syn.cuh:63-68) */
+enum { CB_SYN_TRACE0 = 0, CB_SYN_TRACE1 = 1, CB_SYN_TRACE2 = 2, CB_SYN_WEIGHT = 3, CB_SYN_VAL = 4,
+       CB_SYN_OLD_V = 5 };
+/* set kinds for cb_syn_build */
+enum { CB_SYN_MFGR = 0, CB_SYN_GOGR = 1, CB_SYN_GRGO = 2, CB_SYN_GRPKJ = 3, CB_SYN_MLIPKJ = 4 };
+
+/* mfgr_create (syn.cu:104-128): 4 mossy-fibre inputs per granule cell. */
+int cb_mfgr_create(cb_pop *gr, uint64_t seed, cb_syn **out);
+/* gogr_create (syn.cu:339-410): Golgi compartment 30 -> granule compartment 16. */
+int cb_gogr_create(cb_pop *go, cb_pop *gr, cb_syn **out);
+/* grgo_create (syn.cu:181-261): granule axon -> Golgi compartment 0. */
+int cb_grgo_create(cb_pop *gr, cb_pop *go, cb_syn **out);
+/* grpkj_create (syn.cu:519-644): reads the compartment x positions from
+ * location_csv (NULL: "./Pkj_location_info.csv"), 25 synapses per pair. */
+int cb_grpkj_create(cb_pop *gr, cb_pop *pkj, const char *location_csv, uint64_t seed, cb_syn **out);
+/* mlipkj_create (syn.cu:690-738): one inert synapse per Purkinje cell. */
+int cb_mlipkj_create(cb_pop *pkj, cb_pop *gr, uint64_t seed, cb_syn **out);
+/* mfgr_finalize / gogr_finalize / ... (syn.cu:130-141 etc.): uncouples and frees. */
+int cb_syn_finalize(cb_syn *s);
+/* gr_synapse_update (syn.cu:821-855): mossy-fibre draws, Golgi spikes. */
+int cb_gr_synapse_update(double t, cb_syn *mfgr, cb_syn *gogr);
+/* go_synapse_update (syn.cu:857-885): granule spikes onto Golgi cells. */
+int cb_go_synapse_update(double t, cb_syn *grgo);
+/* pkj_synapse_update (syn.cu:887-917): granule spikes onto Purkinje cells; the
+ * inhibitory set is not updated, as in the reference. */
+int cb_pkj_synapse_update(double t, cb_syn *grpkj, cb_syn *mlipkj);
+int cb_syn_size(const cb_syn *s, int *n);
+/* pre / post global compartment of every synapse (syn.cuh:62 comp). */
+int cb_syn_comp(const cb_syn *s, int *pre, int *post);
+/* n values of one CB_SYN_* field. */
+int cb_syn_read(const cb_syn *s, int field, double *dst);
+/* The connection rules alone, on the host (no device): the set of the given
+ * kind between an nx_a x ny_a grid and an nx_b x ny_b grid (MFGR a = GR;
+ * GOGR a = GO, b = GR; GRGO a = GR, b = GO; GRPKJ a = GR, b = PKJ with x1600
+ * the location file's x column; MLIPKJ a = PKJ, b = GR).  With pre, post and
+ * weight NULL only *n is set; otherwise they receive *n entries. */
+int cb_syn_build(int kind, int nx_a, int ny_a, int nx_b, int ny_b, const double *x1600, uint64_t seed, int *n,
+                 int *pre, int *post, double *weight);
+
 #ifdef __cplusplus
 }
 #endif

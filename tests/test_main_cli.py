@@ -212,3 +212,70 @@ def test_be_io_rows_against_oracle(built, tmp_path, ref_data):
     # the file holds "%lf" (6 decimals): 5e-7 of print rounding on top of the engine's 1e-9
     np.testing.assert_allclose(got[:, 1:], np.array(ref), rtol=0, atol=6e-7)
     assert (tmp_path / "comptime.csv").read_text().startswith("Gr,0,BE,Go,0,BE,Pkj,0,BE,Io,4,BE,CompTime,")
+
+
+@pytest.mark.gpu
+def test_network_rows_against_oracle(built, tmp_path, ref_data):
+    """./main CN 4 4 CN 1 1 CN 1 1 CN 0 0 --network: the populations draw their
+    initial voltages from libc rand() seeded 0 (GR, GO, PKJ), the mossy-fibre,
+    granule-Purkinje and inhibitory sets take the next three draws as seeds
+    (main.cu:139-145), and the synapses update after every macro step.
+    Against the oracle network built the same way: the first granule row and
+    every Golgi row at 6e-7 (%lf), every Purkinje row at 2e-6 (the bar of the
+    single-Purkinje CN run above: that solve also stops at the 100-iteration
+    cap, observed 9.9e-7 here); the granule rows within 0.1 mV,
+    as the 16-cell granule CN solve stops at the 100-iteration CG cap and its
+    truncated iterate drifts (the oracle's own spread on this network under a
+    1e-12 mV change of the granule initial voltages reaches 1.7e-3 mV after
+    one macro step and 3.6 mV around the mossy-fibre events).  MF_RASTER.csv
+    and MLI_RASTER.csv are created empty."""
+    import ctypes
+    for f in ("Gr.txt", "Go_131.txt", "Pkj.txt", "Pkj_GJ.csv", "Pkj_location_info.csv"):
+        shutil.copy(os.path.join(ref_data, f), tmp_path / f)
+    print("main: GR 4x4, GO 1x1, PKJ 1x1, CN, --network, 20 ms", flush=True)
+    r = run_main(["CN", "4", "4", "CN", "1", "1", "CN", "1", "1", "CN", "0", "0", "--ts", "20", "--network"],
+                 tmp_path, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    assert "# of mfgr = 64" in r.stdout and "# of grpkj = 400" in r.stdout
+    for f in ("MF_RASTER.csv", "MLI_RASTER.csv"):
+        assert (tmp_path / f).read_text() == ""
+    rows = lambda name: np.array([[float(x) for x in l.rstrip("\n").rstrip(",").split(",")]
+                                  for l in open(tmp_path / name)])
+    got_gr, got_go, got_pk = rows("gr_v_by_CN.csv"), rows("go_v_by_CN.csv"), rows("pkj_v_by_CN.csv")
+    assert got_gr.shape == (160, 17) and got_go.shape == (160, 2) and got_pk.shape == (160, 2)
+
+    from test_oracle_cn import L as LC
+    from test_oracle_syn import L as LS, Syn, pkj_x
+    from test_oracle_gr import L as LR
+    from test_oracle_go import L as LG
+    lib = orc.lib()
+    lib.rand.restype = ctypes.c_int
+    lib.srand(0)
+    ogr = orc.Pop(LR().or_gr_create(os.path.join(ref_data, "Gr.txt").encode(), 4, 4, None, 1e-9, 1e-9, 1), 16, 578)
+    ogo = orc.Pop(LG().or_go_create(os.path.join(ref_data, "Go_131.txt").encode(), 1, 1, None, 1e-6, 1e-6, 1), 1, 131)
+    opk = orc.Pop(lib.or_pkj_create(os.path.join(ref_data, "Pkj.txt").encode(), None, 1, 1, None, 1e-6, 1e-6, 1),
+                  1, 1600)
+    seeds = [lib.rand(), lib.rand(), lib.rand()]
+    mf = Syn(LS().or_mfgr_create(16, seeds[0]))
+    gg = Syn(LS().or_gogr_create(ogo.h, ogr.h))
+    rg = Syn(LS().or_grgo_create(ogr.h, ogo.h))
+    rp = Syn(LS().or_grpkj_create(ogr.h, opk.h, orc._dp(pkj_x(ref_data)), seeds[1]))
+    for o, s in ((ogr, mf), (ogr, gg), (ogo, rg), (opk, rp)):
+        assert LS().or_pop_attach_syn(o.h, s.h) == 0
+    for o in (ogr, ogo, opk):
+        assert LC().or_cn_init(o.h) == 0
+    lib.or_net_update.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_double]
+    ref_gr, ref_go, ref_pk = [], [], []
+    for k in range(160):
+        for o in (ogr, ogo, opk):
+            LC().or_cn_advance(o.h, k * 0.125)
+        lib.or_net_update(mf.h, gg.h, rg.h, rp.h, k * 0.125)
+        ref_gr.append(ogr.get(orc.V)[16::578])
+        ref_go.append(ogo.get(orc.V)[30])
+        ref_pk.append(opk.get(orc.V)[1599])
+    assert np.max(mf.get(4)) > 0.0                   # mossy-fibre events happened (three in 20 ms)
+    np.testing.assert_allclose(got_gr[0, 1:], ref_gr[0], rtol=0, atol=6e-7)
+    print(f"granule rows: max |main - oracle| = {np.max(np.abs(got_gr[:, 1:] - np.array(ref_gr))):.3e} mV", flush=True)
+    np.testing.assert_allclose(got_gr[:, 1:], np.array(ref_gr), rtol=0, atol=0.1)
+    np.testing.assert_allclose(got_go[:, 1], ref_go, rtol=0, atol=6e-7)
+    np.testing.assert_allclose(got_pk[:, 1], ref_pk, rtol=0, atol=2e-6)
