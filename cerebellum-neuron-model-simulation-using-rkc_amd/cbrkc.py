@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(HERE, "libcbrkc.so")
 CB_OK = 0
 GR, GO, PKJ, IO = 0, 1, 2, 3
 BE, CN, RKC = 0, 1, 2
-ENGINE_GLOBAL_FP64, ENGINE_CELL_FP64, ENGINE_CELL_MIXED = 0, 1, 2
+ENGINE_GLOBAL_FP64, ENGINE_CELL_FP64, ENGINE_CELL_MIXED, ENGINE_TREE_FP64 = 0, 1, 2, 3
 F_VNEW, F_V, F_CA, F_REV, F_B, F_IEXT, F_Y, F_YN, F_FN, F_CA2, F_CA_OLD, F_VOLD = range(12)
 F_GATE0 = 100
 H_MAX = 0.125

@@ -98,7 +98,7 @@ PoissonSpec poisson_of(const cb_pop_config &cfg) {
 int create_io(const cb_pop_config &cfg, cb_pop **out) {
   const int n = cfg.nx * cfg.ny;
   if (n == 0) return fail(CB_E_ARG, "empty population");
-  if (cfg.engine != CB_ENGINE_GLOBAL_FP64)
+  if (cfg.engine != CB_ENGINE_GLOBAL_FP64 && cfg.engine != CB_ENGINE_TREE_FP64)
     return fail(CB_E_UNSUP, "inferior-olive cells are gap-junction coupled: global engine only");
   IoHost host;
   std::string err;
@@ -131,10 +131,11 @@ int create_io(const cb_pop_config &cfg, cb_pop **out) {
   p->engine = cfg.engine;
   p->comp = io::COMP;
   p->device = cfg.device;
+  const bool tree = cfg.engine == CB_ENGINE_TREE_FP64;
   if (cfg.solver == CB_SOLVER_CN)
-    p->eng.reset(make_cn_io(host, stim));
+    p->eng.reset(make_cn_io(host, stim, tree));
   else if (cfg.solver == CB_SOLVER_BE)
-    p->eng.reset(make_be_io(host, stim));
+    p->eng.reset(make_be_io(host, stim, tree));
   else
     p->eng.reset(make_global_io(host, sol, stim, poisson_of(cfg)));
   *out = p.release();
@@ -146,7 +147,7 @@ int create_io(const cb_pop_config &cfg, cb_pop **out) {
 int create_go(const cb_pop_config &cfg, cb_pop **out) {
   const int n = cfg.nx * cfg.ny;
   if (n == 0) return fail(CB_E_ARG, "empty population");
-  if (cfg.engine != CB_ENGINE_GLOBAL_FP64)
+  if (cfg.engine != CB_ENGINE_GLOBAL_FP64 && cfg.engine != CB_ENGINE_TREE_FP64)
     return fail(CB_E_UNSUP, "Golgi cells are gap-junction coupled: global engine only");
   GoHost host;
   std::string err;
@@ -179,10 +180,11 @@ int create_go(const cb_pop_config &cfg, cb_pop **out) {
   p->engine = cfg.engine;
   p->comp = go::COMP;
   p->device = cfg.device;
+  const bool tree = cfg.engine == CB_ENGINE_TREE_FP64;
   if (cfg.solver == CB_SOLVER_CN)
-    p->eng.reset(make_cn_go(host, stim));
+    p->eng.reset(make_cn_go(host, stim, tree));
   else if (cfg.solver == CB_SOLVER_BE)
-    p->eng.reset(make_be_go(host, stim));
+    p->eng.reset(make_be_go(host, stim, tree));
   else
     p->eng.reset(make_global_go(host, sol, stim, poisson_of(cfg)));
   *out = p.release();
@@ -194,7 +196,8 @@ int create_go(const cb_pop_config &cfg, cb_pop **out) {
 int create_gr(const cb_pop_config &cfg, cb_pop **out) {
   const int n = cfg.nx * cfg.ny;
   if (n == 0) return fail(CB_E_ARG, "empty population");
-  if (cfg.engine != CB_ENGINE_GLOBAL_FP64) return fail(CB_E_UNSUP, "granule cells: global engine only");
+  if (cfg.engine != CB_ENGINE_GLOBAL_FP64 && cfg.engine != CB_ENGINE_TREE_FP64)
+    return fail(CB_E_UNSUP, "granule cells: global engine only");
   GrHost host;
   std::string err;
   const std::string path = cfg.morph_path ? cfg.morph_path : "./Gr.txt";
@@ -226,10 +229,11 @@ int create_gr(const cb_pop_config &cfg, cb_pop **out) {
   p->engine = cfg.engine;
   p->comp = gr::COMP;
   p->device = cfg.device;
+  const bool tree = cfg.engine == CB_ENGINE_TREE_FP64;
   if (cfg.solver == CB_SOLVER_CN)
-    p->eng.reset(make_cn_gr(host, stim));
+    p->eng.reset(make_cn_gr(host, stim, tree));
   else if (cfg.solver == CB_SOLVER_BE)
-    p->eng.reset(make_be_gr(host, stim));
+    p->eng.reset(make_be_gr(host, stim, tree));
   else
     p->eng.reset(make_global_gr(host, sol, stim, poisson_of(cfg)));
   *out = p.release();
@@ -401,8 +405,10 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     if (cfg->cell_type < CB_GR || cfg->cell_type > CB_IO) return fail(CB_E_ARG, "unknown cell type");
     if (cfg->solver != CB_SOLVER_RKC && cfg->solver != CB_SOLVER_CN && cfg->solver != CB_SOLVER_BE)
       return fail(CB_E_ARG, "unknown solver");
-    if (cfg->solver != CB_SOLVER_RKC && cfg->engine != CB_ENGINE_GLOBAL_FP64)
-      return fail(CB_E_UNSUP, "the CN and BE solvers run on the global FP64 engine");
+    if (cfg->solver != CB_SOLVER_RKC && cfg->engine != CB_ENGINE_GLOBAL_FP64 && cfg->engine != CB_ENGINE_TREE_FP64)
+      return fail(CB_E_UNSUP, "the CN and BE solvers run on the global FP64 engine or the per-cell tree solve");
+    if (cfg->solver == CB_SOLVER_RKC && cfg->engine == CB_ENGINE_TREE_FP64)
+      return fail(CB_E_UNSUP, "the per-cell tree solve is provided for the CN and BE solvers");
     if (cfg->solver != CB_SOLVER_RKC && cfg->poisson_on)
       return fail(CB_E_UNSUP, "the Poisson conductance drive is provided for the RKC solver");
     CB_HIP(hipSetDevice(cfg->device));
@@ -467,7 +473,8 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     p->comp = pkj::COMP;
     p->device = cfg->device;
     if (cfg->solver != CB_SOLVER_RKC) {
-      p->eng.reset(cfg->solver == CB_SOLVER_CN ? make_cn_pkj(host, stim) : make_be_pkj(host, stim));
+      const bool tree = cfg->engine == CB_ENGINE_TREE_FP64;
+      p->eng.reset(cfg->solver == CB_SOLVER_CN ? make_cn_pkj(host, stim, tree) : make_be_pkj(host, stim, tree));
       *out = p.release();
       return CB_OK;
     }

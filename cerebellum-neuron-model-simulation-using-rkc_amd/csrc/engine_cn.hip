@@ -34,6 +34,7 @@
 // buffered), so an iteration needs no separate vector update launch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -70,6 +71,7 @@ struct CnSys {
   double *part, *scal;
   int *flag;
   int nc, bicg;
+  int direct;                        // 1: the per-cell tree solve replaces the Krylov loop
   SynIn syn;                         // incoming synapses (none: syn.on() false)
 };
 
@@ -175,6 +177,7 @@ __device__ inline double be_row_b(const CnSys &S, int i, double gs, double bb) {
 
 // end of the prep kernel: |b|^2 (= the first r.r) and the solve's flags
 __device__ inline void cn_prep_finish(const CnSys &S, double bb, double *lds) {
+  if (S.direct) return;              // uniform over the grid: no Krylov state to set up
   double tot;
   if (block_sum_last(S, bb * bb, lds, &tot) && threadIdx.x == 0) {
     S.scal[S_BNORM] = tot;
@@ -567,6 +570,169 @@ __global__ void k_cn_record(const double *__restrict__ v, double *__restrict__ o
   if (c < n) out[c] = v[rec + (size_t)c * comp];
 }
 
+// ---- the exact per-cell solve (Hines elimination on each cell's tree) ----
+//
+// Each cell's compartments are renumbered breadth-first from the centre of
+// its tree (the middle of a longest path, which halves the number of levels
+// against a leaf root), so every level is a contiguous range of positions and
+// a node's children are contiguous too.  One 64-lane wavefront per cell
+// stages the cell's diagonal, right-hand side and off-diagonal pairs in LDS,
+// eliminates level by level from the deepest parents to the root (each parent
+// folds in its children in position order: d_p -= A(p,c) A(c,p) / d_c,
+// r_p -= A(p,c) r_c / d_c), then substitutes back from the root
+// (x_c = (r_c - A(c,p) x_p) / d_c).  No fill-in, no pivoting (the CN and BE
+// systems are diagonally dominant), a fixed order, so the result does not
+// depend on the launch.
+struct HinesPlan {
+  const int *perm;       // position -> global compartment, [cell * comp + pos]
+  const double *up, *dn; // A(node, parent), A(parent, node) per position (root: 0)
+  const int *meta;       // first child position << 8 | child count
+  const int *lev;        // per cell: level start positions, then comp
+  const int *lev_off;    // cell -> its first entry in lev (n + 1 entries)
+  int comp;
+};
+
+constexpr int HINES_WAVE = 64;
+constexpr int HINES_MAX_COMP = 65536 / (4 * 8 + 4);   // the LDS a cell's staging takes (64 KB)
+
+__global__ void __launch_bounds__(HINES_WAVE) k_hines(HinesPlan P, const double *__restrict__ diag,
+                                                      const double *__restrict__ b, double *__restrict__ v) {
+  extern __shared__ double sm[];
+  const int comp = P.comp, lane = threadIdx.x;
+  double *d = sm, *r = sm + comp, *up = sm + 2 * comp, *dn = sm + 3 * comp;
+  int *meta = (int *)(sm + 4 * comp);
+  const size_t base = (size_t)blockIdx.x * comp;
+  for (int k = lane; k < comp; k += HINES_WAVE) {
+    const int g = P.perm[base + k];
+    d[k] = diag[g];
+    r[k] = b[g];
+    up[k] = P.up[base + k];
+    dn[k] = P.dn[base + k];
+    meta[k] = P.meta[base + k];
+  }
+  __syncthreads();
+  const int *lev = P.lev + P.lev_off[blockIdx.x];
+  const int nl = P.lev_off[blockIdx.x + 1] - P.lev_off[blockIdx.x] - 1;   // levels
+  for (int l = nl - 2; l >= 0; l--) {
+    for (int k = lev[l] + lane; k < lev[l + 1]; k += HINES_WAVE) {
+      const int m = meta[k], fc = m >> 8, nch = m & 255;
+      double dd = d[k], rr = r[k];
+      for (int j = fc; j < fc + nch; j++) {
+        const double f = dn[j] / d[j];
+        dd -= f * up[j];
+        rr -= f * r[j];
+      }
+      d[k] = dd;
+      r[k] = rr;
+    }
+    __syncthreads();
+  }
+  if (lane == 0) r[0] = r[0] / d[0];
+  __syncthreads();
+  for (int l = 0; l < nl - 1; l++) {
+    for (int k = lev[l] + lane; k < lev[l + 1]; k += HINES_WAVE) {
+      const int m = meta[k], fc = m >> 8, nch = m & 255;
+      const double xp = r[k];
+      for (int j = fc; j < fc + nch; j++) r[j] = (r[j] - up[j] * xp) / d[j];
+    }
+    __syncthreads();
+  }
+  for (int k = lane; k < comp; k += HINES_WAVE) v[P.perm[base + k]] = r[k];
+}
+
+struct HinesHost {
+  std::vector<int> perm, meta, lev, lev_off;
+  std::vector<double> up, dn;
+};
+
+// breadth-first distances from s over adj (ascending neighbour order)
+void tree_bfs(const std::vector<std::vector<int>> &adj, int s, std::vector<int> &dist, std::vector<int> &par,
+              std::vector<int> &order) {
+  const int n = (int)adj.size();
+  dist.assign(n, -1);
+  par.assign(n, -1);
+  order.clear();
+  dist[s] = 0;
+  order.push_back(s);
+  for (size_t h = 0; h < order.size(); h++) {
+    const int u = order[h];
+    for (int w : adj[u])
+      if (dist[w] < 0) {
+        dist[w] = dist[u] + 1;
+        par[w] = u;
+        order.push_back(w);
+      }
+  }
+}
+
+// the plan of every cell from the system's CSR (val = the path's signed
+// matrix with its stored diagonal); throws when a cell is not a tree or is
+// coupled to another cell
+void build_hines(int n, int comp, const std::vector<int> &row, const std::vector<int> &col,
+                 const std::vector<double> &val, HinesHost &H) {
+  if (comp > HINES_MAX_COMP) throw Error("the per-cell tree solve is provided up to 1820 compartments per cell", -5);
+  const size_t nc = (size_t)n * comp;
+  H.perm.resize(nc);
+  H.meta.resize(nc);
+  H.up.assign(nc, 0.0);
+  H.dn.assign(nc, 0.0);
+  H.lev.clear();
+  H.lev_off.assign(n + 1, 0);
+  std::vector<std::vector<int>> adj(comp);
+  std::vector<int> dist, par, order, pos(comp);
+  auto at = [&](int gi, int gj) -> double {
+    for (int j = row[gi]; j < row[gi + 1]; j++)
+      if (col[j] == gj) return val[j];
+    throw Error("the per-cell tree solve needs a structurally symmetric matrix", -5);
+  };
+  for (int c = 0; c < n; c++) {
+    const int base = c * comp;
+    int edges = 0;
+    for (int i = 0; i < comp; i++) {
+      adj[i].clear();
+      for (int j = row[base + i]; j < row[base + i + 1]; j++) {
+        const int k = col[j] - base;
+        if (k == i) continue;
+        if (k < 0 || k >= comp)
+          throw Error("the per-cell tree solve needs cells without intercell coupling (gap junctions)", -5);
+        adj[i].push_back(k);
+        edges++;
+      }
+      std::sort(adj[i].begin(), adj[i].end());
+    }
+    tree_bfs(adj, 0, dist, par, order);
+    if ((int)order.size() != comp || edges != 2 * (comp - 1))
+      throw Error("the per-cell tree solve needs every cell to be a tree", -5);
+    // centre: the middle of a longest path (two sweeps)
+    const int a = order.back();
+    tree_bfs(adj, a, dist, par, order);
+    std::vector<int> path;
+    for (int u = order.back(); u >= 0; u = par[u]) path.push_back(u);
+    const int root = path[path.size() / 2];
+    tree_bfs(adj, root, dist, par, order);
+    for (int k = 0; k < comp; k++) pos[order[k]] = k;
+    H.lev_off[c] = (int)H.lev.size();
+    int next_child = 1;
+    for (int k = 0; k < comp; k++) {
+      const int u = order[k];
+      if (k == 0 || dist[u] != dist[order[k - 1]]) H.lev.push_back(k);
+      int nch = 0;
+      for (int w : adj[u])
+        if (par[w] == u) nch++;
+      if (nch > 255) throw Error("the per-cell tree solve takes at most 255 children per compartment", -5);
+      H.meta[(size_t)base + k] = (next_child << 8) | nch;
+      next_child += nch;
+      H.perm[(size_t)base + k] = base + u;
+      if (par[u] >= 0) {
+        H.up[(size_t)base + k] = at(base + u, base + par[u]);
+        H.dn[(size_t)base + k] = at(base + par[u], base + u);
+      }
+    }
+    H.lev.push_back(comp);
+  }
+  H.lev_off[n] = (int)H.lev.size();
+}
+
 template <typename T>
 T *dal(size_t n) {
   void *p = nullptr;
@@ -588,9 +754,8 @@ class CnEngine : public Engine {
  public:
   CnEngine(int n, int comp, const std::vector<double> &cm, const std::vector<double> &v,
            const std::vector<double> &iext, const std::vector<int> &row, const std::vector<int> &col,
-           const std::vector<double> &rkc_val, const StimSpec &stim, bool bicg, bool be)
-      : n_(n), comp_(comp), nc_((int)cm.size()), be_(be), dt_(be ? BE_DT : CN_DT), stim_(stim) {
-    create_stream(&st_);
+           const std::vector<double> &rkc_val, const StimSpec &stim, bool bicg, bool be, bool direct)
+      : n_(n), comp_(comp), nc_((int)cm.size()), be_(be), direct_(direct), dt_(be ? BE_DT : CN_DT), stim_(stim) {
     const int nnz = (int)col.size();
     // the reference's unflipped matrix = -(RKC-signed); halved for CN
     // (pkj_solve.cu:238-245 and the other xx_solve.cu files)
@@ -602,6 +767,19 @@ class CnEngine : public Engine {
         if (col[j] == i) dpos[i] = j;
     for (int i = 0; i < nc_; i++)
       if (dpos[i] < 0) throw Error("the CN and BE paths need a stored diagonal in every row");
+    HinesHost hh;
+    if (direct_) build_hines(n_, comp_, row, col, val, hh);   // before any device allocation
+    create_stream(&st_);
+    if (direct_) {
+      P_.perm = dup(hh.perm);
+      P_.up = dup(hh.up);
+      P_.dn = dup(hh.dn);
+      P_.meta = dup(hh.meta);
+      P_.lev = dup(hh.lev);
+      P_.lev_off = dup(hh.lev_off);
+      P_.comp = comp_;
+      S_.direct = 1;
+    }
     // transpose for BiCG: rows of A in order, so transposed rows are in column order
     std::vector<int> trow(nc_ + 1, 0), tcol(nnz), tsrc(nnz), fill(nc_, 0);
     for (int j = 0; j < nnz; j++) trow[col[j] + 1]++;
@@ -647,8 +825,9 @@ class CnEngine : public Engine {
                           (const void *)S_.ap, (const void *)S_.rs, (const void *)S_.atps, (const void *)S_.p[0],
                           (const void *)S_.p[1], (const void *)S_.ps[0], (const void *)S_.ps[1],
                           (const void *)S_.part, (const void *)S_.scal, (const void *)S_.flag, (const void *)amp_,
-                          (const void *)rec_})
-      (void)hipFree(const_cast<void *>(p));
+                          (const void *)rec_, (const void *)P_.perm, (const void *)P_.up, (const void *)P_.dn,
+                          (const void *)P_.meta, (const void *)P_.lev, (const void *)P_.lev_off})
+      if (p) (void)hipFree(const_cast<void *>(p));
     (void)hipStreamDestroy(st_);
   }
   int nc() const override { return nc_; }
@@ -668,9 +847,14 @@ class CnEngine : public Engine {
   // one CN (BE) step of CN_DT (BE_DT): xx_solve_update_v with that solver
   void advance() override {
     prep();
-    for (int k = 0; k < 100; k++) {
-      k_kry_a<<<nb_, NB, 0, st_>>>(S_, k);
-      k_kry_b<<<nb_, NB, 0, st_>>>(S_, k);
+    if (direct_) {
+      const size_t lds = (size_t)comp_ * (4 * sizeof(double) + sizeof(int));
+      k_hines<<<n_, HINES_WAVE, lds, st_>>>(P_, S_.diag, S_.b, S_.v);
+    } else {
+      for (int k = 0; k < 100; k++) {
+        k_kry_a<<<nb_, NB, 0, st_>>>(S_, k);
+        k_kry_b<<<nb_, NB, 0, st_>>>(S_, k);
+      }
     }
     st.steps++;
   }
@@ -739,10 +923,11 @@ class CnEngine : public Engine {
     up(S_.omega1, o1);
   }
   int n_, comp_, nc_, nb_ = 1;
-  bool be_;
+  bool be_, direct_;
   double dt_;
   StimSpec stim_;
   CnSys S_{};
+  HinesPlan P_{};
   double *amp_ = nullptr, *rec_ = nullptr;
   hipStream_t st_{};
   Stats st;
@@ -750,8 +935,8 @@ class CnEngine : public Engine {
 
 class CnPkj : public CnEngine {
  public:
-  CnPkj(const PkjHost &H, const StimSpec &stim, bool be)
-      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false, be) {
+  CnPkj(const PkjHost &H, const StimSpec &stim, bool be, bool direct)
+      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false, be, direct) {
     using namespace pkj;
     M_.ca = dup(H.ca);
     M_.rev = dup(H.rev);
@@ -793,8 +978,8 @@ class CnPkj : public CnEngine {
 
 class CnIo : public CnEngine {
  public:
-  CnIo(const IoHost &H, const StimSpec &stim, bool be)
-      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, true, be) {
+  CnIo(const IoHost &H, const StimSpec &stim, bool be, bool direct)
+      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, true, be, direct) {
     using namespace io;
     M_.ca = dup(H.ca);
     M_.cond = dup(H.cond);
@@ -835,8 +1020,8 @@ class CnIo : public CnEngine {
 
 class CnGo : public CnEngine {
  public:
-  CnGo(const GoHost &H, const StimSpec &stim, bool be)
-      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false, be) {
+  CnGo(const GoHost &H, const StimSpec &stim, bool be, bool direct)
+      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false, be, direct) {
     using namespace go;
     M_.ca = dup(H.ca);
     M_.ca2 = dup(H.ca2);
@@ -882,8 +1067,8 @@ class CnGo : public CnEngine {
 
 class CnGr : public CnEngine {
  public:
-  CnGr(const GrHost &H, const StimSpec &stim, bool be)
-      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false, be) {
+  CnGr(const GrHost &H, const StimSpec &stim, bool be, bool direct)
+      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false, be, direct) {
     using namespace gr;
     ca_ = dup(H.ca);
     M_.ca = ca_;
@@ -929,13 +1114,13 @@ class CnGr : public CnEngine {
 
 }  // namespace
 
-Engine *make_cn_pkj(const PkjHost &h, const StimSpec &s) { return new CnPkj(h, s, false); }
-Engine *make_cn_io(const IoHost &h, const StimSpec &s) { return new CnIo(h, s, false); }
-Engine *make_cn_go(const GoHost &h, const StimSpec &s) { return new CnGo(h, s, false); }
-Engine *make_cn_gr(const GrHost &h, const StimSpec &s) { return new CnGr(h, s, false); }
-Engine *make_be_pkj(const PkjHost &h, const StimSpec &s) { return new CnPkj(h, s, true); }
-Engine *make_be_io(const IoHost &h, const StimSpec &s) { return new CnIo(h, s, true); }
-Engine *make_be_go(const GoHost &h, const StimSpec &s) { return new CnGo(h, s, true); }
-Engine *make_be_gr(const GrHost &h, const StimSpec &s) { return new CnGr(h, s, true); }
+Engine *make_cn_pkj(const PkjHost &h, const StimSpec &s, bool direct) { return new CnPkj(h, s, false, direct); }
+Engine *make_cn_io(const IoHost &h, const StimSpec &s, bool direct) { return new CnIo(h, s, false, direct); }
+Engine *make_cn_go(const GoHost &h, const StimSpec &s, bool direct) { return new CnGo(h, s, false, direct); }
+Engine *make_cn_gr(const GrHost &h, const StimSpec &s, bool direct) { return new CnGr(h, s, false, direct); }
+Engine *make_be_pkj(const PkjHost &h, const StimSpec &s, bool direct) { return new CnPkj(h, s, true, direct); }
+Engine *make_be_io(const IoHost &h, const StimSpec &s, bool direct) { return new CnIo(h, s, true, direct); }
+Engine *make_be_go(const GoHost &h, const StimSpec &s, bool direct) { return new CnGo(h, s, true, direct); }
+Engine *make_be_gr(const GrHost &h, const StimSpec &s, bool direct) { return new CnGr(h, s, true, direct); }
 
 }  // namespace cbrkc

@@ -20,6 +20,11 @@
 //                    reference's network_sim, main.cu:38, false there); the
 //                    synapse sets are created and coupled either way, as
 //                    main.cu:139-145 does, so without it they stay silent
+//   --tree           CN and BE populations solve each step exactly, per cell
+//                    on its tree (Hines elimination), instead of the
+//                    reference's 100-iteration CG / BiCG; cells coupled by gap
+//                    junctions (Purkinje grids with Pkj_GJ.csv, Golgi grids)
+//                    are refused
 //
 // This build provides all four populations with all three solvers.  Each
 // population advances one H_MAX = 0.125 ms output interval per row, in
@@ -52,6 +57,21 @@ int solver_of(const char *s) {
 // a solver's step: one RKC call covers H_MAX, CN and BE step 0.025 ms (param.h:10-12)
 double dt_of(int solver) { return solver == CB_SOLVER_RKC ? H_MAX : 0.025; }
 
+// xx_initialize + xx_solve_initialize of a granule, Golgi or olive
+// population; with --tree a CN / BE population takes the exact per-cell solve
+int init_pop(bool tree, int type, int solver, int nx, int ny, const char *name, cb_pop **out) {
+  if (tree && solver != CB_SOLVER_RKC) {
+    cb_pop_config cfg;
+    const int rc = cb_config_default(&cfg, type, solver, nx, ny);
+    if (rc != CB_OK) return rc;
+    cfg.engine = CB_ENGINE_TREE_FP64;
+    return cb_pop_create(&cfg, out);
+  }
+  if (type == CB_GR) return cb_gr_initialize(nx, ny, name, out);
+  if (type == CB_GO) return cb_go_initialize(nx, ny, name, out);
+  return cb_io_initialize(nx, ny, name, out);
+}
+
 int check(int rc, const char *what) {
   if (rc != CB_OK) {
     std::fprintf(stderr, "%s failed (%d): %s\n", what, rc, cb_last_error());
@@ -67,12 +87,12 @@ int main(int argc, char *argv[]) {
   for (int i = 0; i < argc; i++) std::printf("argv[%d] = %s\n", i, argv[i]);
   if (argc < 13) {
     std::printf("usage: %s S_gr gx gy S_go ox oy S_pkj px py S_io ix iy [--ts MS] [--engine global|cell|mixed] "
-                "[--stable] [--chunk K] [--network]\n", argv[0]);
+                "[--stable] [--chunk K] [--network] [--tree]\n", argv[0]);
     return 0;
   }
   double ts = 1500.0;
   int engine = CB_ENGINE_GLOBAL_FP64, chunk = 64;
-  bool stable = false, network = false;
+  bool stable = false, network = false, tree = false;
   for (int i = 13; i < argc; i++) {
     std::string o = argv[i];
     if (o == "--ts" && i + 1 < argc) {
@@ -87,6 +107,8 @@ int main(int argc, char *argv[]) {
       if (chunk < 1) chunk = 1;
     } else if (o == "--network") {
       network = true;
+    } else if (o == "--tree") {
+      tree = true;
     } else {
       std::printf("unknown option %s\n", o.c_str());
       return 1;
@@ -118,11 +140,11 @@ int main(int argc, char *argv[]) {
   std::srand(0);
   const int ngr = nx[CB_GR] * ny[CB_GR];
   cb_pop *grp = nullptr;
-  if (ngr > 0) check(cb_gr_initialize(nx[CB_GR], ny[CB_GR], argv[1], &grp), "gr_initialize");
+  if (ngr > 0) check(init_pop(tree, CB_GR, solver[CB_GR], nx[CB_GR], ny[CB_GR], argv[1], &grp), "gr_initialize");
   std::vector<double> gr_v(ngr > 0 ? (size_t)ngr * 578 : 0), gr_prev(ngr, 100.0);
   const int ngo = nx[CB_GO] * ny[CB_GO];
   cb_pop *gop = nullptr;
-  if (ngo > 0) check(cb_go_initialize(nx[CB_GO], ny[CB_GO], argv[4], &gop), "go_initialize");
+  if (ngo > 0) check(init_pop(tree, CB_GO, solver[CB_GO], nx[CB_GO], ny[CB_GO], argv[4], &gop), "go_initialize");
   const int n = nx[CB_PKJ] * ny[CB_PKJ];
   cb_pop *pkj = nullptr;
   int comp = 0;
@@ -130,6 +152,7 @@ int main(int argc, char *argv[]) {
     cb_pop_config cfg;
     check(cb_config_default(&cfg, CB_PKJ, solver[CB_PKJ], nx[CB_PKJ], ny[CB_PKJ]), "config");
     if (solver[CB_PKJ] == CB_SOLVER_RKC) cfg.engine = engine;
+    if (tree && solver[CB_PKJ] != CB_SOLVER_RKC) cfg.engine = CB_ENGINE_TREE_FP64;
     if (stable && solver[CB_PKJ] == CB_SOLVER_RKC) {
       cfg.ion_interval_ms = std::ldexp(1.0, -8);
       cfg.refresh_fn = 1;
@@ -144,7 +167,7 @@ int main(int argc, char *argv[]) {
   // as io_initialize follows pkj_initialize in main.cu:125-137
   const int nio = nx[CB_IO] * ny[CB_IO];
   cb_pop *iop = nullptr;
-  if (nio > 0) check(cb_io_initialize(nx[CB_IO], ny[CB_IO], argv[10], &iop), "io_initialize");
+  if (nio > 0) check(init_pop(tree, CB_IO, solver[CB_IO], nx[CB_IO], ny[CB_IO], argv[10], &iop), "io_initialize");
   std::vector<double> io_v((size_t)nio * 3), io_prev(nio, 100.0);
   std::printf("\n");
 

@@ -36,7 +36,12 @@ enum { CB_SOLVER_BE = 0, CB_SOLVER_CN = 1, CB_SOLVER_RKC = 2 };
 enum {
   CB_ENGINE_GLOBAL_FP64 = 0,  /* one controller for the whole population, FP64 (reference semantics) */
   CB_ENGINE_CELL_FP64 = 1,    /* one controller per cell, persistent launch, FP64 */
-  CB_ENGINE_CELL_MIXED = 2    /* one controller per cell, FP32 stages/gates, compensated V, FP64 control */
+  CB_ENGINE_CELL_MIXED = 2,   /* one controller per cell, FP32 stages/gates, compensated V, FP64 control */
+  CB_ENGINE_TREE_FP64 = 3     /* CN and BE only: the exact per-cell tree (Hines) solve in place of the
+                                 reference's 100-iteration CG / BiCG (cg_cusparse_crs, solve_cnm.cu:20-98);
+                                 cells must be uncoupled trees of at most 1820 compartments (CB_E_UNSUP
+                                 otherwise); no counterpart in the reference, which has only the Krylov
+                                 solves */
 };
 /* fields readable with cb_pop_read (nc doubles each) */
 enum {
@@ -60,7 +65,8 @@ typedef struct cb_pop cb_pop;
 typedef struct cb_pop_config {
   int cell_type;            /* CB_PKJ, ... */
   int solver;               /* CB_SOLVER_* */
-  int engine;               /* CB_ENGINE_* (RKC only; CN and BE run on CB_ENGINE_GLOBAL_FP64) */
+  int engine;               /* CB_ENGINE_* (CN and BE: CB_ENGINE_GLOBAL_FP64, the reference's Krylov
+                               solve, or CB_ENGINE_TREE_FP64) */
   int nx, ny;               /* cell grid; nx * ny cells */
   const char *morph_path;   /* morphology file; NULL: "./Pkj.txt" */
   const char *gj_path;      /* gap-junction pair list; NULL: "./Pkj_GJ.csv" (grid with nx > 1 only) */

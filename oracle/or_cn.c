@@ -190,6 +190,93 @@ static int bicg(or_pop *p, const double *diag, double *x, const double *b)
   return k;
 }
 
+/* The exact solve of A x = b for a matrix whose graph is a forest (every
+   cell a tree, no intercell entries), by Gaussian elimination in leaf order:
+   leaves are eliminated into their one remaining neighbour (ascending index
+   first, then each node as its degree drops to one), which creates no
+   fill-in, and the solution follows in the reverse order.  This is the
+   direct counterpart of the reference's truncated CG / BiCG (the CN / BE
+   systems are diagonally dominant, so no pivoting is needed); the GPU's
+   per-cell solver eliminates level by level from the tree centre, a
+   different order, so the two agree to rounding.  Returns -1 when the graph
+   has a cycle or is not structurally symmetric. */
+static double entry(const or_pop *p, const double *diag, int i, int j)
+{
+  for (int k = p->row[i]; k < p->row[i + 1]; k++)
+    if (p->col[k] == j) return (k == p->cn_dpos[i]) ? diag[i] : p->cn_val[k];
+  return NAN;
+}
+
+static int direct(or_pop *p, const double *diag, double *x, const double *b)
+{
+  const int n = p->nc;
+  int *deg = (int *)calloc(n + 1, sizeof(int)), *gone = (int *)calloc(n + 1, sizeof(int));
+  int *order = (int *)malloc(sizeof(int) * (n + 1)), *nbr = (int *)malloc(sizeof(int) * (n + 1));
+  int *queue = (int *)malloc(sizeof(int) * (n + 1));
+  double *d = zal(n), *r = zal(n);
+  int bad = 0, nq = 0, head = 0, ne = 0;
+  for (int i = 0; i < n; i++) {
+    d[i] = diag[i];
+    r[i] = b[i];
+    for (int k = p->row[i]; k < p->row[i + 1]; k++)
+      if (p->col[k] != i) {
+        deg[i]++;
+        if (isnan(entry(p, diag, p->col[k], i))) bad = 1;
+      }
+  }
+  for (int i = 0; i < n && !bad; i++)
+    if (deg[i] <= 1) queue[nq++] = i;
+  while (!bad && head < nq) {
+    const int i = queue[head++];
+    int q = -1;
+    for (int k = p->row[i]; k < p->row[i + 1]; k++)
+      if (p->col[k] != i && !gone[p->col[k]]) q = p->col[k];
+    gone[i] = 1;
+    order[ne] = i;
+    nbr[ne++] = q;
+    if (q < 0) continue;               /* the last node of its tree */
+    const double f = entry(p, diag, q, i) / d[i];
+    d[q] -= f * entry(p, diag, i, q);
+    r[q] -= f * r[i];
+    if (--deg[q] == 1) queue[nq++] = q;
+  }
+  if (ne < n) bad = 1;                 /* nodes left on a cycle */
+  for (int e = ne - 1; e >= 0 && !bad; e--) {
+    const int i = order[e], q = nbr[e];
+    x[i] = (q < 0) ? r[i] / d[i] : (r[i] - entry(p, diag, i, q) * x[q]) / d[i];
+  }
+  free(deg); free(gone); free(order); free(nbr); free(queue); free(d); free(r);
+  return bad ? -1 : 0;
+}
+
+/* the path's linear solve: CG / BiCG as the reference, or the exact one */
+static int solve(or_pop *p, const double *diag, double *x, const double *b)
+{
+  if (p->cn_direct) return direct(p, diag, x, b) == 0 ? 0 : -1;
+  return (p->type == OR_IO) ? bicg(p, diag, x, b) : cg(p, diag, x, b);
+}
+
+/* switches the CN / BE paths to the exact solve (on = 1); 1 when the
+   population's matrix is not a forest (gap junctions closing cycles) */
+int or_cn_set_direct(or_pop *p, int on)
+{
+  if (on) {
+    double *dg = zal(p->nc), *bb = zal(p->nc), *xx = zal(p->nc);
+    for (int i = 0; i < p->nc; i++) { dg[i] = 1.0 + fabs(p->cn_val[p->cn_dpos[i]]) * 4.0; bb[i] = 1.0; }
+    const int r = direct(p, dg, xx, bb);
+    free(dg); free(bb); free(xx);
+    if (r != 0) return 1;
+  }
+  p->cn_direct = on ? 1 : 0;
+  return 0;
+}
+
+/* test hook: the exact solve for the given diagonal; 0, or -1 as above */
+int or_cn_direct_test(or_pop *p, const double *diag, const double *b, double *x)
+{
+  return direct(p, diag, x, b);
+}
+
 /* the matrix (CN: the unflipped matrix halved, BE: unhalved), the diagonal
    positions and the transpose; 1 when a row lacks its stored diagonal */
 static int sys_init(or_pop *p, int be)
@@ -261,7 +348,7 @@ int or_cn_step(or_pop *p)
     b[i] = bb;
   }
   for (int i = 0; i < n; i++) { p->cn_omega1[i] = omega2[i]; p->cn_gamma1[i] = gamma2[i]; }
-  const int it = (p->type == OR_IO) ? bicg(p, diag, p->v, b) : cg(p, diag, p->v, b);
+  const int it = solve(p, diag, p->v, b);
   p->n_steps++;
   p->n_stages += it;
   free(gamma2); free(omega2); free(diag); free(b);
@@ -326,7 +413,7 @@ int or_be_step(or_pop *p)
   be_ion(p, DT);
   double *diag = zal(n), *b = zal(n);
   be_assemble(p, DT, diag, b);
-  const int it = (p->type == OR_IO) ? bicg(p, diag, p->v, b) : cg(p, diag, p->v, b);
+  const int it = solve(p, diag, p->v, b);
   p->n_steps++;
   p->n_stages += it;
   free(diag); free(b);

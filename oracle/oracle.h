@@ -68,6 +68,7 @@ typedef struct or_pop {
      each row's diagonal and the transpose's structure */
   double *cn_vold, *cn_gamma1, *cn_omega1, *cn_val;
   int *cn_dpos, *cn_trow, *cn_tcol, *cn_tsrc;
+  int cn_direct;      /* 1: exact tree-elimination solve in place of CG / BiCG */
   int *compart;       /* compartment type */
   double *cond[OR_MAX_COND];
   double *ion[OR_MAX_ION];
@@ -152,6 +153,8 @@ int or_cn_init(or_pop *p);
 int or_cn_step(or_pop *p);
 int or_cn_advance(or_pop *p, double t);
 int or_cn_solve_test(or_pop *p, const double *diag, const double *b, double *x, int sym);
+int or_cn_set_direct(or_pop *p, int on);
+int or_cn_direct_test(or_pop *p, const double *diag, const double *b, double *x);
 void or_pkj_cn_ion(or_pop *p, double dt);
 void or_pkj_cn_ge(const or_pop *p, int id, double *gsum, double *esum);
 void or_io_cn_ion(or_pop *p, double dt);
