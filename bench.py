@@ -9,6 +9,13 @@ interval) of all cells.  A compartment-step is one compartment advanced over
 one macro step, independent of the adaptive internal steps.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+`--workload scale` runs the scale-stress comparison instead: 8192 synthetic
+Purkinje cells with per-cell diameter factors U(0.5, 2.0), 50 ms, the mixed
+engine against the per-cell FP64 RKC engine, the plain FP64 global RKC and
+the FP64 Crank-Nicolson engine with the exact per-cell tree solve at several
+steps, each with its error against the FP64 RKC run (one JSON line, not the
+headline metric).
 """
 import argparse
 import json
@@ -59,6 +66,17 @@ def philox_params(seed, n):
         r2 = cbrkc.philox4x32_10([c, 1, 0, 0], [seed & 0xffffffff, 0x43454C4C])
         amp[c] = (0.2 + 0.8 * (((r2[0] >> 5) * 67108864.0 + (r2[1] >> 6)) / 9007199254740992.0)) * 1e-3
     return v0, amp
+
+
+def philox_dscale(seed, n):
+    """Per-cell diameter factors U(0.5, 2.0) from Philox4x32-10
+    (ctr = (cell, 2, 0, 0), key = (seed, 0x43454c4c))."""
+    import cbrkc
+    d = np.zeros(n)
+    for c in range(n):
+        r = cbrkc.philox4x32_10([c, 2, 0, 0], [seed & 0xffffffff, 0x43454C4C])
+        d[c] = 0.5 + 1.5 * (((r[0] >> 5) * 67108864.0 + (r[1] >> 6)) / 9007199254740992.0)
+    return d
 
 
 def make_pop(cbrkc, engine, morph, n, v0, amp, args):
@@ -171,6 +189,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--morph", default=None, help="reuse an existing morphology file")
+    ap.add_argument("--workload", choices=("precision", "scale"), default="precision",
+                    help="precision: the headline 1024-cell run; scale: the 8192-cell scale-stress comparison")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -192,6 +212,8 @@ def main():
     import cbrkc
     from trace_metrics import compare
 
+    if args.workload == "scale":
+        return scale_stress(cbrkc, compare, args)
     n = args.cells
     morph = args.morph or f"/tmp/cbrkc_bench_pkj{n}_s{args.seed}.txt"
     if not os.path.exists(morph):
@@ -289,6 +311,82 @@ def main():
     if not args.no_cpu:
         print("[bench] CPU oracle sample", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(morph, v0, amp, args)
+    print(json.dumps(out), flush=True)
+
+
+def scale_stress(cbrkc, compare, args):
+    """8192 synthetic Purkinje cells (diameters x U(0.5, 2.0) per cell, so the
+    cells' spectral radii and stage counts spread), soma currents U(0.2, 1.0)
+    nA, no dendritic drive, 50 ms: the mixed engine (measured), the per-cell
+    FP64 RKC engine (the accuracy reference), the plain FP64 global RKC on a
+    sample of macro steps, and the FP64 CN engine with the exact per-cell tree
+    solve at the reference's 0.025 ms step and at 2^-5 .. 2^-8 ms; the CN
+    comparison point is the fastest step whose median per-cell subthreshold
+    error against the FP64 RKC run is within 0.1 mV."""
+    n = 8192 if args.cells == 1024 else args.cells
+    steps = 400 if args.steps == 1600 else args.steps
+    morph = args.morph or f"/tmp/cbrkc_scale_pkj{n}_s{args.seed}.txt"
+    if not os.path.exists(morph):
+        cbrkc.synth_pkj_morph(morph, seed=args.seed, ncell=n, dscale=philox_dscale(args.seed, n))
+    v0, amp = philox_params(args.seed, n)
+    stim = (1599, 0.0, 1e9, amp)
+
+    def rkc(engine):
+        return cbrkc.Population(cbrkc.PKJ, n, 1, morph=morph, v0=v0, atol=args.tol, rtol=args.tol, stim=stim,
+                                engine=engine, **cbrkc.STABLE)
+
+    out = {"metric": f"compartment-steps/s, {n} PCs scale stress (diameters x U(0.5,2.0)), {steps * H_MAX:g} ms",
+           "unit": "compartment-steps/s", "n_gpus": 1, "steps": steps, "higher_is_better": True, "data": "synthetic",
+           "config": {"workload": f"{n} synthetic Purkinje cells x 1600 compartments, diameters scaled per cell by "
+                                  f"U(0.5,2.0), soma current U(0.2,1.0) nA, RKC atol=rtol={args.tol:g} with the "
+                                  "safeguards; CN: DT 0.025 ms, exact per-cell tree solve",
+                      "cells": n, "compartments": n * COMP, "sim_ms": steps * H_MAX}}
+    print(f"[scale] mixed engine, {n} cells, {steps} macro steps", file=sys.stderr, flush=True)
+    pm = rkc(cbrkc.ENGINE_CELL_MIXED)
+    nt = pm.autotune(4, threads=args.threads)
+    tr_m, wall_m = timed_run(pm, steps, args.chunk, 0.0)
+    out["value"] = n * COMP * steps / wall_m
+    out["mixed"] = {"value": out["value"], "wall_s": wall_m, "threads_per_cell": nt, "work": pm.stats()}
+    del pm
+    print("[scale] per-cell FP64 RKC engine", file=sys.stderr, flush=True)
+    pd = rkc(cbrkc.ENGINE_CELL_FP64)
+    pd.autotune(0, threads=256)
+    tr_d, wall_d = timed_run(pd, steps, args.chunk, 0.0)
+    out["fp64_cell"] = {"value": n * COMP * steps / wall_d, "wall_s": wall_d, "work": pd.stats()}
+    del pd
+    out["error_mixed_vs_fp64"] = error_report(compare, tr_d, tr_m, "mixed")
+    print(f"[scale] plain FP64 global RKC, {args.plain_steps} macro steps", file=sys.stderr, flush=True)
+    pg = rkc(cbrkc.ENGINE_GLOBAL_FP64)
+    _, wall_g = timed_run(pg, args.plain_steps, args.plain_steps, 0.0, record=False)
+    out["plain_fp64_global"] = {"value": n * COMP * args.plain_steps / wall_g, "wall_s": wall_g,
+                                "macro_steps": args.plain_steps}
+    del pg
+    # FP64 CN with the exact per-cell tree solve at the reference's step and at
+    # finer ones: its error against the FP64 RKC run and its throughput; the
+    # reference's 0.025 ms step is not stable on these stiff synthetic somata
+    # (non-finite cells are counted instead of compared)
+    cn = []
+    for dt in (0.025, 2.0 ** -5, 2.0 ** -6, 2.0 ** -7, 2.0 ** -8):
+        print(f"[scale] FP64 CN engine, exact per-cell tree solve, step {dt:g} ms", file=sys.stderr, flush=True)
+        pc = cbrkc.Population(cbrkc.PKJ, n, 1, solver=cbrkc.CN, engine=cbrkc.ENGINE_TREE_FP64, morph=morph, v0=v0,
+                              stim=stim, step_ms=None if dt == 0.025 else dt)
+        tr_c, wall_c = timed_run(pc, steps, args.chunk, 0.0)
+        entry = {"step_ms": dt, "value": n * COMP * steps / wall_c, "wall_s": wall_c,
+                 "cn_steps": pc.stats()["steps"]}
+        del pc
+        bad = ~np.all(np.isfinite(tr_c), axis=0) | np.any(np.abs(tr_c) > 1e3, axis=0)
+        entry["cells_nonfinite_or_beyond_1V"] = int(bad.sum())
+        if not bad.any():
+            entry["error_vs_fp64_rkc"] = error_report(compare, tr_d, tr_c, "cn_tree")
+        cn.append(entry)
+    out["fp64_cn_tree"] = cn
+    ok = [e for e in cn if "error_vs_fp64_rkc" in e
+          and e["error_vs_fp64_rkc"]["median_per_cell_subthreshold_abs_dV_mV"] <= 0.1]
+    out["speedup_mixed_vs_plain_fp64"] = out["value"] / out["plain_fp64_global"]["value"]
+    if ok:
+        best = max(ok, key=lambda e: e["value"])
+        out["fp64_cn_tree_at_0.1mV"] = {"step_ms": best["step_ms"], "value": best["value"]}
+        out["speedup_mixed_vs_fp64_cn_tree_at_0.1mV"] = out["value"] / best["value"]
     print(json.dumps(out), flush=True)
 
 

@@ -85,6 +85,7 @@ class Config(ctypes.Structure):
         ("refresh_fn", ctypes.c_int),
         ("rho_floor", ctypes.c_int),
         ("device", ctypes.c_int),
+        ("step_ms", ctypes.c_double),
     ]
 
 
@@ -198,7 +199,7 @@ class Population:
 
     def __init__(self, cell_type=PKJ, nx=1, ny=1, solver=RKC, engine=ENGINE_GLOBAL_FP64, morph=None, gj=None,
                  atol=None, rtol=None, v0=None, stim=None, poisson=None, ion_interval=None, refresh_fn=False,
-                 rho_floor=False, device=0):
+                 rho_floor=False, device=0, step_ms=None):
         L = lib()
         cfg = Config()
         _check(L.cb_config_default(ctypes.byref(cfg), cell_type, solver, nx, ny))
@@ -234,6 +235,8 @@ class Population:
         cfg.refresh_fn = 1 if refresh_fn else 0
         cfg.rho_floor = 1 if rho_floor else 0
         cfg.device = device
+        if step_ms:
+            cfg.step_ms = step_ms
         h = ctypes.c_void_p()
         _check(L.cb_pop_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h

@@ -133,9 +133,9 @@ int create_io(const cb_pop_config &cfg, cb_pop **out) {
   p->device = cfg.device;
   const bool tree = cfg.engine == CB_ENGINE_TREE_FP64;
   if (cfg.solver == CB_SOLVER_CN)
-    p->eng.reset(make_cn_io(host, stim, tree));
+    p->eng.reset(make_cn_io(host, stim, tree, cfg.step_ms));
   else if (cfg.solver == CB_SOLVER_BE)
-    p->eng.reset(make_be_io(host, stim, tree));
+    p->eng.reset(make_be_io(host, stim, tree, cfg.step_ms));
   else
     p->eng.reset(make_global_io(host, sol, stim, poisson_of(cfg)));
   *out = p.release();
@@ -182,9 +182,9 @@ int create_go(const cb_pop_config &cfg, cb_pop **out) {
   p->device = cfg.device;
   const bool tree = cfg.engine == CB_ENGINE_TREE_FP64;
   if (cfg.solver == CB_SOLVER_CN)
-    p->eng.reset(make_cn_go(host, stim, tree));
+    p->eng.reset(make_cn_go(host, stim, tree, cfg.step_ms));
   else if (cfg.solver == CB_SOLVER_BE)
-    p->eng.reset(make_be_go(host, stim, tree));
+    p->eng.reset(make_be_go(host, stim, tree, cfg.step_ms));
   else
     p->eng.reset(make_global_go(host, sol, stim, poisson_of(cfg)));
   *out = p.release();
@@ -231,9 +231,9 @@ int create_gr(const cb_pop_config &cfg, cb_pop **out) {
   p->device = cfg.device;
   const bool tree = cfg.engine == CB_ENGINE_TREE_FP64;
   if (cfg.solver == CB_SOLVER_CN)
-    p->eng.reset(make_cn_gr(host, stim, tree));
+    p->eng.reset(make_cn_gr(host, stim, tree, cfg.step_ms));
   else if (cfg.solver == CB_SOLVER_BE)
-    p->eng.reset(make_be_gr(host, stim, tree));
+    p->eng.reset(make_be_gr(host, stim, tree, cfg.step_ms));
   else
     p->eng.reset(make_global_gr(host, sol, stim, poisson_of(cfg)));
   *out = p.release();
@@ -409,6 +409,8 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
       return fail(CB_E_UNSUP, "the CN and BE solvers run on the global FP64 engine or the per-cell tree solve");
     if (cfg->solver == CB_SOLVER_RKC && cfg->engine == CB_ENGINE_TREE_FP64)
       return fail(CB_E_UNSUP, "the per-cell tree solve is provided for the CN and BE solvers");
+    if (cfg->solver == CB_SOLVER_RKC && cfg->step_ms > 0.0)
+      return fail(CB_E_ARG, "step_ms applies to the CN and BE solvers (RKC chooses its own steps)");
     if (cfg->solver != CB_SOLVER_RKC && cfg->poisson_on)
       return fail(CB_E_UNSUP, "the Poisson conductance drive is provided for the RKC solver");
     CB_HIP(hipSetDevice(cfg->device));
@@ -474,7 +476,8 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     p->device = cfg->device;
     if (cfg->solver != CB_SOLVER_RKC) {
       const bool tree = cfg->engine == CB_ENGINE_TREE_FP64;
-      p->eng.reset(cfg->solver == CB_SOLVER_CN ? make_cn_pkj(host, stim, tree) : make_be_pkj(host, stim, tree));
+      p->eng.reset(cfg->solver == CB_SOLVER_CN ? make_cn_pkj(host, stim, tree, cfg->step_ms)
+                                               : make_be_pkj(host, stim, tree, cfg->step_ms));
       *out = p.release();
       return CB_OK;
     }

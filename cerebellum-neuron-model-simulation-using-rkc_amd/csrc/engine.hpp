@@ -129,16 +129,17 @@ Engine *make_global_go(const GoHost &host, const SolverSpec &sol, const StimSpec
 Engine *make_global_gr(const GrHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);
 // the Crank-Nicolson engines (solve_cnm.cu), FP64, population-wide CG / BiCG;
 // direct: an exact per-cell tree (Hines) solve instead, for populations whose
-// cells are uncoupled trees (throws Error code -5 otherwise)
-Engine *make_cn_pkj(const PkjHost &host, const StimSpec &stim, bool direct = false);
-Engine *make_cn_io(const IoHost &host, const StimSpec &stim, bool direct = false);
-Engine *make_cn_go(const GoHost &host, const StimSpec &stim, bool direct = false);
-Engine *make_cn_gr(const GrHost &host, const StimSpec &stim, bool direct = false);
+// cells are uncoupled trees (throws Error code -5 otherwise); dt > 0: the step
+// [ms] in place of the reference's 0.025, a whole fraction of H_MAX
+Engine *make_cn_pkj(const PkjHost &host, const StimSpec &stim, bool direct = false, double dt = 0.0);
+Engine *make_cn_io(const IoHost &host, const StimSpec &stim, bool direct = false, double dt = 0.0);
+Engine *make_cn_go(const GoHost &host, const StimSpec &stim, bool direct = false, double dt = 0.0);
+Engine *make_cn_gr(const GrHost &host, const StimSpec &stim, bool direct = false, double dt = 0.0);
 // the backward-Euler solver of solve_bem.cu on the same Krylov core
-Engine *make_be_pkj(const PkjHost &h, const StimSpec &s, bool direct = false);
-Engine *make_be_io(const IoHost &h, const StimSpec &s, bool direct = false);
-Engine *make_be_go(const GoHost &h, const StimSpec &s, bool direct = false);
-Engine *make_be_gr(const GrHost &h, const StimSpec &s, bool direct = false);
+Engine *make_be_pkj(const PkjHost &h, const StimSpec &s, bool direct = false, double dt = 0.0);
+Engine *make_be_io(const IoHost &h, const StimSpec &s, bool direct = false, double dt = 0.0);
+Engine *make_be_go(const GoHost &h, const StimSpec &s, bool direct = false, double dt = 0.0);
+Engine *make_be_gr(const GrHost &h, const StimSpec &s, bool direct = false, double dt = 0.0);
 Engine *make_cell_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn,
                       bool mixed);
 

@@ -72,6 +72,7 @@ struct CnSys {
   int *flag;
   int nc, bicg;
   int direct;                        // 1: the per-cell tree solve replaces the Krylov loop
+  double dt;                         // the step [ms]: CN_DT / BE_DT, or a whole fraction of H_MAX
   SynIn syn;                         // incoming synapses (none: syn.on() false)
 };
 
@@ -130,7 +131,7 @@ __device__ inline double set_rhs(const CnSys &S, int i, double bb) {
 // conductance and reversal sums (gs, es) and the initial omega2 (the IO gap
 // term, else 0); writes the diagonal and the right-hand side; returns b_i
 __device__ inline double cn_row(const CnSys &S, int i, double gs, double es, double om0) {
-  const double DT = CN_DT;
+  const double DT = S.dt;
   // add_*_val (solve_cnm.cu:100-128, 245-257, 406-418): gamma2 += val / 2,
   // omega2 += val E / 2 per synapse, before the ion sums
   double g0 = 0.0, o0 = om0;
@@ -213,8 +214,8 @@ __global__ void __launch_bounds__(NB) k_cn_prep_pkj(CnSys S, PkjCn M) {
     const double vm = (1.5 * v - 0.5 * S.vold[i]);
     double ca = M.ca[i], rev = M.rev[i];
     const double caq = ca_rk4(v, ca, rev, M.shell[i], c[g_CaP] * g[m_CaP] * g[h_CaP], c[g_CaT] * g[m_CaT] * g[h_CaT],
-                              CN_DT, i % 1599 == 0);
-    gates_cn<double>(vm, caq, CN_DT, g);
+                              S.dt, i % 1599 == 0);
+    gates_cn<double>(vm, caq, S.dt, g);
 #pragma unroll
     for (int k = 0; k < NGATE; k++) M.gate[(size_t)k * nc + i] = g[k];
     M.ca[i] = ca;
@@ -250,7 +251,7 @@ __global__ void __launch_bounds__(NB) k_cn_prep_io(CnSys S, IoCn M) {
     const double v = S.v[i];
     // io_update_ion_2nd (io_ion.cu:53-89)
     double ca = M.ca[i];
-    update_ion_at<double>(v, 1.5 * v - 0.5 * S.vold[i], ca, c[g_CaH], M.compart[i], CN_DT, g);
+    update_ion_at<double>(v, 1.5 * v - 0.5 * S.vold[i], ca, c[g_CaH], M.compart[i], S.dt, g);
 #pragma unroll
     for (int k = 0; k < NGATE; k++) M.gate[(size_t)k * nc + i] = g[k];
     M.ca[i] = ca;
@@ -295,8 +296,8 @@ __global__ void __launch_bounds__(NB) k_cn_prep_go(CnSys S, GoCn M) {
     // go_update_ion_exp_imp (go_ion.cu:319-377): Ca currents at V_old, gates
     // at (V + V_old) / 2; then the KAHP step (go_ion.cu:88-213)
     double ca = M.ca[i], ca2 = M.ca2[i], rev = M.rev[i];
-    update_ion_at(vo, v, vo, M.area[i], c, ca, ca2, rev, CN_DT, g);
-    kahp_step(M.ca_old[i], ca, CN_DT, g);
+    update_ion_at(vo, v, vo, M.area[i], c, ca, ca2, rev, S.dt, g);
+    kahp_step(M.ca_old[i], ca, S.dt, g);
 #pragma unroll
     for (int k = 0; k < NGATE; k++) M.gate[(size_t)k * nc + i] = g[k];
     M.ca[i] = ca;
@@ -337,9 +338,9 @@ __global__ void __launch_bounds__(NB) k_cn_prep_gr(CnSys S, GrCn M) {
       const NaType T = M.na[t];
       double x[12];
       na_gather(g, x);
-      if (na_step(T, vo, v, CN_DT, x)) na_scatter(x, g);
+      if (na_step(T, vo, v, S.dt, x)) na_scatter(x, g);
     }
-    update_gates(v, vo, M.ca[i], CN_DT, g);
+    update_gates(v, vo, M.ca[i], S.dt, g);
 #pragma unroll
     for (int k = 0; k < NGATE; k++) M.gate[(size_t)k * nc + i] = g[k];
     S.vold[i] = v;
@@ -368,13 +369,13 @@ __global__ void __launch_bounds__(NB) k_be_prep_pkj(CnSys S, PkjCn M) {
     // (solve_bem.cu:407-445)
     double ca = M.ca[i], rev = M.rev[i];
     update_ion_be(v, ca, rev, M.shell[i], M.area[i], M.compart[i] == SOMA, c[g_CaP] * g[m_CaP] * g[h_CaP],
-                  c[g_CaT] * g[m_CaT] * g[h_CaT], BE_DT, g);
+                  c[g_CaT] * g[m_CaT] * g[h_CaT], S.dt, g);
 #pragma unroll
     for (int k = 0; k < NGATE; k++) M.gate[(size_t)k * nc + i] = g[k];
     M.ca[i] = ca;
     M.rev[i] = rev;
     double gs, es;
-    be_ge(c, g, rev, S.iext[i], S.cm[i] / BE_DT, v, gs, es);
+    be_ge(c, g, rev, S.iext[i], S.cm[i] / S.dt, v, gs, es);
     bb = be_row_b(S, i, gs, 0.0 + es);   // pkj_solve_by_bem couples no synapses
   }
   cn_prep_finish(S, bb, lds);
@@ -395,12 +396,12 @@ __global__ void __launch_bounds__(NB) k_be_prep_io(CnSys S, IoCn M) {
     // io_update_ion (io_ion.cu:128-161) then io_update_matrix
     // (solve_bem.cu:476-508)
     double ca = M.ca[i];
-    update_ion_be<double>(v, ca, c[g_CaH], M.compart[i], BE_DT, g);
+    update_ion_be<double>(v, ca, c[g_CaH], M.compart[i], S.dt, g);
 #pragma unroll
     for (int k = 0; k < NGATE; k++) M.gate[(size_t)k * nc + i] = g[k];
     M.ca[i] = ca;
     double gs, es;
-    be_ge(c, g, S.iext[i], S.cm[i] / BE_DT, v, gs, es);
+    be_ge(c, g, S.iext[i], S.cm[i] / S.dt, v, gs, es);
     bb = 0.0 + es;
     // io_gap_update + add_io_gap_val (gap.cu:142-160, solve_bem.cu:463-473):
     // minus each gap current into the dendrite, partners in slot order
@@ -433,15 +434,15 @@ __global__ void __launch_bounds__(NB) k_be_prep_go(CnSys S, GoCn M) {
     // go_KAHP_update at the present Ca (go_ion.cu:214-317), go_update_ion
     // (go_ion.cu:438-511), then go_update_matrix (solve_bem.cu:323-347)
     double ca = M.ca[i], ca2 = M.ca2[i], rev = M.rev[i];
-    kahp_be(ca, BE_DT, g);
-    update_ion_be(v, M.area[i], c, ca, ca2, rev, BE_DT, g);
+    kahp_be(ca, S.dt, g);
+    update_ion_be(v, M.area[i], c, ca, ca2, rev, S.dt, g);
 #pragma unroll
     for (int k = 0; k < NGATE; k++) M.gate[(size_t)k * nc + i] = g[k];
     M.ca[i] = ca;
     M.ca2[i] = ca2;
     M.rev[i] = rev;
     double gs, es;
-    be_ge(c, g, rev, S.iext[i], S.cm[i] / BE_DT, v, gs, es);
+    be_ge(c, g, rev, S.iext[i], S.cm[i] / S.dt, v, gs, es);
     bb = be_row(S, i, gs, es);
   }
   cn_prep_finish(S, bb, lds);
@@ -466,13 +467,13 @@ __global__ void __launch_bounds__(NB) k_be_prep_gr(CnSys S, GrCn M) {
       const NaType T = M.na[t];
       double x[12];
       na_gather(g, x);
-      if (na_be(T, v, BE_DT, x)) na_scatter(x, g);
+      if (na_be(T, v, S.dt, x)) na_scatter(x, g);
     }
-    update_gates_be(v, M.ca[i], BE_DT, g);
+    update_gates_be(v, M.ca[i], S.dt, g);
 #pragma unroll
     for (int k = 0; k < NGATE; k++) M.gate[(size_t)k * nc + i] = g[k];
     double gs, es;
-    be_ge(c, g, S.iext[i], S.cm[i] / BE_DT, v, gs, es);
+    be_ge(c, g, S.iext[i], S.cm[i] / S.dt, v, gs, es);
     bb = be_row(S, i, gs, es);
   }
   cn_prep_finish(S, bb, lds);
@@ -754,8 +755,14 @@ class CnEngine : public Engine {
  public:
   CnEngine(int n, int comp, const std::vector<double> &cm, const std::vector<double> &v,
            const std::vector<double> &iext, const std::vector<int> &row, const std::vector<int> &col,
-           const std::vector<double> &rkc_val, const StimSpec &stim, bool bicg, bool be, bool direct)
+           const std::vector<double> &rkc_val, const StimSpec &stim, bool bicg, bool be, bool direct, double dt)
       : n_(n), comp_(comp), nc_((int)cm.size()), be_(be), direct_(direct), dt_(be ? BE_DT : CN_DT), stim_(stim) {
+    if (dt > 0.0) {
+      const double k = H_MAX / dt;
+      if (k < 1.0 || k > 4096.0 || std::fabs(k - std::nearbyint(k)) > 1e-9)
+        throw Error("the CN / BE step must divide the 0.125 ms output interval into 1 to 4096 steps", -2);
+      dt_ = dt;
+    }
     const int nnz = (int)col.size();
     // the reference's unflipped matrix = -(RKC-signed); halved for CN
     // (pkj_solve.cu:238-245 and the other xx_solve.cu files)
@@ -813,6 +820,7 @@ class CnEngine : public Engine {
     CB_HIP(hipMemset(S_.flag, 0, sizeof(int) * F_NFLAG));
     S_.nc = nc_;
     S_.bicg = bicg ? 1 : 0;
+    S_.dt = dt_;
     amp_ = dup(stim_.amp);
     rec_ = dal<double>(n_);
   }
@@ -935,8 +943,8 @@ class CnEngine : public Engine {
 
 class CnPkj : public CnEngine {
  public:
-  CnPkj(const PkjHost &H, const StimSpec &stim, bool be, bool direct)
-      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false, be, direct) {
+  CnPkj(const PkjHost &H, const StimSpec &stim, bool be, bool direct, double dt)
+      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false, be, direct, dt) {
     using namespace pkj;
     M_.ca = dup(H.ca);
     M_.rev = dup(H.rev);
@@ -978,8 +986,8 @@ class CnPkj : public CnEngine {
 
 class CnIo : public CnEngine {
  public:
-  CnIo(const IoHost &H, const StimSpec &stim, bool be, bool direct)
-      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, true, be, direct) {
+  CnIo(const IoHost &H, const StimSpec &stim, bool be, bool direct, double dt)
+      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, true, be, direct, dt) {
     using namespace io;
     M_.ca = dup(H.ca);
     M_.cond = dup(H.cond);
@@ -1020,8 +1028,8 @@ class CnIo : public CnEngine {
 
 class CnGo : public CnEngine {
  public:
-  CnGo(const GoHost &H, const StimSpec &stim, bool be, bool direct)
-      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false, be, direct) {
+  CnGo(const GoHost &H, const StimSpec &stim, bool be, bool direct, double dt)
+      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false, be, direct, dt) {
     using namespace go;
     M_.ca = dup(H.ca);
     M_.ca2 = dup(H.ca2);
@@ -1067,8 +1075,8 @@ class CnGo : public CnEngine {
 
 class CnGr : public CnEngine {
  public:
-  CnGr(const GrHost &H, const StimSpec &stim, bool be, bool direct)
-      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false, be, direct) {
+  CnGr(const GrHost &H, const StimSpec &stim, bool be, bool direct, double dt)
+      : CnEngine(H.n, H.comp, H.cm, H.v, H.iext, H.row, H.col, H.val, stim, false, be, direct, dt) {
     using namespace gr;
     ca_ = dup(H.ca);
     M_.ca = ca_;
@@ -1114,13 +1122,29 @@ class CnGr : public CnEngine {
 
 }  // namespace
 
-Engine *make_cn_pkj(const PkjHost &h, const StimSpec &s, bool direct) { return new CnPkj(h, s, false, direct); }
-Engine *make_cn_io(const IoHost &h, const StimSpec &s, bool direct) { return new CnIo(h, s, false, direct); }
-Engine *make_cn_go(const GoHost &h, const StimSpec &s, bool direct) { return new CnGo(h, s, false, direct); }
-Engine *make_cn_gr(const GrHost &h, const StimSpec &s, bool direct) { return new CnGr(h, s, false, direct); }
-Engine *make_be_pkj(const PkjHost &h, const StimSpec &s, bool direct) { return new CnPkj(h, s, true, direct); }
-Engine *make_be_io(const IoHost &h, const StimSpec &s, bool direct) { return new CnIo(h, s, true, direct); }
-Engine *make_be_go(const GoHost &h, const StimSpec &s, bool direct) { return new CnGo(h, s, true, direct); }
-Engine *make_be_gr(const GrHost &h, const StimSpec &s, bool direct) { return new CnGr(h, s, true, direct); }
+Engine *make_cn_pkj(const PkjHost &h, const StimSpec &s, bool direct, double dt) {
+  return new CnPkj(h, s, false, direct, dt);
+}
+Engine *make_cn_io(const IoHost &h, const StimSpec &s, bool direct, double dt) {
+  return new CnIo(h, s, false, direct, dt);
+}
+Engine *make_cn_go(const GoHost &h, const StimSpec &s, bool direct, double dt) {
+  return new CnGo(h, s, false, direct, dt);
+}
+Engine *make_cn_gr(const GrHost &h, const StimSpec &s, bool direct, double dt) {
+  return new CnGr(h, s, false, direct, dt);
+}
+Engine *make_be_pkj(const PkjHost &h, const StimSpec &s, bool direct, double dt) {
+  return new CnPkj(h, s, true, direct, dt);
+}
+Engine *make_be_io(const IoHost &h, const StimSpec &s, bool direct, double dt) {
+  return new CnIo(h, s, true, direct, dt);
+}
+Engine *make_be_go(const GoHost &h, const StimSpec &s, bool direct, double dt) {
+  return new CnGo(h, s, true, direct, dt);
+}
+Engine *make_be_gr(const GrHost &h, const StimSpec &s, bool direct, double dt) {
+  return new CnGr(h, s, true, direct, dt);
+}
 
 }  // namespace cbrkc

@@ -84,6 +84,9 @@ typedef struct cb_pop_config {
   int rho_floor;            /* 1: spectral-radius estimates are raised to at least 1.2 x the exact
                                cable spectral radius (see cb_pkj_cable_spectral_radius); 0: reference */
   int device;               /* HIP device ordinal */
+  double step_ms;           /* CN and BE: the step [ms]; <= 0: the reference's 0.025 (param.h:10-11);
+                               otherwise 0.125 / step_ms must be a whole number from 1 to 4096 (no
+                               counterpart in the reference, whose step is a compile-time constant) */
 } cb_pop_config;
 
 /* Fills a config with the reference defaults for one cell type and solver. */

@@ -326,11 +326,25 @@ int or_cn_init(or_pop *p)
   return 0;
 }
 
+/* the step of the path: the reference's DT, or the one set with or_cn_set_dt */
+static double step_dt(const or_pop *p, int be) { return p->cn_dt > 0.0 ? p->cn_dt : (be ? BE_DT_ : CN_DT_); }
+
+/* sets the CN / BE step [ms] (dt <= 0: the reference's 0.025); 1 unless
+   H_MAX / dt is a whole number of steps */
+int or_cn_set_dt(or_pop *p, double dt)
+{
+  if (dt <= 0.0) { p->cn_dt = 0.0; return 0; }
+  const double k = H_MAX_ / dt;
+  if (k < 1.0 || fabs(k - floor(k + 0.5)) > 1e-9) return 1;
+  p->cn_dt = dt;
+  return 0;
+}
+
 /* one CN step of CN_DT (xx_solve_by_cnm) */
 int or_cn_step(or_pop *p)
 {
   const int n = p->nc;
-  const double DT = CN_DT_;
+  const double DT = step_dt(p, 0);
   cn_ion(p, DT);
   double *gamma2 = zal(n), *omega2 = zal(n), *diag = zal(n), *b = zal(n);
   if (p->type == OR_IO && p->n > 1)
@@ -362,7 +376,7 @@ int or_cn_advance(or_pop *p, double t)
   while (l_dt < H_MAX_) {
     or_set_current(p, t + l_dt);
     or_cn_step(p);
-    l_dt += CN_DT_;
+    l_dt += step_dt(p, 0);
   }
   return 0;
 }
@@ -401,7 +415,7 @@ static void be_assemble(const or_pop *p, double DT, double *diag, double *b)
 /* test hook: the BE system of the present state, without the ion update */
 int or_be_assemble_test(or_pop *p, double *diag, double *b)
 {
-  be_assemble(p, BE_DT_, diag, b);
+  be_assemble(p, step_dt(p, 1), diag, b);
   return 0;
 }
 
@@ -409,7 +423,7 @@ int or_be_assemble_test(or_pop *p, double *diag, double *b)
 int or_be_step(or_pop *p)
 {
   const int n = p->nc;
-  const double DT = BE_DT_;
+  const double DT = step_dt(p, 1);
   be_ion(p, DT);
   double *diag = zal(n), *b = zal(n);
   be_assemble(p, DT, diag, b);
@@ -427,7 +441,7 @@ int or_be_advance(or_pop *p, double t)
   while (l_dt < H_MAX_) {
     or_set_current(p, t + l_dt);
     or_be_step(p);
-    l_dt += BE_DT_;
+    l_dt += step_dt(p, 1);
   }
   return 0;
 }

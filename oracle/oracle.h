@@ -69,6 +69,7 @@ typedef struct or_pop {
   double *cn_vold, *cn_gamma1, *cn_omega1, *cn_val;
   int *cn_dpos, *cn_trow, *cn_tcol, *cn_tsrc;
   int cn_direct;      /* 1: exact tree-elimination solve in place of CG / BiCG */
+  double cn_dt;       /* CN / BE step [ms]; 0: the reference's 0.025 (param.h:10-11) */
   int *compart;       /* compartment type */
   double *cond[OR_MAX_COND];
   double *ion[OR_MAX_ION];
@@ -154,6 +155,7 @@ int or_cn_step(or_pop *p);
 int or_cn_advance(or_pop *p, double t);
 int or_cn_solve_test(or_pop *p, const double *diag, const double *b, double *x, int sym);
 int or_cn_set_direct(or_pop *p, int on);
+int or_cn_set_dt(or_pop *p, double dt);
 int or_cn_direct_test(or_pop *p, const double *diag, const double *b, double *x);
 void or_pkj_cn_ion(or_pop *p, double dt);
 void or_pkj_cn_ge(const or_pop *p, int id, double *gsum, double *esum);

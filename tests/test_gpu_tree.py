@@ -150,3 +150,37 @@ def test_tree_solve_refusals(ref_data):
     with pytest.raises(cbrkc.CbError):        # RKC has no linear solve to replace
         cbrkc.Population(cbrkc.GR, 1, 1, solver=cbrkc.RKC, engine=cbrkc.ENGINE_TREE_FP64,
                          morph=os.path.join(ref_data, "Gr.txt"), v0=[-70.0])
+
+
+def test_tree_finer_step_pkj(ref_data):
+    """step_ms = 2^-7 ms (16 CN steps per output interval) against the oracle
+    with the same step."""
+    import ctypes
+    LD().or_cn_set_dt.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    v0 = np.array([-65.0])
+    stim = (1599, 0.5, 1e9, 2e-3)
+    dt = 2.0 ** -7
+    p = cbrkc.Population(cbrkc.PKJ, 1, 1, solver=cbrkc.CN, engine=cbrkc.ENGINE_TREE_FP64,
+                         morph=os.path.join(ref_data, "Pkj.txt"), v0=v0, stim=stim, step_ms=dt)
+    o = oracle(orc.Pop.pkj(os.path.join(ref_data, "Pkj.txt"), 1, 1, v0=v0, libc_rand=False), cbrkc.CN)
+    assert LD().or_cn_set_dt(o.h, dt) == 0
+    o.set_stim(*stim)
+    print("tree solve, PKJ 1 cell, CN with 2^-7 ms steps, 16 output intervals", flush=True)
+    worst = 0.0
+    for k in range(16):
+        p.run(1, k * 0.125, 1599, record=False)
+        LD().or_cn_advance(o.h, k * 0.125)
+        d = np.max(np.abs(p.read(cbrkc.F_V) - o.get(orc.V)))
+        worst = max(worst, d)
+        assert d <= 1e-9, (k, d)
+    print(f"worst interval: {worst:.3e} mV", flush=True)
+    assert p.stats()["steps"] == 256
+
+
+def test_step_validation(ref_data):
+    with pytest.raises(cbrkc.CbError):        # 0.125 / 0.03 is not a whole number of steps
+        cbrkc.Population(cbrkc.GR, 1, 1, solver=cbrkc.CN, morph=os.path.join(ref_data, "Gr.txt"), v0=[-70.0],
+                         step_ms=0.03)
+    with pytest.raises(cbrkc.CbError):        # RKC chooses its own steps
+        cbrkc.Population(cbrkc.GR, 1, 1, solver=cbrkc.RKC, morph=os.path.join(ref_data, "Gr.txt"), v0=[-70.0],
+                         step_ms=2.0 ** -7)

@@ -196,3 +196,34 @@ def test_be_path_with_the_exact_solve(ref_data):
     L().or_be_advance(p.h, 0.0)
     assert np.all(np.isfinite(p.get(orc.V)))
     assert p.stats()["stages"] == 0                    # no Krylov iterations
+
+
+def test_step_setting_and_a_finer_passive_step():
+    """or_cn_set_dt: a whole fraction of the 0.125 ms interval is accepted
+    (the sub-step loop then runs 0.125 / dt steps), anything else refused; the
+    CN step at dt = 2^-7 ms is the dense solve with that dt."""
+    L().or_cn_set_dt.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    L().or_cn_set_dt.restype = ctypes.c_int
+    rng = np.random.default_rng(8)
+    n = 20
+    Lr = tree_laplacian(n, rng)
+    cm = rng.uniform(0.5, 2.0, n) * 1e-6
+    v0 = rng.uniform(-70.0, -60.0, n)
+    p = generic(Lr, cm, v0)
+    iext = np.zeros(n)
+    iext[2] = 2e-5
+    p.set(orc.IEXT, iext)
+    assert L().or_cn_init(p.h) == 0
+    assert L().or_cn_set_direct(p.h, 1) == 0
+    for bad in (0.03, 0.2, 0.125 / 3.5):
+        assert L().or_cn_set_dt(p.h, bad) == 1
+    dt = 2.0 ** -7
+    assert L().or_cn_set_dt(p.h, dt) == 0
+    Lpos = -Lr
+    A = np.diag(cm / dt) + Lpos / 2
+    L().or_cn_advance(p.h, 0.0)                        # one output interval = 16 steps of 2^-7 ms
+    v = v0.copy()
+    for _ in range(16):
+        v = np.linalg.solve(A, (np.diag(cm / dt) - Lpos / 2) @ v + iext)
+    np.testing.assert_allclose(p.get(orc.V), v, rtol=1e-12)
+    assert p.stats()["steps"] == 16
