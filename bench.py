@@ -189,8 +189,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--morph", default=None, help="reuse an existing morphology file")
-    ap.add_argument("--workload", choices=("precision", "scale"), default="precision",
-                    help="precision: the headline 1024-cell run; scale: the 8192-cell scale-stress comparison")
+    ap.add_argument("--workload", choices=("precision", "scale", "rows"), default="precision",
+                    help="precision: the headline 1024-cell run; scale: the 8192-cell scale-stress comparison; "
+                         "rows: every cell type x solver on the reference's morphologies, with the oracle beside it")
+    ap.add_argument("--rows", default="PKJ,GR,GO,IO", help="cell types of the rows workload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -214,6 +216,8 @@ def main():
 
     if args.workload == "scale":
         return scale_stress(cbrkc, compare, args)
+    if args.workload == "rows":
+        return rows(cbrkc, args)
     n = args.cells
     morph = args.morph or f"/tmp/cbrkc_bench_pkj{n}_s{args.seed}.txt"
     if not os.path.exists(morph):
@@ -388,6 +392,98 @@ def scale_stress(cbrkc, compare, args):
         out["fp64_cn_tree_at_0.1mV"] = {"step_ms": best["step_ms"], "value": best["value"]}
         out["speedup_mixed_vs_fp64_cn_tree_at_0.1mV"] = out["value"] / best["value"]
     print(json.dumps(out), flush=True)
+
+
+REF_DATA = os.path.join(ROOT, "tests", "golden", "ref_data")
+
+
+def rows(cbrkc, args):
+    """Every cell type with every solver on the reference's own morphology
+    files (reference semantics: population-global RKC, the CN / BE Krylov
+    solves, and the exact per-cell tree solve), each timed on the GPU and, on
+    a bounded sample, in the oracle on the host cores."""
+    types = {  # name: (cell type, file, nx, ny, initial V, macro steps)
+        "PKJ": (cbrkc.PKJ, "Pkj.txt", 1, 64, -65.0, 8),
+        "GR": (cbrkc.GR, "Gr.txt", 16, 16, -70.0, 4),
+        "GO": (cbrkc.GO, "Go_131.txt", 16, 16, -60.0, 8),
+        "IO": (cbrkc.IO, "IO.txt", 32, 32, -60.0, 8),
+    }
+    solvers = (("RKC", cbrkc.RKC, cbrkc.ENGINE_GLOBAL_FP64), ("CN", cbrkc.CN, cbrkc.ENGINE_GLOBAL_FP64),
+               ("CN tree", cbrkc.CN, cbrkc.ENGINE_TREE_FP64), ("BE", cbrkc.BE, cbrkc.ENGINE_GLOBAL_FP64),
+               ("BE tree", cbrkc.BE, cbrkc.ENGINE_TREE_FP64))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out = []
+    for name in args.rows.split(","):
+        ctype, fname, nx, ny, v, k = types[name]
+        n = nx * ny
+        morph = os.path.join(REF_DATA, fname)
+        for sname, solver, engine in solvers:
+            row = {"cells": name, "solver": sname, "n": n}
+            print(f"[rows] {name} x {n}, {sname}", file=sys.stderr, flush=True)
+            try:
+                p = cbrkc.Population(ctype, nx, ny, solver=solver, engine=engine, morph=morph, v0=np.full(n, v))
+            except cbrkc.CbError as e:
+                row["refused"] = str(e)
+                out.append(row)
+                continue
+            rec = {cbrkc.PKJ: 1599, cbrkc.GR: 16, cbrkc.GO: 30, cbrkc.IO: 0}[ctype]
+            p.run(1, 0.0, rec)                   # recording copies back, so each call ends synchronised
+            t = time.perf_counter()
+            p.run(k, H_MAX, rec)
+            wall = time.perf_counter() - t
+            row.update(compartments=p.nc, macro_steps=k, wall_s=wall, value=p.nc * k / wall,
+                       work=p.stats())
+            del p
+            if not args.no_cpu:
+                row["cpu_baseline"] = rows_cpu(ctype, morph, nx, ny, v, solver, engine, args, threads)
+            out.append(row)
+    print(json.dumps({"workload": "rows", "unit": "compartment-steps/s",
+                      "note": "reference morphologies, reference semantics; cpu_baseline = the oracle (FP64 C, "
+                              "OpenMP where it has loops over cells) on the same population",
+                      "rows": out}), flush=True)
+
+
+def rows_cpu(ctype, morph, nx, ny, v, solver, engine, args, threads):
+    """The oracle on the same population for up to 4 macro steps or
+    --cpu-seconds, whichever ends first."""
+    import cbrkc
+    import orc
+    from test_oracle_direct import L
+    from test_oracle_gr import gr_pop
+    from test_oracle_go import go_pop
+    from test_oracle_io import io_pop
+    n = nx * ny
+    v0 = np.full(n, v)
+    ref = os.path.dirname(morph)
+    if ctype == cbrkc.PKJ:
+        o = orc.Pop.pkj(morph, nx, ny, v0=v0, libc_rand=False)
+    elif ctype == cbrkc.GR:
+        o = gr_pop(ref, nx, ny, v0=v0)
+    elif ctype == cbrkc.GO:
+        o = go_pop(ref, nx, ny, v0=v0)
+    else:
+        o = io_pop(ref, nx, ny, v0=v0)
+    o.threads(threads)
+    if solver == cbrkc.CN:
+        assert L().or_cn_init(o.h) == 0
+        adv = L().or_cn_advance
+    elif solver == cbrkc.BE:
+        assert L().or_be_init(o.h) == 0
+        adv = L().or_be_advance
+    if solver != cbrkc.RKC and engine == cbrkc.ENGINE_TREE_FP64:
+        assert L().or_cn_set_direct(o.h, 1) == 0
+    t = time.perf_counter()
+    k = 0
+    while k < 4 and (k == 0 or time.perf_counter() - t < args.cpu_seconds):
+        if solver == cbrkc.RKC:
+            o.set_current(k * H_MAX)
+            o.advance()
+        else:
+            adv(o.h, k * H_MAX)
+        k += 1
+    dt = time.perf_counter() - t
+    return {"value": o.nc * k / dt, "cores": threads, "kind": "port", "macro_steps": k, "wall_s": dt,
+            "threaded": "all vector loops" if solver == cbrkc.RKC else "ion updates (the linear solve is serial)"}
 
 
 if __name__ == "__main__":
