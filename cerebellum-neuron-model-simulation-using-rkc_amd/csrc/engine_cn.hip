@@ -556,6 +556,123 @@ __global__ void __launch_bounds__(NB) k_kry_b(CnSys S, int k) {
   }
 }
 
+// ---- the whole Krylov solve of a small system in one workgroup ----
+// For populations of at most KRY_SMALL_BLOCKS x 256 compartments the 200
+// launches of a step cost more than the arithmetic, and the reference's
+// solves often converge early (the olive's BiCG in about eight iterations),
+// after which the multi-block launches only return.  This kernel runs the
+// same iteration in one workgroup: rows in 256-wide chunks (chunk c = block c
+// of k_kry_a / k_kry_b), every dot product as the chunk partials of
+// block_tree_sum reduced once more with the same tree (the order of
+// block_sum_last), so the iterates are bit-identical to the multi-block path;
+// it stops at convergence.
+constexpr int KRY_SMALL_BLOCKS = 16;
+
+// The total of the values the threads left in v[c * NB + t] for chunks
+// c < nch: every chunk's stride-halving tree (the additions of
+// block_tree_sum), all chunks' levels at once, then the chunk partials by the
+// same tree as block_sum_last.  Returned to every thread.
+__device__ inline double chunks_total(double *v, int nch, double *lds, double *bc) {
+  __syncthreads();
+  for (int stride = NB / 2; stride >= 1; stride >>= 1) {
+    for (int q = threadIdx.x; q < nch * stride; q += NB) {
+      const int c = q / stride, t = q - c * stride;
+      v[c * NB + t] = v[c * NB + t] + v[c * NB + t + stride];
+    }
+    __syncthreads();
+  }
+  if (nch > 1) {
+    const double y = (threadIdx.x < nch) ? v[threadIdx.x * NB] : 0.0;
+    const double t = block_tree_sum(y, lds);
+    if (threadIdx.x == 0) *bc = t;
+  } else if (threadIdx.x == 0) {
+    *bc = v[0];
+  }
+  __syncthreads();
+  return *bc;
+}
+
+__global__ void __launch_bounds__(NB) k_kry_small(CnSys S) {
+  __shared__ double lds[NB];
+  __shared__ double part[KRY_SMALL_BLOCKS * NB];
+  __shared__ double bc;
+  const int nc = S.nc, nch = (nc + NB - 1) / NB;
+  const double bnorm = S.scal[S_BNORM];
+  double rr = S.scal[S_RR], alpha = 0.0, beta = 0.0;
+  int k = 0, done = 0;
+  for (; k < 100 && !done; k++) {
+    const double *pp = S.p[(k + 1) & 1], *pps = S.ps[(k + 1) & 1];   // p_{k-1}
+    double *pk = S.p[k & 1], *psk = S.ps[k & 1];
+    double tot = 0.0;
+    for (int c = 0; c < nch; c++) {                  // the k_kry_a half
+      const int i = c * NB + threadIdx.x;
+      double dotv = 0.0;
+      if (i < nc) {
+        if (k == 0) S.v[i] = 0.0 * S.v[i];
+        double sa = 0.0, mine = 0.0;
+        for (int j = S.row[i]; j < S.row[i + 1]; j++) {
+          const int cj = S.col[j];
+          const double pj = (k == 0) ? pk[cj] : (beta * pp[cj] + S.r[cj]);
+          if (cj == i) mine = pj;
+          sa += ((j == S.dpos[i]) ? S.diag[i] : S.val[j]) * pj;
+        }
+        if (k > 0) pk[i] = mine;
+        S.ap[i] = sa;
+        if (S.bicg) {
+          double st = 0.0, mines = 0.0;
+          for (int t = S.trow[i]; t < S.trow[i + 1]; t++) {
+            const int cj = S.tcol[t], src = S.tsrc[t];
+            const double pj = (k == 0) ? psk[cj] : (beta * pps[cj] + S.rs[cj]);
+            if (cj == i) mines = pj;
+            st += ((src == S.dpos[cj]) ? S.diag[cj] : S.val[src]) * pj;
+          }
+          if (k > 0) psk[i] = mines;
+          S.atps[i] = st;
+          dotv = mines * sa;
+        } else {
+          dotv = mine * sa;
+        }
+      }
+      part[c * NB + threadIdx.x] = dotv;
+    }
+    tot = chunks_total(part, nch, lds, &bc);
+    alpha = rr / tot;
+    const double nalpha = -1.0 * alpha;
+    for (int c = 0; c < nch; c++) {                  // the k_kry_b half
+      const int i = c * NB + threadIdx.x;
+      double dotv = 0.0;
+      if (i < nc) {
+        S.v[i] = alpha * S.p[k & 1][i] + S.v[i];
+        const double ri = nalpha * S.ap[i] + S.r[i];
+        S.r[i] = ri;
+        if (S.bicg) {
+          const double rsi = nalpha * S.atps[i] + S.rs[i];
+          S.rs[i] = rsi;
+          dotv = ri * rsi;
+        } else {
+          dotv = ri * ri;
+        }
+      }
+      part[c * NB + threadIdx.x] = dotv;
+    }
+    tot = chunks_total(part, nch, lds, &bc);
+    if (sqrt(tot) <= 1.0e-15 * sqrt(bnorm)) {
+      done = 1;
+    } else {
+      beta = tot / rr;
+      rr = tot;
+    }
+  }
+  if (threadIdx.x == 0) {
+    S.flag[F_ITERS] = k;
+    S.flag[F_DONE] = 1;
+    S.flag[F_TOTAL] += k;
+    S.scal[S_ALPHA] = alpha;
+    S.scal[S_BETA] = beta;
+    S.scal[S_RR] = rr;
+  }
+}
+
 __global__ void k_cn_stim(double *__restrict__ iext, const double *__restrict__ amp, int comp, int stim_comp,
                           double t0, double t1, double t, int n) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -858,6 +975,8 @@ class CnEngine : public Engine {
     if (direct_) {
       const size_t lds = (size_t)comp_ * (4 * sizeof(double) + sizeof(int));
       k_hines<<<n_, HINES_WAVE, lds, st_>>>(P_, S_.diag, S_.b, S_.v);
+    } else if (nb_ <= KRY_SMALL_BLOCKS) {
+      k_kry_small<<<1, NB, 0, st_>>>(S_);
     } else {
       for (int k = 0; k < 100; k++) {
         k_kry_a<<<nb_, NB, 0, st_>>>(S_, k);

@@ -14,6 +14,12 @@ device's and the host's exp / log in the ion updates:
 * a 2 x 2 Golgi grid (CG with gap junctions in the matrix), 60 ms, and two
   granule cells with a step current, 12 ms: first interval to 1e-9 mV, spike
   counts equal, times within one output interval.
+
+Populations of at most 16 x 256 compartments (all of the above) run each
+step's whole solve in one workgroup (k_kry_small); larger ones launch the
+two halves of every iteration over the whole grid (k_kry_a / k_kry_b).  Both
+reduce in the same order, so both match the oracle alike; the 2 x 4 granule
+population (4 624 compartments, 19 blocks) checks the multi-block path.
 """
 import ctypes
 import os
@@ -122,3 +128,15 @@ def test_cn_go_grid_and_gr_cells(ref_data):
     print(f"CN GR: spikes GPU {sg} oracle {so}", flush=True)
     assert len(so) >= 2 and len(sg) == len(so)
     assert all(a[1] == b[1] and abs(a[0] - b[0]) <= 0.125 for a, b in zip(sg, so))
+
+
+def test_cn_multiblock_gr_population(ref_data):
+    from test_oracle_gr import gr_pop
+    v0 = np.linspace(-72.0, -68.0, 8)
+    p = cbrkc.Population(cbrkc.GR, 2, 4, solver=cbrkc.CN, morph=os.path.join(ref_data, "Gr.txt"), v0=v0)
+    assert p.nc > 16 * 256
+    o = oracle_cn(gr_pop(ref_data, 2, 4, v0=v0))
+    print("CN GR 2x4 (multi-block Krylov launches), 4 output intervals", flush=True)
+    run_pair(p, o, 4, 16, 578, 1e-9)
+    st = p.stats()
+    assert st["steps"] == 20 and 0 < st["stages"] <= 2000, st
