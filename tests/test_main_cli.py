@@ -279,3 +279,31 @@ def test_network_rows_against_oracle(built, tmp_path, ref_data):
     np.testing.assert_allclose(got_gr[:, 1:], np.array(ref_gr), rtol=0, atol=0.1)
     np.testing.assert_allclose(got_go[:, 1], ref_go, rtol=0, atol=6e-7)
     np.testing.assert_allclose(got_pk[:, 1], ref_pk, rtol=0, atol=2e-6)
+
+
+@pytest.mark.gpu
+def test_tree_option_rows_against_oracle(built, tmp_path, ref_data):
+    """./main CN 0 0 CN 0 0 CN 1 1 CN 0 0 --tree: the Purkinje cell's CN steps
+    solved exactly per cell, against the oracle's exact solve from the same
+    libc rand() initial voltage (%lf rows, 2e-6); a Purkinje grid coupled by
+    Pkj_GJ.csv is refused with the library's message."""
+    import ctypes
+    for f in ("Pkj.txt", "Pkj_GJ.csv"):
+        shutil.copy(os.path.join(ref_data, f), tmp_path / f)
+    print("main: PKJ 1x1, CN, --tree, 2 ms", flush=True)
+    r = run_main(["CN", "0", "0", "CN", "0", "0", "CN", "1", "1", "CN", "0", "0", "--ts", "2", "--tree"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = [l.rstrip("\n").rstrip(",").split(",") for l in open(tmp_path / "pkj_v_by_CN.csv")]
+    got = np.array([[float(x) for x in row] for row in rows])
+    assert got.shape == (16, 2)
+    from test_oracle_direct import L as LD
+    o = orc.Pop.pkj(os.path.join(ref_data, "Pkj.txt"), 1, 1, libc_rand=True, seed=0)
+    assert LD().or_cn_init(o.h) == 0 and LD().or_cn_set_direct(o.h, 1) == 0
+    ref = []
+    for k in range(16):
+        LD().or_cn_advance(o.h, k * 0.125)
+        ref.append(o.get(orc.V)[1599])
+    np.testing.assert_allclose(got[:, 1], ref, rtol=0, atol=2e-6)
+    print("main: PKJ 2x2 with Pkj_GJ.csv, CN, --tree (refused)", flush=True)
+    r = run_main(["CN", "0", "0", "CN", "0", "0", "CN", "2", "2", "CN", "0", "0", "--ts", "1", "--tree"], tmp_path)
+    assert r.returncode != 0 and "gap junctions" in r.stderr, r.stdout[-500:] + r.stderr
