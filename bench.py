@@ -87,12 +87,13 @@ def make_pop(cbrkc, engine, morph, n, v0, amp, args):
 
 
 def timed_run(p, steps, chunk, t0, record=True):
-    """steps macro steps in launches of `chunk`; returns (trace, wall seconds)."""
+    """steps macro steps in launches of `chunk` (<= 0: one launch for all of
+    them); returns (trace, wall seconds)."""
     tr = np.zeros((steps, p.n)) if record else None
     done = 0
     t = time.perf_counter()
     while done < steps:
-        k = min(chunk, steps - done)
+        k = steps - done if chunk <= 0 else min(chunk, steps - done)
         out = p.run(k, t0 + done * H_MAX, 1599, record=record)
         if record:
             tr[done:done + k] = out
@@ -175,7 +176,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1600, help="timed macro steps (1600 = 200 ms)")
     ap.add_argument("--warmup", type=int, default=8)
-    ap.add_argument("--chunk", type=int, default=80, help="macro steps per persistent launch")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="macro steps per persistent launch (0: the whole timed run in one launch)")
     ap.add_argument("--cells", type=int, default=1024)
     ap.add_argument("--tol", type=float, default=1e-6, help="atol = rtol (the reference's TOL_PKJ)")
     ap.add_argument("--seed", type=int, default=2024)
