@@ -261,6 +261,14 @@ def main():
                  "frac": round(tfl / PEAK_FP32_TFLOPS, 5),
                  "algorithmic_flops_per_launch": round(flops / nlaunch)}
     roof["binding"] = "fp32-valu" if roof_valu["frac"] > roof["frac"] else "hbm"
+    if traffic:
+        # the bytes that do reach HBM (rocprofv3 FETCH_SIZE / WRITE_SIZE per
+        # cell-step, scaled to this launch) over the same duration: the
+        # algorithmic figure prices the on-chip stage loop as streamed, so it
+        # can exceed the peak while the measured traffic cannot
+        rate = traffic / (kms / nlaunch * 1e-3) / 1e9
+        roof["traffic_rate_GBs"] = round(rate, 1)
+        roof["traffic_frac"] = round(rate / PEAK_HBM_GBS, 4)
 
     out = {
         "metric": METRIC, "value": value, "unit": "compartment-steps/s", "n_gpus": 1, "steps": args.steps,
