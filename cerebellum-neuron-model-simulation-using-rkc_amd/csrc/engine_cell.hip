@@ -93,7 +93,9 @@ enum Plane {
   P_Y0HI, P_Y0LO, P_F0, P_VY, P_EDE,             // integrator state at launch edges
   P_CA                                           // FP64 Ca, 8 / sizeof(T) planes
 };
-template <typename T> constexpr int n_planes() { return P_CA + (int)(8 / sizeof(T)); }
+// the network synaptic current of the macro step follows the Ca planes
+template <typename T> constexpr int p_nets() { return P_CA + (int)(8 / sizeof(T)); }
+template <typename T> constexpr int n_planes() { return p_nets<T>() + 1; }
 
 template <typename T>
 struct CellArgs {
@@ -125,6 +127,9 @@ struct CellArgs {
   double *rec;            // [nsteps][ncell] or null
   int *queue;             // work counter
   const int *order;       // queue position -> cell (the costliest cells of the last run first)
+  // network synapses onto the cells (granule -> Purkinje): CSR by global
+  // compartment cell * comp + original index; off when net.row is null
+  SynIn net;
 };
 
 template <typename T> __device__ inline T absT(T x) { return x < T(0) ? -x : x; }
@@ -244,7 +249,7 @@ __device__ inline void two_sum_acc(T &hi, T &lo, T d) {
 
 // The body of both kernels: INIT runs RKC_vec_initialize for every cell,
 // otherwise nsteps macro steps are advanced from t0.
-template <typename T, int NT, int K, bool INIT>
+template <typename T, int NT, int K, bool INIT, bool NET>
 __device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, double t0, int step0) {
   using namespace pkj;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -356,6 +361,10 @@ __device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, doub
     // channel arithmetic does not multiply register pressure by K.
     auto ion_and_b = [&](double dt, bool do_ion, bool stim_live) {
       PHASE_T0(tI);
+      // the synaptic current of the macro step, decayed over the interval
+      // since the last ion update (solve_rkc.cu:344-367)
+      const bool net_on = NET && a.net.row != nullptr;
+      const T net_decay = net_on ? (T)exp(-dt / 0.6) : T(0);
       // the voltage for the update goes through LDS: each thread reads back
       // only what it wrote, so no barrier is needed
       T *vyl = L.d1;
@@ -409,6 +418,7 @@ __device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, doub
                            ? (T)a.amp[cell] : T(0);
         T bb = current<T>(vn, iext, rev, c, g);
         if (a.syn_on) bb += S.ld(P_SYN, i) * (vn - a.syn_e);
+        if (net_on) bb += S.ld(p_nets<T>(), i) * net_decay;
         S.st(P_B, i, bb);
       }
       a_valid = false;
@@ -795,16 +805,18 @@ __device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, doub
 }
 
 // Occupancy target: 4 waves per SIMD for FP32 (4 cells per CU), 2 for FP64.
-template <typename T, int NT, int K>
+// NET: the variant with network synapses (their current enters b at every
+// ion update); the plain variant carries none of that code
+template <typename T, int NT, int K, bool NET>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2, 8)))
 k_cell_run(CellArgs<T> a, int nsteps, double t0, int step0) {
-  cell_body<T, NT, K, false>(a, nsteps, t0, step0);
+  cell_body<T, NT, K, false, NET>(a, nsteps, t0, step0);
 }
 
 template <typename T, int NT, int K>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2, 8)))
 k_cell_init(CellArgs<T> a) {
-  cell_body<T, NT, K, true>(a, 0, 0.0, 0);
+  cell_body<T, NT, K, true, false>(a, 0, 0.0, 0);
 }
 
 template <typename T>
@@ -817,6 +829,27 @@ T *dalloc(size_t n) {
 template <typename T>
 void h2d(T *d, const std::vector<T> &h) {
   CB_HIP(hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+}
+
+// The macro step's synaptic current from its starting y, before the launch
+// of that one macro step: per compartment the sum over its synapses of
+// val (y - E) in entry order (the current the reference adds at each ion
+// update, decayed; solve_rkc.cu:344-367), into the P_NETS plane.  One thread
+// per compartment position; y = hi + lo from the launch-edge planes.
+template <typename T>
+__global__ void __launch_bounds__(256) k_cell_net(CellArgs<T> a) {
+  const int cell = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.comp) return;
+  const Slab<T> S(a.slab + (size_t)cell * a.slab_stride, (int)a.slab_stride);
+  const int g = cell * a.comp + (int)(S.meta(i) & 0xFFFFu);
+  const double y = (double)S.ld(P_Y0HI, i) + (double)S.ld(P_Y0LO, i);
+  double acc = 0.0;
+  for (int j = a.net.row[g]; j < a.net.row[g + 1]; j++) {
+    const int sl = a.net.slot[j];
+    const int st = sl >> 30;
+    acc += a.net.val[st][sl & 0x3fffffff] * (y - a.net.E[st]);
+  }
+  S.st(p_nets<T>(), i, (T)acc);
 }
 
 template <typename T>
@@ -984,6 +1017,11 @@ class CellPkj : public Engine {
   }
 
   int nc() const override { return nc_; }
+  // granule -> Purkinje synapses (the inhibitory set couples nothing): the
+  // CSR by global compartment, read at every macro-step start
+  void set_synapses(const SynIn &in) override { a_.net = in; }
+  const double *device_v() const override { return vnew_; }
+  hipStream_t stream() const override { return st_; }
   int ncell() const override { return n_; }
   Stats stats() const override {
     std::vector<double> ctl((size_t)n_ * NCTL);
@@ -1018,6 +1056,12 @@ class CellPkj : public Engine {
       a_.rec_orig = rec_comp;
     }
     a_.rec = rec;
+    if (a_.net.on()) {
+      // the synaptic current is fixed per macro step from that step's start,
+      // and the synapse sets update between macro steps
+      if (nsteps != 1) throw Error("with synapses the per-cell engines advance one macro step per call");
+      k_cell_net<T><<<dim3((C_ + 255) / 256, n_), 256, 0, st_>>>(a_);
+    }
     launch(0, nsteps, t0);
     a_.rec = nullptr;
     macro_ += nsteps;
@@ -1217,8 +1261,11 @@ class CellPkj : public Engine {
   void go(int mode, int nsteps, double t0, int grid) {
     if (mode == 1)
       hipLaunchKernelGGL((k_cell_init<T, NT, K>), dim3(grid), dim3(NT), lds_bytes(), st_, a_);
+    else if (a_.net.on())
+      hipLaunchKernelGGL((k_cell_run<T, NT, K, true>), dim3(grid), dim3(NT), lds_bytes(), st_, a_, nsteps, t0,
+                         (int)macro_);
     else
-      hipLaunchKernelGGL((k_cell_run<T, NT, K>), dim3(grid), dim3(NT), lds_bytes(), st_, a_, nsteps, t0,
+      hipLaunchKernelGGL((k_cell_run<T, NT, K, false>), dim3(grid), dim3(NT), lds_bytes(), st_, a_, nsteps, t0,
                          (int)macro_);
   }
 

@@ -246,8 +246,10 @@ int cb_io_finalize(cb_pop *p);
  * RKC granule cells b += val (y - E) at each ion update, Golgi / Purkinje
  * cells the macro step's val (y - E) from its start at each ion update
  * (Purkinje: x exp(-dt / 0.6)); CN gamma2 / omega2 += val / 2, val E / 2; BE
- * diagonal / b += val, val E (granule and Golgi cells only).  Finalize every
- * set before its populations.  The random draws (mossy-fibre and inhibitory
+ * diagonal / b += val, val E (granule and Golgi cells only); the per-cell
+ * Purkinje engines (CB_ENGINE_CELL_*) take the RKC Purkinje term and then
+ * advance one macro step per call.  Finalize every set before its
+ * populations.  The random draws (mossy-fibre and inhibitory
  * Poisson inputs, contact jitter) come from Philox4x32-10 keyed by seed; the
  * reference's cuRAND streams are not reproduced. */
 typedef struct cb_syn cb_syn;

@@ -167,12 +167,59 @@ def test_network_rkc(ref_data):
     run(ref_data, cbrkc.RKC, (2, 2))
 
 
-def test_synapses_rejected_by_per_cell_engines(ref_data):
+@pytest.mark.parametrize("engine", [cbrkc.ENGINE_CELL_FP64, cbrkc.ENGINE_CELL_MIXED])
+def test_granule_purkinje_synapses_on_the_per_cell_engines(ref_data, engine):
+    """A Purkinje cell on a per-cell engine driven by granule -> Purkinje
+    synapses from a 2 x 2 granule population (global engine, step current,
+    spikes near 6 ms), against the same Purkinje cell on the global FP64
+    engine with its own set built from the same granule cells and seed (so
+    the same contacts and, updated from the same presynaptic voltages, the
+    same conductances).  For one cell the per-cell controller is the global
+    one up to the order of its norm reductions, so the two take different
+    step sequences within the 1e-6 tolerance: identical spike counts, spike
+    times within 0.01 ms, subthreshold within 0.03 mV (FP64; measured 0.014 mV
+    with the synaptic input moving the soma by 7 mV) / 0.1 mV (mixed).  A
+    third, uncoupled Purkinje cell shows the synapses act."""
+    amp = np.linspace(0.012e-3, 0.02e-3, 4)
+    gr = cbrkc.Population(cbrkc.GR, 2, 2, morph=os.path.join(ref_data, "Gr.txt"), v0=np.full(4, -70.0),
+                          stim=(16, 2.0, 1e9, amp))
+    mk = lambda e: cbrkc.Population(cbrkc.PKJ, 1, 1, engine=e, morph=os.path.join(ref_data, "Pkj.txt"), v0=[-65.0])
+    pc, pg, pn = mk(engine), mk(cbrkc.ENGINE_GLOBAL_FP64), mk(cbrkc.ENGINE_GLOBAL_FP64)
+    loc = os.path.join(ref_data, "Pkj_location_info.csv")
+    sc = (cbrkc.grpkj_create(gr, pc, loc, 5), cbrkc.mlipkj_create(pc, gr, 7))
+    sg = (cbrkc.grpkj_create(gr, pg, loc, 5), cbrkc.mlipkj_create(pg, gr, 7))
+    np.testing.assert_array_equal(sc[0].comp()[1], sg[0].comp()[1])
+    print(f"granule -> Purkinje ({sc[0].n} synapses), PKJ on engine {engine} vs the global FP64 engine, 12 ms",
+          flush=True)
+    tc, tg, tn, tgr = [], [], [], []
+    for k in range(96):
+        t = k * 0.125
+        for p in (gr, pc, pg, pn):
+            p.set_current(t)
+            p.advance()
+        cbrkc.pkj_synapse_update(t, *sc)
+        cbrkc.pkj_synapse_update(t, *sg)
+        tc.append(pc.read(cbrkc.F_VNEW)[1599])
+        tg.append(pg.read(cbrkc.F_VNEW)[1599])
+        tn.append(pn.read(cbrkc.F_VNEW)[1599])
+        tgr.append(gr.read(cbrkc.F_VNEW)[77::578])
+    np.testing.assert_allclose(sc[0].read(cbrkc.SYN_VAL), sg[0].read(cbrkc.SYN_VAL), rtol=0, atol=0)
+    assert np.max(sg[0].read(cbrkc.SYN_VAL)) > 0.0 and len(spikes(np.array(tgr), 10.0)) > 0
+    from trace_metrics import compare
+    a, b = np.array(tg)[:, None], np.array(tc)[:, None]
+    counts, dt, sub = compare(a, b)
+    effect = np.max(np.abs(np.array(tg) - np.array(tn)))
+    print(f"spikes {counts.tolist()}, max spike-time diff {dt.max():.3e} ms, subthreshold {sub.max():.3e} mV; "
+          f"synaptic effect on the soma {effect:.3e} mV", flush=True)
+    assert effect > 1e-3
+    assert np.all(counts[:, 0] == counts[:, 1]) and dt.max() <= 0.01
+    assert sub.max() <= (0.03 if engine == cbrkc.ENGINE_CELL_FP64 else 0.1)
+    for x in sc + sg:
+        x.close()
+
+
+def test_synapses_rejected_by_populations_without_them(ref_data):
     gr = cbrkc.Population(cbrkc.GR, 1, 1, morph=os.path.join(ref_data, "Gr.txt"), v0=[-70.0])
-    pk = cbrkc.Population(cbrkc.PKJ, 1, 1, engine=cbrkc.ENGINE_CELL_FP64, morph=os.path.join(ref_data, "Pkj.txt"),
-                          v0=[-65.0])
-    with pytest.raises(cbrkc.CbError):
-        cbrkc.grpkj_create(gr, pk, os.path.join(ref_data, "Pkj_location_info.csv"), 1)
     io = cbrkc.Population(cbrkc.IO, 1, 1, morph=os.path.join(ref_data, "IO.txt"), v0=[-60.0])
     with pytest.raises(cbrkc.CbError):
         cbrkc.grgo_create(gr, io)             # not a Golgi population
