@@ -307,3 +307,35 @@ def test_tree_option_rows_against_oracle(built, tmp_path, ref_data):
     print("main: PKJ 2x2 with Pkj_GJ.csv, CN, --tree (refused)", flush=True)
     r = run_main(["CN", "0", "0", "CN", "0", "0", "CN", "2", "2", "CN", "0", "0", "--ts", "1", "--tree"], tmp_path)
     assert r.returncode != 0 and "gap junctions" in r.stderr, r.stdout[-500:] + r.stderr
+
+
+@pytest.mark.gpu
+def test_network_with_the_per_cell_purkinje_engine(built, tmp_path, ref_data):
+    """./main CN 4 4 CN 1 1 RKC 1 1 CN 0 0 --network --engine cell against the
+    same command on the global engine: the Purkinje cell (per-cell FP64 RKC
+    engine, one macro step per call) receives the same granule -> Purkinje
+    input, so its rows agree within the per-cell engines' synaptic bar
+    (0.03 mV); the granule and Golgi rows (CN, the same engine in both runs)
+    are identical.  70 ms, so that the granule cells fire under the
+    reference's 50 ms step current; a run without --network shows the input
+    reaches the Purkinje soma."""
+    outs = {}
+    for eng, net in (("global", True), ("cell", True), ("global", False)):
+        d = tmp_path / f"{eng}_{net}"
+        d.mkdir()
+        for f in ("Gr.txt", "Go_131.txt", "Pkj.txt", "Pkj_GJ.csv", "Pkj_location_info.csv"):
+            shutil.copy(os.path.join(ref_data, f), d / f)
+        print(f"main: GR 4x4 CN, GO 1x1 CN, PKJ 1x1 RKC --engine {eng}, network {net}, 70 ms", flush=True)
+        r = run_main(["CN", "4", "4", "CN", "1", "1", "RKC", "1", "1", "CN", "0", "0", "--ts", "70", "--engine", eng]
+                     + (["--network"] if net else []), d, timeout=600)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+        rows = lambda name: np.array([[float(x) for x in l.rstrip("\n").rstrip(",").split(",")]
+                                      for l in open(d / name)])
+        outs[(eng, net)] = (rows("gr_v_by_CN.csv"), rows("go_v_by_CN.csv"), rows("pkj_v_by_RKC.csv"))
+    (gg, og, pg), (gc, oc, pc) = outs[("global", True)], outs[("cell", True)]
+    np.testing.assert_array_equal(gg, gc)
+    np.testing.assert_array_equal(og, oc)
+    d = np.max(np.abs(pg[:, 1] - pc[:, 1]))
+    effect = np.max(np.abs(pg[:, 1] - outs[("global", False)][2][:, 1]))
+    print(f"Purkinje rows: max |cell - global| = {d:.3e} mV; synaptic effect {effect:.3e} mV", flush=True)
+    assert pg.shape == pc.shape == (560, 2) and d <= 0.03 and effect > 1e-3
