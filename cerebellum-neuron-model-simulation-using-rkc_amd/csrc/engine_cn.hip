@@ -879,6 +879,7 @@ class CnEngine : public Engine {
       if (k < 1.0 || k > 4096.0 || std::fabs(k - std::nearbyint(k)) > 1e-9)
         throw Error("the CN / BE step must divide the 0.125 ms output interval into 1 to 4096 steps", -2);
       dt_ = dt;
+      nsub_ = (int)std::nearbyint(k);
     }
     const int nnz = (int)col.size();
     // the reference's unflipped matrix = -(RKC-signed); halved for CN
@@ -994,11 +995,19 @@ class CnEngine : public Engine {
     if (rec_out) drec = dal<double>((size_t)nsteps * n_);
     for (int k = 0; k < nsteps; k++) {
       const double t = t0 + k * H_MAX;
-      double l_dt = 0.0;
-      while (l_dt < H_MAX) {
-        set_current(t + l_dt);
-        advance();
-        l_dt += dt_;
+      if (nsub_ > 0) {
+        // a set step: exactly H_MAX / dt steps, at whole multiples of dt
+        for (int j = 0; j < nsub_; j++) {
+          set_current(t + j * dt_);
+          advance();
+        }
+      } else {
+        double l_dt = 0.0;
+        while (l_dt < H_MAX) {
+          set_current(t + l_dt);
+          advance();
+          l_dt += dt_;
+        }
       }
       st.macro_steps++;
       if (drec) k_cn_record<<<(n_ + 127) / 128, 128, 0, st_>>>(S_.v, drec + (size_t)k * n_, comp_, rec_comp, n_);
@@ -1052,6 +1061,7 @@ class CnEngine : public Engine {
   int n_, comp_, nc_, nb_ = 1;
   bool be_, direct_;
   double dt_;
+  int nsub_ = 0;   // steps per output interval when the step is set (0: the reference's loop)
   StimSpec stim_;
   CnSys S_{};
   HinesPlan P_{};

@@ -372,6 +372,14 @@ int or_cn_step(or_pop *p)
 /* the reference's sub-step loop for one H_MAX output interval from time t */
 int or_cn_advance(or_pop *p, double t)
 {
+  if (p->cn_dt > 0.0) {                 /* a set step: exactly H_MAX / dt steps */
+    const int nsub = (int)floor(H_MAX_ / p->cn_dt + 0.5);
+    for (int j = 0; j < nsub; j++) {
+      or_set_current(p, t + j * p->cn_dt);
+      or_cn_step(p);
+    }
+    return 0;
+  }
   double l_dt = 0.0;
   while (l_dt < H_MAX_) {
     or_set_current(p, t + l_dt);
@@ -437,6 +445,14 @@ int or_be_step(or_pop *p)
 /* the sub-step loop for one H_MAX output interval from time t */
 int or_be_advance(or_pop *p, double t)
 {
+  if (p->cn_dt > 0.0) {                 /* a set step: exactly H_MAX / dt steps */
+    const int nsub = (int)floor(H_MAX_ / p->cn_dt + 0.5);
+    for (int j = 0; j < nsub; j++) {
+      or_set_current(p, t + j * p->cn_dt);
+      or_be_step(p);
+    }
+    return 0;
+  }
   double l_dt = 0.0;
   while (l_dt < H_MAX_) {
     or_set_current(p, t + l_dt);

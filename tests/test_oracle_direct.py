@@ -227,3 +227,19 @@ def test_step_setting_and_a_finer_passive_step():
         v = np.linalg.solve(A, (np.diag(cm / dt) - Lpos / 2) @ v + iext)
     np.testing.assert_allclose(p.get(orc.V), v, rtol=1e-12)
     assert p.stats()["steps"] == 16
+
+
+def test_a_set_step_runs_a_whole_number_of_steps():
+    """dt = 0.125 / 3 ms (not a binary fraction): each output interval runs
+    exactly three steps at t, t + dt, t + 2 dt, where accumulating dt until
+    0.125 is reached could run a fourth."""
+    L().or_cn_set_dt.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    L().or_cn_set_dt.restype = ctypes.c_int
+    rng = np.random.default_rng(9)
+    n = 6
+    p = generic(tree_laplacian(n, rng), np.full(n, 1e-6), np.full(n, -65.0))
+    assert L().or_cn_init(p.h) == 0
+    assert L().or_cn_set_dt(p.h, 0.125 / 3) == 0
+    for k in range(4):
+        L().or_cn_advance(p.h, k * 0.125)
+    assert p.stats()["steps"] == 12
