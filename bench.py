@@ -153,22 +153,34 @@ def cpu_baseline(morph_file, v0, amp, args):
     sample = morph_file + f".first{ncell}"
     with open(sample, "w") as f:
         f.writelines(lines)
-    o = orc.Pop.pkj(sample, ncell, 1, v0=v0[:ncell], atol=args.tol, rtol=args.tol, libc_rand=False)
-    o.stable_split()
-    o.enable_poisson(seed=args.seed)
-    o.set_stim(1599, 0.0, 1e9, amp[:ncell])
+    def timed(threads, seconds):
+        o = orc.Pop.pkj(sample, ncell, 1, v0=v0[:ncell], atol=args.tol, rtol=args.tol, libc_rand=False)
+        o.stable_split()
+        o.enable_poisson(seed=args.seed)
+        o.set_stim(1599, 0.0, 1e9, amp[:ncell])
+        o.threads(threads)
+        t = time.perf_counter()
+        steps = 0
+        while steps < args.steps and time.perf_counter() - t < seconds:
+            o.set_current(steps * H_MAX)
+            o.advance()
+            steps += 1
+        return steps, time.perf_counter() - t
+
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    o.threads(threads)
-    t = time.perf_counter()
-    steps = 0
-    while steps < args.steps and time.perf_counter() - t < args.cpu_seconds:
-        o.set_current(steps * H_MAX)
-        o.advance()
-        steps += 1
-    dt = time.perf_counter() - t
+    steps, dt = timed(threads, args.cpu_seconds)
+    steps1, dt1 = timed(1, args.cpu_seconds / 3)       # the same run on one thread
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": ncell * COMP * steps / dt, "unit": "compartment-steps/s", "cores": threads, "kind": "port",
             "sample": f"{ncell} of the 1024 cells as one population (reference global controller, FP64), "
-                      f"{steps} macro steps ({steps * H_MAX:g} ms simulated) in {dt:.1f} s"}
+                      f"{steps} macro steps ({steps * H_MAX:g} ms simulated) in {dt:.1f} s",
+            "value_1_thread": ncell * COMP * steps1 / dt1, "sample_1_thread": f"{steps1} macro steps in {dt1:.1f} s",
+            "cpu_model": model, "nproc": os.cpu_count()}
 
 
 def main():
