@@ -834,12 +834,13 @@ void h2d(T *d, const std::vector<T> &h) {
 // The macro step's synaptic current from its starting y, before the launch
 // of that one macro step: per compartment the sum over its synapses of
 // val (y - E) in entry order (the current the reference adds at each ion
-// update, decayed; solve_rkc.cu:344-367), into the P_NETS plane.  One thread
-// per compartment position; y = hi + lo from the launch-edge planes.
+// update, decayed; solve_rkc.cu:344-367), into its slab plane.  One thread
+// per (cell, compartment position); y = hi + lo from the launch-edge planes.
 template <typename T>
 __global__ void __launch_bounds__(256) k_cell_net(CellArgs<T> a) {
-  const int cell = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= a.comp) return;
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= (size_t)a.ncell * a.comp) return;
+  const int cell = (int)(q / a.comp), i = (int)(q % a.comp);
   const Slab<T> S(a.slab + (size_t)cell * a.slab_stride, (int)a.slab_stride);
   const int g = cell * a.comp + (int)(S.meta(i) & 0xFFFFu);
   const double y = (double)S.ld(P_Y0HI, i) + (double)S.ld(P_Y0LO, i);
@@ -1060,7 +1061,7 @@ class CellPkj : public Engine {
       // the synaptic current is fixed per macro step from that step's start,
       // and the synapse sets update between macro steps
       if (nsteps != 1) throw Error("with synapses the per-cell engines advance one macro step per call");
-      k_cell_net<T><<<dim3((C_ + 255) / 256, n_), 256, 0, st_>>>(a_);
+      k_cell_net<T><<<(unsigned)(((size_t)n_ * C_ + 255) / 256), 256, 0, st_>>>(a_);
     }
     launch(0, nsteps, t0);
     a_.rec = nullptr;
