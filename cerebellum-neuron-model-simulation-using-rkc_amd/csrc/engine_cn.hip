@@ -711,6 +711,7 @@ struct HinesPlan {
 };
 
 constexpr int HINES_WAVE = 64;
+constexpr int CHAIN_SEGS = 64;   // segments per cell of the chain solve (one per lane)
 constexpr int HINES_MAX_COMP = 65536 / (4 * 8 + 4);   // the LDS a cell's staging takes (64 KB)
 
 __global__ void __launch_bounds__(HINES_WAVE) k_hines(HinesPlan P, const double *__restrict__ diag,
@@ -758,9 +759,97 @@ __global__ void __launch_bounds__(HINES_WAVE) k_hines(HinesPlan P, const double 
   for (int k = lane; k < comp; k += HINES_WAVE) v[P.perm[base + k]] = r[k];
 }
 
+// The same elimination by chains, for trees made of few long unbranched
+// runs (the granule cell's 578 compartments are 9 chains, the Golgi cell's 131
+// are 6; the level walk pays one barrier per compartment of depth there).  A
+// segment is a maximal run bottom -> top in which every compartment above the
+// bottom has exactly one child; positions are numbered segment by segment
+// (bottom first), the segments by height (0: the bottom is a leaf, else one
+// more than its child segments).  One lane per segment: at height h it folds
+// the tops of its child segments into its bottom (child order as in k_hines),
+// then walks up its chain in registers, one division per compartment, no
+// barrier; back substitution walks down from the top.  Every compartment sees
+// the same operations in the same order as in k_hines, so the two agree bit
+// for bit.
+struct ChainPlan {
+  const int *perm;       // position -> global compartment
+  const double *up, *dn; // per position
+  const int *seg;        // [cell][CHAIN_SEGS][4]: start, len, kf | kc << 16, (ptop + 1) | h << 16
+  const int *kids;       // [cell][CHAIN_SEGS]: positions of the child segments' tops
+  const int *nseg, *nh;  // per cell: segments, heights
+  int comp;
+};
+
+__global__ void __launch_bounds__(HINES_WAVE) k_hines_chain(ChainPlan P, const double *__restrict__ diag,
+                                                            const double *__restrict__ b, double *__restrict__ v) {
+  extern __shared__ double sm[];
+  const int comp = P.comp, lane = threadIdx.x, cell = blockIdx.x;
+  double *d = sm, *r = sm + comp, *up = sm + 2 * comp, *dn = sm + 3 * comp;
+  int *kids = (int *)(sm + 4 * comp);
+  const size_t base = (size_t)cell * comp;
+  for (int k = lane; k < comp; k += HINES_WAVE) {
+    const int g = P.perm[base + k];
+    d[k] = diag[g];
+    r[k] = b[g];
+    up[k] = P.up[base + k];
+    dn[k] = P.dn[base + k];
+  }
+  kids[lane] = P.kids[(size_t)cell * CHAIN_SEGS + lane];
+  const int nseg = P.nseg[cell], nh = P.nh[cell];
+  int start = 0, len = 0, kf = 0, kc = 0, ptop = -1, h = -1;
+  if (lane < nseg) {
+    const int *q = P.seg + ((size_t)cell * CHAIN_SEGS + lane) * 4;
+    start = q[0];
+    len = q[1];
+    kf = q[2] & 0xffff;
+    kc = q[2] >> 16;
+    ptop = (q[3] & 0xffff) - 1;
+    h = q[3] >> 16;
+  }
+  __syncthreads();
+  for (int hh = 0; hh < nh; hh++) {
+    if (h == hh) {
+      int c = start;
+      double dd = d[c], rr = r[c];
+      for (int q = kf; q < kf + kc; q++) {
+        const int j = kids[q];
+        const double f = dn[j] / d[j];
+        dd -= f * up[j];
+        rr -= f * r[j];
+      }
+      for (int i = 1; i < len; i++, c++) {
+        const double f = dn[c] / dd;
+        d[c] = dd;
+        r[c] = rr;
+        dd = d[c + 1] - f * up[c];
+        rr = r[c + 1] - f * rr;
+      }
+      d[c] = dd;
+      r[c] = rr;
+    }
+    __syncthreads();
+  }
+  for (int hh = nh - 1; hh >= 0; hh--) {
+    if (h == hh) {
+      int c = start + len - 1;
+      double x = (ptop >= 0) ? (r[c] - up[c] * r[ptop]) / d[c] : r[c] / d[c];
+      r[c] = x;
+      for (c = c - 1; c >= start; c--) {
+        x = (r[c] - up[c] * x) / d[c];
+        r[c] = x;
+      }
+    }
+    __syncthreads();
+  }
+  for (int k = lane; k < comp; k += HINES_WAVE) v[P.perm[base + k]] = r[k];
+}
+
 struct HinesHost {
   std::vector<int> perm, meta, lev, lev_off;
   std::vector<double> up, dn;
+  // the chain plan (every cell at most CHAIN_SEGS segments), else empty
+  std::vector<int> cperm, seg, kids, nseg, nh;
+  std::vector<double> cup, cdn;
 };
 
 // breadth-first distances from s over adj (ascending neighbour order)
@@ -783,6 +872,68 @@ void tree_bfs(const std::vector<std::vector<int>> &adj, int s, std::vector<int> 
   }
 }
 
+// the chain plan of cell c from its tree (par, breadth-first order from the
+// root); false when the cell has more than CHAIN_SEGS segments
+template <typename At>
+bool chain_plan(int c, int comp, const std::vector<std::vector<int>> &adj, const std::vector<int> &par,
+                const std::vector<int> &order, At &at, HinesHost &H) {
+  std::vector<int> nch(comp, 0), seg_of(comp, -1);
+  for (int u = 0; u < comp; u++)
+    if (par[u] >= 0) nch[par[u]]++;
+  std::vector<std::vector<int>> segs;   // bottom -> top, deepest bottoms first
+  for (int k = comp - 1; k >= 0; k--) {
+    const int u = order[k];
+    if (nch[u] == 1) continue;
+    std::vector<int> nodes{u};
+    for (int w = u; par[w] >= 0 && nch[par[w]] == 1;) {
+      w = par[w];
+      nodes.push_back(w);
+    }
+    if ((int)segs.size() == CHAIN_SEGS) return false;
+    for (int w : nodes) seg_of[w] = (int)segs.size();
+    segs.push_back(std::move(nodes));
+  }
+  const int ns = (int)segs.size();
+  std::vector<int> ht(ns, 0);
+  for (int s = 0; s < ns; s++)              // child segments were made earlier
+    for (int w : adj[segs[s][0]])
+      if (par[w] == segs[s][0]) ht[s] = std::max(ht[s], ht[seg_of[w]] + 1);
+  std::vector<int> ord(ns);
+  for (int s = 0; s < ns; s++) ord[s] = s;
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return ht[x] < ht[y]; });
+  std::vector<int> pos(comp), start(ns);
+  int next = 0;
+  for (int s : ord) {
+    start[s] = next;
+    for (int w : segs[s]) pos[w] = next++;
+  }
+  const size_t base = (size_t)c * comp;
+  for (int u = 0; u < comp; u++) {
+    const size_t k = base + pos[u];
+    H.cperm[k] = (int)base + u;
+    if (par[u] >= 0) {
+      H.cup[k] = at((int)base + u, (int)base + par[u]);
+      H.cdn[k] = at((int)base + par[u], (int)base + u);
+    }
+  }
+  int nkid = 0, hmax = 0;
+  for (int i = 0; i < ns; i++) {
+    const int s = ord[i], bottom = segs[s][0], top = segs[s].back();
+    const int kf = nkid;
+    for (int w : adj[bottom])                 // the bottom's children in the level walk's order
+      if (par[w] == bottom) H.kids[(size_t)c * CHAIN_SEGS + nkid++] = pos[w];
+    int *q = &H.seg[((size_t)c * CHAIN_SEGS + i) * 4];
+    q[0] = start[s];
+    q[1] = (int)segs[s].size();
+    q[2] = kf | ((nkid - kf) << 16);
+    q[3] = (par[top] >= 0 ? pos[par[top]] + 1 : 0) | (ht[s] << 16);
+    hmax = std::max(hmax, ht[s]);
+  }
+  H.nseg[c] = ns;
+  H.nh[c] = hmax + 1;
+  return true;
+}
+
 // the plan of every cell from the system's CSR (val = the path's signed
 // matrix with its stored diagonal); throws when a cell is not a tree or is
 // coupled to another cell
@@ -796,6 +947,14 @@ void build_hines(int n, int comp, const std::vector<int> &row, const std::vector
   H.dn.assign(nc, 0.0);
   H.lev.clear();
   H.lev_off.assign(n + 1, 0);
+  bool chain = true;
+  H.cperm.assign(nc, 0);
+  H.cup.assign(nc, 0.0);
+  H.cdn.assign(nc, 0.0);
+  H.seg.assign((size_t)n * CHAIN_SEGS * 4, 0);
+  H.kids.assign((size_t)n * CHAIN_SEGS, 0);
+  H.nseg.assign(n, 0);
+  H.nh.assign(n, 0);
   std::vector<std::vector<int>> adj(comp);
   std::vector<int> dist, par, order, pos(comp);
   auto at = [&](int gi, int gj) -> double {
@@ -847,8 +1006,14 @@ void build_hines(int n, int comp, const std::vector<int> &row, const std::vector
       }
     }
     H.lev.push_back(comp);
+    if (chain) chain = chain_plan(c, comp, adj, par, order, at, H);
   }
   H.lev_off[n] = (int)H.lev.size();
+  if (!chain) {
+    for (auto *x : {&H.cperm, &H.seg, &H.kids, &H.nseg, &H.nh}) x->clear();
+    H.cup.clear();
+    H.cdn.clear();
+  }
 }
 
 template <typename T>
@@ -895,6 +1060,17 @@ class CnEngine : public Engine {
     HinesHost hh;
     if (direct_) build_hines(n_, comp_, row, col, val, hh);   // before any device allocation
     create_stream(&st_);
+    if (direct_ && !hh.nseg.empty()) {
+      chain_ = true;
+      Q_.perm = dup(hh.cperm);
+      Q_.up = dup(hh.cup);
+      Q_.dn = dup(hh.cdn);
+      Q_.seg = dup(hh.seg);
+      Q_.kids = dup(hh.kids);
+      Q_.nseg = dup(hh.nseg);
+      Q_.nh = dup(hh.nh);
+      Q_.comp = comp_;
+    }
     if (direct_) {
       P_.perm = dup(hh.perm);
       P_.up = dup(hh.up);
@@ -952,7 +1128,9 @@ class CnEngine : public Engine {
                           (const void *)S_.p[1], (const void *)S_.ps[0], (const void *)S_.ps[1],
                           (const void *)S_.part, (const void *)S_.scal, (const void *)S_.flag, (const void *)amp_,
                           (const void *)rec_, (const void *)P_.perm, (const void *)P_.up, (const void *)P_.dn,
-                          (const void *)P_.meta, (const void *)P_.lev, (const void *)P_.lev_off})
+                          (const void *)P_.meta, (const void *)P_.lev, (const void *)P_.lev_off,
+                          (const void *)Q_.perm, (const void *)Q_.up, (const void *)Q_.dn, (const void *)Q_.seg,
+                          (const void *)Q_.kids, (const void *)Q_.nseg, (const void *)Q_.nh})
       if (p) (void)hipFree(const_cast<void *>(p));
     (void)hipStreamDestroy(st_);
   }
@@ -973,7 +1151,10 @@ class CnEngine : public Engine {
   // one CN (BE) step of CN_DT (BE_DT): xx_solve_update_v with that solver
   void advance() override {
     prep();
-    if (direct_) {
+    if (chain_) {
+      const size_t lds = (size_t)comp_ * 4 * sizeof(double) + CHAIN_SEGS * sizeof(int);
+      k_hines_chain<<<n_, HINES_WAVE, lds, st_>>>(Q_, S_.diag, S_.b, S_.v);
+    } else if (direct_) {
       const size_t lds = (size_t)comp_ * (4 * sizeof(double) + sizeof(int));
       k_hines<<<n_, HINES_WAVE, lds, st_>>>(P_, S_.diag, S_.b, S_.v);
     } else if (nb_ <= KRY_SMALL_BLOCKS) {
@@ -1065,6 +1246,8 @@ class CnEngine : public Engine {
   StimSpec stim_;
   CnSys S_{};
   HinesPlan P_{};
+  ChainPlan Q_{};
+  bool chain_ = false;   // every cell at most CHAIN_SEGS segments: the chain solve
   double *amp_ = nullptr, *rec_ = nullptr;
   hipStream_t st_{};
   Stats st;
