@@ -15,7 +15,9 @@ Purkinje 1599 through -10 mV) in equal numbers per cell with times within
 one macro step; the mossy-fibre conductances, which depend on the draws
 alone, equal at every step; every set's conductances within 1e-6 relative at
 the end when the spike trains agree to the macro step (they do on these
-workloads).
+workloads).  The granule -> Purkinje set also drives a Purkinje cell on the
+per-cell engines, checked against the global FP64 engine (its own bars in
+the test's docstring).
 """
 import ctypes
 import os
