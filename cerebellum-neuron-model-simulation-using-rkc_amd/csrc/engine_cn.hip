@@ -714,19 +714,25 @@ constexpr int HINES_WAVE = 64;
 constexpr int CHAIN_SEGS = 64;   // segments per cell of the chain solve (one per lane)
 constexpr int HINES_MAX_COMP = 65536 / (4 * 8 + 4);   // the LDS a cell's staging takes (64 KB)
 
+// SYM: A(node, parent) = A(parent, node) everywhere (the CN and BE tree
+// matrices of the Purkinje, granule and Golgi cells), so one off-diagonal
+// plane is staged: 28 instead of 36 bytes of LDS per compartment, three
+// Purkinje cells per CU instead of two
+template <bool SYM>
 __global__ void __launch_bounds__(HINES_WAVE) k_hines(HinesPlan P, const double *__restrict__ diag,
                                                       const double *__restrict__ b, double *__restrict__ v) {
   extern __shared__ double sm[];
   const int comp = P.comp, lane = threadIdx.x;
-  double *d = sm, *r = sm + comp, *up = sm + 2 * comp, *dn = sm + 3 * comp;
-  int *meta = (int *)(sm + 4 * comp);
+  double *d = sm, *r = sm + comp, *up = sm + 2 * comp;
+  double *dn = SYM ? up : sm + 3 * comp;
+  int *meta = (int *)(sm + (SYM ? 3 : 4) * comp);
   const size_t base = (size_t)blockIdx.x * comp;
   for (int k = lane; k < comp; k += HINES_WAVE) {
     const int g = P.perm[base + k];
     d[k] = diag[g];
     r[k] = b[g];
     up[k] = P.up[base + k];
-    dn[k] = P.dn[base + k];
+    if (!SYM) dn[k] = P.dn[base + k];
     meta[k] = P.meta[base + k];
   }
   __syncthreads();
@@ -847,6 +853,7 @@ __global__ void __launch_bounds__(HINES_WAVE) k_hines_chain(ChainPlan P, const d
 struct HinesHost {
   std::vector<int> perm, meta, lev, lev_off;
   std::vector<double> up, dn;
+  bool sym = true;   // up == dn at every position
   // the chain plan (every cell at most CHAIN_SEGS segments), else empty
   std::vector<int> cperm, seg, kids, nseg, nh;
   std::vector<double> cup, cdn;
@@ -1003,6 +1010,7 @@ void build_hines(int n, int comp, const std::vector<int> &row, const std::vector
       if (par[u] >= 0) {
         H.up[(size_t)base + k] = at(base + u, base + par[u]);
         H.dn[(size_t)base + k] = at(base + par[u], base + u);
+        if (H.up[(size_t)base + k] != H.dn[(size_t)base + k]) H.sym = false;
       }
     }
     H.lev.push_back(comp);
@@ -1072,6 +1080,7 @@ class CnEngine : public Engine {
       Q_.comp = comp_;
     }
     if (direct_) {
+      sym_ = hh.sym;
       P_.perm = dup(hh.perm);
       P_.up = dup(hh.up);
       P_.dn = dup(hh.dn);
@@ -1154,9 +1163,12 @@ class CnEngine : public Engine {
     if (chain_) {
       const size_t lds = (size_t)comp_ * 4 * sizeof(double) + CHAIN_SEGS * sizeof(int);
       k_hines_chain<<<n_, HINES_WAVE, lds, st_>>>(Q_, S_.diag, S_.b, S_.v);
+    } else if (direct_ && sym_) {
+      const size_t lds = (size_t)comp_ * (3 * sizeof(double) + sizeof(int));
+      k_hines<true><<<n_, HINES_WAVE, lds, st_>>>(P_, S_.diag, S_.b, S_.v);
     } else if (direct_) {
       const size_t lds = (size_t)comp_ * (4 * sizeof(double) + sizeof(int));
-      k_hines<<<n_, HINES_WAVE, lds, st_>>>(P_, S_.diag, S_.b, S_.v);
+      k_hines<false><<<n_, HINES_WAVE, lds, st_>>>(P_, S_.diag, S_.b, S_.v);
     } else if (nb_ <= KRY_SMALL_BLOCKS) {
       k_kry_small<<<1, NB, 0, st_>>>(S_);
     } else {
@@ -1247,6 +1259,7 @@ class CnEngine : public Engine {
   CnSys S_{};
   HinesPlan P_{};
   ChainPlan Q_{};
+  bool sym_ = false;     // the level walk stages one off-diagonal plane
   bool chain_ = false;   // every cell at most CHAIN_SEGS segments: the chain solve
   double *amp_ = nullptr, *rec_ = nullptr;
   hipStream_t st_{};
