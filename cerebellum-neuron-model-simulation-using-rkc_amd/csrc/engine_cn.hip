@@ -80,15 +80,16 @@ struct CnSys {
 // part[blockIdx.x]; the block that finishes last reduces all partials level by
 // level with the same tree and returns true with the total in *out (thread 0).
 // The partials cross workgroups (and the XCDs' separate L2 caches), so they
-// are stored and loaded with device-scope atomics and the ticket is an
-// acquire-release counter.
+// are stored and loaded with device-scope atomics; every block releases its
+// partial with the ticket increment, and only the block that turns out last
+// acquires (the others read nothing afterwards).
 __device__ bool block_sum_last(const CnSys &S, double x, double *lds, double *out) {
   const int nb = gridDim.x;
   const double bs = block_tree_sum(x, lds);
   __shared__ bool last;
   if (threadIdx.x == 0) {
     __hip_atomic_store(&S.part[blockIdx.x], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int prev = __hip_atomic_fetch_add(&S.flag[F_TICKET], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    const int prev = __hip_atomic_fetch_add(&S.flag[F_TICKET], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     last = (prev == nb - 1);
   }
   __syncthreads();
