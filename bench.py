@@ -222,7 +222,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(cells would shard by population across GPUs; that path is not built)", file=sys.stderr)
+              "(the cells would be sharded across GPUs; that path is not built)", file=sys.stderr)
         sys.exit(2)
 
     import cbrkc
