@@ -25,6 +25,7 @@ BE, CN, RKC = 0, 1, 2
 ENGINE_GLOBAL_FP64, ENGINE_CELL_FP64, ENGINE_CELL_MIXED, ENGINE_TREE_FP64 = 0, 1, 2, 3
 F_VNEW, F_V, F_CA, F_REV, F_B, F_IEXT, F_Y, F_YN, F_FN, F_CA2, F_CA_OLD, F_VOLD = range(12)
 F_GATE0 = 100
+STIM_NONE = -2   # cb_pop_config.stim_comp: no injected current (network_sim on)
 H_MAX = 0.125
 
 # every symbol include/cbrkc.h declares
@@ -86,6 +87,7 @@ class Config(ctypes.Structure):
         ("rho_floor", ctypes.c_int),
         ("device", ctypes.c_int),
         ("step_ms", ctypes.c_double),
+        ("hmin_ms", ctypes.c_double),
     ]
 
 
@@ -197,9 +199,9 @@ def cable_spectral_radius(path, ncell):
 class Population:
     """One cell population with its solver (the reference's neuron_t + neuron_solve_t)."""
 
-    def __init__(self, cell_type=PKJ, nx=1, ny=1, solver=RKC, engine=ENGINE_GLOBAL_FP64, morph=None, gj=None,
+    def __init__(self, cell_type=PKJ, nx=1, ny=1, solver=RKC, engine=ENGINE_GLOBAL_FP64, morph=None, gj="",
                  atol=None, rtol=None, v0=None, stim=None, poisson=None, ion_interval=None, refresh_fn=False,
-                 rho_floor=False, device=0, step_ms=None):
+                 rho_floor=False, device=0, step_ms=None, hmin_ms=None):
         L = lib()
         cfg = Config()
         _check(L.cb_config_default(ctypes.byref(cfg), cell_type, solver, nx, ny))
@@ -207,7 +209,10 @@ class Population:
         self._keep = []
         if morph:
             cfg.morph_path = morph.encode()
-        if gj:
+        # Purkinje grids: gj names the gap-junction pair list; "" (the default
+        # here) builds uncoupled cells, None reads ./Pkj_GJ.csv as the
+        # reference does and fails when it is missing
+        if gj is not None:
             cfg.gj_path = gj.encode()
         if atol:
             cfg.atol = atol
@@ -217,7 +222,9 @@ class Population:
             v0 = np.ascontiguousarray(v0, dtype=np.float64)
             self._keep.append(v0)
             cfg.v0 = _dp(v0)
-        if stim is not None:
+        if stim == "none":
+            cfg.stim_comp = STIM_NONE
+        elif stim is not None:
             comp, t0, t1, amp = stim
             amp = np.ascontiguousarray(np.broadcast_to(np.asarray(amp, dtype=np.float64), (nx * ny,)))
             self._keep.append(amp)
@@ -237,6 +244,8 @@ class Population:
         cfg.device = device
         if step_ms:
             cfg.step_ms = step_ms
+        if hmin_ms:
+            cfg.hmin_ms = hmin_ms
         h = ctypes.c_void_p()
         _check(L.cb_pop_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
@@ -312,13 +321,6 @@ def pkj_initialize(nx, ny, solver="RKC"):
     _check(lib().cb_pop_size(h, ctypes.byref(n), ctypes.byref(nc)))
     p.n, p.nc, p.comp, p.engine, p._keep = n.value, nc.value, nc.value // max(n.value, 1), ENGINE_GLOBAL_FP64, []
     return p
-
-
-def io_initialize(nx, ny, solver="RKC"):
-    """io_initialize + io_solve_initialize + io_gap_create; reads ./IO.txt."""
-    h = ctypes.c_void_p()
-    _check(lib().cb_io_initialize(nx, ny, solver.encode(), ctypes.byref(h)))
-    return _wrap(h)
 
 
 def pkj_set_current(p, t):

@@ -117,11 +117,13 @@ int create_io(const cb_pop_config &cfg, cb_pop **out) {
   stim.amp.assign(n, io::STIM);
   if (cfg.stim_amp)
     for (int c = 0; c < n; c++) stim.amp[c] = cfg.stim_amp[c];
+  if (cfg.stim_comp == CB_STIM_NONE) stim.amp.assign(n, 0.0);
   SolverSpec sol;
   const double tol = 1.0e-6;   // TOL_IO, solve_rkc.cuh:31
   sol.atol = cfg.atol > 0 ? cfg.atol : tol;
   sol.rtol = cfg.rtol > 0 ? cfg.rtol : tol;
   sol.ion_interval = cfg.ion_interval_ms;
+  sol.hmin = cfg.hmin_ms;
   sol.refresh_fn = cfg.refresh_fn != 0;
   if (cfg.rho_floor) return fail(CB_E_UNSUP, "the spectral-radius floor is provided for Purkinje cells");
   if (!(sol.atol > 0.0) || !(sol.rtol >= 0.0)) return fail(CB_E_ARG, "atol must be > 0 and rtol >= 0");
@@ -131,6 +133,7 @@ int create_io(const cb_pop_config &cfg, cb_pop **out) {
   p->engine = cfg.engine;
   p->comp = io::COMP;
   p->device = cfg.device;
+  CB_HIP(hipSetDevice(cfg.device));
   const bool tree = cfg.engine == CB_ENGINE_TREE_FP64;
   if (cfg.solver == CB_SOLVER_CN)
     p->eng.reset(make_cn_io(host, stim, tree, cfg.step_ms));
@@ -166,11 +169,13 @@ int create_go(const cb_pop_config &cfg, cb_pop **out) {
   stim.amp.assign(n, go::STIM);
   if (cfg.stim_amp)
     for (int c = 0; c < n; c++) stim.amp[c] = cfg.stim_amp[c];
+  if (cfg.stim_comp == CB_STIM_NONE) stim.amp.assign(n, 0.0);
   SolverSpec sol;
   const double tol = 1.0e-6;   // TOL_GO, solve_rkc.cuh:29
   sol.atol = cfg.atol > 0 ? cfg.atol : tol;
   sol.rtol = cfg.rtol > 0 ? cfg.rtol : tol;
   sol.ion_interval = cfg.ion_interval_ms;
+  sol.hmin = cfg.hmin_ms;
   sol.refresh_fn = cfg.refresh_fn != 0;
   if (cfg.rho_floor) return fail(CB_E_UNSUP, "the spectral-radius floor is provided for Purkinje cells");
   if (!(sol.atol > 0.0) || !(sol.rtol >= 0.0)) return fail(CB_E_ARG, "atol must be > 0 and rtol >= 0");
@@ -180,6 +185,7 @@ int create_go(const cb_pop_config &cfg, cb_pop **out) {
   p->engine = cfg.engine;
   p->comp = go::COMP;
   p->device = cfg.device;
+  CB_HIP(hipSetDevice(cfg.device));
   const bool tree = cfg.engine == CB_ENGINE_TREE_FP64;
   if (cfg.solver == CB_SOLVER_CN)
     p->eng.reset(make_cn_go(host, stim, tree, cfg.step_ms));
@@ -215,11 +221,13 @@ int create_gr(const cb_pop_config &cfg, cb_pop **out) {
   stim.amp.assign(n, gr::STIM);
   if (cfg.stim_amp)
     for (int c = 0; c < n; c++) stim.amp[c] = cfg.stim_amp[c];
+  if (cfg.stim_comp == CB_STIM_NONE) stim.amp.assign(n, 0.0);
   SolverSpec sol;
   const double tol = 1.0e-9;   // TOL_GR, solve_rkc.cuh:28
   sol.atol = cfg.atol > 0 ? cfg.atol : tol;
   sol.rtol = cfg.rtol > 0 ? cfg.rtol : tol;
   sol.ion_interval = cfg.ion_interval_ms;
+  sol.hmin = cfg.hmin_ms;
   sol.refresh_fn = cfg.refresh_fn != 0;
   if (cfg.rho_floor) return fail(CB_E_UNSUP, "the spectral-radius floor is provided for Purkinje cells");
   if (!(sol.atol > 0.0) || !(sol.rtol >= 0.0)) return fail(CB_E_ARG, "atol must be > 0 and rtol >= 0");
@@ -229,6 +237,7 @@ int create_gr(const cb_pop_config &cfg, cb_pop **out) {
   p->engine = cfg.engine;
   p->comp = gr::COMP;
   p->device = cfg.device;
+  CB_HIP(hipSetDevice(cfg.device));
   const bool tree = cfg.engine == CB_ENGINE_TREE_FP64;
   if (cfg.solver == CB_SOLVER_CN)
     p->eng.reset(make_cn_gr(host, stim, tree, cfg.step_ms));
@@ -411,9 +420,11 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
       return fail(CB_E_UNSUP, "the per-cell tree solve is provided for the CN and BE solvers");
     if (cfg->solver == CB_SOLVER_RKC && cfg->step_ms > 0.0)
       return fail(CB_E_ARG, "step_ms applies to the CN and BE solvers (RKC chooses its own steps)");
+    if (cfg->hmin_ms > 0.125) return fail(CB_E_ARG, "hmin_ms must not exceed H_MAX = 0.125 ms");
     if (cfg->solver != CB_SOLVER_RKC && cfg->poisson_on)
       return fail(CB_E_UNSUP, "the Poisson conductance drive is provided for the RKC solver");
-    CB_HIP(hipSetDevice(cfg->device));
+    // the device is selected right before an engine is built, so argument
+    // and file errors are reported without touching a GPU
     if (cfg->cell_type == CB_IO) return create_io(*cfg, out);
     if (cfg->cell_type == CB_GO) return create_go(*cfg, out);
     if (cfg->cell_type == CB_GR) return create_gr(*cfg, out);
@@ -424,13 +435,13 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     std::vector<int> gpre, gpost;
     const int n = cfg->nx * cfg->ny;
     if (n == 0) return fail(CB_E_ARG, "empty population");
-    bool use_gj = n > 1;
+    // a grid reads its gap-junction pairs from ./Pkj_GJ.csv unless told
+    // otherwise, and a missing file is an error as in pkj_solve.cu:86-87;
+    // gj_path = "" asks for uncoupled cells
+    bool use_gj = n > 1 && !(cfg->gj_path && cfg->gj_path[0] == '\0');
     if (use_gj) {
       const std::string gpath = cfg->gj_path ? cfg->gj_path : "./Pkj_GJ.csv";
-      if (!read_pkj_gj(gpath, gpre, gpost, err)) {
-        if (cfg->gj_path) return fail(CB_E_IO, err);
-        use_gj = false;   // the default file is optional
-      }
+      if (!read_pkj_gj(gpath, gpre, gpost, err)) return fail(CB_E_IO, err);
     }
     PkjHost host;
     if (!build_pkj_host(rows, cfg->nx, cfg->ny, use_gj ? &gpre : nullptr, use_gj ? &gpost : nullptr, cfg->v0, host,
@@ -446,6 +457,7 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     stim.amp.assign(n, pkj::CURRENT);
     if (cfg->stim_amp)
       for (int c = 0; c < n; c++) stim.amp[c] = cfg->stim_amp[c];
+    if (cfg->stim_comp == CB_STIM_NONE) stim.amp.assign(n, 0.0);
     PoissonSpec syn;
     syn.on = cfg->poisson_on != 0;
     syn.rate_hz = cfg->poisson_rate_hz;
@@ -458,6 +470,7 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     sol.atol = cfg->atol > 0 ? cfg->atol : tol;
     sol.rtol = cfg->rtol > 0 ? cfg->rtol : tol;
     sol.ion_interval = cfg->ion_interval_ms;
+    sol.hmin = cfg->hmin_ms;
     sol.refresh_fn = cfg->refresh_fn != 0;
     if (cfg->rho_floor) {
       std::vector<double> rho;
@@ -474,6 +487,10 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     p->engine = cfg->engine;
     p->comp = pkj::COMP;
     p->device = cfg->device;
+    if (cfg->solver == CB_SOLVER_RKC &&
+        (cfg->engine == CB_ENGINE_CELL_FP64 || cfg->engine == CB_ENGINE_CELL_MIXED) && host.n_gj > 0)
+      return fail(CB_E_UNSUP, "per-cell engines need cells without intercell gap junctions");
+    CB_HIP(hipSetDevice(cfg->device));
     if (cfg->solver != CB_SOLVER_RKC) {
       const bool tree = cfg->engine == CB_ENGINE_TREE_FP64;
       p->eng.reset(cfg->solver == CB_SOLVER_CN ? make_cn_pkj(host, stim, tree, cfg->step_ms)
@@ -487,7 +504,6 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
         break;
       case CB_ENGINE_CELL_FP64:
       case CB_ENGINE_CELL_MIXED:
-        if (host.n_gj > 0) return fail(CB_E_UNSUP, "per-cell engines need cells without intercell gap junctions");
         p->eng.reset(make_cell_pkj(host, sol, stim, syn, cfg->engine == CB_ENGINE_CELL_MIXED));
         break;
       default:
@@ -547,22 +563,26 @@ int cb_pop_read(cb_pop *p, int field, double *dst) {
 }
 
 int cb_pop_size(const cb_pop *p, int *ncell, int *nc) {
-  if (!p) return fail(CB_E_ARG, "null population");
-  if (ncell) *ncell = p->eng->ncell();
-  if (nc) *nc = p->eng->nc();
-  return CB_OK;
+  return guarded([&]() -> int {
+    if (!p) return fail(CB_E_ARG, "null population");
+    if (ncell) *ncell = p->eng->ncell();
+    if (nc) *nc = p->eng->nc();
+    return CB_OK;
+  });
 }
 
 int cb_pop_stats(const cb_pop *p, int64_t *out6) {
-  if (!p || !out6) return fail(CB_E_ARG, "null argument");
-  Stats s = p->eng->stats();
-  out6[0] = s.steps;
-  out6[1] = s.rejects;
-  out6[2] = s.stages;
-  out6[3] = s.rho_iters;
-  out6[4] = s.ion_updates;
-  out6[5] = s.macro_steps;
-  return CB_OK;
+  return guarded([&]() -> int {
+    if (!p || !out6) return fail(CB_E_ARG, "null argument");
+    Stats s = p->eng->stats();
+    out6[0] = s.steps;
+    out6[1] = s.rejects;
+    out6[2] = s.stages;
+    out6[3] = s.rho_iters;
+    out6[4] = s.ion_updates;
+    out6[5] = s.macro_steps;
+    return CB_OK;
+  });
 }
 
 int cb_pop_kernel_time(cb_pop *p, double *ms, int64_t *launches, int reset) {
@@ -589,10 +609,12 @@ int cb_pop_autotune(cb_pop *p, int trial_steps, int *threads) {
 }
 
 int cb_pop_controller(const cb_pop *p, double *o, double *hw) {
-  if (!p || !o || !hw) return fail(CB_E_ARG, "null argument");
-  if (p->engine != CB_ENGINE_GLOBAL_FP64) return fail(CB_E_UNSUP, "per-cell engines have no global controller");
-  p->eng->controller(o, hw);
-  return CB_OK;
+  return guarded([&]() -> int {
+    if (!p || !o || !hw) return fail(CB_E_ARG, "null argument");
+    if (p->engine != CB_ENGINE_GLOBAL_FP64) return fail(CB_E_UNSUP, "per-cell engines have no global controller");
+    p->eng->controller(o, hw);
+    return CB_OK;
+  });
 }
 
 int cb_synth_pkj_morph(const char *path, uint64_t seed, int ncell, double soma_r_um, double lmin_um, double lmax_um,

@@ -25,6 +25,10 @@ struct Error {
   explicit Error(std::string m, int c = -1) : msg(std::move(m)), code(c) {}
 };
 
+// RKC status beyond the reference's idid values (1, 2, 3, 4, 6): the state
+// of a cell became NaN or infinite
+constexpr int NONFINITE = 7;
+
 // field ids shared with the C-ABI (cb_pop_read)
 enum Field {
   F_VNEW = 0,     // interpolated V at the end of the last macro step (vec[v_new])
@@ -116,6 +120,8 @@ struct SolverSpec {
   double ion_interval = -1.0;   // <= 0: 2^log10(atol) (solve_rkc.cu:1265)
   bool refresh_fn = false;      // recompute fn = F(y_n) after each ion update (off in the reference)
   std::vector<double> rho_floor;  // per cell: lower bound applied to every spectral-radius estimate (empty: none)
+  double hmin = -1.0;           // <= 0: the cell type's H_MIN (solve_rkc.cuh:24-27)
+  double hmin_or(double dflt) const { return hmin > 0.0 ? hmin : dflt; }
 };
 
 // RKC stage coefficients per stage count m = 2..mtab: row m holds mtab

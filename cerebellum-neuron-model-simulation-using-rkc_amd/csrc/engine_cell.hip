@@ -538,7 +538,11 @@ __device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, doub
           absh = fabs(tend - lt);
           last = true;
         }
-        int m = 1 + (int)(sqrt(1.54 * absh * sprad + 1.0));
+        // (a non-finite h * rho takes the mmax branch instead of an undefined
+        // conversion, so the coefficient table is always indexed in range;
+        // the error estimate then stops the cell)
+        const double mr = sqrt(1.54 * absh * sprad + 1.0);
+        int m = (mr < mmax) ? 1 + (int)mr : (int)mmax + 1;
         if (m > mmax) {
           m = (int)mmax;
           absh = (m * m - 1) / (1.54 * sprad);
@@ -641,6 +645,11 @@ __device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, doub
         }
         double err = wg_sum<NT>(sq, red, slot);
         err = sqrt(err / (double)C);
+        if (!(err <= 1.0e300)) {   // NaN or infinite state: stop this cell
+          idid = NONFINITE;
+          PHASE_ADD(PH_ERR, tE);
+          return;
+        }
         if (err > 1.0) {
           count(1, 1.0);
           absh = 0.8 * absh / ctl_pow<T>(err, 1);
@@ -950,7 +959,7 @@ class CellPkj : public Engine {
     a_.comp = C;
     a_.atol = sol.atol;
     a_.rtol = sol.rtol;
-    a_.hmin = std::pow(2.0, -10.0);
+    a_.hmin = sol.hmin_or(std::pow(2.0, -10.0));   // H_MIN_PKJ, solve_rkc.cuh:26
     a_.hmax = H_MAX;
     a_.integ = sol.ion_interval > 0 ? sol.ion_interval : std::pow(2.0, std::log10(sol.atol));
     a_.refresh_fn = sol.refresh_fn ? 1 : 0;
@@ -1220,8 +1229,9 @@ class CellPkj : public Engine {
     for (auto &e : s.saved) (void)hipFree(e.second.first);
     s.saved.clear();
   }
+  // everything a run writes: the slabs, the controllers and V at the launch end
   std::vector<std::pair<void *, size_t>> mutable_bufs() {
-    return {{a_.slab, stride_ * n_}, {a_.ctl, sizeof(double) * (size_t)n_ * NCTL}};
+    return {{a_.slab, stride_ * n_}, {a_.ctl, sizeof(double) * (size_t)n_ * NCTL}, {a_.vnew_out, sizeof(double) * nc_}};
   }
 
   static int nint_ori(double x) {
@@ -1322,6 +1332,7 @@ class CellPkj : public Engine {
     CB_HIP(hipStreamSynchronize(st_));
     for (int c = 0; c < n_; c++) {
       int s = (int)ctl[(size_t)c * NCTL + C_STATUS];
+      if (s == NONFINITE) throw Error("RKC: non-finite state in cell " + std::to_string(c), s);
       if (s != 0) throw Error("RKC failed in cell " + std::to_string(c) + ": idid = " + std::to_string(s), s);
     }
   }

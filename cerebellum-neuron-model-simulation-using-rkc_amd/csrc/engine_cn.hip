@@ -82,7 +82,9 @@ struct CnSys {
 // The partials cross workgroups (and the XCDs' separate L2 caches), so they
 // are stored and loaded with device-scope atomics; every block releases its
 // partial with the ticket increment, and only the block that turns out last
-// acquires (the others read nothing afterwards).
+// acquires: its thread 0, right after reading the ticket, so the acquire
+// follows the atomic read it pairs with; the workgroup barrier then orders
+// the other threads' loads after it (the other blocks read nothing).
 __device__ bool block_sum_last(const CnSys &S, double x, double *lds, double *out) {
   const int nb = gridDim.x;
   const double bs = block_tree_sum(x, lds);
@@ -91,10 +93,10 @@ __device__ bool block_sum_last(const CnSys &S, double x, double *lds, double *ou
     __hip_atomic_store(&S.part[blockIdx.x], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int prev = __hip_atomic_fetch_add(&S.flag[F_TICKET], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     last = (prev == nb - 1);
+    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
   if (!last) return false;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   int m = nb;
   while (m > 1) {
     const int chunks = (m + NB - 1) / NB;

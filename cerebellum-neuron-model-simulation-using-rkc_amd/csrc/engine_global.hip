@@ -801,7 +801,10 @@ class GlobalRkc : public Engine {
         o_[ABSH] = std::fabs(o_[TEND] - o_[LT]);
         last_ = true;
       }
-      int m = 1 + (int)(std::sqrt(1.54 * o_[ABSH] * o_[SPRAD] + 1.0));
+      // (a non-finite h * rho takes the mmax branch instead of an undefined
+      // conversion; its error estimate then stops the run below)
+      const double mr = std::sqrt(1.54 * o_[ABSH] * o_[SPRAD] + 1.0);
+      int m = (mr < o_[MMAX]) ? 1 + (int)mr : (int)o_[MMAX] + 1;
       if (m > o_[MMAX]) {
         m = (int)o_[MMAX];
         o_[ABSH] = (m * m - 1) / (1.54 * o_[SPRAD]);
@@ -820,6 +823,9 @@ class GlobalRkc : public Engine {
         return;
       }
       err = std::sqrt(err / (double)nc_);
+      // a NaN or infinite state: the reference's main stops on a NaN voltage
+      // (main.cu:200, 228, 257, 287); here the run stops with the status
+      if (!std::isfinite(err)) throw Error("RKC: non-finite state (error estimate " + std::to_string(err) + ")", NONFINITE);
       if (err > 1.0) {
         st.rejects++;
         o_[ABSH] = 0.8 * o_[ABSH] / (std::pow(err, 1.0 / 3.0));
@@ -895,7 +901,7 @@ class GlobalPkj : public GlobalRkc {
     shell_ = dalloc<double>(nc_); h2d(shell_, H.shell);
     cond_ = dalloc<double>((size_t)NCOND * nc_); h2d(cond_, H.cond);
     gate_ = dalloc<double>((size_t)NGATE * nc_); h2d(gate_, H.gate);
-    setup(sol, std::pow(2.0, -10.0));   // H_MIN_PKJ, solve_rkc.cuh:26
+    setup(sol, sol.hmin_or(std::pow(2.0, -10.0)));   // H_MIN_PKJ, solve_rkc.cuh:26
   }
   ~GlobalPkj() override {
     (void)hipStreamSynchronize(st_);
@@ -941,7 +947,7 @@ class GlobalIo : public GlobalRkc {
     gate_ = dalloc<double>((size_t)NGATE * nc_); h2d(gate_, H.gate);
     compart_ = dalloc<int8_t>(nc_); h2d(compart_, H.compart);
     nbr_ = dalloc<int>(H.nbr.size()); h2d(nbr_, H.nbr);
-    setup(sol, std::pow(2.0, -10.0));   // H_MIN_IO, solve_rkc.cuh:27
+    setup(sol, sol.hmin_or(std::pow(2.0, -10.0)));   // H_MIN_IO, solve_rkc.cuh:27
   }
   ~GlobalIo() override {
     (void)hipStreamSynchronize(st_);
@@ -982,7 +988,7 @@ class GlobalGo : public GlobalRkc {
     area_ = dalloc<double>(nc_); h2d(area_, H.area);
     cond_ = dalloc<double>((size_t)NCOND * nc_); h2d(cond_, H.cond);
     gate_ = dalloc<double>((size_t)NGATE * nc_); h2d(gate_, H.gate);
-    setup(sol, std::pow(2.0, -10.0));   // H_MIN_GO, solve_rkc.cuh:25
+    setup(sol, sol.hmin_or(std::pow(2.0, -10.0)));   // H_MIN_GO, solve_rkc.cuh:25
   }
   ~GlobalGo() override {
     (void)hipStreamSynchronize(st_);
@@ -1032,7 +1038,7 @@ class GlobalGr : public GlobalRkc {
     const NaTable T = na_table();
     na_ = dalloc<NaType>(NTYPE);
     CB_HIP(hipMemcpy(na_, T.t, sizeof(T.t), hipMemcpyHostToDevice));
-    setup(sol, std::pow(2.0, -12.0));   // H_MIN_GR, solve_rkc.cuh:24
+    setup(sol, sol.hmin_or(std::pow(2.0, -12.0)));   // H_MIN_GR, solve_rkc.cuh:24
   }
   ~GlobalGr() override {
     (void)hipStreamSynchronize(st_);

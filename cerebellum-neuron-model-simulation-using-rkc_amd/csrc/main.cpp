@@ -16,9 +16,11 @@
 //   --stable         the safeguards for synthetic trees (ion interval 2^-8 ms,
 //                    fn refresh, spectral-radius floor)
 //   --chunk K        macro steps per device launch between host writes (64)
-//   --network        update the synapses after every macro step (the
-//                    reference's network_sim, main.cu:38, false there); the
-//                    synapse sets are created and coupled either way, as
+//   --network        update the synapses after every macro step and inject
+//                    no step current (the reference's network_sim, main.cu:38,
+//                    false there, which skips every xx_set_current,
+//                    main.cu:192-196, 220-223, 248-251, 279-283); the synapse
+//                    sets are created and coupled either way, as
 //                    main.cu:139-145 does, so without it they stay silent
 //   --tree           CN and BE populations solve each step exactly, per cell
 //                    on its tree (Hines elimination), instead of the
@@ -58,13 +60,15 @@ int solver_of(const char *s) {
 double dt_of(int solver) { return solver == CB_SOLVER_RKC ? H_MAX : 0.025; }
 
 // xx_initialize + xx_solve_initialize of a granule, Golgi or olive
-// population; with --tree a CN / BE population takes the exact per-cell solve
-int init_pop(bool tree, int type, int solver, int nx, int ny, const char *name, cb_pop **out) {
-  if (tree && solver != CB_SOLVER_RKC) {
+// population; with --tree a CN / BE population takes the exact per-cell
+// solve, with --network it has no step current
+int init_pop(bool tree, bool network, int type, int solver, int nx, int ny, const char *name, cb_pop **out) {
+  if ((tree && solver != CB_SOLVER_RKC) || network) {
     cb_pop_config cfg;
     const int rc = cb_config_default(&cfg, type, solver, nx, ny);
     if (rc != CB_OK) return rc;
-    cfg.engine = CB_ENGINE_TREE_FP64;
+    if (tree && solver != CB_SOLVER_RKC) cfg.engine = CB_ENGINE_TREE_FP64;
+    if (network) cfg.stim_comp = CB_STIM_NONE;
     return cb_pop_create(&cfg, out);
   }
   if (type == CB_GR) return cb_gr_initialize(nx, ny, name, out);
@@ -140,11 +144,11 @@ int main(int argc, char *argv[]) {
   std::srand(0);
   const int ngr = nx[CB_GR] * ny[CB_GR];
   cb_pop *grp = nullptr;
-  if (ngr > 0) check(init_pop(tree, CB_GR, solver[CB_GR], nx[CB_GR], ny[CB_GR], argv[1], &grp), "gr_initialize");
+  if (ngr > 0) check(init_pop(tree, network, CB_GR, solver[CB_GR], nx[CB_GR], ny[CB_GR], argv[1], &grp), "gr_initialize");
   std::vector<double> gr_v(ngr > 0 ? (size_t)ngr * 578 : 0), gr_prev(ngr, 100.0);
   const int ngo = nx[CB_GO] * ny[CB_GO];
   cb_pop *gop = nullptr;
-  if (ngo > 0) check(init_pop(tree, CB_GO, solver[CB_GO], nx[CB_GO], ny[CB_GO], argv[4], &gop), "go_initialize");
+  if (ngo > 0) check(init_pop(tree, network, CB_GO, solver[CB_GO], nx[CB_GO], ny[CB_GO], argv[4], &gop), "go_initialize");
   const int n = nx[CB_PKJ] * ny[CB_PKJ];
   cb_pop *pkj = nullptr;
   int comp = 0;
@@ -153,6 +157,7 @@ int main(int argc, char *argv[]) {
     check(cb_config_default(&cfg, CB_PKJ, solver[CB_PKJ], nx[CB_PKJ], ny[CB_PKJ]), "config");
     if (solver[CB_PKJ] == CB_SOLVER_RKC) cfg.engine = engine;
     if (tree && solver[CB_PKJ] != CB_SOLVER_RKC) cfg.engine = CB_ENGINE_TREE_FP64;
+    if (network) cfg.stim_comp = CB_STIM_NONE;
     if (stable && solver[CB_PKJ] == CB_SOLVER_RKC) {
       cfg.ion_interval_ms = std::ldexp(1.0, -8);
       cfg.refresh_fn = 1;
@@ -167,7 +172,7 @@ int main(int argc, char *argv[]) {
   // as io_initialize follows pkj_initialize in main.cu:125-137
   const int nio = nx[CB_IO] * ny[CB_IO];
   cb_pop *iop = nullptr;
-  if (nio > 0) check(init_pop(tree, CB_IO, solver[CB_IO], nx[CB_IO], ny[CB_IO], argv[10], &iop), "io_initialize");
+  if (nio > 0) check(init_pop(tree, network, CB_IO, solver[CB_IO], nx[CB_IO], ny[CB_IO], argv[10], &iop), "io_initialize");
   std::vector<double> io_v((size_t)nio * 3), io_prev(nio, 100.0);
   std::printf("\n");
 
@@ -220,7 +225,7 @@ int main(int argc, char *argv[]) {
     for (int s = 0; s < k && ngr > 0; s++) {
       const double ts_ = t + (s + 1) * H_MAX;
       for (double l_dt = 0.0; l_dt < H_MAX; l_dt += dt_of(solver[CB_GR])) {
-        check(cb_gr_set_current(grp, ts_ - H_MAX + l_dt), "gr_set_current");
+        if (!network) check(cb_gr_set_current(grp, ts_ - H_MAX + l_dt), "gr_set_current");
         check(cb_gr_solve_update_v(grp), "gr_solve_update_v");
       }
       check(cb_pop_read(grp, CB_F_VNEW, gr_v.data()), "gr read");
@@ -290,7 +295,7 @@ int main(int argc, char *argv[]) {
     for (int s = 0; s < k && nio > 0; s++) {
       const double ts_ = t + (s + 1) * H_MAX;
       for (double l_dt = 0.0; l_dt < H_MAX; l_dt += dt_of(solver[CB_IO])) {
-        check(cb_io_set_current(iop, ts_ - H_MAX + l_dt), "io_set_current");
+        if (!network) check(cb_io_set_current(iop, ts_ - H_MAX + l_dt), "io_set_current");
         check(cb_io_solve_update_v(iop), "io_solve_update_v");
       }
       check(cb_pop_read(iop, CB_F_VNEW, io_v.data()), "io read");

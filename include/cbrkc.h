@@ -25,7 +25,10 @@ extern "C" {
 #define CB_E_ARG (-1)      /* bad argument or configuration */
 #define CB_E_IO (-2)       /* missing or malformed input file */
 #define CB_E_HIP (-3)      /* HIP runtime failure */
-#define CB_E_SOLVER (-4)   /* RKC controller failure (idid 3, 4 or 6) */
+#define CB_E_SOLVER (-4)   /* RKC failure: a zero error weight (the reference's idid 3) or a NaN /
+                              infinite state (the reference's main stops on a NaN voltage); idid 4
+                              (h below hmin) and 6 (power method not converged) fall through to an
+                              accepted step, as in the reference */
 #define CB_E_UNSUP (-5)    /* combination not provided */
 
 /* cell types, param.h:15 */
@@ -60,6 +63,9 @@ enum {
   CB_F_GATE0 = 100 /* + gate index in the cell type's gate order */
 };
 
+/* cb_pop_config.stim_comp: no injected current at all */
+#define CB_STIM_NONE (-2)
+
 typedef struct cb_pop cb_pop;
 
 typedef struct cb_pop_config {
@@ -69,10 +75,14 @@ typedef struct cb_pop_config {
                                solve, or CB_ENGINE_TREE_FP64) */
   int nx, ny;               /* cell grid; nx * ny cells */
   const char *morph_path;   /* morphology file; NULL: "./Pkj.txt" */
-  const char *gj_path;      /* gap-junction pair list; NULL: "./Pkj_GJ.csv" (grid with nx > 1 only) */
+  const char *gj_path;      /* gap-junction pair list of a Purkinje grid (nx * ny > 1): NULL reads
+                               "./Pkj_GJ.csv", and a missing file is an error (pkj_solve.cu:86-87);
+                               "" builds the grid without gap junctions */
   double atol, rtol;        /* <= 0: the reference's value for the cell type */
   const double *v0;         /* per-cell initial V [mV]; NULL: -68 + 2 (U - 0.5) from libc rand() */
-  int stim_comp;            /* local compartment of the step current; < 0: reference protocol */
+  int stim_comp;            /* local compartment of the step current; -1: reference protocol;
+                               CB_STIM_NONE: no injected current, as the reference runs with
+                               network_sim on (main.cu:192-196, 220-223, 248-251, 279-283) */
   double stim_t0, stim_t1;  /* [ms] window of the step current */
   const double *stim_amp;   /* per-cell amplitude [uA]; NULL: reference amplitude */
   int poisson_on;           /* dendritic Poisson conductance drive on every compartment */
@@ -87,6 +97,8 @@ typedef struct cb_pop_config {
   double step_ms;           /* CN and BE: the step [ms]; <= 0: the reference's 0.025 (param.h:10-11);
                                otherwise 0.125 / step_ms must be a whole number from 1 to 4096 (no
                                counterpart in the reference, whose step is a compile-time constant) */
+  double hmin_ms;           /* RKC: the smallest step [ms]; <= 0: the reference's H_MIN of the cell type
+                               (2^-10, granule 2^-12, solve_rkc.cuh:24-27); at most H_MAX = 0.125 */
 } cb_pop_config;
 
 /* Fills a config with the reference defaults for one cell type and solver. */

@@ -54,3 +54,61 @@ def test_oracle_philox_known_answers():
         [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
     assert [int(x) for x in orc.philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344],
                                        [0xa4093822, 0x299f31d0])] == [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def _create(tmp_path, monkeypatch, **kw):
+    """cb_pop_create with cb_config_default plus the given fields, in tmp_path
+    as working directory; returns (status, message)."""
+    L = cbrkc.lib()
+    cfg = cbrkc.Config()
+    assert L.cb_config_default(ctypes.byref(cfg), kw.pop("cell_type", cbrkc.PKJ), kw.pop("solver", cbrkc.RKC),
+                               kw.pop("nx", 1), kw.pop("ny", 1)) == 0
+    keep = []
+    for k, v in kw.items():
+        if isinstance(v, str):
+            v = v.encode()
+            keep.append(v)
+        setattr(cfg, k, v)
+    monkeypatch.chdir(tmp_path)
+    h = ctypes.c_void_p()
+    rc = L.cb_pop_create(ctypes.byref(cfg), ctypes.byref(h))
+    assert not h.value
+    return rc, L.cb_last_error().decode()
+
+
+CB_E_ARG, CB_E_IO, CB_E_UNSUP = -1, -2, -5
+
+
+def test_failure_paths_before_the_device(tmp_path, monkeypatch, ref_data):
+    """Argument and input-file failures come back as status codes with a
+    message, without a GPU: the reference prints and exits on each of these
+    (pkj.cu:28 missing morphology, pkj_solve.cu:86-87 missing Pkj_GJ.csv).
+    The reference's idid = 3 (an error weight of zero, solve_rkc.cu:50-53)
+    needs atol = 0; a non-positive atol selects the cell type's TOL here, so
+    every weight is positive by construction."""
+    import shutil
+    shutil.copy(os.path.join(ref_data, "Pkj.txt"), tmp_path / "Pkj.txt")
+    # a Purkinje grid needs ./Pkj_GJ.csv, as the reference does
+    rc, msg = _create(tmp_path, monkeypatch, nx=2, ny=2)
+    assert rc == CB_E_IO and "Pkj_GJ.csv" in msg, (rc, msg)
+    rc, msg = _create(tmp_path, monkeypatch, morph_path="/nonexistent/Pkj.txt")
+    assert rc == CB_E_IO, (rc, msg)
+    rc, msg = _create(tmp_path, monkeypatch, cell_type=cbrkc.IO, morph_path="/nonexistent/IO.txt")
+    assert rc == CB_E_IO, (rc, msg)
+    # an explicit pair list: the per-cell engines refuse coupled cells
+    shutil.copy(os.path.join(ref_data, "Pkj_GJ.csv"), tmp_path / "Pkj_GJ.csv")
+    rc, msg = _create(tmp_path, monkeypatch, nx=2, ny=2, engine=cbrkc.ENGINE_CELL_MIXED)
+    assert rc == CB_E_UNSUP and "gap junctions" in msg, (rc, msg)
+    rc, msg = _create(tmp_path, monkeypatch, solver=7)
+    assert rc == CB_E_ARG, (rc, msg)
+    rc, msg = _create(tmp_path, monkeypatch, cell_type=9)
+    assert rc == CB_E_ARG, (rc, msg)
+    rc, msg = _create(tmp_path, monkeypatch, solver=cbrkc.CN, poisson_on=1)
+    assert rc == CB_E_UNSUP, (rc, msg)
+    rc, msg = _create(tmp_path, monkeypatch, step_ms=0.0625)
+    assert rc == CB_E_ARG and "step_ms" in msg, (rc, msg)
+    rc, msg = _create(tmp_path, monkeypatch, stim_comp=1600, stim_t0=0.0, stim_t1=1.0)
+    assert rc == CB_E_ARG and "out of range" in msg, (rc, msg)
+    # reference-named entry point: an unknown solver name (main.cu:60-102 prefix match)
+    h = ctypes.c_void_p()
+    assert cbrkc.lib().cb_pkj_initialize(1, 1, b"XYZ", ctypes.byref(h)) == CB_E_ARG

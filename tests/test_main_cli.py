@@ -263,6 +263,7 @@ def test_network_rows_against_oracle(built, tmp_path, ref_data):
     for o, s in ((ogr, mf), (ogr, gg), (ogo, rg), (opk, rp)):
         assert LS().or_pop_attach_syn(o.h, s.h) == 0
     for o in (ogr, ogo, opk):
+        o.set_stim(0, 0.0, 0.0, np.zeros(o.n))    # network_sim: no xx_set_current, i_ext stays 0
         assert LC().or_cn_init(o.h) == 0
     lib.or_net_update.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_double]
     ref_gr, ref_go, ref_pk = [], [], []
