@@ -31,7 +31,7 @@ H_MAX = 0.125
 # every symbol include/cbrkc.h declares
 EXPORTS = [
     "cb_config_default", "cb_pop_create", "cb_pop_destroy", "cb_pop_set_current", "cb_pop_advance",
-    "cb_pop_run", "cb_pop_read", "cb_pop_size", "cb_pop_stats", "cb_pop_controller",
+    "cb_pop_run", "cb_pop_run_noalloc", "cb_pop_read", "cb_pop_size", "cb_pop_stats", "cb_pop_controller",
     "cb_synth_pkj_morph", "cb_philox4x32_10", "cb_last_error", "cb_pkj_initialize",
     "cb_pkj_solve_update_v", "cb_pkj_set_current", "cb_pkj_output_file", "cb_pkj_finalize",
     "cb_pop_kernel_time", "cb_pop_autotune", "cb_pkj_cable_spectral_radius",
@@ -110,6 +110,7 @@ def lib():
         L.cb_pop_set_current.argtypes = [vp, ctypes.c_double]
         L.cb_pop_advance.argtypes = [vp]
         L.cb_pop_run.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp]
+        L.cb_pop_run_noalloc.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp]
         L.cb_pop_read.argtypes = [vp, ctypes.c_int, dp]
         L.cb_pop_size.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.cb_pop_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
@@ -276,6 +277,12 @@ class Population:
     def run(self, nsteps, t0=0.0, rec_comp=1599, record=True):
         out = np.zeros((nsteps, self.n)) if record else None
         _check(lib().cb_pop_run(self.h, nsteps, t0, rec_comp, _dp(out) if record else None))
+        return out
+
+    def run_noalloc(self, nsteps, t0=0.0, rec_comp=1599, record=True):
+        """run() through cb_pop_run_noalloc (no device allocation)"""
+        out = np.zeros((nsteps, self.n)) if record else None
+        _check(lib().cb_pop_run_noalloc(self.h, nsteps, t0, rec_comp, _dp(out) if record else None))
         return out
 
     def read(self, field=F_VNEW):

@@ -554,6 +554,17 @@ int cb_pop_run(cb_pop *p, int nsteps, double t0, int rec_comp, double *rec_out) 
   });
 }
 
+int cb_pop_run_noalloc(cb_pop *p, int nsteps, double t0, int rec_comp, double *rec_out) {
+  return guarded([&]() -> int {
+    if (!p) return fail(CB_E_ARG, "null population");
+    if (nsteps < 0) return fail(CB_E_ARG, "negative step count");
+    if (rec_out && (rec_comp < 0 || rec_comp >= p->comp)) return fail(CB_E_ARG, "record compartment out of range");
+    CB_HIP(hipSetDevice(p->device));
+    p->eng->run_noalloc(nsteps, t0, rec_comp, rec_out);
+    return CB_OK;
+  });
+}
+
 int cb_pop_read(cb_pop *p, int field, double *dst) {
   return guarded([&]() -> int {
     if (!p || !dst) return fail(CB_E_ARG, "null argument");

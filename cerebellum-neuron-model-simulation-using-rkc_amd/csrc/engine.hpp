@@ -94,6 +94,16 @@ class Engine {
   // many macro steps without host round trips; soma V and spikes are
   // buffered on the device.  Default: loop over advance().
   virtual void run(int nsteps, double t0, int rec_comp, double *rec_out) ;
+  // run() without any device allocation: the per-call device memory (work
+  // counter, recording staging) lives in the library's static per-device
+  // scratch array.  Only the per-cell engines provide it.
+  virtual void run_noalloc(int nsteps, double t0, int rec_comp, double *rec_out) {
+    (void)nsteps;
+    (void)t0;
+    (void)rec_comp;
+    (void)rec_out;
+    throw Error("the allocation-free run is provided by the per-cell RKC engines", -5);
+  }
   // couples the population's incoming synapses (in.on() false: none); the
   // device arrays stay owned by the caller and must outlive the coupling
   virtual void set_synapses(const SynIn &in) {

@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -828,6 +829,39 @@ k_cell_init(CellArgs<T> a) {
   cell_body<T, NT, K, true, false>(a, 0, 0.0, 0);
 }
 
+// Device scratch of cb_pop_run_noalloc: bytes [0, 256) hold the work
+// counter of the launch, the rest the recording of rec_comp before its copy
+// to the caller's buffer.  A file-scope array, so a call allocates nothing;
+// every device the library runs on holds one copy until the process ends,
+// shared by all populations and calls on that device (overlapping calls on
+// one device race on it).  Host code uses the device address that
+// hipGetSymbolAddress returns for the current device, never the name.
+constexpr size_t SCRATCH_BYTES = 8u << 20;
+constexpr size_t SCRATCH_REC = 256;   // offset of the recording
+__device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch[MAX_DEVICES] = {};
+
+// the scratch array's address on the current device, looked up once per device
+unsigned char *scratch_on_current_device() {
+  int dev = 0;
+  CB_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) throw Error("device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *ptr = nullptr;
+    CB_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = (unsigned char *)ptr;
+  }
+  return g_scratch[dev];
+}
+
+// clears the work counter with a kernel rather than a memset node, so a
+// caller may capture the launch sequence in a graph
+__global__ void k_zero_int(int *p) { *p = 0; }
+
 template <typename T>
 T *dalloc(size_t n) {
   void *p = nullptr;
@@ -1010,6 +1044,7 @@ class CellPkj : public Engine {
     }
     variants_ = variants();
     choose_default();
+    scratch_ = scratch_on_current_device();
     launch(1, 0, 0.0);
     check_status();
   }
@@ -1079,6 +1114,44 @@ class CellPkj : public Engine {
       CB_HIP(hipMemcpyAsync(rec_out, rec, sizeof(double) * (size_t)nsteps * n_, hipMemcpyDeviceToHost, st_));
     }
     CB_HIP(hipStreamSynchronize(st_));
+    collect_time();
+    check_status();
+    t_next_ = t0 + nsteps * H_MAX;
+    if (nsteps >= 4) reorder();
+  }
+
+  // run() with the work counter and the recording in the device's scratch
+  // array: no hipMalloc / hipFree on this path (the population's own arrays
+  // were allocated at creation).  nsteps x ncell recorded doubles must fit
+  // the scratch after its first 256 bytes.
+  void run_noalloc(int nsteps, double t0, int rec_comp, double *rec_out) override {
+    if (nsteps <= 0) return;
+    const size_t need = rec_out ? sizeof(double) * (size_t)nsteps * n_ : 0;
+    if (need > SCRATCH_BYTES - SCRATCH_REC)
+      throw Error("allocation-free run: " + std::to_string(nsteps) + " steps x " + std::to_string(n_) +
+                  " cells of recording exceed the " + std::to_string((SCRATCH_BYTES - SCRATCH_REC) >> 20) +
+                  " MiB scratch");
+    int *own_queue = a_.queue;
+    a_.queue = (int *)scratch_;
+    a_.rec = rec_out ? (double *)(scratch_ + SCRATCH_REC) : nullptr;
+    if (rec_out) a_.rec_orig = rec_comp;
+    try {
+      if (a_.net.on()) {
+        if (nsteps != 1) throw Error("with synapses the per-cell engines advance one macro step per call");
+        k_cell_net<T><<<(unsigned)(((size_t)n_ * C_ + 255) / 256), 256, 0, st_>>>(a_);
+      }
+      launch(0, nsteps, t0);
+      if (rec_out)
+        CB_HIP(hipMemcpyAsync(rec_out, a_.rec, need, hipMemcpyDeviceToHost, st_));
+      CB_HIP(hipStreamSynchronize(st_));
+    } catch (...) {
+      a_.queue = own_queue;
+      a_.rec = nullptr;
+      throw;
+    }
+    a_.queue = own_queue;
+    a_.rec = nullptr;
+    macro_ += nsteps;
     collect_time();
     check_status();
     t_next_ = t0 + nsteps * H_MAX;
@@ -1283,7 +1356,7 @@ class CellPkj : public Engine {
   void launch(int mode, int nsteps, double t0) {
     const Variant &v = variants_[cur_];
     if ((long)v.nt * v.k < C_) throw Error("per-cell mapping too small for the cell");
-    CB_HIP(hipMemsetAsync(a_.queue, 0, sizeof(int), st_));
+    k_zero_int<<<1, 1, 0, st_>>>(a_.queue);
     int dev = 0;
     CB_HIP(hipGetDevice(&dev));
     int ncu = 0;
@@ -1348,6 +1421,7 @@ class CellPkj : public Engine {
   size_t rec_cap_ = 0;
   std::vector<int> perm_;
   int *order_ = nullptr;
+  unsigned char *scratch_ = nullptr;   // this device's scratch array
   std::vector<double> cost_seen_;
   std::vector<Variant> variants_;
   int cur_ = 0;

@@ -133,6 +133,18 @@ int cb_pop_advance(cb_pop *p);
  * step where the engine allows. */
 int cb_pop_run(cb_pop *p, int nsteps, double t0, int rec_comp, double *rec_out);
 
+/* cb_pop_run without any device allocation or free, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The per-call device
+ * memory (the work counter and the recording before its copy to rec_out)
+ * lives in a file-scope device array of 8 MiB per device, so nsteps x ncell
+ * must stay within 1048544 recorded doubles (CB_E_ARG beyond).  All calls on
+ * one device share that array: calls that overlap on one device (several
+ * host threads, or populations on different streams used concurrently)
+ * race.  Per-cell RKC engines only (CB_E_UNSUP otherwise); results equal
+ * cb_pop_run's. */
+int cb_pop_run_noalloc(cb_pop *p, int nsteps, double t0, int rec_comp, double *rec_out);
+
 /* Copies nc values of a state field to host memory. */
 int cb_pop_read(cb_pop *p, int field, double *dst);
 
