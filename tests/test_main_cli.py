@@ -317,17 +317,20 @@ def test_network_with_the_per_cell_purkinje_engine(built, tmp_path, ref_data):
     engine, one macro step per call) receives the same granule -> Purkinje
     input, so its rows agree within the per-cell engines' synaptic bar
     (0.03 mV); the granule and Golgi rows (CN, the same engine in both runs)
-    are identical.  70 ms, so that the granule cells fire under the
-    reference's 50 ms step current; a run without --network shows the input
-    reaches the Purkinje soma."""
+    are identical.  With --network no step current is injected
+    (network_sim), so the granule cells fire on mossy-fibre input alone, from
+    about 15 ms on; 45 ms, because the reference's Purkinje RKC scheme itself
+    turns non-finite on this network at 62.4 ms (the oracle run of the same
+    network, without the synthetic-tree safeguards); a run without --network
+    shows the input reaches the Purkinje soma."""
     outs = {}
     for eng, net in (("global", True), ("cell", True), ("global", False)):
         d = tmp_path / f"{eng}_{net}"
         d.mkdir()
         for f in ("Gr.txt", "Go_131.txt", "Pkj.txt", "Pkj_GJ.csv", "Pkj_location_info.csv"):
             shutil.copy(os.path.join(ref_data, f), d / f)
-        print(f"main: GR 4x4 CN, GO 1x1 CN, PKJ 1x1 RKC --engine {eng}, network {net}, 70 ms", flush=True)
-        r = run_main(["CN", "4", "4", "CN", "1", "1", "RKC", "1", "1", "CN", "0", "0", "--ts", "70", "--engine", eng]
+        print(f"main: GR 4x4 CN, GO 1x1 CN, PKJ 1x1 RKC --engine {eng}, network {net}, 45 ms", flush=True)
+        r = run_main(["CN", "4", "4", "CN", "1", "1", "RKC", "1", "1", "CN", "0", "0", "--ts", "45", "--engine", eng]
                      + (["--network"] if net else []), d, timeout=600)
         assert r.returncode == 0, r.stdout[-2000:] + r.stderr
         rows = lambda name: np.array([[float(x) for x in l.rstrip("\n").rstrip(",").split(",")]
@@ -339,4 +342,4 @@ def test_network_with_the_per_cell_purkinje_engine(built, tmp_path, ref_data):
     d = np.max(np.abs(pg[:, 1] - pc[:, 1]))
     effect = np.max(np.abs(pg[:, 1] - outs[("global", False)][2][:, 1]))
     print(f"Purkinje rows: max |cell - global| = {d:.3e} mV; synaptic effect {effect:.3e} mV", flush=True)
-    assert pg.shape == pc.shape == (560, 2) and d <= 0.03 and effect > 1e-3
+    assert pg.shape == pc.shape == (360, 2) and d <= 0.03 and effect > 1e-3
