@@ -203,10 +203,13 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--morph", default=None, help="reuse an existing morphology file")
-    ap.add_argument("--workload", choices=("precision", "scale", "rows"), default="precision",
+    ap.add_argument("--workload", choices=("precision", "scale", "rows", "batching"), default="precision",
                     help="precision: the headline 1024-cell run; scale: the 8192-cell scale-stress comparison; "
                          "rows: every cell type x solver on the reference's morphologies, with the oracle beside it")
     ap.add_argument("--rows", default="PKJ,GR,GO,IO", help="cell types of the rows workload")
+    ap.add_argument("--mix-scale", type=int, default=1, help="batching: divide every population by this")
+    ap.add_argument("--mix-intervals", type=int, default=1000, help="batching: timed 0.1 ms intervals (100 ms)")
+    ap.add_argument("--mix-trial", type=int, default=8, help="batching: autotuner trial intervals")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -232,6 +235,8 @@ def main():
         return scale_stress(cbrkc, compare, args)
     if args.workload == "rows":
         return rows(cbrkc, args)
+    if args.workload == "batching":
+        return batching(cbrkc, args)
     n = args.cells
     morph = args.morph or f"/tmp/cbrkc_bench_pkj{n}_s{args.seed}.txt"
     if not os.path.exists(morph):
@@ -420,6 +425,154 @@ def scale_stress(cbrkc, compare, args):
 
 
 REF_DATA = os.path.join(ROOT, "tests", "golden", "ref_data")
+
+
+def uniform(seed, stream, n, lo, hi):
+    """n draws of U(lo, hi) from Philox4x32-10 (ctr = (i, stream, 0, 0), key = (seed, 0x4d495820))"""
+    import cbrkc
+    out = np.zeros(n)
+    for i in range(n):
+        r = cbrkc.philox4x32_10([i, stream, 0, 0], [seed & 0xffffffff, 0x4D495820])
+        out[i] = lo + (hi - lo) * (((r[0] >> 5) * 67108864.0 + (r[1] >> 6)) / 9007199254740992.0)
+    return out
+
+
+def batching(cbrkc, args):
+    """The uneven mix in one persistent launch per 0.1 ms exchange interval:
+    16384 granule cells (4-compartment synthetic chains), 2048 Golgi cells
+    (synthetic 131-compartment trees), 1024 Purkinje cells (1600-compartment
+    synthetic trees, mixed precision) and 2048 olive cells (3 compartments,
+    each joined to 6 random partners, g ~ U(0.5, 1.5) nS), constant per-cell
+    currents from per-type uniform ranges, 100 ms.  The lanes per small cell
+    are chosen by the autotuner; chunks of 64 intervals replay a captured
+    graph."""
+    scale = args.mix_scale
+    n = {cbrkc.GR: 16384 // scale, cbrkc.GO: 2048 // scale, cbrkc.PKJ: 1024 // scale, cbrkc.IO: 2048 // scale}
+    comp = {cbrkc.GR: 4, cbrkc.GO: 131, cbrkc.PKJ: 1600, cbrkc.IO: 3}
+    interval = 0.1
+    nint = args.mix_intervals
+    seed = args.seed
+    d = "/tmp/cbrkc_mix"
+    os.makedirs(d, exist_ok=True)
+    grf, gof, pkf, gjf = (os.path.join(d, f) for f in (f"gr{n[0]}.txt", f"go{n[1]}.txt", f"pkj{n[2]}.txt",
+                                                        f"io{n[3]}_gj.csv"))
+    cbrkc.synth_gr_chains(grf, os.path.join(REF_DATA, "Gr.txt"), seed, n[cbrkc.GR])
+    cbrkc.synth_go_trees(gof, os.path.join(REF_DATA, "Go_131.txt"), seed, n[cbrkc.GO])
+    if not os.path.exists(pkf):
+        cbrkc.synth_pkj_morph(pkf, seed=seed, ncell=n[cbrkc.PKJ])
+    cbrkc.synth_io_gj(gjf, seed, n[cbrkc.IO], k=6, g_min_nS=0.5, g_max_nS=1.5)
+    pv0, pamp = philox_params(seed, n[cbrkc.PKJ])
+    pops = {
+        cbrkc.GR: dict(n=n[cbrkc.GR], morph=grf, comp=4, v0=uniform(seed, 10, n[cbrkc.GR], -72.0, -68.0),
+                       amp=uniform(seed, 11, n[cbrkc.GR], 0.008e-3, 0.016e-3)),
+        cbrkc.GO: dict(n=n[cbrkc.GO], morph=gof, v0=uniform(seed, 12, n[cbrkc.GO], -62.0, -58.0),
+                       amp=uniform(seed, 13, n[cbrkc.GO], 0.1e-3, 0.3e-3)),
+        cbrkc.PKJ: dict(n=n[cbrkc.PKJ], morph=pkf, v0=pv0, amp=pamp),
+        cbrkc.IO: dict(n=n[cbrkc.IO], morph=os.path.join(REF_DATA, "IO.txt"),
+                       v0=uniform(seed, 14, n[cbrkc.IO], -62.0, -58.0), amp=uniform(seed, 15, n[cbrkc.IO], 3.0, 6.0)),
+    }
+    print(f"[batching] building the mix {n}", file=sys.stderr, flush=True)
+    m = cbrkc.Mix(pops, io_gj=gjf, interval_ms=interval, pkj_tol=args.tol)
+    lanes = m.autotune(args.mix_trial)
+    print(f"[batching] lanes per cell (GR, GO, PKJ, IO) {lanes}", file=sys.stderr, flush=True)
+    m.run(args.warmup, record=False)
+    s0 = {t: m.stats(t) for t in range(4)}
+    m.kernel_time(reset=True)
+    t = time.perf_counter()
+    rec, spk = m.run(nint)
+    wall = time.perf_counter() - t
+    kms, nl = m.kernel_time(reset=True)
+    s1 = {t: m.stats(t) for t in range(4)}
+    total_comp = sum(n[t] * comp[t] for t in range(4))
+    sim_ms = nint * interval
+    value = total_comp * sim_ms / 0.125 / wall
+    names = {cbrkc.GR: "granule", cbrkc.GO: "Golgi", cbrkc.PKJ: "Purkinje", cbrkc.IO: "olive"}
+    per = {}
+    for tt in range(4):
+        w = work(s0[tt], s1[tt])
+        sp = int(m.spikes(spk, tt).sum())
+        per[names[tt]] = {"cells": n[tt], "compartments": n[tt] * comp[tt], "lanes_per_cell": lanes[tt],
+                          "spikes": sp, "work": w}
+    out = {"metric": "compartment-steps/s, batched uneven mix (GR/GO/PKJ/IO), one launch per 0.1 ms interval",
+           "value": value, "unit": "compartment-steps/s (one compartment over 0.125 ms simulated)", "n_gpus": 1,
+           "higher_is_better": True, "data": "synthetic", "dtype": "f32 (Purkinje) / f64 (granule, Golgi, olive)",
+           "config": {"workload": f"{n[0]} granule (4-compartment chains), {n[1]} Golgi (131), {n[2]} Purkinje "
+                                  f"(1600), {n[3]} olive (3, 6 random junctions per cell, U(0.5,1.5) nS); constant "
+                                  f"currents; 0.1 ms exchange intervals; {sim_ms:g} ms",
+                      "compartments": total_comp, "intervals": nint, "interval_ms": interval},
+           "wall_s": wall, "sim_ms_per_s": sim_ms / wall,
+           "kernel_ms": kms, "launch_chunks": nl, "populations": per,
+           "all_finite": bool(np.all(np.isfinite(rec)))}
+    if not args.no_cpu:
+        out["cpu_baseline"] = batching_cpu(cbrkc, pops, gjf, interval, args)
+    print(json.dumps(out), flush=True)
+
+
+def batching_cpu(cbrkc, pops, gjf, interval, args):
+    """The oracle on a bounded sample of the mix, each cell the reference's
+    RKC on a population of one (as the engine advances it), for up to
+    --cpu-seconds per type: compartment-steps/s of the sample on one host
+    thread per cell, all cells of the sample in parallel host threads."""
+    import ctypes
+    from concurrent.futures import ThreadPoolExecutor
+    import orc
+    from test_gpu_mix import _L, _blocks
+    lib = _L()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    d = "/tmp/cbrkc_mix_cpu"
+    os.makedirs(d, exist_ok=True)
+    sample = {cbrkc.GR: 16, cbrkc.GO: 4, cbrkc.PKJ: 2, cbrkc.IO: 16}
+    comp = {cbrkc.GR: 4, cbrkc.GO: 131, cbrkc.PKJ: 1600, cbrkc.IO: 3}
+    tot_cs, tot_s = 0.0, 0.0
+    per = {}
+    for t, k in sample.items():
+        spec = pops[t]
+        if t == cbrkc.GR:
+            files = _blocks(spec["morph"], 4, spec["n"], d, "gr")[:k]
+            mk = lambda c: orc.Pop(lib.or_gr_create_comp(files[c].encode(), 4, 1, 1, orc._dp(spec["v0"][c:c + 1]),
+                                                         1e-9, 1e-9, 0), 1, 4)
+        elif t == cbrkc.GO:
+            files = _blocks(spec["morph"], 131, spec["n"], d, "go")[:k]
+            mk = lambda c: orc.Pop(lib.or_go_create(files[c].encode(), 1, 1, orc._dp(spec["v0"][c:c + 1]), 1e-6,
+                                                    1e-6, 0), 1, 131)
+        elif t == cbrkc.PKJ:
+            files = orc.split_morph(spec["morph"], spec["n"], 1600, d)[:k]
+            def mk(c, files=files, spec=spec):
+                o = orc.Pop.pkj(files[c], 1, 1, v0=spec["v0"][c:c + 1], atol=args.tol, rtol=args.tol,
+                                libc_rand=False)
+                o.stable_split()
+                return o
+        else:
+            mk = lambda c: orc.Pop(lib.or_io_create(spec["morph"].encode(), 1, 1, orc._dp(spec["v0"][c:c + 1]),
+                                                    1e-6, 1e-6, 0), 1, 3)
+
+        stim_comp = {cbrkc.GR: 1, cbrkc.GO: 30, cbrkc.PKJ: 1599, cbrkc.IO: 0}[t]
+
+        def one(c, mk=mk, t=t, stim_comp=stim_comp, spec=spec):
+            o = mk(c)
+            o.set_stim(stim_comp, 0.0, 1e300, np.array([spec["amp"][c]]))
+            o.set_current(0.0)
+            o.threads(1)
+            steps = 0
+            t0 = time.perf_counter()
+            while time.perf_counter() - t0 < args.cpu_seconds / 4:
+                o.advance()
+                steps += 1
+            return steps, time.perf_counter() - t0
+
+        with ThreadPoolExecutor(max_workers=min(k, threads)) as ex:
+            res = list(ex.map(one, range(k)))
+        cs = sum(st * comp[t] for st, _ in res)
+        sec = max(w for _, w in res)
+        per[t] = {"cells": k, "macro_steps": [st for st, _ in res], "wall_s": sec}
+        tot_cs += cs
+        tot_s += sec
+    return {"value": tot_cs / tot_s, "unit": "compartment-steps/s (0.125 ms output intervals)", "cores": threads,
+            "kind": "port", "sample": "oracle, each cell its own population (the reference's RKC per cell): "
+                                      "16 granule chains, 4 Golgi trees, 2 Purkinje cells, 16 olive cells "
+                                      "(uncoupled), one host thread per cell, the four types one after another, "
+                                      f"about {args.cpu_seconds / 4:g} s each",
+            "per_type": {str(k): v for k, v in per.items()}}
 
 
 def rows(cbrkc, args):

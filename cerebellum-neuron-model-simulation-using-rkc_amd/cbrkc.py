@@ -25,13 +25,14 @@ BE, CN, RKC = 0, 1, 2
 ENGINE_GLOBAL_FP64, ENGINE_CELL_FP64, ENGINE_CELL_MIXED, ENGINE_TREE_FP64 = 0, 1, 2, 3
 F_VNEW, F_V, F_CA, F_REV, F_B, F_IEXT, F_Y, F_YN, F_FN, F_CA2, F_CA_OLD, F_VOLD = range(12)
 F_GATE0 = 100
+REC_ALL = -1     # cb_pop_run: record every compartment
 STIM_NONE = -2   # cb_pop_config.stim_comp: no injected current (network_sim on)
 H_MAX = 0.125
 
 # every symbol include/cbrkc.h declares
 EXPORTS = [
     "cb_config_default", "cb_pop_create", "cb_pop_destroy", "cb_pop_set_current", "cb_pop_advance",
-    "cb_pop_run", "cb_pop_run_noalloc", "cb_pop_read", "cb_pop_size", "cb_pop_stats", "cb_pop_controller",
+    "cb_pop_run", "cb_pop_run_noalloc", "cb_pop_set_raster", "cb_pop_spikes", "cb_pop_read", "cb_pop_size", "cb_pop_stats", "cb_pop_controller",
     "cb_synth_pkj_morph", "cb_philox4x32_10", "cb_last_error", "cb_pkj_initialize",
     "cb_pkj_solve_update_v", "cb_pkj_set_current", "cb_pkj_output_file", "cb_pkj_finalize",
     "cb_pop_kernel_time", "cb_pop_autotune", "cb_pkj_cable_spectral_radius",
@@ -41,6 +42,9 @@ EXPORTS = [
     "cb_mfgr_create", "cb_gogr_create", "cb_grgo_create", "cb_grpkj_create", "cb_mlipkj_create",
     "cb_syn_finalize", "cb_gr_synapse_update", "cb_go_synapse_update", "cb_pkj_synapse_update",
     "cb_syn_size", "cb_syn_comp", "cb_syn_read", "cb_syn_build",
+    "cb_mix_config_default", "cb_mix_create", "cb_mix_run", "cb_mix_layout", "cb_mix_stats", "cb_mix_autotune",
+    "cb_mix_kernel_time", "cb_mix_use_graph", "cb_mix_destroy", "cb_synth_gr_chains", "cb_synth_go_trees",
+    "cb_synth_io_gj",
 ]
 
 # synapse kinds and fields (cb_syn_build, cb_syn_read)
@@ -91,6 +95,31 @@ class Config(ctypes.Structure):
     ]
 
 
+class MixPop(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int),
+        ("morph_path", ctypes.c_char_p),
+        ("comp", ctypes.c_int),
+        ("v0", ctypes.POINTER(ctypes.c_double)),
+        ("amp", ctypes.POINTER(ctypes.c_double)),
+        ("stim_comp", ctypes.c_int),
+        ("rec_comp", ctypes.c_int),
+        ("thr", ctypes.c_double),
+    ]
+
+
+class MixConfig(ctypes.Structure):
+    _fields_ = [
+        ("pop", MixPop * 4),
+        ("io_gj_path", ctypes.c_char_p),
+        ("interval_ms", ctypes.c_double),
+        ("pkj_atol", ctypes.c_double),
+        ("pkj_rtol", ctypes.c_double),
+        ("pkj_stable", ctypes.c_int),
+        ("device", ctypes.c_int),
+    ]
+
+
 _lib = None
 
 
@@ -111,6 +140,23 @@ def lib():
         L.cb_pop_advance.argtypes = [vp]
         L.cb_pop_run.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp]
         L.cb_pop_run_noalloc.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp]
+        L.cb_pop_set_raster.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.cb_pop_spikes.argtypes = [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.cb_mix_config_default.argtypes = [ctypes.POINTER(MixConfig)]
+        L.cb_mix_create.argtypes = [ctypes.POINTER(MixConfig), ctypes.POINTER(vp)]
+        L.cb_mix_run.argtypes = [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_uint32)]
+        L.cb_mix_layout.argtypes = [vp, ip, ip, ip, ip]
+        L.cb_mix_stats.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+        L.cb_mix_autotune.argtypes = [vp, ctypes.c_int, ip]
+        L.cb_mix_kernel_time.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int]
+        L.cb_mix_use_graph.argtypes = [vp, ctypes.c_int]
+        L.cb_mix_destroy.argtypes = [vp]
+        L.cb_synth_gr_chains.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int, ip,
+                                         ctypes.c_int]
+        L.cb_synth_go_trees.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int]
+        L.cb_synth_io_gj.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_double]
         L.cb_pop_read.argtypes = [vp, ctypes.c_int, dp]
         L.cb_pop_size.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.cb_pop_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
@@ -188,6 +234,127 @@ def synth_pkj_morph(path, seed, ncell, soma_r_um=10.0, lmin_um=5.0, lmax_um=20.0
         dscale = np.ascontiguousarray(dscale, dtype=np.float64)
         ds = _dp(dscale)
     _check(lib().cb_synth_pkj_morph(path.encode(), seed, ncell, soma_r_um, lmin_um, lmax_um, d0_um, d1_um, ds))
+
+
+def synth_gr_chains(path, template, seed, ncell, types=None):
+    """granule chains in the Gr.txt format (default: dendrite, soma, axon
+    initial segment, axon), geometry from the template's rows x U(0.8, 1.25)"""
+    t = None
+    if types is not None:
+        arr = (ctypes.c_int * len(types))(*types)
+        t = arr
+    _check(lib().cb_synth_gr_chains(path.encode(), template.encode(), seed, ncell, t, len(types) if types else 0))
+
+
+def synth_go_trees(path, template, seed, ncell):
+    """Golgi trees with the template's topology, geometry x U(0.8, 1.25)"""
+    _check(lib().cb_synth_go_trees(path.encode(), template.encode(), seed, ncell))
+
+
+def synth_io_gj(path, seed, ncell, k=6, g_min_nS=0.5, g_max_nS=1.5):
+    """olive gap junctions: every cell joined to k random partners"""
+    _check(lib().cb_synth_io_gj(path.encode(), seed, ncell, k, g_min_nS, g_max_nS))
+
+
+class Mix:
+    """The batched engine over a mix of cell types (cb_mix_*).  pops maps a
+    cell type (GR, GO, PKJ, IO) to a dict with n, morph, v0 and optionally
+    amp, comp, stim_comp, rec_comp, thr."""
+
+    def __init__(self, pops, io_gj=None, interval_ms=0.125, pkj_tol=1e-6, pkj_stable=True, device=0):
+        L = lib()
+        cfg = MixConfig()
+        _check(L.cb_mix_config_default(ctypes.byref(cfg)))
+        self._keep = []
+        for t, spec in pops.items():
+            p = cfg.pop[t]
+            p.n = spec["n"]
+            mp = spec["morph"].encode()
+            self._keep.append(mp)
+            p.morph_path = mp
+            v0 = np.ascontiguousarray(spec["v0"], dtype=np.float64)
+            self._keep.append(v0)
+            p.v0 = _dp(v0)
+            if spec.get("amp") is not None:
+                amp = np.ascontiguousarray(np.broadcast_to(spec["amp"], (p.n,)), dtype=np.float64)
+                self._keep.append(amp)
+                p.amp = _dp(amp)
+            for k in ("comp", "stim_comp", "rec_comp", "thr"):
+                if k in spec:
+                    setattr(p, k, spec[k])
+        if io_gj:
+            g = io_gj.encode()
+            self._keep.append(g)
+            cfg.io_gj_path = g
+        cfg.interval_ms = interval_ms
+        cfg.pkj_atol = cfg.pkj_rtol = pkj_tol
+        cfg.pkj_stable = 1 if pkj_stable else 0
+        cfg.device = device
+        h = ctypes.c_void_p()
+        _check(L.cb_mix_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        n4, w4, l4 = (ctypes.c_int * 4)(), (ctypes.c_int * 4)(), (ctypes.c_int * 4)()
+        w = ctypes.c_int()
+        _check(L.cb_mix_layout(h, n4, w4, ctypes.byref(w), l4))
+        self.ncell = list(n4)
+        self.word_off = list(w4)
+        self.words = w.value
+        self.offset = [int(sum(self.ncell[:t])) for t in range(4)]
+        self.total = int(sum(self.ncell))
+
+    def close(self):
+        if getattr(self, "h", None):
+            _check(lib().cb_mix_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, nint, record=True):
+        """(rec [nint, total cells], spikes [nint, words] uint32)"""
+        rec = np.zeros((nint, self.total)) if record else None
+        spk = np.zeros((nint, max(self.words, 1)), dtype=np.uint32) if record else None
+        _check(lib().cb_mix_run(self.h, nint, _dp(rec) if record else None,
+                                spk.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if record else None))
+        return rec, spk
+
+    def trace(self, rec, t):
+        """the recorded voltages of type t, [nint, cells]"""
+        return rec[:, self.offset[t]:self.offset[t] + self.ncell[t]]
+
+    def spikes(self, spk, t):
+        """[nint, cells] booleans: an upward threshold crossing in that interval"""
+        n, w0 = self.ncell[t], self.word_off[t]
+        c = np.arange(n)
+        return ((spk[:, w0 + c // 32] >> (c % 32).astype(np.uint32)) & 1).astype(bool)
+
+    def stats(self, t):
+        s = (ctypes.c_int64 * 6)()
+        _check(lib().cb_mix_stats(self.h, t, s))
+        return dict(steps=s[0], rejects=s[1], stages=s[2], rho_iters=s[3], ion_updates=s[4], macro_steps=s[5])
+
+    def lanes(self):
+        n4, w4, l4 = (ctypes.c_int * 4)(), (ctypes.c_int * 4)(), (ctypes.c_int * 4)()
+        w = ctypes.c_int()
+        _check(lib().cb_mix_layout(self.h, n4, w4, ctypes.byref(w), l4))
+        return list(l4)
+
+    def autotune(self, trial=8):
+        l4 = (ctypes.c_int * 4)()
+        _check(lib().cb_mix_autotune(self.h, trial, l4))
+        return list(l4)
+
+    def kernel_time(self, reset=False):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        _check(lib().cb_mix_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(n), 1 if reset else 0))
+        return ms.value, n.value
+
+    def use_graph(self, on):
+        _check(lib().cb_mix_use_graph(self.h, 1 if on else 0))
 
 
 def cable_spectral_radius(path, ncell):
@@ -275,9 +442,28 @@ class Population:
         _check(lib().cb_pop_advance(self.h))
 
     def run(self, nsteps, t0=0.0, rec_comp=1599, record=True):
-        out = np.zeros((nsteps, self.n)) if record else None
+        out = (np.zeros((nsteps, self.n)) if rec_comp >= 0 else np.zeros((nsteps, self.nc))) if record else None
         _check(lib().cb_pop_run(self.h, nsteps, t0, rec_comp, _dp(out) if record else None))
         return out
+
+    def set_raster(self, comp, thr):
+        """device raster on compartment comp (upward crossings of thr mV); comp < 0 stops it"""
+        _check(lib().cb_pop_set_raster(self.h, comp, thr))
+
+    def spikes(self):
+        """all spikes found so far, as (times [ms], cells), taken out of the list"""
+        ts, cs = [], []
+        while True:
+            t = np.zeros(4096)
+            c = np.zeros(4096, dtype=np.int32)
+            n = ctypes.c_int()
+            _check(lib().cb_pop_spikes(self.h, 4096, _dp(t), c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       ctypes.byref(n)))
+            ts.append(t[:n.value])
+            cs.append(c[:n.value])
+            if n.value < 4096:
+                break
+        return np.concatenate(ts), np.concatenate(cs)
 
     def run_noalloc(self, nsteps, t0=0.0, rec_comp=1599, record=True):
         """run() through cb_pop_run_noalloc (no device allocation)"""

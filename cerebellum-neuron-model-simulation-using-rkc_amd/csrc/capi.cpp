@@ -548,8 +548,26 @@ int cb_pop_run(cb_pop *p, int nsteps, double t0, int rec_comp, double *rec_out) 
   return guarded([&]() -> int {
     if (!p) return fail(CB_E_ARG, "null population");
     if (nsteps < 0) return fail(CB_E_ARG, "negative step count");
-    if (rec_out && (rec_comp < 0 || rec_comp >= p->comp)) return fail(CB_E_ARG, "record compartment out of range");
+    if (rec_out && (rec_comp < CB_REC_ALL || rec_comp >= p->comp))
+      return fail(CB_E_ARG, "record compartment out of range");
     p->eng->run(nsteps, t0, rec_comp, rec_out);
+    return CB_OK;
+  });
+}
+
+int cb_pop_set_raster(cb_pop *p, int comp, double thr) {
+  return guarded([&]() -> int {
+    if (!p) return fail(CB_E_ARG, "null population");
+    if (comp >= p->comp) return fail(CB_E_ARG, "raster compartment out of range");
+    p->eng->set_raster(comp, thr);
+    return CB_OK;
+  });
+}
+
+int cb_pop_spikes(cb_pop *p, int cap, double *t_out, int *cell_out, int *n_out) {
+  return guarded([&]() -> int {
+    if (!p || !n_out || (cap > 0 && (!t_out || !cell_out))) return fail(CB_E_ARG, "null argument");
+    *n_out = p->eng->take_spikes(cap, t_out, cell_out);
     return CB_OK;
   });
 }
@@ -947,6 +965,197 @@ int cb_syn_build(int kind, int nx_a, int ny_a, int nx_b, int ny_b, const double 
       std::memcpy(post, h.post.data(), sizeof(int) * h.n);
       std::memcpy(weight, h.weight.data(), sizeof(double) * h.n);
     }
+    return CB_OK;
+  });
+}
+
+struct cb_mix {
+  std::unique_ptr<Mix> m;
+  int device = 0;
+};
+
+// cb_mix_create validates its configuration and then refuses while this is
+// set (see there)
+constexpr bool mix_engine_switched_off = true;
+
+int cb_mix_config_default(cb_mix_config *cfg) {
+  if (!cfg) return fail(CB_E_ARG, "null config");
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->interval_ms = 0.125;
+  cfg->pkj_atol = 1e-6;
+  cfg->pkj_rtol = 1e-6;
+  cfg->pkj_stable = 1;
+  cfg->pop[CB_GR].comp = 4;
+  cfg->pop[CB_GR].stim_comp = 1;
+  cfg->pop[CB_GR].rec_comp = 3;
+  cfg->pop[CB_GR].thr = 10.0;
+  cfg->pop[CB_GO].stim_comp = 30;
+  cfg->pop[CB_GO].rec_comp = 30;
+  cfg->pop[CB_GO].thr = 10.0;
+  cfg->pop[CB_PKJ].stim_comp = 1599;
+  cfg->pop[CB_PKJ].rec_comp = 1599;
+  cfg->pop[CB_PKJ].thr = -10.0;
+  cfg->pop[CB_IO].stim_comp = 0;
+  cfg->pop[CB_IO].rec_comp = 2;
+  cfg->pop[CB_IO].thr = 10.0;
+  return CB_OK;
+}
+
+int cb_mix_create(const cb_mix_config *cfg, cb_mix **out) {
+  return guarded([&]() -> int {
+    if (!cfg || !out) return fail(CB_E_ARG, "null argument");
+    *out = nullptr;
+    MixSpec S;
+    for (int t = 0; t < 4; t++) {
+      const cb_mix_pop &q = cfg->pop[t];
+      if (q.n < 0) return fail(CB_E_ARG, "negative cell count");
+      if (q.n == 0) continue;
+      if (!q.morph_path || !q.v0) return fail(CB_E_ARG, "a population needs a morphology file and initial voltages");
+      MixPopSpec &p = S.pop[t];
+      p.n = q.n;
+      p.morph = q.morph_path;
+      p.comp = q.comp;
+      p.v0.assign(q.v0, q.v0 + q.n);
+      if (q.amp) p.amp.assign(q.amp, q.amp + q.n);
+      p.stim_comp = q.stim_comp;
+      p.rec_comp = q.rec_comp;
+      p.thr = q.thr;
+      std::FILE *f = std::fopen(q.morph_path, "r");
+      if (!f) return fail(CB_E_IO, std::string("no such file ") + q.morph_path);
+      std::fclose(f);
+    }
+    if (cfg->pop[CB_PKJ].n > 0 &&
+        (cfg->pop[CB_PKJ].stim_comp < 0 || cfg->pop[CB_PKJ].stim_comp >= pkj::COMP ||
+         cfg->pop[CB_PKJ].rec_comp < 0 || cfg->pop[CB_PKJ].rec_comp >= pkj::COMP))
+      return fail(CB_E_ARG, "Purkinje stimulated or recorded compartment out of range");
+    if (cfg->io_gj_path) S.io_gj = cfg->io_gj_path;
+    S.interval = cfg->interval_ms > 0 ? cfg->interval_ms : 0.125;
+    S.pkj_atol = cfg->pkj_atol > 0 ? cfg->pkj_atol : 1e-6;
+    S.pkj_rtol = cfg->pkj_rtol > 0 ? cfg->pkj_rtol : 1e-6;
+    S.pkj_stable = cfg->pkj_stable != 0;
+    // The batched engine is switched off: the first device run of its tests
+    // ended with a GPU memory fault whose cause has not been found (the
+    // small-cell body is in bounds in a host emulation of the same source).
+    // It stays off until that cause is shown and fixed.
+    if (mix_engine_switched_off)
+      return fail(CB_E_UNSUP, "the batched mix engine is switched off: its first GPU run faulted and the cause "
+                              "has not been found");
+    CB_HIP(hipSetDevice(cfg->device));
+    std::unique_ptr<cb_mix> m(new cb_mix);
+    m->device = cfg->device;
+    m->m.reset(make_mix(S));
+    *out = m.release();
+    return CB_OK;
+  });
+}
+
+int cb_mix_run(cb_mix *m, int nint, double *rec_out, uint32_t *spk_out) {
+  return guarded([&]() -> int {
+    if (!m) return fail(CB_E_ARG, "null mix");
+    if (nint < 0) return fail(CB_E_ARG, "negative interval count");
+    CB_HIP(hipSetDevice(m->device));
+    m->m->run(nint, rec_out, spk_out);
+    return CB_OK;
+  });
+}
+
+int cb_mix_layout(const cb_mix *m, int *ncell4, int *word_off4, int *words, int *lanes4) {
+  return guarded([&]() -> int {
+    if (!m) return fail(CB_E_ARG, "null mix");
+    for (int t = 0; t < 4; t++) {
+      if (ncell4) ncell4[t] = m->m->ncell(t);
+      if (word_off4) word_off4[t] = m->m->word_off(t);
+      if (lanes4) lanes4[t] = m->m->lanes(t);
+    }
+    if (words) *words = m->m->words();
+    return CB_OK;
+  });
+}
+
+int cb_mix_stats(const cb_mix *m, int type, int64_t *out6) {
+  return guarded([&]() -> int {
+    if (!m || !out6) return fail(CB_E_ARG, "null argument");
+    if (type < 0 || type > 3) return fail(CB_E_ARG, "unknown cell type");
+    const Stats s = m->m->stats(type);
+    out6[0] = s.steps;
+    out6[1] = s.rejects;
+    out6[2] = s.stages;
+    out6[3] = s.rho_iters;
+    out6[4] = s.ion_updates;
+    out6[5] = s.macro_steps;
+    return CB_OK;
+  });
+}
+
+int cb_mix_autotune(cb_mix *m, int trial_intervals, int *lanes4) {
+  return guarded([&]() -> int {
+    if (!m) return fail(CB_E_ARG, "null mix");
+    CB_HIP(hipSetDevice(m->device));
+    m->m->autotune(trial_intervals);
+    if (lanes4)
+      for (int t = 0; t < 4; t++) lanes4[t] = m->m->lanes(t);
+    return CB_OK;
+  });
+}
+
+int cb_mix_kernel_time(cb_mix *m, double *ms, int64_t *launches, int reset) {
+  return guarded([&]() -> int {
+    if (!m || !ms || !launches) return fail(CB_E_ARG, "null argument");
+    long n = 0;
+    m->m->kernel_time(ms, &n, reset != 0);
+    *launches = n;
+    return CB_OK;
+  });
+}
+
+int cb_mix_use_graph(cb_mix *m, int on) {
+  return guarded([&]() -> int {
+    if (!m) return fail(CB_E_ARG, "null mix");
+    m->m->use_graph(on != 0);
+    return CB_OK;
+  });
+}
+
+int cb_mix_destroy(cb_mix *m) {
+  return guarded([&]() -> int {
+    if (!m) return CB_OK;
+    CB_HIP(hipSetDevice(m->device));
+    delete m;
+    return CB_OK;
+  });
+}
+
+int cb_synth_gr_chains(const char *path, const char *template_path, uint64_t seed, int ncell, const int *types,
+                       int comp) {
+  return guarded([&]() -> int {
+    if (!path || !template_path || ncell <= 0) return fail(CB_E_ARG, "bad argument");
+    std::vector<int> ty;
+    if (types) {
+      if (comp <= 0 || comp > gr::COMP) return fail(CB_E_ARG, "bad compartment count");
+      ty.assign(types, types + comp);
+    } else {
+      ty = {3, 4, 6, 7};   // dendrite, soma, axon initial segment, axon (gr.cuh:20-28 types)
+    }
+    std::string err;
+    if (!synth_gr_chains(path, template_path, seed, ncell, ty, err)) return fail(CB_E_IO, err);
+    return CB_OK;
+  });
+}
+
+int cb_synth_go_trees(const char *path, const char *template_path, uint64_t seed, int ncell) {
+  return guarded([&]() -> int {
+    if (!path || !template_path || ncell <= 0) return fail(CB_E_ARG, "bad argument");
+    std::string err;
+    if (!synth_go_trees(path, template_path, seed, ncell, err)) return fail(CB_E_IO, err);
+    return CB_OK;
+  });
+}
+
+int cb_synth_io_gj(const char *path, uint64_t seed, int ncell, int k, double g_min_nS, double g_max_nS) {
+  return guarded([&]() -> int {
+    if (!path || !(g_min_nS >= 0.0) || g_max_nS < g_min_nS) return fail(CB_E_ARG, "bad argument");
+    std::string err;
+    if (!synth_io_gj(path, seed, ncell, k, g_min_nS, g_max_nS, err)) return fail(CB_E_ARG, err);
     return CB_OK;
   });
 }

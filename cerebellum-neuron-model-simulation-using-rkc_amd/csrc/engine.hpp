@@ -67,7 +67,7 @@ struct PoissonSpec {
 
 class Engine {
  public:
-  virtual ~Engine() {}
+  virtual ~Engine() { free_raster(); }
   virtual void set_current(double t) = 0;      // xx_set_current
   virtual void advance() = 0;                  // one H_MAX macro step (xx_solve_by_rkc)
   virtual void read(int field, double *dst) = 0;  // nc values
@@ -114,6 +114,31 @@ class Engine {
   // works on
   virtual const double *device_v() const { return nullptr; }
   virtual hipStream_t stream() const { return nullptr; }
+  // the output voltage on the device (what read(F_VNEW) returns), or null
+  virtual const double *device_vnew() const { return nullptr; }
+
+  // Raster: with comp >= 0 every run detects, on the device after each
+  // output interval, the upward crossings of thr by compartment comp of every
+  // cell and appends (time at the interval's end, cell) to the spike list;
+  // comp < 0 stops it.
+  void set_raster(int comp, double thr);
+  // moves up to cap spikes (time order, cells ascending within a time) out of
+  // the list; returns how many
+  int take_spikes(int cap, double *t_out, int *cell_out);
+
+ protected:
+  int raster_comp_ = -1;
+  double raster_thr_ = 0.0;
+  double *raster_prev_ = nullptr;     // device, one per cell, 100 mV at the start
+  std::vector<std::pair<double, int>> spikes_;
+  // the previous-value array, allocated at the first use
+  double *raster_prev();
+  // appends the spikes of nsteps intervals of bit words (words per interval)
+  void append_spikes(const std::vector<uint32_t> &bits, int nsteps, int words, double t0, double interval);
+  // the device recorder after output interval k of a run: recorded values
+  // into rec (null: none), spike bits into bits (null: none)
+  void record_interval(hipStream_t st, const double *v, double *rec, int rec_comp, uint32_t *bits);
+  void free_raster();
 };
 
 // hipStreamCreate re-seeds the C library's rand() state.  The reference
@@ -131,6 +156,7 @@ struct SolverSpec {
   bool refresh_fn = false;      // recompute fn = F(y_n) after each ion update (off in the reference)
   std::vector<double> rho_floor;  // per cell: lower bound applied to every spectral-radius estimate (empty: none)
   double hmin = -1.0;           // <= 0: the cell type's H_MIN (solve_rkc.cuh:24-27)
+  double interval = 0.125;      // output / exchange interval, also the largest step (the reference's H_MAX)
   double hmin_or(double dflt) const { return hmin > 0.0 ? hmin : dflt; }
 };
 
@@ -138,6 +164,51 @@ struct SolverSpec {
 // entries of 4 doubles; entry 0 = {mu~_1}, entry j = {mu_j, nu_j, mu~_j,
 // a_{j-1}} for j = 1..m-1 (solve_rkc.cu:815-906).
 std::vector<double> rkc_stage_table(int mtab);
+
+// The batched engine over an uneven mix of cell types (granule, Golgi,
+// Purkinje, inferior olive; index = the C-ABI's cell type): every cell on
+// its own adaptive RKC step, all of them in one kernel per launch.
+struct MixPopSpec {
+  int n = 0;
+  std::string morph;
+  int comp = 0;                     // granule chain length (others: the format's)
+  std::vector<double> v0, amp;      // per cell
+  int stim_comp = 0, rec_comp = 0;
+  double thr = 0.0;
+};
+struct MixSpec {
+  MixPopSpec pop[4];                // CB_GR, CB_GO, CB_PKJ, CB_IO order (0, 1, 2, 3)
+  std::string io_gj;                // "pre,post,g_nS" pairs, or empty
+  double interval = 0.125;
+  double pkj_atol = 1e-6, pkj_rtol = 1e-6;
+  bool pkj_stable = true;           // ion interval 2^-8 ms, fn refresh, spectral-radius floor
+};
+
+class Mix {
+ public:
+  virtual ~Mix() {}
+  // nint intervals; rec_out [nint][total cells], cells in type order;
+  // spk_out [nint][words()] spike bits; either may be null
+  virtual void run(int nint, double *rec_out, uint32_t *spk_out) = 0;
+  virtual int ncell(int type) const = 0;
+  virtual int total_cells() const = 0;
+  virtual int words() const = 0;
+  virtual int word_off(int type) const = 0;
+  virtual int lanes(int type) const = 0;   // lanes (threads) per cell
+  virtual Stats stats(int type) const = 0;
+  virtual void autotune(int trial_intervals) = 0;
+  virtual void kernel_time(double *ms, long *launches, bool reset) = 0;
+  virtual void use_graph(bool on) = 0;
+};
+Mix *make_mix(const MixSpec &spec);
+
+// synthetic inputs in the reference formats (synth.cpp)
+bool synth_gr_chains(const std::string &path, const std::string &template_path, uint64_t seed, int ncell,
+                     const std::vector<int> &types, std::string &err);
+bool synth_go_trees(const std::string &path, const std::string &template_path, uint64_t seed, int ncell,
+                    std::string &err);
+bool synth_io_gj(const std::string &path, uint64_t seed, int ncell, int k, double g_min, double g_max,
+                 std::string &err);
 
 Engine *make_global_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);
 Engine *make_global_io(const IoHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn);

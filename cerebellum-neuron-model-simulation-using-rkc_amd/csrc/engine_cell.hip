@@ -33,19 +33,22 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <memory>
+#include <string>
 #include <mutex>
 #include <type_traits>
 #include <vector>
 
 #include "engine.hpp"
 #include "philox.hpp"
+#include "ion_state.hpp"
+#include "morph.hpp"
 #include "pkj_model.hpp"
 
 namespace cbrkc {
 namespace {
 
 constexpr double UROUND = 2.22e-16;
-constexpr double H_MAX = 0.125;
 constexpr int NO_PARENT = 0x7FF;
 // per-compartment flag byte: compartment type in the low bits, plus the
 // reference's index-keyed fixed somatic Ca reversal (pkj_ion.cu:329-331)
@@ -106,6 +109,7 @@ struct CellArgs {
   const double *rho_floor;  // per cell lower bound of the spectral radius (null: none)
   const double *coef;     // RKC stage coefficients by stage count (rkc_stage_table)
   int coef_mmax;          // largest tabulated stage count
+  double interval;        // output / exchange interval [ms] (the reference's H_MAX = 0.125)
   // static, per compartment, cell-major, breadth-first order (read into LDS)
   const uint32_t *topo;   // parent | first_child << 11 | nchild << 22
   const T *gpar;          // conductance to the parent [mS]
@@ -126,6 +130,14 @@ struct CellArgs {
   // recording
   int rec_orig;           // original local index of the recorded compartment
   double *rec;            // [nsteps][ncell] or null
+  // spikes: upward crossings of thr by the recorded compartment between two
+  // intervals set bit (cell & 31) of spk[s * spk_words + cell / 32], s the
+  // interval of the launch;
+  // vprev holds each cell's previous value (null: no detection)
+  double thr;
+  double *vprev;
+  uint32_t *spk;
+  int spk_words;
   int *queue;             // work counter
   const int *order;       // queue position -> cell (the costliest cells of the last run first)
   // network synapses onto the cells (granule -> Purkinje): CSR by global
@@ -742,7 +754,7 @@ __device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, doub
       __syncthreads();
     } else {
       for (int s = 0; s < nsteps && status == 0; s++) {
-        t_cur = t0 + s * H_MAX;
+        t_cur = t0 + s * a.interval;
         if (a.syn_on) {
           for (int i = tid; i < C; i += NT) {
             const int og = (int)(S.meta(i) & 0xFFFFu);
@@ -776,13 +788,22 @@ __device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, doub
           status = (int)idid;
           break;
         }
-        tend += H_MAX;
+        tend += a.interval;
         count(5, 1.0);
         // v_new = y_{n+1} (rkcint at s = 1); record the chosen compartment
-        if (a.rec) {
+        // and mark its upward threshold crossing
+        if (a.rec || a.spk) {
 #pragma unroll
           for (int k = 0; k < K; k++)
-            if (rec_mask >> k & 1u) a.rec[(size_t)s * a.ncell + cell] = (double)hi[k] + (double)lo[k];
+            if (rec_mask >> k & 1u) {
+              const double v = (double)hi[k] + (double)lo[k];
+              if (a.rec) a.rec[(size_t)s * a.ncell + cell] = v;
+              if (a.spk) {
+                if (v > a.thr && a.vprev[cell] <= a.thr)
+                  atomicOr(&a.spk[(size_t)s * a.spk_words + (cell >> 5)], 1u << (cell & 31));
+                a.vprev[cell] = v;
+              }
+            }
         }
       }
     }
@@ -994,7 +1015,8 @@ class CellPkj : public Engine {
     a_.atol = sol.atol;
     a_.rtol = sol.rtol;
     a_.hmin = sol.hmin_or(std::pow(2.0, -10.0));   // H_MIN_PKJ, solve_rkc.cuh:26
-    a_.hmax = H_MAX;
+    a_.hmax = sol.interval;
+    a_.interval = sol.interval;
     a_.integ = sol.ion_interval > 0 ? sol.ion_interval : std::pow(2.0, std::log10(sol.atol));
     a_.refresh_fn = sol.refresh_fn ? 1 : 0;
     a_.rho_floor = nullptr;
@@ -1009,7 +1031,7 @@ class CellPkj : public Engine {
     std::vector<double> ctl((size_t)n * NCTL, 0.0);
     for (int c = 0; c < n; c++) {
       double *q = &ctl[(size_t)c * NCTL];
-      q[C_TEND] = H_MAX;
+      q[C_TEND] = sol.interval;
       q[C_LAST] = 1;
       q[C_NEWSPC] = 1;
       q[C_JACATT] = 0;
@@ -1025,10 +1047,10 @@ class CellPkj : public Engine {
     amp.resize(n, pkj::CURRENT);
     a_.amp = put(amp);
     a_.syn_on = syn.on ? 1 : 0;
-    a_.syn_decay = (T)std::exp(-H_MAX / syn.tau_ms);
+    a_.syn_decay = (T)std::exp(-sol.interval / syn.tau_ms);
     a_.syn_g_event = (T)syn.g_event;
     a_.syn_e = (T)syn.e_syn;
-    a_.syn_p = 1.0 - std::exp(-syn.rate_hz * H_MAX * 1e-3);
+    a_.syn_p = 1.0 - std::exp(-syn.rate_hz * sol.interval * 1e-3);
     a_.syn_k0 = (uint32_t)syn.seed;
     a_.syn_k1 = (uint32_t)(syn.seed >> 32);
     a_.rec_orig = 1599;
@@ -1056,6 +1078,7 @@ class CellPkj : public Engine {
     (void)hipFree(a_.queue);
     (void)hipFree(order_);
     if (rec_) (void)hipFree(rec_);
+    if (spk_) (void)hipFree(spk_);
     (void)hipEventDestroy(ev0_);
     (void)hipEventDestroy(ev1_);
     (void)hipStreamDestroy(st_);
@@ -1066,6 +1089,7 @@ class CellPkj : public Engine {
   // CSR by global compartment, read at every macro-step start
   void set_synapses(const SynIn &in) override { a_.net = in; }
   const double *device_v() const override { return vnew_; }
+  const double *device_vnew() const override { return vnew_; }
   hipStream_t stream() const override { return st_; }
   int ncell() const override { return n_; }
   Stats stats() const override {
@@ -1090,6 +1114,11 @@ class CellPkj : public Engine {
 
   void run(int nsteps, double t0, int rec_comp, double *rec_out) override {
     if (nsteps <= 0) return;
+    if (rec_out && rec_comp < 0) throw Error("the per-cell engines record one compartment per cell");
+    // spikes are found in the kernel on the recorded compartment
+    const bool raster = raster_comp_ >= 0;
+    if (raster && rec_out && rec_comp != raster_comp_)
+      throw Error("the per-cell engines detect spikes on the recorded compartment");
     double *rec = nullptr;
     if (rec_out) {
       if (rec_cap_ < (size_t)nsteps * n_) {
@@ -1100,6 +1129,20 @@ class CellPkj : public Engine {
       rec = rec_;
       a_.rec_orig = rec_comp;
     }
+    const int words = (n_ + 31) / 32;
+    if (raster) {
+      if (spk_cap_ < (size_t)nsteps * words) {
+        if (spk_) CB_HIP(hipFree(spk_));
+        spk_cap_ = (size_t)nsteps * words;
+        spk_ = dalloc<uint32_t>(spk_cap_);
+      }
+      CB_HIP(hipMemsetAsync(spk_, 0, sizeof(uint32_t) * (size_t)nsteps * words, st_));
+      a_.rec_orig = raster_comp_;
+      a_.thr = raster_thr_;
+      a_.vprev = raster_prev();
+      a_.spk = spk_;
+      a_.spk_words = words;
+    }
     a_.rec = rec;
     if (a_.net.on()) {
       // the synaptic current is fixed per macro step from that step's start,
@@ -1109,14 +1152,18 @@ class CellPkj : public Engine {
     }
     launch(0, nsteps, t0);
     a_.rec = nullptr;
+    a_.spk = nullptr;
     macro_ += nsteps;
     if (rec_out) {
       CB_HIP(hipMemcpyAsync(rec_out, rec, sizeof(double) * (size_t)nsteps * n_, hipMemcpyDeviceToHost, st_));
     }
+    std::vector<uint32_t> bits((size_t)(raster ? nsteps : 0) * words);
+    if (raster) CB_HIP(hipMemcpyAsync(bits.data(), spk_, sizeof(uint32_t) * bits.size(), hipMemcpyDeviceToHost, st_));
     CB_HIP(hipStreamSynchronize(st_));
+    if (raster) append_spikes(bits, nsteps, words, t0, a_.interval);
     collect_time();
     check_status();
-    t_next_ = t0 + nsteps * H_MAX;
+    t_next_ = t0 + nsteps * a_.interval;
     if (nsteps >= 4) reorder();
   }
 
@@ -1154,7 +1201,7 @@ class CellPkj : public Engine {
     macro_ += nsteps;
     collect_time();
     check_status();
-    t_next_ = t0 + nsteps * H_MAX;
+    t_next_ = t0 + nsteps * a_.interval;
     if (nsteps >= 4) reorder();
   }
 
@@ -1275,6 +1322,12 @@ class CellPkj : public Engine {
     int nt, k;
     double sec;
   };
+  // for the batched mix engine, which launches this population's cells
+  // inside its own kernel: the kernel arguments, the dynamic LDS a cell
+  // needs and the device buffers a run changes
+  CellArgs<T> &args() { return a_; }
+  size_t cell_lds_bytes() const { return lds_bytes(); }
+  std::vector<std::pair<void *, size_t>> state_buffers() { return mutable_bufs(); }
   std::vector<TuneRec> tune_log_;
   int chosen_nt() const { return variants_[cur_].nt; }
 
@@ -1419,6 +1472,8 @@ class CellPkj : public Engine {
   double *vnew_ = nullptr;
   double *rec_ = nullptr;
   size_t rec_cap_ = 0;
+  uint32_t *spk_ = nullptr;
+  size_t spk_cap_ = 0;
   std::vector<int> perm_;
   int *order_ = nullptr;
   unsigned char *scratch_ = nullptr;   // this device's scratch array
@@ -1433,7 +1488,1164 @@ class CellPkj : public Engine {
   double t_next_ = 0.0;
 };
 
+// ---------------------------------------------------------------------------
+// Small cells (granule, Golgi, inferior olive), G lanes of one wavefront per
+// cell, 64 / G cells per wave, each with its own FP64 controller: the
+// reference's algorithm on a population of one cell (rkclow, step, rkcrho
+// and the per-type drivers, solve_rkc.cu:562-1362), the same structure as
+// the Purkinje body above.  Lane r of a cell holds compartments r, r + G, ...
+// in registers; the cell's stage vector sits in the wave's slice of LDS for
+// the cable stencil (the cell's CSR row as a neighbour list); norms are
+// butterfly sums over the cell's G lanes.  Each cell takes its own step
+// sequence, so control flow diverges between the cells of a wave but never
+// inside one cell's lanes: the shuffles and LDS reads of a cell only touch
+// lanes on the same path.  Stages and ion updates are FP64 (the models'
+// per-compartment functions shared with the population-global engine).
+
+constexpr int SMALL_NB = 4;      // cable neighbours per compartment
+constexpr int SMALL_KMAX = 4;    // compartments per lane
+constexpr int SMALL_LDS = 256;   // stage-buffer doubles per wave: cells per wave x compartments
+enum SmallType { SM_GR = 0, SM_GO = 1, SM_IO = 2 };
+
+struct SmallPop {
+  int type, n, comp, G;
+  IoState io;
+  GoState go;
+  GrState gr;
+  const int *nb;        // [nc][SMALL_NB] cell-local neighbour index or -1
+  const double *nbg;    // [nc][SMALL_NB] conductance to it [mS] (the CSR's off-diagonal)
+  const double *invc;   // [nc] 1 / Cm
+  double *bb;           // [nc] frozen current (the model state's b)
+  double *y0, *f0, *vy, *ede, *vnew;
+  double *ctl;          // [n][NCTL]
+  const double *gj;     // [n] gap-junction current of compartment 0 over the interval, or null
+  double atol, rtol, hmin, hmax, integ, interval;
+  int strict;           // 1: ion update once sum_dt > integ (granule, olive); 0: >= (Golgi)
+  int refresh_fn;
+  int rec_comp;         // recorded / raster compartment
+  double *rec;          // [nsteps][n] or null
+  double thr;           // raster threshold
+  double *vprev;        // [n] previous recorded value
+  uint32_t *spk;        // [nsteps][spk_words] or null
+  int spk_words;
+};
+
+__device__ inline double small_ion(const SmallPop &P, int gi, double vn, double dt, bool do_ion, bool with_iext) {
+  if (P.type == SM_GR) return gr_ion_current(P.gr, gi, vn, dt, do_ion, with_iext);
+  if (P.type == SM_GO) return go_ion_current(P.go, gi, vn, dt, do_ion, with_iext);
+  return io_ion_current(P.io, gi, vn, dt, do_ion, with_iext);
+}
+
+// sum over the G lanes of one cell (G a power of two, groups aligned): every
+// lane of the group ends with the same value (each level adds a pair in
+// either order, and IEEE addition commutes)
+__device__ inline double gsum(double x, int G) {
+  for (int off = 1; off < G; off <<= 1) x += __shfl_xor(x, off);
+  return x;
+}
+
+template <bool INIT>
+__device__ void small_cells(const SmallPop &P, int c0, int nsteps, double *lds) {
+  const int lane = threadIdx.x & 63;
+  const int G = P.G, C = P.comp;
+  const int slot = lane / G, r = lane - slot * G;
+  const int cell = c0 + slot;
+  if (cell >= P.n) return;             // the group has no cell (a population's last item)
+  const int K = (C + G - 1) / G;       // <= SMALL_KMAX (checked on the host)
+  double *cb[2] = {lds + slot * C, lds + SMALL_LDS + slot * C};
+  const size_t base = (size_t)cell * C;
+  double *ctl = P.ctl + (size_t)cell * NCTL;
+  auto comp_of = [&](int k) { return r + k * G; };
+  auto valid = [&](int k) { return k < K && r + k * G < C; };
+
+  double y[SMALL_KMAX], F0[SMALL_KMAX], A[SMALL_KMAX], D1[SMALL_KMAX], D2[SMALL_KMAX], VY[SMALL_KMAX],
+      bq[SMALL_KMAX];
+#pragma unroll
+  for (int k = 0; k < SMALL_KMAX; k++) {
+    y[k] = F0[k] = VY[k] = bq[k] = A[k] = D1[k] = D2[k] = 0.0;
+    if (valid(k)) {
+      const size_t gi = base + comp_of(k);
+      y[k] = P.y0[gi];
+      F0[k] = P.f0[gi];
+      VY[k] = P.vy[gi];
+      bq[k] = P.bb[gi];
+    }
+  }
+  // sum_nb g (x_nb - x_i) over the compartment's neighbours, x from LDS
+  auto lap = [&](const double *x, int k, double xi) {
+    const size_t gi = base + comp_of(k);
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < SMALL_NB; q++) {
+      const int j = P.nb[gi * SMALL_NB + q];
+      if (j >= 0) s += P.nbg[gi * SMALL_NB + q] * (x[j] - xi);
+    }
+    return s;
+  };
+  auto put = [&](double *buf, const double *v) {
+#pragma unroll
+    for (int k = 0; k < SMALL_KMAX; k++)
+      if (valid(k)) buf[comp_of(k)] = v[k];
+    __builtin_amdgcn_wave_barrier();
+  };
+
+  double absh = ctl[C_ABSH], sprad = ctl[C_SPRAD], naccpt = ctl[C_NACCPT], errold = ctl[C_ERROLD],
+         hold = ctl[C_HOLD], lt = ctl[C_LT], tend = ctl[C_TEND], gt = ctl[C_GT], gt_old = ctl[C_GTOLD],
+         sum_dt = ctl[C_SUMDT], mmax = ctl[C_MMAX];
+  bool newspc = ctl[C_NEWSPC] != 0.0, jacatt = ctl[C_JACATT] != 0.0, last = ctl[C_LAST] != 0.0;
+  double idid = ctl[C_IDID];
+  double cnt[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) cnt[q] = ctl[C_STEPS + q];
+  int status = (int)ctl[C_STATUS];
+  const double at = P.atol, rt = P.rtol, hmin = P.hmin, hmax = P.hmax;
+  bool a_valid = false;
+
+  // A = F(y_n) = (-b + L y_n) / C
+  auto compute_A = [&]() {
+    put(cb[0], y);
+#pragma unroll
+    for (int k = 0; k < SMALL_KMAX; k++)
+      if (valid(k)) A[k] = (-bq[k] + lap(cb[0], k, y[k])) * P.invc[base + comp_of(k)];
+    __builtin_amdgcn_wave_barrier();
+    a_valid = true;
+  };
+
+  // the type's ion update (when do_ion) and frozen current at V = Y_{m-2};
+  // the initial one (!do_ion) has neither stimulus nor junction current: the
+  // reference initialises RKC before the output loop's first set_current
+  auto ion_and_b = [&](double dt, bool do_ion) {
+#pragma unroll
+    for (int k = 0; k < SMALL_KMAX; k++)
+      if (valid(k)) {
+        const int i = comp_of(k);
+        double b = small_ion(P, (int)(base + i), VY[k], dt, do_ion, do_ion);
+        if (i == 0 && P.gj && do_ion) b += P.gj[cell];
+        P.bb[base + i] = b;
+        bq[k] = b;
+      }
+    a_valid = false;
+  };
+
+  // rkcrho, solve_rkc.cu:562-754, on the perturbation delta = v - y_n (in D1)
+  auto rkcrho = [&]() {
+    const int itmax = 1000;
+    double sprad_old[3] = {0.0, 0.0, 0.0};
+    const double sqrtu = sqrt(UROUND), small = 1.0 / 0.125;
+    double sy = 0.0, se = 0.0;
+#pragma unroll
+    for (int k = 0; k < SMALL_KMAX; k++) {
+      D1[k] = valid(k) ? P.ede[base + comp_of(k)] : 0.0;
+      sy += y[k] * y[k];
+      se += D1[k] * D1[k];
+    }
+    const double ynrm = sqrt(gsum(sy, G)), vnrm = sqrt(gsum(se, G));
+    double dynrm;
+    if (ynrm != 0.0 && vnrm != 0.0) {
+      dynrm = ynrm * sqrtu;
+#pragma unroll
+      for (int k = 0; k < SMALL_KMAX; k++) D1[k] = (dynrm / vnrm) * D1[k];
+    } else if (ynrm != 0.0) {
+      dynrm = ynrm * sqrtu;
+#pragma unroll
+      for (int k = 0; k < SMALL_KMAX; k++) D1[k] = sqrtu * y[k];
+    } else if (vnrm != 0.0) {
+      dynrm = UROUND;
+#pragma unroll
+      for (int k = 0; k < SMALL_KMAX; k++) D1[k] = (dynrm / vnrm) * D1[k] - y[k];
+    } else {
+      dynrm = UROUND;
+#pragma unroll
+      for (int k = 0; k < SMALL_KMAX; k++) D1[k] = dynrm - y[k];
+    }
+    double sigma = 0.0, sigmal;
+    for (int iter = 0; iter < itmax; iter++) {
+      cnt[3] += 1.0;
+      put(cb[0], D1);
+      double sq = 0.0;
+#pragma unroll
+      for (int k = 0; k < SMALL_KMAX; k++) {
+        D2[k] = 0.0;
+        if (valid(k)) {
+          D2[k] = (A[k] - F0[k]) + lap(cb[0], k, D1[k]) * P.invc[base + comp_of(k)];
+          sq += D2[k] * D2[k];
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      const double dfnrm = sqrt(gsum(sq, G));
+      sigmal = sigma;
+      sigma = dfnrm / dynrm;
+      sprad_old[0] = sprad_old[1];
+      sprad_old[1] = sprad_old[2];
+      sprad_old[2] = sprad;
+      sprad = 1.2 * sigma;
+      bool done = iter >= 1 && fabs(sigma - sigmal) <= fmax(sigma, small) * 0.001;
+      if (!done && iter > itmax / 2 && fabs(sprad_old[2] - sprad_old[0]) < 0.00001 &&
+          fabs(sprad_old[1] - sprad) < 0.00001) {
+        sprad = (sprad + sprad_old[2]) / 2.0;
+        done = true;
+      }
+      if (done) {
+#pragma unroll
+        for (int k = 0; k < SMALL_KMAX; k++)
+          if (valid(k)) P.ede[base + comp_of(k)] = D1[k];
+        return;
+      }
+      if (dfnrm != 0.0) {
+#pragma unroll
+        for (int k = 0; k < SMALL_KMAX; k++) D1[k] = (dynrm / dfnrm) * (-D2[k]);
+      } else {
+        const int idx = iter % C;
+#pragma unroll
+        for (int k = 0; k < SMALL_KMAX; k++)
+          if (valid(k) && comp_of(k) == idx) D1[k] = -D1[k];
+      }
+    }
+    idid = 6;
+  };
+
+  // rkclow, solve_rkc.cu:908-1096, on increments D_j = Y_j - Y_0
+  auto rkclow = [&]() {
+    int loop_num = 0;
+    while (true) {
+      if (++loop_num > 1000) return;
+      if (!a_valid) compute_A();
+      if (newspc) {
+        rkcrho();
+        jacatt = true;
+      }
+      last = false;
+      if (1.1 * absh >= fabs(tend - lt)) {
+        absh = fabs(tend - lt);
+        last = true;
+      }
+      const double mr = sqrt(1.54 * absh * sprad + 1.0);
+      int m = (mr < mmax) ? 1 + (int)mr : (int)mmax + 1;
+      if (m > mmax) {
+        m = (int)mmax;
+        absh = (m * m - 1) / (1.54 * sprad);
+        last = false;
+      }
+      double h = absh;
+      if (h <= hmin) h = hmin;
+      const double w0 = 1.0 + 2.0 / (13.0 * m * m);
+      const double temp1 = w0 * w0 - 1.0, temp2 = sqrt(temp1);
+      const double arg = m * log(w0 + temp2);
+      const double w1 = sinh(arg) * temp1 / (cosh(arg) * m * temp2 - w0 * sinh(arg));
+      double bjm1 = 1.0 / ((2.0 * w0) * (2.0 * w0)), bjm2 = bjm1;
+      const double mus1 = w1 * bjm1;
+      int cur = 0;
+#pragma unroll
+      for (int k = 0; k < SMALL_KMAX; k++) {
+        D2[k] = 0.0;
+        D1[k] = (h * mus1) * F0[k];
+        VY[k] = y[k];   // Y_{m-2} when m == 2
+      }
+      put(cb[cur], D1);
+      double zjm1 = w0, zjm2 = 1.0, dzjm1 = 1.0, dzjm2 = 0.0, d2zjm1 = 0.0, d2zjm2 = 0.0;
+      for (int j = 1; j < m; j++) {
+        const double zj = 2.0 * w0 * zjm1 - zjm2;
+        const double dzj = 2.0 * w0 * dzjm1 - dzjm2 + 2.0 * zjm1;
+        const double d2zj = 2.0 * w0 * d2zjm1 - d2zjm2 + 4.0 * dzjm1;
+        const double bj = d2zj / (dzj * dzj);
+        const double ajm1 = 1.0 - zjm1 * bjm1;
+        const double mu = 2.0 * w0 * bj / bjm1;
+        const double nu = -bj / bjm2;
+        const double hmus = h * (mu * w1 / w0);
+        double Dn[SMALL_KMAX];
+#pragma unroll
+        for (int k = 0; k < SMALL_KMAX; k++) {
+          Dn[k] = 0.0;
+          if (valid(k)) {
+            const double F = A[k] + lap(cb[cur], k, D1[k]) * P.invc[base + comp_of(k)];
+            Dn[k] = mu * D1[k] + nu * D2[k] + hmus * (F - ajm1 * F0[k]);
+            if (j == m - 1) VY[k] = y[k] + D2[k];   // Y_{m-2}
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < SMALL_KMAX; k++) {
+          D2[k] = D1[k];
+          D1[k] = Dn[k];
+        }
+        put(cb[cur ^ 1], D1);
+        cur ^= 1;
+        bjm2 = bjm1; bjm1 = bj;
+        zjm2 = zjm1; zjm1 = zj;
+        dzjm2 = dzjm1; dzjm1 = dzj;
+        d2zjm2 = d2zjm1; d2zjm1 = d2zj;
+      }
+      cnt[2] += m + 1;
+      // f(Y_m) (into D2) and the error estimate, solve_rkc.cu:58-73
+      double sq = 0.0;
+#pragma unroll
+      for (int k = 0; k < SMALL_KMAX; k++) {
+        const double Dm = D1[k];
+        D2[k] = 0.0;
+        if (valid(k)) {
+          const double Fm = A[k] + lap(cb[cur], k, Dm) * P.invc[base + comp_of(k)];
+          D2[k] = Fm;
+          const double yn = y[k], yy = y[k] + Dm;
+          const double wt = at + rt * fmax(fabs(yy), fabs(yn));
+          const double est = 0.8 * (yn - yy) + 0.4 * h * (F0[k] + Fm);
+          const double q = est / wt;
+          sq += q * q;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      double err = sqrt(gsum(sq, G) / (double)C);
+      if (!(err <= 1.0e300)) {
+        idid = NONFINITE;
+        return;
+      }
+      if (err > 1.0) {
+        cnt[1] += 1.0;
+        absh = 0.8 * absh / pow(err, 1.0 / 3.0);
+        if (absh < hmin) {
+          idid = 4;   // falls through to the accept branch, as the reference does
+        } else {
+          newspc = !jacatt;
+          continue;
+        }
+      }
+      cnt[0] += 1.0;
+      naccpt += 1;
+      lt += h;
+      gt = lt;
+      jacatt = false;
+      newspc = false;
+#pragma unroll
+      for (int k = 0; k < SMALL_KMAX; k++) {
+        y[k] = y[k] + D1[k];
+        F0[k] = D2[k];
+      }
+      a_valid = false;
+      double fac = 10.0, tp1, tp2;
+      if (naccpt == 1) {
+        tp2 = pow(err, 1.0 / 3.0);
+        if (0.8 < fac * tp2) fac = 0.8 / tp2;
+      } else {
+        tp1 = 0.8 * absh * pow(errold, 1.0 / 3.0);
+        tp2 = fabs(hold) * pow(err, 2.0 / 3.0);
+        if (tp1 < fac * tp2) fac = tp1 / tp2;
+      }
+      absh = fmax(0.1, fac) * absh;
+      absh = fmax(hmin, fmin(hmax, absh));
+      errold = err;
+      hold = h;
+      idid = last ? 1 : 2;
+      return;
+    }
+  };
+
+  if constexpr (INIT) {
+    // RKC_vec_initialize (solve_rkc.cu:756-791)
+    ion_and_b(0.0, false);
+    compute_A();
+#pragma unroll
+    for (int k = 0; k < SMALL_KMAX; k++) {
+      F0[k] = A[k];
+      if (valid(k)) P.ede[base + comp_of(k)] = F0[k];
+    }
+    rkcrho();
+    // set_initial_absh (solve_rkc.cu:489-560)
+    double ab = hmax;
+    if (sprad * ab > 1.0) ab = 1.0 / sprad;
+    ab = fmax(ab, hmin);
+#pragma unroll
+    for (int k = 0; k < SMALL_KMAX; k++) D1[k] = ab * F0[k];
+    put(cb[0], D1);
+    double sq = 0.0;
+#pragma unroll
+    for (int k = 0; k < SMALL_KMAX; k++)
+      if (valid(k)) {
+        const double F = A[k] + lap(cb[0], k, D1[k]) * P.invc[base + comp_of(k)];
+        const double wt = at + rt * fabs(y[k]);
+        const double d = (F - F0[k]) * (1.0 / wt);
+        sq += d * d;
+      }
+    __builtin_amdgcn_wave_barrier();
+    double est = gsum(sq, G);
+    est = ab * sqrt(est / C);
+    if (0.1 * ab < hmax * sqrt(est))
+      ab = fmax(0.1 * ab / sqrt(est), hmin);
+    else
+      ab = hmax;
+    absh = ab;
+  } else {
+    for (int s = 0; s < nsteps && status == 0; s++) {
+      // the driver of xx_solve_by_rkc (solve_rkc.cu:1100-1362)
+      auto ion_step = [&]() {
+        cnt[4] += 1.0;
+        ion_and_b(sum_dt, true);
+        compute_A();
+        if (P.refresh_fn) {
+#pragma unroll
+          for (int k = 0; k < SMALL_KMAX; k++) F0[k] = A[k];
+        }
+        sum_dt = 0.0;
+      };
+      if (sum_dt != 0.0) ion_step();
+      while (true) {
+        rkclow();
+        sum_dt += gt - gt_old;
+        gt_old = gt;
+        if ((P.strict ? sum_dt > P.integ : sum_dt >= P.integ) && idid == 2.0) ion_step();
+        if (idid == 2.0) continue;
+        if (idid == 1.0) break;
+        status = (int)idid;
+        break;
+      }
+      tend += P.interval;
+      cnt[5] += 1.0;
+#pragma unroll
+      for (int k = 0; k < SMALL_KMAX; k++)
+        if (valid(k) && comp_of(k) == P.rec_comp) {
+          const double v = y[k];
+          if (P.rec) P.rec[(size_t)s * P.n + cell] = v;
+          if (P.spk) {
+            if (v > P.thr && P.vprev[cell] <= P.thr)
+              atomicOr(&P.spk[(size_t)s * P.spk_words + (cell >> 5)], 1u << (cell & 31));
+            P.vprev[cell] = v;
+          }
+        }
+    }
+  }
+
+  // write back
+#pragma unroll
+  for (int k = 0; k < SMALL_KMAX; k++)
+    if (valid(k)) {
+      const size_t gi = base + comp_of(k);
+      P.y0[gi] = y[k];
+      P.f0[gi] = F0[k];
+      P.vy[gi] = VY[k];
+      P.vnew[gi] = y[k];
+    }
+  if (r == 0) {
+    ctl[C_ABSH] = absh; ctl[C_SPRAD] = sprad; ctl[C_NACCPT] = naccpt; ctl[C_ERROLD] = errold;
+    ctl[C_HOLD] = hold; ctl[C_LT] = lt; ctl[C_TEND] = tend; ctl[C_GT] = gt; ctl[C_GTOLD] = gt_old;
+    ctl[C_SUMDT] = sum_dt; ctl[C_MMAX] = mmax;
+    ctl[C_NEWSPC] = newspc; ctl[C_JACATT] = jacatt; ctl[C_LAST] = last; ctl[C_IDID] = idid;
+    for (int q = 0; q < 6; q++) ctl[C_STEPS + q] = cnt[q];
+    ctl[C_STATUS] = status;
+  }
+}
+
+// One persistent launch over a mix of cell types: the Purkinje cells first,
+// one workgroup per cell from their own work counter (the costliest first),
+// then, wave by wave, the small-cell items (a population and its first
+// cell; 64 / G cells each) from a second counter.  INIT runs the RKC
+// initialisation instead of nsteps intervals.
+struct MixArgs {
+  CellArgs<float> pkj;
+  int pkj_on;
+  const SmallPop *pops;   // device array
+  const int *item_pop, *item_cell;
+  int nitems;
+  int *small_queue;
+};
+
+template <int NT, int K, bool INIT>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_mix(MixArgs m, int nsteps, double t0, int step0) {
+  if (m.pkj_on) cell_body<float, NT, K, INIT, false>(m.pkj, nsteps, t0, step0);
+  extern __shared__ __align__(16) unsigned char smem[];
+  double *lds = (double *)smem + (size_t)(threadIdx.x >> 6) * 2 * SMALL_LDS;
+  while (true) {
+    int it = 0;
+    if ((threadIdx.x & 63) == 0) it = atomicAdd(m.small_queue, 1);
+    it = __shfl(it, 0);
+    if (it >= m.nitems) break;
+    const SmallPop *P = m.pops + m.item_pop[it];
+    small_cells<INIT>(*P, m.item_cell[it], nsteps, lds);
+  }
+}
+
+// gap-junction current of every inferior-olive cell for the next interval,
+// from compartment 0's voltage at the interval's start: for each partner j
+// in CSR order, g_ij (0.8 exp(-0.01 dV^2) + 0.2) dV with dV = V_i - V_j (the
+// reference's nonlinear junction, gap.cu:149-160, with a per-pair
+// conductance in place of G_C)
+__global__ void __launch_bounds__(256) k_io_gj(const int *row, const int *col, const double *g, const double *vnew,
+                                               int comp, double *gj, int n) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  const double vi = vnew[(size_t)c * comp];
+  double s = 0.0;
+  for (int e = row[c]; e < row[c + 1]; e++) {
+    const double dv = vi - vnew[(size_t)col[e] * comp];
+    s += g[e] * (0.8 * exp(-0.01 * dv * dv) + 0.2) * dv;
+  }
+  gj[c] = s;
+}
+
+// fills a spike buffer with zeros (a kernel, so it may sit in a graph)
+__global__ void k_zero_u32(uint32_t *p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Batched engine over an uneven mix of cell types (host side): Purkinje cells
+// on the per-cell mixed-precision body, granule / Golgi / olive cells on the
+// small-cell lane-group body, all in one kernel per launch.  Olive cells
+// exchange gap-junction currents once per interval, so with olive coupling
+// every interval is one launch followed by the exchange; the launches of a
+// chunk of intervals are captured once in a HIP graph and replayed.
+class MixEngine : public Mix {
+ public:
+  static constexpr int T_GR = 0, T_GO = 1, T_PKJ = 2, T_IO = 3;
+  static constexpr int NT = 512, KP = 4;
+
+  explicit MixEngine(const MixSpec &S) : spec_(S) {
+    create_stream(&st_);
+    CB_HIP(hipEventCreate(&ev0_));
+    CB_HIP(hipEventCreate(&ev1_));
+    for (int t = 0; t < 4; t++) {
+      n_[t] = S.pop[t].n;
+      if (n_[t] < 0) throw Error("negative cell count");
+      if (n_[t] > 0 && (int)S.pop[t].v0.size() != n_[t]) throw Error("a population needs one initial voltage per cell");
+    }
+    if (S.interval <= 0.0 || S.interval > 0.125) throw Error("interval must be in (0, 0.125] ms");
+    // Purkinje cells: the per-cell mixed engine's state and initialisation
+    if (n_[T_PKJ] > 0) {
+      std::vector<MorphRow> rows;
+      std::string err;
+      if (!read_pkj_morph(S.pop[T_PKJ].morph, pkj::COMP, rows, err)) throw Error(err);
+      PkjHost H;
+      if (!build_pkj_host(rows, n_[T_PKJ], 1, nullptr, nullptr, S.pop[T_PKJ].v0.data(), H, err)) throw Error(err);
+      SolverSpec sol;
+      sol.atol = S.pkj_atol;
+      sol.rtol = S.pkj_rtol;
+      sol.interval = S.interval;
+      if (S.pkj_stable) {
+        sol.ion_interval = std::ldexp(1.0, -8);
+        sol.refresh_fn = true;
+        std::vector<double> rho;
+        cable_spectral_radius(H, rho);
+        sol.rho_floor.resize(rho.size());
+        for (size_t c = 0; c < rho.size(); c++) sol.rho_floor[c] = 1.2 * rho[c];
+      }
+      StimSpec stim;
+      stim.comp = S.pop[T_PKJ].stim_comp;
+      stim.t0 = 0.0;
+      stim.t1 = 1e300;
+      stim.amp = S.pop[T_PKJ].amp;
+      stim.amp.resize(n_[T_PKJ], 0.0);
+      pkj_.reset(new CellPkj<float>(H, sol, stim, PoissonSpec()));
+      if (!pkj_->select_mapping(NT)) throw Error("no 512-thread Purkinje mapping");
+      pkj_->sync();
+    }
+    for (int t : {T_GR, T_GO, T_IO})
+      if (n_[t] > 0) build_small(t);
+    if (!S.io_gj.empty() && n_[T_IO] > 0) build_io_gj();
+    // device copies of the small populations, the work items, the counter
+    small_queue_ = dalloc<int>(1);
+    bufs_.push_back(small_queue_);
+    pops_dev_ = dalloc<SmallPop>(3);
+    bufs_.push_back(pops_dev_);
+    upload_pops();
+    int dev = 0;
+    CB_HIP(hipGetDevice(&dev));
+    CB_HIP(hipDeviceGetAttribute(&ncu_, hipDeviceAttributeMultiprocessorCount, dev));
+    lds_ = std::max(pkj_ ? pkj_->cell_lds_bytes() : 0, (size_t)(NT / 64) * 2 * SMALL_LDS * sizeof(double));
+    // RKC_vec_initialize of every small cell
+    if (!items_pop_.empty()) {
+      MixArgs a = args(0, nullptr, nullptr);
+      a.pkj_on = 0;
+      k_zero_int<<<1, 1, 0, st_>>>(small_queue_);
+      hipLaunchKernelGGL((k_mix<NT, KP, true>), dim3(grid()), dim3(NT), lds_, st_, a, 0, 0.0, 0);
+      CB_HIP(hipGetLastError());
+      CB_HIP(hipStreamSynchronize(st_));
+      check_status();
+    }
+  }
+
+  ~MixEngine() {
+    (void)hipStreamSynchronize(st_);
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    for (void *p : bufs_) (void)hipFree(p);
+    (void)hipFree(item_pop_dev_);
+    (void)hipFree(item_cell_dev_);
+    (void)hipEventDestroy(ev0_);
+    (void)hipEventDestroy(ev1_);
+    (void)hipStreamDestroy(st_);
+  }
+
+  int ncell(int t) const override { return n_[t]; }
+  int total_cells() const override { return n_[0] + n_[1] + n_[2] + n_[3]; }
+  int words() const override { return words_total(); }
+  int word_off(int t) const override {
+    int w = 0;
+    for (int u = 0; u < t; u++) w += (n_[u] + 31) / 32;
+    return w;
+  }
+  int lanes(int t) const override { return t == T_PKJ ? NT : pop_[t].G; }
+
+  // nint intervals; rec_out [nint][total cells] (type order GR, GO, PKJ,
+  // IO), spk_out [nint][words()] bits; either may be null
+  void run(int nint, double *rec_out, uint32_t *spk_out) override {
+    if (nint < 0) throw Error("negative interval count");
+    int done = 0;
+    while (done < nint) {
+      const int k = std::min(CHUNK, nint - done);
+      ensure_chunk_buffers();
+      launch_chunk(k, true);
+      CB_HIP(hipStreamSynchronize(st_));
+      collect_time();
+      check_status();
+      if (rec_out) {
+        const int tot = total_cells();
+        std::vector<double> tmp;
+        for (int t = 0, off = 0; t < 4; off += n_[t], t++) {
+          if (!n_[t]) continue;
+          tmp.resize((size_t)k * n_[t]);
+          CB_HIP(hipMemcpy(tmp.data(), rec_[t], sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+          for (int s = 0; s < k; s++)
+            std::memcpy(rec_out + (size_t)(done + s) * tot + off, tmp.data() + (size_t)s * n_[t],
+                        sizeof(double) * n_[t]);
+        }
+      }
+      if (spk_out)
+        CB_HIP(hipMemcpy(spk_out + (size_t)done * words_total(), spk_, sizeof(uint32_t) * (size_t)k * words_total(),
+                         hipMemcpyDeviceToHost));
+      intervals_ += k;
+      done += k;
+    }
+  }
+
+  Stats stats(int t) const override {
+    Stats s;
+    if (t == T_PKJ) {
+      if (pkj_) s = pkj_->stats();
+      s.macro_steps = intervals_;
+      return s;
+    }
+    if (!n_[t]) return s;
+    std::vector<double> ctl((size_t)n_[t] * NCTL);
+    CB_HIP(hipMemcpy(ctl.data(), pop_[t].ctl, sizeof(double) * ctl.size(), hipMemcpyDeviceToHost));
+    for (int c = 0; c < n_[t]; c++) {
+      const double *q = &ctl[(size_t)c * NCTL];
+      s.steps += (long)q[C_STEPS];
+      s.rejects += (long)q[C_REJECTS];
+      s.stages += (long)q[C_STAGES];
+      s.rho_iters += (long)q[C_RHO];
+      s.ion_updates += (long)q[C_IONS];
+    }
+    s.macro_steps = intervals_;
+    return s;
+  }
+
+  // Lanes per cell of each small population (cells per wavefront = 64 / G):
+  // every candidate runs the same trial from the same state (restored after
+  // each), one population at a time, and the fastest is kept.
+  void autotune(int trial) override {
+    if (trial <= 0) return;
+    Snapshot snap = snapshot();
+    for (int t : {T_GR, T_IO, T_GO}) {
+      if (!n_[t]) continue;
+      std::vector<int> cand;
+      for (int G = 1; G <= 64; G *= 2)
+        if ((pop_[t].comp + G - 1) / G <= SMALL_KMAX && (64 / G) * pop_[t].comp <= SMALL_LDS) cand.push_back(G);
+      double best = 1e30;
+      int bestG = pop_[t].G;
+      for (int G : cand) {
+        set_lanes(t, G);
+        launch_chunk(std::min(trial, CHUNK), false);   // warm
+        restore(snap);
+        CB_HIP(hipStreamSynchronize(st_));
+        const auto t0 = std::chrono::steady_clock::now();
+        launch_chunk(std::min(trial, CHUNK), false);
+        CB_HIP(hipStreamSynchronize(st_));
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        restore(snap);
+        CB_HIP(hipStreamSynchronize(st_));
+        tune_.push_back({t, G, sec});
+        if (sec < best) {
+          best = sec;
+          bestG = G;
+        }
+      }
+      set_lanes(t, bestG);
+    }
+    release(snap);
+    timed_pending_ = false;
+  }
+  struct TuneRec {
+    int type, lanes;
+    double sec;
+  };
+  std::vector<TuneRec> tune_;
+
+  void kernel_time(double *ms, long *launches, bool reset) override {
+    *ms = kt_ms_;
+    *launches = kt_n_;
+    if (reset) {
+      kt_ms_ = 0.0;
+      kt_n_ = 0;
+    }
+  }
+
+ private:
+  static constexpr int CHUNK = 64;   // intervals per captured graph
+
+  struct Small {
+    std::vector<void *> mut;         // device buffers a run changes, with sizes below
+    std::vector<size_t> mut_bytes;
+  };
+
+  template <typename U>
+  U *put(const std::vector<U> &h) {
+    U *d = dalloc<U>(h.size());
+    h2d(d, h);
+    bufs_.push_back((void *)d);
+    return d;
+  }
+  template <typename U>
+  U *zeros(size_t n) {
+    U *d = dalloc<U>(n);
+    CB_HIP(hipMemset(d, 0, sizeof(U) * (n ? n : 1)));
+    bufs_.push_back((void *)d);
+    return d;
+  }
+  template <typename U>
+  U *mut(const std::vector<U> &h, int t) {
+    U *d = put(h);
+    small_[t].mut.push_back((void *)d);
+    small_[t].mut_bytes.push_back(sizeof(U) * h.size());
+    return d;
+  }
+
+  // stencil, RKC state and controller records common to the small types
+  template <typename Host>
+  void common_small(int t, const Host &H, double atol, double rtol, double hmin, bool strict) {
+    SmallPop &P = pop_[t];
+    const int n = H.n, C = H.comp, nc = H.nc;
+    P.n = n;
+    P.comp = C;
+    std::vector<int> nb((size_t)nc * SMALL_NB, -1);
+    std::vector<double> nbg((size_t)nc * SMALL_NB, 0.0), invc(nc);
+    for (int gi = 0; gi < nc; gi++) {
+      const int cell = gi / C;
+      int q = 0;
+      for (int e = H.row[gi]; e < H.row[gi + 1]; e++) {
+        if (H.col[e] == gi) continue;
+        if (H.col[e] / C != cell) throw Error("intercell cable entries are not provided by the batched engine");
+        if (q >= SMALL_NB) throw Error("a compartment with more than 4 cable neighbours");
+        nb[(size_t)gi * SMALL_NB + q] = H.col[e] - cell * C;
+        nbg[(size_t)gi * SMALL_NB + q] = H.val[e];
+        q++;
+      }
+      invc[gi] = 1.0 / H.cm[gi];
+    }
+    P.nb = put(nb);
+    P.nbg = put(nbg);
+    P.invc = put(invc);
+    P.y0 = mut(H.v, t);
+    P.f0 = mut(std::vector<double>(nc, 0.0), t);
+    P.vy = mut(H.v, t);
+    P.ede = mut(std::vector<double>(nc, 0.0), t);
+    P.vnew = mut(H.v, t);
+    std::vector<double> ctl((size_t)n * NCTL, 0.0);
+    for (int c = 0; c < n; c++) {
+      double *q = &ctl[(size_t)c * NCTL];
+      q[C_TEND] = spec_.interval;
+      q[C_LAST] = 1;
+      q[C_NEWSPC] = 1;
+      q[C_MMAX] = nint_small(std::sqrt(rtol / (10.0 * UROUND)));
+    }
+    P.ctl = mut(ctl, t);
+    P.atol = atol;
+    P.rtol = rtol;
+    P.hmin = hmin;
+    P.hmax = spec_.interval;
+    P.interval = spec_.interval;
+    P.integ = std::pow(2.0, std::log10(atol));   // solve_rkc.cu:1141, 1205, 1340
+    P.strict = strict ? 1 : 0;
+    P.refresh_fn = 0;
+    const MixPopSpec &ps = spec_.pop[t];
+    if (ps.rec_comp < 0 || ps.rec_comp >= C || ps.stim_comp < 0 || ps.stim_comp >= C)
+      throw Error("recorded or stimulated compartment out of range");
+    P.rec_comp = ps.rec_comp;
+    P.thr = ps.thr;
+    P.vprev = mut(std::vector<double>(n, 100.0), t);   // main.cu:162-169
+    P.gj = nullptr;
+    P.G = 1;
+    while ((C + P.G - 1) / P.G > SMALL_KMAX) P.G *= 2;
+    if (P.G > 64) throw Error("cell too large for the small-cell mapping (at most 256 compartments)");
+    if ((64 / P.G) * C > SMALL_LDS) throw Error("cells too large for the wave's stage buffer");
+  }
+
+  std::vector<double> stim_vec(int t, int nc, int C) {
+    std::vector<double> iext(nc, 0.0);
+    const MixPopSpec &ps = spec_.pop[t];
+    for (int c = 0; c < ps.n && c < (int)ps.amp.size(); c++) iext[(size_t)c * C + ps.stim_comp] = ps.amp[c];
+    return iext;
+  }
+
+  void build_small(int t) {
+    const MixPopSpec &ps = spec_.pop[t];
+    std::string err;
+    SmallPop &P = pop_[t];
+    P.type = t == T_GR ? SM_GR : t == T_GO ? SM_GO : SM_IO;
+    if (t == T_GR) {
+      GrHost H;
+      if (!build_gr_host(ps.morph, ps.n, 1, ps.v0.data(), H, err, ps.comp > 0 ? ps.comp : gr::COMP)) throw Error(err);
+      common_small(t, H, 1.0e-9, 1.0e-9, std::pow(2.0, -12.0), true);   // TOL_GR, H_MIN_GR (solve_rkc.cuh:24, 28)
+      GrState &S = P.gr;
+      S = GrState{};
+      S.v = mut(H.v, t);
+      S.iext = put(stim_vec(t, H.nc, H.comp));
+      S.ca = put(H.ca);
+      S.compart = put(H.compart);
+      S.cond = put(H.cond);
+      S.gate = mut(H.gate, t);
+      const gr::NaTable tab = gr::na_table();
+      S.na = put(std::vector<gr::NaType>(tab.t, tab.t + gr::NTYPE));
+      S.b = mut(std::vector<double>(H.nc, 0.0), t);
+      S.nc = H.nc;
+      P.bb = S.b;
+    } else if (t == T_GO) {
+      GoHost H;
+      if (!build_go_host(ps.morph, ps.n, 1, ps.v0.data(), H, err, go::COMP, false)) throw Error(err);
+      common_small(t, H, 1.0e-6, 1.0e-6, std::pow(2.0, -10.0), false);  // TOL_GO, H_MIN_GO
+      GoState &S = P.go;
+      S = GoState{};
+      S.v = mut(H.v, t);
+      S.ca = mut(H.ca, t);
+      S.ca2 = mut(H.ca2, t);
+      S.rev = mut(H.rev, t);
+      S.ca_old = mut(H.ca_old, t);
+      S.iext = put(stim_vec(t, H.nc, H.comp));
+      S.area = put(H.area);
+      S.cond = put(H.cond);
+      S.gate = mut(H.gate, t);
+      S.b = mut(std::vector<double>(H.nc, 0.0), t);
+      S.nc = H.nc;
+      P.bb = S.b;
+    } else {
+      IoHost H;
+      if (!build_io_host(ps.morph, ps.n, 1, ps.v0.data(), H, err)) throw Error(err);
+      common_small(t, H, 1.0e-6, 1.0e-6, std::pow(2.0, -10.0), true);   // TOL_IO, H_MIN_IO
+      IoState &S = P.io;
+      S = IoState{};
+      S.v = mut(H.v, t);
+      S.ca = mut(H.ca, t);
+      S.iext = put(stim_vec(t, H.nc, H.comp));
+      S.cond = put(H.cond);
+      S.gate = mut(H.gate, t);
+      S.compart = put(H.compart);
+      S.b = mut(std::vector<double>(H.nc, 0.0), t);
+      S.nc = H.nc;
+      S.n = H.n;
+      P.bb = S.b;
+    }
+  }
+
+  // the olive pair list as a CSR by cell, both directions, with each pair's
+  // conductance per unit dendritic membrane: g [nS] 1e-6 / (pi d L) of
+  // compartment 0 in IO.txt (the model's currents are per cm^2)
+  void build_io_gj() {
+    std::FILE *f = std::fopen(spec_.pop[T_IO].morph.c_str(), "r");
+    if (!f) throw Error("no such file " + spec_.pop[T_IO].morph);
+    double area0 = 0.0;
+    for (int k = 0; k < io::COMP; k++) {
+      int i1, i2, i3;
+      double d, len, cm;
+      if (std::fscanf(f, "%d %d %lf %lf %lf %d", &i1, &i2, &d, &len, &cm, &i3) != 6) {
+        std::fclose(f);
+        throw Error("IO_PARAM_FILE_READING_ERROR in " + spec_.pop[T_IO].morph);
+      }
+      if (i1 == 0) area0 = M_PI * d * 1e-4 * len * 1e-4;
+    }
+    std::fclose(f);
+    if (!(area0 > 0.0)) throw Error("olive compartment 0 has no membrane area");
+    std::FILE *g = std::fopen(spec_.io_gj.c_str(), "r");
+    if (!g) throw Error("no such file " + spec_.io_gj);
+    const int n = n_[T_IO];
+    std::vector<std::vector<std::pair<int, double>>> adj(n);
+    int a, b;
+    double gns;
+    while (std::fscanf(g, " %d , %d , %lf", &a, &b, &gns) == 3) {
+      if (a < 0 || b < 0 || a >= n || b >= n || a == b) {
+        std::fclose(g);
+        throw Error("gap-junction pair out of range in " + spec_.io_gj);
+      }
+      const double gpa = gns * 1e-6 / area0;
+      adj[a].push_back({b, gpa});
+      adj[b].push_back({a, gpa});
+    }
+    std::fclose(g);
+    std::vector<int> row(n + 1, 0), col;
+    std::vector<double> gv;
+    for (int c = 0; c < n; c++) {
+      for (auto &e : adj[c]) {
+        col.push_back(e.first);
+        gv.push_back(e.second);
+      }
+      row[c + 1] = (int)col.size();
+    }
+    io_row_ = put(row);
+    io_col_ = put(col.empty() ? std::vector<int>(1, 0) : col);
+    io_g_ = put(gv.empty() ? std::vector<double>(1, 0.0) : gv);
+    pop_[T_IO].gj = mut(std::vector<double>(n, 0.0), T_IO);
+    io_gj_on_ = true;
+  }
+
+  void set_lanes(int t, int G) {
+    pop_[t].G = G;
+    upload_pops();
+    graph_ready_ = false;
+  }
+
+  // the costliest items first: Golgi cells, then granule, then olive
+  void upload_pops() {
+    SmallPop host[3];
+    const int order[3] = {T_GO, T_GR, T_IO};
+    std::vector<int> ip, ic;
+    int slot = 0;
+    for (int t : order) {
+      if (!n_[t]) continue;
+      host[slot] = pop_[t];
+      const int W = 64 / pop_[t].G;
+      for (int c0 = 0; c0 < n_[t]; c0 += W) {
+        ip.push_back(slot);
+        ic.push_back(c0);
+      }
+      pop_slot_[t] = slot++;
+    }
+    CB_HIP(hipMemcpy(pops_dev_, host, sizeof(SmallPop) * 3, hipMemcpyHostToDevice));
+    upload_tables();
+    graph_ready_ = false;
+    items_pop_ = ip;
+    if (item_pop_dev_) {
+      CB_HIP(hipStreamSynchronize(st_));
+      (void)hipFree(item_pop_dev_);
+      (void)hipFree(item_cell_dev_);
+    }
+    item_pop_dev_ = dalloc<int>(std::max<size_t>(1, ip.size()));
+    item_cell_dev_ = dalloc<int>(std::max<size_t>(1, ic.size()));
+    if (!ip.empty()) {
+      CB_HIP(hipMemcpy(item_pop_dev_, ip.data(), sizeof(int) * ip.size(), hipMemcpyHostToDevice));
+      CB_HIP(hipMemcpy(item_cell_dev_, ic.data(), sizeof(int) * ic.size(), hipMemcpyHostToDevice));
+    }
+  }
+
+  MixArgs args(int s, double *const *rec, uint32_t *spk) {
+    MixArgs a;
+    a.pkj_on = pkj_ ? 1 : 0;
+    if (pkj_) {
+      a.pkj = pkj_->args();
+      a.pkj.rec = rec ? rec[T_PKJ] + (size_t)s * n_[T_PKJ] : nullptr;
+      a.pkj.rec_orig = spec_.pop[T_PKJ].rec_comp;
+      a.pkj.thr = spec_.pop[T_PKJ].thr;
+      a.pkj.vprev = pkj_vprev_;
+      a.pkj.spk = spk ? spk + (size_t)s * words_total() + word_off(T_PKJ) : nullptr;
+      a.pkj.spk_words = words_total();
+    }
+    a.pops = pops_dev_;
+    a.item_pop = item_pop_dev_;
+    a.item_cell = item_cell_dev_;
+    a.nitems = (int)items_pop_.size();
+    a.small_queue = small_queue_;
+    return a;
+  }
+
+  // the recording and spike pointers of one launch: the small populations
+  // take theirs from the device copy, so each interval's launch gets its own
+  // copy of the population table
+  void ensure_chunk_buffers() {
+    if (spk_) return;
+    for (int t = 0; t < 4; t++)
+      if (n_[t]) rec_[t] = zeros<double>((size_t)CHUNK * n_[t]);
+    spk_ = zeros<uint32_t>((size_t)CHUNK * words_total());
+    if (pkj_) pkj_vprev_ = put(std::vector<double>(n_[T_PKJ], 100.0));
+    pops_iv_ = dalloc<SmallPop>((size_t)3 * CHUNK);
+    bufs_.push_back(pops_iv_);
+    upload_tables();
+    graph_ready_ = false;
+  }
+
+  int words_total() const { return word_off(4); }
+
+  // the population tables of the CHUNK launches of a chunk, each with its
+  // interval's recording and spike pointers (uploaded once, so a captured
+  // graph holds kernels only)
+  void upload_tables() {
+    if (!spk_) return;
+    std::vector<SmallPop> tabs((size_t)3 * CHUNK);
+    for (int l = 0; l < CHUNK; l++)
+      for (int t : {T_GR, T_GO, T_IO}) {
+        if (!n_[t]) continue;
+        SmallPop P = pop_[t];
+        P.rec = rec_[t] + (size_t)l * n_[t];
+        P.spk = spk_ + (size_t)l * words_total() + word_off(t);
+        P.spk_words = words_total();
+        tabs[(size_t)l * 3 + pop_slot_[t]] = P;
+      }
+    CB_HIP(hipMemcpy(pops_iv_, tabs.data(), sizeof(SmallPop) * tabs.size(), hipMemcpyHostToDevice));
+  }
+
+  // one chunk of k intervals: per interval (with olive coupling) the gap
+  // currents, the two work counters cleared by a kernel and one mixed
+  // launch; without coupling one launch for the whole chunk.  With record,
+  // the launches write the chunk's recording and spike buffers.  The mix
+  // has no time-dependent input (constant currents, no Poisson drive), so a
+  // chunk's launches are the same for every chunk and can be replayed.
+  void enqueue(int k, bool record) {
+    const bool per = io_gj_on_;
+    const int nl = per ? k : 1;
+    if (record)
+      k_zero_u32<<<(unsigned)(((size_t)k * words_total() + 255) / 256), 256, 0, st_>>>(spk_, (size_t)k * words_total());
+    for (int l = 0; l < nl; l++) {
+      if (per)
+        k_io_gj<<<(unsigned)((n_[T_IO] + 255) / 256), 256, 0, st_>>>(io_row_, io_col_, io_g_, pop_[T_IO].vnew,
+                                                                     io::COMP, (double *)pop_[T_IO].gj, n_[T_IO]);
+      MixArgs a = args(l, record ? rec_ : nullptr, record ? spk_ : nullptr);
+      a.pops = record ? pops_iv_ + (size_t)l * 3 : pops_dev_;
+      if (pkj_) k_zero_int<<<1, 1, 0, st_>>>(a.pkj.queue);
+      k_zero_int<<<1, 1, 0, st_>>>(small_queue_);
+      hipLaunchKernelGGL((k_mix<NT, KP, false>), dim3(grid()), dim3(NT), lds_, st_, a, per ? 1 : k, 0.0, 0);
+      CB_HIP(hipGetLastError());
+    }
+  }
+
+  void launch_chunk(int k, bool record) {
+    if (record && k == CHUNK && use_graph_) {
+      if (!graph_ready_) {
+        if (graph_exec_) CB_HIP(hipGraphExecDestroy(graph_exec_));
+        graph_exec_ = nullptr;
+        hipGraph_t g = nullptr;
+        CB_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+        enqueue(k, record);
+        CB_HIP(hipStreamEndCapture(st_, &g));
+        CB_HIP(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0));
+        CB_HIP(hipGraphDestroy(g));
+        graph_ready_ = true;
+      }
+      CB_HIP(hipEventRecord(ev0_, st_));
+      CB_HIP(hipGraphLaunch(graph_exec_, st_));
+      CB_HIP(hipEventRecord(ev1_, st_));
+    } else {
+      CB_HIP(hipEventRecord(ev0_, st_));
+      enqueue(k, record);
+      CB_HIP(hipEventRecord(ev1_, st_));
+    }
+    timed_pending_ = true;
+  }
+
+ public:
+  // false: every chunk is enqueued launch by launch instead of replaying the
+  // captured graph (the same kernels and arguments)
+  void use_graph(bool on) override { use_graph_ = on; }
+
+ private:
+  unsigned grid() const {
+    const int per_cu = std::max(1, std::min(2, (int)(160 * 1024 / (lds_ + 1024))));
+    return (unsigned)(ncu_ * per_cu);
+  }
+
+  void collect_time() {
+    if (!timed_pending_) return;
+    float ms = 0.0f;
+    CB_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    kt_ms_ += ms;
+    kt_n_ += 1;
+    timed_pending_ = false;
+  }
+
+  void check_status() {
+    if (pkj_) {
+      std::vector<double> ctl((size_t)n_[T_PKJ] * NCTL);
+      CB_HIP(hipMemcpy(ctl.data(), pkj_->args().ctl, sizeof(double) * ctl.size(), hipMemcpyDeviceToHost));
+      for (int c = 0; c < n_[T_PKJ]; c++) {
+        const int s = (int)ctl[(size_t)c * NCTL + C_STATUS];
+        if (s) throw Error("RKC failed in Purkinje cell " + std::to_string(c) + ": status " + std::to_string(s), s);
+      }
+    }
+    const char *name[4] = {"granule", "Golgi", "Purkinje", "olive"};
+    for (int t : {T_GR, T_GO, T_IO}) {
+      if (!n_[t]) continue;
+      std::vector<double> ctl((size_t)n_[t] * NCTL);
+      CB_HIP(hipMemcpy(ctl.data(), pop_[t].ctl, sizeof(double) * ctl.size(), hipMemcpyDeviceToHost));
+      for (int c = 0; c < n_[t]; c++) {
+        const int s = (int)ctl[(size_t)c * NCTL + C_STATUS];
+        if (s == NONFINITE)
+          throw Error(std::string("RKC: non-finite state in ") + name[t] + " cell " + std::to_string(c), s);
+        if (s) throw Error(std::string("RKC failed in ") + name[t] + " cell " + std::to_string(c) + ": idid = " +
+                               std::to_string(s), s);
+      }
+    }
+  }
+
+  struct Snapshot {
+    std::vector<std::pair<void *, std::pair<void *, size_t>>> saved;
+  };
+  std::vector<std::pair<void *, size_t>> state_buffers() {
+    std::vector<std::pair<void *, size_t>> v;
+    if (pkj_) v = pkj_->state_buffers();
+    for (int t : {T_GR, T_GO, T_IO})
+      for (size_t q = 0; q < small_[t].mut.size(); q++) v.push_back({small_[t].mut[q], small_[t].mut_bytes[q]});
+    return v;
+  }
+  Snapshot snapshot() {
+    Snapshot s;
+    for (auto &m : state_buffers()) {
+      void *p = nullptr;
+      CB_HIP(hipMalloc(&p, m.second));
+      CB_HIP(hipMemcpyAsync(p, m.first, m.second, hipMemcpyDeviceToDevice, st_));
+      s.saved.push_back({m.first, {p, m.second}});
+    }
+    return s;
+  }
+  void restore(const Snapshot &s) {
+    for (auto &e : s.saved)
+      CB_HIP(hipMemcpyAsync(e.first, e.second.first, e.second.second, hipMemcpyDeviceToDevice, st_));
+  }
+  void release(Snapshot &s) {
+    CB_HIP(hipStreamSynchronize(st_));
+    for (auto &e : s.saved) (void)hipFree(e.second.first);
+    s.saved.clear();
+  }
+  static int nint_small(double x) {
+    if (x - (double)(int)x < 0.5) return (int)(double)(int)x;
+    return (int)(double)(int)(x + 0.9);
+  }
+
+  MixSpec spec_;
+  int n_[4] = {0, 0, 0, 0};
+  std::unique_ptr<CellPkj<float>> pkj_;
+  SmallPop pop_[4] = {};
+  Small small_[4];
+  int pop_slot_[4] = {0, 0, 0, 0};
+  SmallPop *pops_dev_ = nullptr, *pops_iv_ = nullptr;
+  std::vector<int> items_pop_;
+  int *item_pop_dev_ = nullptr, *item_cell_dev_ = nullptr;
+  int *small_queue_ = nullptr;
+  const int *io_row_ = nullptr, *io_col_ = nullptr;
+  const double *io_g_ = nullptr;
+  bool io_gj_on_ = false;
+  double *rec_[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t *spk_ = nullptr;
+  double *pkj_vprev_ = nullptr;
+  std::vector<void *> bufs_;
+  hipStream_t st_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  bool graph_ready_ = false;
+  bool use_graph_ = true;
+  bool timed_pending_ = false;
+  double kt_ms_ = 0.0;
+  long kt_n_ = 0;
+  long intervals_ = 0;
+  int ncu_ = 0;
+  size_t lds_ = 0;
+};
+
+
 }  // namespace
+
+Mix *make_mix(const MixSpec &spec) { return new MixEngine(spec); }
 
 Engine *make_cell_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn,
                       bool mixed) {

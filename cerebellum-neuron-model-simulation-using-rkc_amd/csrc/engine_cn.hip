@@ -42,6 +42,7 @@
 
 #include "detsum.hpp"
 #include "engine.hpp"
+#include "record.hpp"
 #include "go_model.hpp"
 #include "gr_model.hpp"
 #include "io_model.hpp"
@@ -1186,9 +1187,17 @@ class CnEngine : public Engine {
   // (main.cu:193-200: set_current(t + l_dt), one CN step, l_dt += DT while
   // l_dt < H_MAX); records compartment rec_comp of every cell after it
   void run(int nsteps, double t0, int rec_comp, double *rec_out) override {
-    std::vector<double> host((size_t)(rec_out ? nsteps : 0) * n_);
+    if (nsteps <= 0) return;
+    const int per = rec_comp >= 0 ? 1 : comp_;
+    const int words = (n_ + 31) / 32;
+    const bool raster = raster_comp_ >= 0;
     double *drec = nullptr;
-    if (rec_out) drec = dal<double>((size_t)nsteps * n_);
+    uint32_t *dbits = nullptr;
+    if (rec_out) drec = dal<double>((size_t)nsteps * n_ * per);
+    if (raster) {
+      dbits = dal<uint32_t>((size_t)nsteps * words);
+      CB_HIP(hipMemsetAsync(dbits, 0, sizeof(uint32_t) * (size_t)nsteps * words, st_));
+    }
     for (int k = 0; k < nsteps; k++) {
       const double t = t0 + k * H_MAX;
       if (nsub_ > 0) {
@@ -1206,15 +1215,20 @@ class CnEngine : public Engine {
         }
       }
       st.macro_steps++;
-      if (drec) k_cn_record<<<(n_ + 127) / 128, 128, 0, st_>>>(S_.v, drec + (size_t)k * n_, comp_, rec_comp, n_);
+      if (drec || dbits)
+        record_interval(st_, S_.v, drec ? drec + (size_t)k * n_ * per : nullptr, rec_comp,
+                        dbits ? dbits + (size_t)k * words : nullptr);
     }
-    if (drec) {
-      CB_HIP(hipMemcpyAsync(host.data(), drec, sizeof(double) * host.size(), hipMemcpyDeviceToHost, st_));
-      CB_HIP(hipStreamSynchronize(st_));
-      CB_HIP(hipFree(drec));
-      std::memcpy(rec_out, host.data(), sizeof(double) * host.size());
-    }
+    std::vector<uint32_t> bits((size_t)(raster ? nsteps : 0) * words);
+    if (drec)
+      CB_HIP(hipMemcpyAsync(rec_out, drec, sizeof(double) * (size_t)nsteps * n_ * per, hipMemcpyDeviceToHost, st_));
+    if (dbits) CB_HIP(hipMemcpyAsync(bits.data(), dbits, sizeof(uint32_t) * bits.size(), hipMemcpyDeviceToHost, st_));
+    CB_HIP(hipStreamSynchronize(st_));
+    if (drec) CB_HIP(hipFree(drec));
+    if (dbits) CB_HIP(hipFree(dbits));
+    if (raster) append_spikes(bits, nsteps, words, t0, H_MAX);
   }
+  const double *device_vnew() const override { return S_.v; }
   const double *device_v() const override { return S_.v; }
   hipStream_t stream() const override { return st_; }
   // add_*_val of the CN and BE paths (GR, GO, PKJ; the BE Purkinje path

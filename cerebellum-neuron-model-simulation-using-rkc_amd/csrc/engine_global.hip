@@ -17,6 +17,7 @@
 // header for the list).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 
 #include <cmath>
@@ -29,7 +30,9 @@
 #include "go_model.hpp"
 #include "gr_model.hpp"
 #include "io_model.hpp"
+#include "ion_state.hpp"
 #include "pkj_model.hpp"
+#include "record.hpp"
 
 namespace cbrkc {
 
@@ -244,17 +247,6 @@ __global__ void __launch_bounds__(256) k_pkj_ion_b(PkjState S, const double *__r
   S.b[i] = bb;
 }
 
-struct IoState {
-  double *v, *ca, *iext;
-  const double *cond;   // [NCOND][nc]
-  double *gate;         // [NGATE][nc]
-  const int8_t *compart;
-  const int *nbr;       // [n][4] gap-junction partner cell or -1
-  double *b;
-  const double *syn_g;  // may be null
-  double syn_e;
-  int nc, n;
-};
 
 // io_update_ion_RKC (io_ion.cu:90-127), io_update_matrix (solve_rkc.cu:412-434)
 // and, on each dendrite, the gap currents of io_gap_update / add_io_gap_val
@@ -266,21 +258,8 @@ __global__ void __launch_bounds__(256) k_io_ion_b(IoState S, const double *__res
   const int nc = S.nc;
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nc) return;
-  double g[NGATE], c[NCOND];
-#pragma unroll
-  for (int k = 0; k < NGATE; k++) g[k] = S.gate[(size_t)k * nc + i];
-#pragma unroll
-  for (int k = 0; k < NCOND; k++) c[k] = S.cond[(size_t)k * nc + i];
   const double vn = vnew[i];
-  if (do_ion) {
-    double ca = S.ca[i];
-    update_ion<double>(S.v[i], vn, ca, c[g_CaH], S.compart[i], DT, g);
-#pragma unroll
-    for (int k = 0; k < NGATE; k++) S.gate[(size_t)k * nc + i] = g[k];
-    S.ca[i] = ca;
-    S.v[i] = vn;
-  }
-  double bb = current<double>(vn, S.iext[i], c, g);
+  double bb = io_ion_current(S, i, vn, DT, do_ion != 0);
   if (do_ion && S.n > 1 && i % COMP == 0) {
     const int cell = i / COMP;
 #pragma unroll
@@ -294,17 +273,6 @@ __global__ void __launch_bounds__(256) k_io_ion_b(IoState S, const double *__res
   S.b[i] = bb;
 }
 
-struct GoState {
-  double *v, *ca, *ca2, *rev, *ca_old, *iext;
-  const double *area;
-  const double *cond;   // [NCOND][nc], density x area
-  double *gate;         // [NGATE][nc]
-  double *b;
-  const double *syn_g;  // may be null
-  double syn_e;
-  int nc;
-  const double *net_s;  // synaptic current of the macro step (granule input), may be null
-};
 
 // go_update_ion_RKC_exp_imp (go_ion.cu:379-437), go_KAHP_update_2order
 // (go_ion.cu:88-213) and go_update_matrix (solve_rkc.cu:288-314), fused: the
@@ -312,50 +280,17 @@ struct GoState {
 // (solve_rkc.cu:1174-1179).  The intercell gap junctions are in the cable
 // matrix, not here.
 __global__ void __launch_bounds__(256) k_go_ion_b(GoState S, const double *__restrict__ vnew, double DT, int do_ion) {
-  using namespace go;
   const int nc = S.nc;
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nc) return;
-  double g[NGATE], c[NCOND];
-#pragma unroll
-  for (int k = 0; k < NGATE; k++) g[k] = S.gate[(size_t)k * nc + i];
-#pragma unroll
-  for (int k = 0; k < NCOND; k++) c[k] = S.cond[(size_t)k * nc + i];
   const double vn = vnew[i];
-  double rev = S.rev[i];
-  if (do_ion) {
-    double ca = S.ca[i], ca2 = S.ca2[i];
-    update_ion(S.v[i], vn, S.area[i], c, ca, ca2, rev, DT, g);
-    const double ca_prev = S.ca_old[i];
-    kahp_step(ca_prev, ca, DT, g);
-#pragma unroll
-    for (int k = 0; k < NGATE; k++) S.gate[(size_t)k * nc + i] = g[k];
-    S.ca[i] = ca;
-    S.ca2[i] = ca2;
-    S.rev[i] = rev;
-    S.ca_old[i] = ca;
-    S.v[i] = vn;
-  }
-  double bb = current(vn, S.iext[i], rev, c, g);
+  double bb = go_ion_current(S, i, vn, DT, do_ion != 0);
   if (S.syn_g) bb += S.syn_g[i] * (vn - S.syn_e);
   // go_update_matrix_new (solve_rkc.cu:279-286, 1181)
   if (S.net_s) bb += S.net_s[i];
   S.b[i] = bb;
 }
 
-struct GrState {
-  double *v, *iext;
-  const double *ca;
-  const int8_t *compart;
-  const double *cond;   // [NCOND][nc], density x area
-  double *gate;         // [NGATE][nc]
-  const gr::NaType *na; // per compartment type
-  double *b;
-  const double *syn_g;  // may be null
-  double syn_e;
-  int nc;
-  SynIn net;            // mossy-fibre and Golgi input, per compartment
-};
 
 // gr_Na_update_2order (gr_ion.cu:45-207), gr_update_ion_RKC_exp_imp
 // (gr_ion.cu:461-480) and gr_update_matrix (solve_rkc.cu:226-249), fused in
@@ -363,31 +298,11 @@ struct GrState {
 // voltage of the previous ion update and the new one, then the gates move and
 // the stored voltage becomes the new one.
 __global__ void __launch_bounds__(256) k_gr_ion_b(GrState S, const double *__restrict__ vnew, double DT, int do_ion) {
-  using namespace gr;
   const int nc = S.nc;
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nc) return;
-  double g[NGATE], c[NCOND];
-#pragma unroll
-  for (int k = 0; k < NGATE; k++) g[k] = S.gate[(size_t)k * nc + i];
-#pragma unroll
-  for (int k = 0; k < NCOND; k++) c[k] = S.cond[(size_t)k * nc + i];
   const double vn = vnew[i];
-  if (do_ion) {
-    const double vo = S.v[i];
-    const int t = S.compart[i];
-    if (t >= HILL) {
-      const NaType T = S.na[t];
-      double x[12];
-      na_gather(g, x);
-      if (na_step(T, vo, vn, DT, x)) na_scatter(x, g);
-    }
-    update_gates(vo, vn, S.ca[i], DT, g);
-#pragma unroll
-    for (int k = 0; k < NGATE; k++) S.gate[(size_t)k * nc + i] = g[k];
-    S.v[i] = vn;
-  }
-  double bb = current(vn, S.iext[i], c, g);
+  double bb = gr_ion_current(S, i, vn, DT, do_ion != 0);
   if (S.syn_g) bb += S.syn_g[i] * (vn - S.syn_e);
   // add_mfgr_val / add_gogr_val (solve_rkc.cu:200-224, 1117-1118), one
   // synapse after the other
@@ -544,6 +459,7 @@ class GlobalRkc : public Engine {
   }
 
   const double *device_v() const override { return v_; }
+  const double *device_vnew() const override { return vec_[VNEW]; }
   hipStream_t stream() const override { return st_; }
 
   void read(int field, double *dst) override {
@@ -1080,16 +996,108 @@ void create_stream(hipStream_t *st) {
   CB_HIP(e);
 }
 
-void Engine::run(int nsteps, double t0, int rec_comp, double *rec_out) {
-  std::vector<double> buf(nc());
-  for (int k = 0; k < nsteps; k++) {
-    set_current(t0 + k * H_MAX);
-    advance();
-    if (rec_out) {
-      read(F_VNEW, buf.data());
-      for (int c = 0; c < ncell(); c++) rec_out[(size_t)k * ncell() + c] = buf[(size_t)c * (nc() / ncell()) + rec_comp];
-    }
+void Engine::set_raster(int comp, double thr) {
+  if (comp >= 0 && comp >= nc() / std::max(1, ncell())) throw Error("raster compartment out of range");
+  raster_comp_ = comp;
+  raster_thr_ = thr;
+  if (raster_prev_) {   // a new raster starts from the reference's 100 mV
+    std::vector<double> h(ncell(), 100.0);
+    CB_HIP(hipMemcpy(raster_prev_, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
   }
+}
+
+double *Engine::raster_prev() {
+  if (!raster_prev_) {
+    CB_HIP(hipMalloc(&raster_prev_, sizeof(double) * std::max(1, ncell())));
+    std::vector<double> h(ncell(), 100.0);
+    CB_HIP(hipMemcpy(raster_prev_, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+  }
+  return raster_prev_;
+}
+
+void Engine::free_raster() {
+  if (raster_prev_) (void)hipFree(raster_prev_);
+  raster_prev_ = nullptr;
+}
+
+int Engine::take_spikes(int cap, double *t_out, int *cell_out) {
+  const int k = (int)std::min<size_t>(spikes_.size(), (size_t)std::max(0, cap));
+  for (int i = 0; i < k; i++) {
+    t_out[i] = spikes_[i].first;
+    cell_out[i] = spikes_[i].second;
+  }
+  spikes_.erase(spikes_.begin(), spikes_.begin() + k);
+  return k;
+}
+
+void Engine::append_spikes(const std::vector<uint32_t> &bits, int nsteps, int words, double t0, double interval) {
+  const int n = ncell();
+  for (int k = 0; k < nsteps; k++)
+    for (int c = 0; c < n; c++)
+      if (bits[(size_t)k * words + (c >> 5)] >> (c & 31) & 1u) spikes_.push_back({t0 + (k + 1) * interval, c});
+}
+
+void Engine::record_interval(hipStream_t st, const double *v, double *rec, int rec_comp, uint32_t *bits) {
+  const int n = ncell();
+  k_rec_step<<<(unsigned)((n + 127) / 128), 128, 0, st>>>(v, nc() / std::max(1, n), n, rec_comp, rec, raster_comp_,
+                                                          raster_thr_, bits ? raster_prev() : nullptr, bits);
+  CB_HIP(hipGetLastError());
+}
+
+// The macro-step loop of main.cu:182-311 for one population: set_current and
+// advance per output interval, with the recorded compartment (rec_comp >= 0)
+// or every compartment (rec_comp < 0) and the raster bits kept on the device
+// and copied back once.
+void Engine::run(int nsteps, double t0, int rec_comp, double *rec_out) {
+  if (nsteps <= 0) return;
+  const double *v = device_vnew();
+  const int n = ncell(), per = rec_comp >= 0 ? 1 : nc() / std::max(1, n);
+  const bool raster = raster_comp_ >= 0;
+  if (!v) {   // an engine without a device output array: copy V per interval
+    if (raster) throw Error("device raster not provided by this engine");
+    std::vector<double> buf(nc());
+    for (int k = 0; k < nsteps; k++) {
+      set_current(t0 + k * H_MAX);
+      advance();
+      if (rec_out) {
+        read(F_VNEW, buf.data());
+        for (int c = 0; c < n; c++)
+          for (int q = 0; q < per; q++)
+            rec_out[((size_t)k * n + c) * per + q] = buf[(size_t)c * (nc() / n) + (rec_comp >= 0 ? rec_comp : q)];
+      }
+    }
+    return;
+  }
+  const int words = (n + 31) / 32;
+  double *drec = nullptr;
+  uint32_t *dbits = nullptr;
+  if (rec_out) CB_HIP(hipMalloc(&drec, sizeof(double) * (size_t)nsteps * n * per));
+  if (raster) {
+    CB_HIP(hipMalloc(&dbits, sizeof(uint32_t) * (size_t)nsteps * words));
+    CB_HIP(hipMemsetAsync(dbits, 0, sizeof(uint32_t) * (size_t)nsteps * words, stream()));
+  }
+  try {
+    for (int k = 0; k < nsteps; k++) {
+      set_current(t0 + k * H_MAX);
+      advance();
+      if (drec || dbits)
+        record_interval(stream(), device_vnew(), drec ? drec + (size_t)k * n * per : nullptr, rec_comp,
+                        dbits ? dbits + (size_t)k * words : nullptr);
+    }
+    std::vector<uint32_t> bits((size_t)(raster ? nsteps : 0) * words);
+    if (drec)
+      CB_HIP(hipMemcpyAsync(rec_out, drec, sizeof(double) * (size_t)nsteps * n * per, hipMemcpyDeviceToHost, stream()));
+    if (dbits)
+      CB_HIP(hipMemcpyAsync(bits.data(), dbits, sizeof(uint32_t) * bits.size(), hipMemcpyDeviceToHost, stream()));
+    CB_HIP(hipStreamSynchronize(stream()));
+    if (raster) append_spikes(bits, nsteps, words, t0, H_MAX);
+  } catch (...) {
+    if (drec) (void)hipFree(drec);
+    if (dbits) (void)hipFree(dbits);
+    throw;
+  }
+  if (drec) CB_HIP(hipFree(drec));
+  if (dbits) CB_HIP(hipFree(dbits));
 }
 
 Engine *make_global_pkj(const PkjHost &host, const SolverSpec &sol, const StimSpec &stim, const PoissonSpec &syn) {

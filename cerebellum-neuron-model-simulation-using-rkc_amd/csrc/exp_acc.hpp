@@ -98,7 +98,16 @@ __host__ __device__ inline double exp_acc(double x) {
 // spike-time shifts over hundreds of milliseconds, so the device follows a
 // correctly rounded exp rather than the device library's (faithful, not
 // correctly rounded) one.
+// CB_HOST_EMU_DEVICE / CB_HOST_EMU_HOST: the host build of the device code
+// used by the tests (a host stand-in for the HIP header, one thread per
+// lane) takes the device's or the host's exponential explicitly.
+#if defined(CB_HOST_EMU_DEVICE)
+static inline double mexp(double x) { return exp_acc(x); }
+#elif defined(CB_HOST_EMU_HOST)
+static inline double mexp(double x) { return ::exp(x); }
+#else
 __host__ inline double mexp(double x) { return ::exp(x); }
 __device__ inline double mexp(double x) { return exp_acc(x); }
+#endif
 
 }  // namespace cbrkc

@@ -47,48 +47,71 @@ NaTable na_table() {
 
 }  // namespace gr
 
-bool build_gr_host(const std::string &param_path, int nx, int ny, const double *v0, GrHost &H, std::string &err) {
+bool build_gr_host(const std::string &param_path, int nx, int ny, const double *v0, GrHost &H, std::string &err,
+                   int comp) {
   using namespace gr;
   std::FILE *f = std::fopen(param_path.c_str(), "r");
   if (!f) {
     err = "no such file " + param_path;
     return false;
   }
-  // gr.cu:29-58 / gr_solve.cu:20-35: idx parent diam len Ra Cm type + 10 densities
-  std::vector<double> rad(COMP, 0.0), len(COMP, 0.0), Ra(COMP, 0.0), cmd(COMP, 0.0), dens((size_t)COMP * NCOND, 0.0);
-  std::vector<int> compart(COMP, 0), l_connect(COMP, -1);
-  std::vector<char> seen(COMP, 0);
-  for (int i = 0; i < COMP; i++) {
-    int i1, i2, i3;
-    double d1, d2, d3, d4, x[NCOND];
-    if (std::fscanf(f, "%d %d %lf %lf %lf %lf %d %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf ", &i1, &i2, &d1, &d2, &d3,
-                    &d4, &i3, &x[0], &x[1], &x[2], &x[3], &x[4], &x[5], &x[6], &x[7], &x[8], &x[9]) != 17 ||
-        i1 < 0 || i1 >= COMP || i2 < -1 || i2 >= COMP || i3 < 0 || i3 >= NTYPE) {
-      std::fclose(f);
-      err = "GR_PARAM_FILE_READING_ERROR in " + param_path;
-      return false;
-    }
-    rad[i1] = 0.5 * d1 * 1e-4;
-    len[i1] = d2 * 1e-4;
-    Ra[i1] = d3 * 1e-3;
-    cmd[i1] = d4;
-    compart[i1] = i3;
-    seen[i1] = 1;
-    for (int q = 0; q < NCOND; q++) dens[(size_t)i1 * NCOND + q] = x[q];
-    l_connect[i] = i2;   // connected by line, gr_solve.cu:34
+  const int n = nx * ny, nc = n * comp;
+  if (comp < 1 || comp > COMP) {
+    std::fclose(f);
+    err = "granule compartment count out of range";
+    return false;
   }
+  // gr.cu:29-58 / gr_solve.cu:20-35: idx parent diam len Ra Cm type + 10
+  // densities; one block of comp rows for every cell, or one block per cell
+  // (idx and parent local to the block)
+  std::vector<double> rad, len, Ra, cmd, dens;
+  std::vector<int> compart, l_connect;
+  std::vector<char> seen;
+  int nblk = 0;
+  for (;; nblk++) {
+    const size_t o = (size_t)nblk * comp;
+    rad.resize(o + comp, 0.0), len.resize(o + comp, 0.0), Ra.resize(o + comp, 0.0), cmd.resize(o + comp, 0.0);
+    dens.resize((o + comp) * NCOND, 0.0);
+    compart.resize(o + comp, 0), l_connect.resize(o + comp, -1), seen.resize(o + comp, 0);
+    for (int i = 0; i < comp; i++) {
+      int i1, i2, i3;
+      double d1, d2, d3, d4, x[NCOND];
+      const int got = std::fscanf(f, "%d %d %lf %lf %lf %lf %d %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf ", &i1, &i2, &d1,
+                                  &d2, &d3, &d4, &i3, &x[0], &x[1], &x[2], &x[3], &x[4], &x[5], &x[6], &x[7], &x[8],
+                                  &x[9]);
+      if (got == EOF && i == 0 && nblk > 0) goto done;
+      if (got != 17 || i1 < 0 || i1 >= comp || i2 < -1 || i2 >= comp || i3 < 0 || i3 >= NTYPE) {
+        std::fclose(f);
+        err = "GR_PARAM_FILE_READING_ERROR in " + param_path;
+        return false;
+      }
+      rad[o + i1] = 0.5 * d1 * 1e-4;
+      len[o + i1] = d2 * 1e-4;
+      Ra[o + i1] = d3 * 1e-3;
+      cmd[o + i1] = d4;
+      compart[o + i1] = i3;
+      seen[o + i1] = 1;
+      for (int q = 0; q < NCOND; q++) dens[(o + i1) * NCOND + q] = x[q];
+      l_connect[o + i] = i2;   // connected by line, gr_solve.cu:34
+    }
+  }
+done:
   std::fclose(f);
-  for (int i = 0; i < COMP; i++)
+  if (nblk != 1 && nblk != n) {
+    err = "GR_PARAM_FILE_READING_ERROR: " + std::to_string(nblk) + " blocks of " + std::to_string(comp) +
+          " rows in " + param_path + " for " + std::to_string(n) + " cells";
+    return false;
+  }
+  for (size_t i = 0; i < (size_t)nblk * comp; i++)
     if (!seen[i]) {
       err = "GR_PARAM_FILE_READING_ERROR: compartment missing in " + param_path;
       return false;
     }
 
-  const int n = nx * ny, nc = n * COMP;
   H.n = n;
   H.nx = nx;
   H.ny = ny;
-  H.comp = COMP;
+  H.comp = comp;
   H.nc = nc;
   H.cm.assign(nc, 0.0);
   H.iext.assign(nc, 0.0);
@@ -98,60 +121,66 @@ bool build_gr_host(const std::string &param_path, int nx, int ny, const double *
   H.cond.assign((size_t)NCOND * nc, 0.0);
   H.gate.assign((size_t)NGATE * nc, 0.0);
   for (int c = 0; c < n; c++)
-    for (int k = 0; k < COMP; k++) {
-      const size_t i = (size_t)c * COMP + k;
-      const double a = 2.0 * M_PI * rad[k] * len[k];   // gr.cu:44-57
-      H.cm[i] = cmd[k] * a;
-      H.compart[i] = (int8_t)compart[k];
-      for (int q = 0; q < NCOND; q++) H.cond[(size_t)q * nc + i] = dens[(size_t)k * NCOND + q] * a;
+    for (int k = 0; k < comp; k++) {
+      const size_t i = (size_t)c * comp + k, o = (size_t)(nblk == 1 ? 0 : c) * comp + k;
+      const double a = 2.0 * M_PI * rad[o] * len[o];   // gr.cu:44-57
+      H.cm[i] = cmd[o] * a;
+      H.compart[i] = (int8_t)compart[o];
+      for (int q = 0; q < NCOND; q++) H.cond[(size_t)q * nc + i] = dens[o * NCOND + q] * a;
     }
   // cable conductances (gr_solve.cu:37-43), CSR per row in column order with
-  // the stored diagonal, RKC sign (gr_solve.cu:73-133)
-  std::vector<double> mat((size_t)COMP * COMP, 0.0);
-  for (int i = 0; i < COMP; i++) {
-    const int d = l_connect[i];
-    if (d >= 0) {
-      mat[d + (size_t)COMP * i] =
-          (2.0 / ((Ra[i] * len[i]) / (rad[i] * rad[i] * M_PI) + (Ra[d] * len[d]) / (rad[d] * rad[d] * M_PI)));
-      mat[i + (size_t)COMP * d] = mat[d + (size_t)COMP * i];
-    }
-  }
-  std::vector<int> rcol;
+  // the stored diagonal, RKC sign (gr_solve.cu:73-133), one block at a time
+  H.row.assign(nc + 1, 0);
+  H.col.clear();
+  H.val.clear();
+  std::vector<double> mat((size_t)comp * comp, 0.0);
+  std::vector<int> rcol, rptr(comp + 1, 0);
   std::vector<double> rval;
-  std::vector<int> rptr(COMP + 1, 0);
-  for (int r = 0; r < COMP; r++) {
-    double dsum = 0.0;
-    for (int c = 0; c < COMP; c++)
-      if (c != r) dsum += mat[c + (size_t)COMP * r];
-    for (int c = 0; c < COMP; c++) {
-      const double m = mat[c + (size_t)COMP * r];
-      if (m > 0.0) {
-        rcol.push_back(c);
-        rval.push_back(m);
-      } else if (c == r) {
-        rcol.push_back(c);
-        rval.push_back(-dsum);
+  for (int cell = 0; cell < n; cell++) {
+    const size_t ob = (size_t)(nblk == 1 ? 0 : cell) * comp;
+    if (cell == 0 || nblk > 1) {
+      std::fill(mat.begin(), mat.end(), 0.0);
+      for (int i = 0; i < comp; i++) {
+        const int d = l_connect[ob + i];
+        if (d >= 0) {
+          const size_t pi = ob + i, pd = ob + d;
+          mat[d + (size_t)comp * i] = (2.0 / ((Ra[pi] * len[pi]) / (rad[pi] * rad[pi] * M_PI) +
+                                              (Ra[pd] * len[pd]) / (rad[pd] * rad[pd] * M_PI)));
+          mat[i + (size_t)comp * d] = mat[d + (size_t)comp * i];
+        }
+      }
+      rcol.clear();
+      rval.clear();
+      for (int r = 0; r < comp; r++) {
+        double dsum = 0.0;
+        for (int c = 0; c < comp; c++)
+          if (c != r) dsum += mat[c + (size_t)comp * r];
+        for (int c = 0; c < comp; c++) {
+          const double m = mat[c + (size_t)comp * r];
+          if (m > 0.0) {
+            rcol.push_back(c);
+            rval.push_back(m);
+          } else if (c == r) {
+            rcol.push_back(c);
+            rval.push_back(-dsum);
+          }
+        }
+        rptr[r + 1] = (int)rcol.size();
       }
     }
-    rptr[r + 1] = (int)rcol.size();
-  }
-  const size_t nnz1 = rcol.size();
-  H.row.assign(nc + 1, 0);
-  H.col.resize(nnz1 * n);
-  H.val.resize(nnz1 * n);
-  for (int cell = 0; cell < n; cell++) {
-    for (size_t j = 0; j < nnz1; j++) {
-      H.col[cell * nnz1 + j] = rcol[j] + cell * COMP;
-      H.val[cell * nnz1 + j] = rval[j];
+    const size_t base = H.col.size();
+    for (size_t j = 0; j < rcol.size(); j++) {
+      H.col.push_back(rcol[j] + cell * comp);
+      H.val.push_back(rval[j]);
     }
-    for (int r = 0; r < COMP; r++) H.row[cell * COMP + r + 1] = (int)(cell * nnz1 + rptr[r + 1]);
+    for (int r = 0; r < comp; r++) H.row[cell * comp + r + 1] = (int)(base + rptr[r + 1]);
   }
 
   // initial state, gr_ion.cu:635-672
   double init_v_rand = 0.0;
   for (int i = 0; i < nc; i++) {
-    if (i % COMP == 0 && !v0) init_v_rand = (((double)std::rand() + 1.0) / ((double)RAND_MAX + 2.0)) - 0.5;
-    const double vv = v0 ? v0[i / COMP] : V_INIT + 2.0 * init_v_rand;
+    if (i % comp == 0 && !v0) init_v_rand = (((double)std::rand() + 1.0) / ((double)RAND_MAX + 2.0)) - 0.5;
+    const double vv = v0 ? v0[i / comp] : V_INIT + 2.0 * init_v_rand;
     H.v[i] = vv;
     auto gate = [&](int q) -> double & { return H.gate[(size_t)q * nc + i]; };
     gate(n_KV) = inf_n_KV(vv);

@@ -145,7 +145,6 @@ int main(int argc, char *argv[]) {
   const int ngr = nx[CB_GR] * ny[CB_GR];
   cb_pop *grp = nullptr;
   if (ngr > 0) check(init_pop(tree, network, CB_GR, solver[CB_GR], nx[CB_GR], ny[CB_GR], argv[1], &grp), "gr_initialize");
-  std::vector<double> gr_v(ngr > 0 ? (size_t)ngr * 578 : 0), gr_prev(ngr, 100.0);
   const int ngo = nx[CB_GO] * ny[CB_GO];
   cb_pop *gop = nullptr;
   if (ngo > 0) check(init_pop(tree, network, CB_GO, solver[CB_GO], nx[CB_GO], ny[CB_GO], argv[4], &gop), "go_initialize");
@@ -173,7 +172,6 @@ int main(int argc, char *argv[]) {
   const int nio = nx[CB_IO] * ny[CB_IO];
   cb_pop *iop = nullptr;
   if (nio > 0) check(init_pop(tree, network, CB_IO, solver[CB_IO], nx[CB_IO], ny[CB_IO], argv[10], &iop), "io_initialize");
-  std::vector<double> io_v((size_t)nio * 3), io_prev(nio, 100.0);
   std::printf("\n");
 
   // create synapse (main.cu:139-145): each Poisson / jittered set draws its
@@ -209,10 +207,51 @@ int main(int argc, char *argv[]) {
   }
   if (network) chunk = 1;   // the synapses are updated between macro steps
 
+  // the rasters of xx_output_file are found on the device (cb_pop_set_raster):
+  // granule compartment 77 above +10 mV (gr.cu:268-276), Golgi 30 above +10
+  // (go.cu:284-292), Purkinje 1599 above -10 (pkj.cu:403-409), olive 2 above
+  // +10 (io.cu:276-284)
+  if (grp) check(cb_pop_set_raster(grp, 77, 10.0), "gr raster");
+  if (gop) check(cb_pop_set_raster(gop, 30, 10.0), "go raster");
+  if (pkj) check(cb_pop_set_raster(pkj, 1599, -10.0), "pkj raster");
+  if (iop) check(cb_pop_set_raster(iop, 2, 10.0), "io raster");
+
+  // one population's chunk of k output intervals: the recorded values come
+  // back once per chunk (per = values per cell and row), the raster from the
+  // device spike list; rows and raster lines in the reference's formats
+  auto chunk_out = [&](cb_pop *p, const char *what, int k, double t, int ncell, int per, int rec_comp,
+                       int nan_index, std::FILE *fv_, std::FILE *fr_, std::vector<double> &rec) -> bool {
+    rec.assign((size_t)k * ncell * per, 0.0);
+    check(cb_pop_run(p, k, t, rec_comp, rec.data()), what);
+    std::vector<double> st(256);
+    std::vector<int> sc(256);
+    std::vector<std::pair<double, int>> spikes;
+    for (;;) {
+      int got = 0;
+      check(cb_pop_spikes(p, (int)st.size(), st.data(), sc.data(), &got), "spikes");
+      for (int q = 0; q < got; q++) spikes.push_back({st[q], sc[q]});
+      if (got < (int)st.size()) break;
+    }
+    size_t next = 0;
+    for (int s = 0; s < k; s++) {
+      const double ts_ = t + (s + 1) * H_MAX;
+      const double *row = rec.data() + (size_t)s * ncell * per;
+      if (std::isnan(row[nan_index])) return false;
+      std::fprintf(fv_, "%lf,", ts_);
+      for (size_t c = 0; c < (size_t)ncell * per; c++) std::fprintf(fv_, "%lf,", row[c]);
+      std::fprintf(fv_, "\n");
+      while (next < spikes.size() && spikes[next].first <= ts_) {
+        std::fprintf(fr_, "%lf,%d\n", spikes[next].first, spikes[next].second);
+        next++;
+      }
+    }
+    return true;
+  };
+
   // the macro-step loop of main.cu:182-311 for the populations this build
-  // provides; the recorded compartment of every cell comes back once per
-  // chunk of macro steps instead of all of V once per step
-  std::vector<double> rec, raster_prev(n, 100.0), go_rec, go_prev(ngo, 100.0);
+  // provides, in the reference's order (GR, GO, PKJ, IO); each population
+  // advances a chunk of macro steps per call
+  std::vector<double> rec;
   const auto start = std::chrono::steady_clock::now();
   auto half = start;
   const long total = (long)std::floor((ts - 0.0001) / H_MAX) + 1;   // while (t < TS - 0.0001)
@@ -220,97 +259,26 @@ int main(int argc, char *argv[]) {
   double t = 0.0;
   while (done < total) {
     const int k = (int)std::min<long>(chunk, total - done);
-    // gr_output_file (gr.cu:256-277): V of compartment 16 per cell, raster on
-    // the upward crossing of +10 mV at compartment 77
-    for (int s = 0; s < k && ngr > 0; s++) {
-      const double ts_ = t + (s + 1) * H_MAX;
-      for (double l_dt = 0.0; l_dt < H_MAX; l_dt += dt_of(solver[CB_GR])) {
-        if (!network) check(cb_gr_set_current(grp, ts_ - H_MAX + l_dt), "gr_set_current");
-        check(cb_gr_solve_update_v(grp), "gr_solve_update_v");
-      }
-      check(cb_pop_read(grp, CB_F_VNEW, gr_v.data()), "gr read");
-      if (std::isnan(gr_v[1])) {
-        std::printf("isnan in gr, time=%f\n", ts_ - H_MAX);
-        return 1;
-      }
-      std::fprintf(fv[CB_GR], "%lf,", ts_);
-      for (int c = 0; c < ngr; c++) std::fprintf(fv[CB_GR], "%lf,", gr_v[16 + (size_t)c * 578]);
-      std::fprintf(fv[CB_GR], "\n");
-      for (int c = 0; c < ngr; c++) {
-        const double v = gr_v[77 + (size_t)c * 578];
-        if (v > 10.0 && gr_prev[c] <= 10.0) std::fprintf(fr[CB_GR], "%lf,%d\n", ts_, c);
-        gr_prev[c] = v;
-      }
+    // gr_output_file (gr.cu:256-277): V of compartment 16 per cell
+    if (ngr > 0 && !chunk_out(grp, "gr_solve_update_v", k, t, ngr, 1, 16, 0, fv[CB_GR], fr[CB_GR], rec)) {
+      std::printf("isnan in gr, time=%f\n", t);
+      return 1;
     }
-    // go_output_file (go.cu:270-293): V of compartment 30 per cell, raster on
-    // its upward crossing of +10 mV
-    if (ngo > 0) {
-      go_rec.assign((size_t)k * ngo, 0.0);
-      check(cb_pop_run(gop, k, t, 30, go_rec.data()), "go_solve_update_v");
-      for (int s = 0; s < k; s++) {
-        const double ts_ = t + (s + 1) * H_MAX;
-        std::fprintf(fv[CB_GO], "%lf,", ts_);
-        for (int c = 0; c < ngo; c++) {
-          const double v = go_rec[(size_t)s * ngo + c];
-          if (std::isnan(v)) {
-            std::printf("isnan in go, time=%f\n", ts_ - H_MAX);
-            return 1;
-          }
-          std::fprintf(fv[CB_GO], "%lf,", v);
-        }
-        std::fprintf(fv[CB_GO], "\n");
-        for (int c = 0; c < ngo; c++) {
-          const double v = go_rec[(size_t)s * ngo + c];
-          if (v > 10.0 && go_prev[c] <= 10.0) std::fprintf(fr[CB_GO], "%lf,%d\n", ts_, c);
-          go_prev[c] = v;
-        }
-      }
+    // go_output_file (go.cu:270-293): V of compartment 30 per cell
+    if (ngo > 0 && !chunk_out(gop, "go_solve_update_v", k, t, ngo, 1, 30, 0, fv[CB_GO], fr[CB_GO], rec)) {
+      std::printf("isnan in go, time=%f\n", t);
+      return 1;
     }
-    if (n > 0) {
-      rec.assign((size_t)k * n, 0.0);
-      check(cb_pop_run(pkj, k, t, 1599, rec.data()), "pkj_solve_update_v");
-      // pkj_output_file (pkj.cu:370-412): V row of compartment 1599 per cell,
-      // raster row per upward crossing of -10 mV (previous value starts at 100)
-      for (int s = 0; s < k; s++) {
-        const double ts_ = t + (s + 1) * H_MAX;
-        std::fprintf(fv[CB_PKJ], "%lf,", ts_);
-        for (int c = 0; c < n; c++) {
-          const double v = rec[(size_t)s * n + c];
-          if (std::isnan(v)) {
-            std::printf("isnan in pkj, time=%f\n", ts_ - H_MAX);
-            return 1;
-          }
-          std::fprintf(fv[CB_PKJ], "%lf,", v);
-        }
-        std::fprintf(fv[CB_PKJ], "\n");
-        for (int c = 0; c < n; c++) {
-          const double v = rec[(size_t)s * n + c];
-          if (v > -10.0 && raster_prev[c] <= -10.0) std::fprintf(fr[CB_PKJ], "%lf,%d\n", ts_, c);
-          raster_prev[c] = v;
-        }
-      }
+    // pkj_output_file (pkj.cu:370-412): V of compartment 1599 per cell
+    if (n > 0 && !chunk_out(pkj, "pkj_solve_update_v", k, t, n, 1, 1599, 0, fv[CB_PKJ], fr[CB_PKJ], rec)) {
+      std::printf("isnan in pkj, time=%f\n", t);
+      return 1;
     }
-    // io_output_file (io.cu:259-285): all three compartments of every cell,
-    // raster on compartment 2 through +10 mV
-    for (int s = 0; s < k && nio > 0; s++) {
-      const double ts_ = t + (s + 1) * H_MAX;
-      for (double l_dt = 0.0; l_dt < H_MAX; l_dt += dt_of(solver[CB_IO])) {
-        if (!network) check(cb_io_set_current(iop, ts_ - H_MAX + l_dt), "io_set_current");
-        check(cb_io_solve_update_v(iop), "io_solve_update_v");
-      }
-      check(cb_pop_read(iop, CB_F_VNEW, io_v.data()), "io read");
-      if (std::isnan(io_v[1])) {
-        std::printf("isnan,time=%f\n", ts_ - H_MAX);
-        return 1;
-      }
-      std::fprintf(fv[CB_IO], "%lf,", ts_);
-      for (double v : io_v) std::fprintf(fv[CB_IO], "%lf,", v);
-      std::fprintf(fv[CB_IO], "\n");
-      for (int c = 0; c < nio; c++) {
-        const double v = io_v[2 + (size_t)c * 3];
-        if (v > 10.0 && io_prev[c] <= 10.0) std::fprintf(fr[CB_IO], "%lf,%d\n", ts_, c);
-        io_prev[c] = v;
-      }
+    // io_output_file (io.cu:259-285): all three compartments of every cell;
+    // the reference tests compartment 1 of cell 0 for NaN
+    if (nio > 0 && !chunk_out(iop, "io_solve_update_v", k, t, nio, 3, CB_REC_ALL, 1, fv[CB_IO], fr[CB_IO], rec)) {
+      std::printf("isnan,time=%f\n", t);
+      return 1;
     }
     // Update synapses (main.cu:302-306), after every population's macro step
     if (network) {

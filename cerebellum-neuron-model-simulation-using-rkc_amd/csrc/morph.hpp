@@ -92,7 +92,11 @@ struct GoHost {
 // Reads a Go_131.txt-format file and builds nx * ny cells; v0 per cell, or
 // nullptr for -60 + 2 (U - 0.5) from libc rand() per cell in order
 // (go_ion.cu:597-600).
-bool build_go_host(const std::string &param_path, int nx, int ny, const double *v0, GoHost &out, std::string &err);
+// comp: compartments per cell (the reference's 131 by default); the file
+// holds one block of comp rows for all cells or one block per cell;
+// grid_gj = false leaves out the grid gap junctions.
+bool build_go_host(const std::string &param_path, int nx, int ny, const double *v0, GoHost &out, std::string &err,
+                   int comp = 131, bool grid_gj = true);
 
 // Granule-cell population on the host: Gr.txt (idx parent diam len Ra Cm type
 // + 10 channel densities, gr.cu:29-58), conductances and Cm as density x
@@ -111,7 +115,10 @@ struct GrHost {
 // Reads a Gr.txt-format file and builds nx * ny cells; v0 per cell, or
 // nullptr for -80 + 2 ((rand() + 1) / (RAND_MAX + 2) - 0.5) from libc rand()
 // per cell in order (gr_ion.cu:642-644).
-bool build_gr_host(const std::string &param_path, int nx, int ny, const double *v0, GrHost &out, std::string &err);
+// comp: compartments per cell (the reference's 578 by default); the file
+// holds one block of comp rows for all cells or one block per cell.
+bool build_gr_host(const std::string &param_path, int nx, int ny, const double *v0, GrHost &out, std::string &err,
+                   int comp = 578);
 
 // Spectral radius of each cell's cable operator C^-1 L (the RKC Jacobian
 // with the ionic current frozen), from the CSR rows of the cell without

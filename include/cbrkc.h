@@ -128,10 +128,28 @@ int cb_pop_advance(cb_pop *p);
 /* nsteps output intervals of H_MAX from t0 (RKC: set_current(t0 + k H_MAX);
  * advance; CN / BE: the five 0.025 ms sub-steps of main.cu:193-200, each with
  * set_current(t + l_dt)), recording compartment rec_comp of every cell after
- * each interval into rec_out[k * ncell + c] (rec_out may be NULL).  The main
- * loop of main.cu:182-311 for one population, without a host round trip per
- * step where the engine allows. */
+ * each interval into rec_out[k * ncell + c] (rec_out may be NULL), or, with
+ * rec_comp = CB_REC_ALL, every compartment into rec_out[k * nc + i] (the
+ * olive's output row, io.cu:259-285; not the per-cell engines).  The main
+ * loop of main.cu:182-311 for one population: the recording (and the raster,
+ * see cb_pop_set_raster) stay on the device and are copied back once per
+ * call instead of all of V after every step (main.cu:263-273). */
+#define CB_REC_ALL (-1)
 int cb_pop_run(cb_pop *p, int nsteps, double t0, int rec_comp, double *rec_out);
+
+/* Raster on the device: from now on every cb_pop_run finds, after each output
+ * interval, the upward crossings of thr [mV] by compartment comp of every cell
+ * (previous value 100 mV at the start, main.cu:162-169) and appends (time of
+ * the interval's end, cell) to the population's spike list; comp < 0 stops
+ * it.  The per-cell engines detect on the recorded compartment, so a run of
+ * theirs that records must record comp.  Replaces the host-side raster of
+ * xx_output_file (pkj.cu:403-409, gr.cu:268-276, go.cu:284-292,
+ * io.cu:276-284). */
+int cb_pop_set_raster(cb_pop *p, int comp, double thr);
+
+/* Moves up to cap spikes out of the population's list, in time order and
+ * cells ascending within a time: t_out[i], cell_out[i]; *n_out = how many. */
+int cb_pop_spikes(cb_pop *p, int cap, double *t_out, int *cell_out, int *n_out);
 
 /* cb_pop_run without any device allocation or free, for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops, harnesses that
@@ -318,6 +336,80 @@ int cb_syn_read(const cb_syn *s, int field, double *dst);
  * weight NULL only *n is set; otherwise they receive *n entries. */
 int cb_syn_build(int kind, int nx_a, int ny_a, int nx_b, int ny_b, const double *x1600, uint64_t seed, int *n,
                  int *pre, int *post, double *weight);
+
+/* ---- batched engine over an uneven mix of cell types ----
+ * Granule, Golgi, Purkinje and inferior-olive cells advanced together, each
+ * cell on its own adaptive RKC step (the reference's algorithm applied to a
+ * population of one cell, solve_rkc.cu:562-1362), all of them in one kernel
+ * per launch from device work counters: a workgroup per Purkinje cell
+ * (mixed precision, as CB_ENGINE_CELL_MIXED), G lanes per granule / Golgi /
+ * olive cell (FP64; 64 / G cells per wavefront).  Olive cells exchange
+ * gap-junction currents once per interval (each pair's current
+ * g (0.8 exp(-0.01 dV^2) + 0.2) dV, the reference's junction shape of
+ * gap.cu:149-160 with a per-pair conductance, frozen over the interval);
+ * chunks of intervals run as a captured HIP graph.  Constant per-cell
+ * currents, no synapses.  No counterpart in the reference, which advances
+ * one population of one type per call (main.cu:182-311). */
+typedef struct cb_mix cb_mix;
+typedef struct cb_mix_pop {
+  int n;                    /* cells (0: none of this type) */
+  const char *morph_path;   /* the type's file format (Gr.txt, Go_131.txt, Pkj.txt, IO.txt): one block of
+                               rows per cell, or one block for all */
+  int comp;                 /* granule: compartments per cell (<= 578); ignored for the other types */
+  const double *v0;         /* per-cell initial V [mV] (required) */
+  const double *amp;        /* per-cell constant current into stim_comp [uA; olive uA/cm^2]; NULL: none */
+  int stim_comp, rec_comp;  /* stimulated compartment; compartment recorded and tested for spikes */
+  double thr;               /* spike threshold on rec_comp [mV] (upward crossing between intervals) */
+} cb_mix_pop;
+typedef struct cb_mix_config {
+  cb_mix_pop pop[4];        /* indexed by CB_GR, CB_GO, CB_PKJ, CB_IO */
+  const char *io_gj_path;   /* olive junctions, lines "pre,post,g_nS" (cells of the olive population);
+                               NULL: uncoupled */
+  double interval_ms;       /* exchange and output interval, also the largest step (0.125 by default) */
+  double pkj_atol, pkj_rtol;/* Purkinje tolerances (the other types use the reference's TOL_xx) */
+  int pkj_stable;           /* 1: the Purkinje safeguards (ion interval 2^-8 ms, fn refresh,
+                               spectral-radius floor) */
+  int device;
+} cb_mix_config;
+
+/* Reference-like defaults: interval 0.125 ms, Purkinje 1e-6 / 1e-6 with the
+ * safeguards, the reference's stimulated and raster compartments and
+ * thresholds per type (granule: chain end, +10 mV; Golgi: 30, +10 mV;
+ * Purkinje: 1599, -10 mV; olive: stimulus on 0, raster on 2, +10 mV). */
+int cb_mix_config_default(cb_mix_config *cfg);
+/* Builds every population and runs RKC_vec_initialize for every cell. */
+int cb_mix_create(const cb_mix_config *cfg, cb_mix **out);
+/* nint intervals: rec_out[k * total + off + c] = V of rec_comp of cell c of
+ * the type at offset off (types in the order GR, GO, PKJ, IO) after interval
+ * k; spk_out[k * words + word_off[type] + c / 32] bit c % 32 = an upward
+ * threshold crossing of that cell in interval k.  Either may be NULL. */
+int cb_mix_run(cb_mix *m, int nint, double *rec_out, uint32_t *spk_out);
+/* cells per type, each type's first spike word, words per interval, lanes
+ * (threads) per cell of each type */
+int cb_mix_layout(const cb_mix *m, int *ncell4, int *word_off4, int *words, int *lanes4);
+/* counters of one type (as cb_pop_stats) */
+int cb_mix_stats(const cb_mix *m, int type, int64_t *out6);
+/* Picks the lanes per cell of each small type by timing trial intervals from
+ * the current state (restored afterwards); lanes4 receives the mapping. */
+int cb_mix_autotune(cb_mix *m, int trial_intervals, int *lanes4);
+/* event-timed duration of the launches of the runs so far (ms) and their count */
+int cb_mix_kernel_time(cb_mix *m, double *ms, int64_t *launches, int reset);
+/* 0: enqueue every chunk launch by launch instead of replaying its graph */
+int cb_mix_use_graph(cb_mix *m, int on);
+int cb_mix_destroy(cb_mix *m);
+
+/* Seeded synthetic inputs in the reference formats (host only):
+ * granule chains of comp compartments (types[k] names the template row type of
+ * compartment k; NULL: dendrite 3, soma 4, axon initial segment 6, axon 7),
+ * Ra, Cm and densities from the template's first row of that type, diameter
+ * and length from it times U(0.8, 1.25); Golgi trees with the template's
+ * topology, types and densities, every diameter and length times
+ * U(0.8, 1.25); olive gap junctions, every cell joined to k distinct random
+ * partners with g ~ U(g_min, g_max) nS, lines "pre,post,g_nS" (pre < post). */
+int cb_synth_gr_chains(const char *path, const char *template_path, uint64_t seed, int ncell, const int *types,
+                       int comp);
+int cb_synth_go_trees(const char *path, const char *template_path, uint64_t seed, int ncell);
+int cb_synth_io_gj(const char *path, uint64_t seed, int ncell, int k, double g_min_nS, double g_max_nS);
 
 #ifdef __cplusplus
 }

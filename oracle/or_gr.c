@@ -254,20 +254,27 @@ int or_gr_na_be(int type, double v, double dt, double *x12)
 
 static double *dal(int n) { return (double *)calloc(n > 0 ? n : 1, sizeof(double)); }
 
-or_pop *or_gr_create(const char *param_file, int nx, int ny, const double *v0_per_cell, double atol, double rtol,
-                     int use_libc_rand)
+/* A granule population whose cells have `comp` compartments (the reference's
+ * GR_COMP = 578 in or_gr_create; fewer for synthetic chains written in the
+ * Gr.txt format, one block of comp rows shared by all cells). */
+or_pop *or_gr_create_comp(const char *param_file, int comp, int nx, int ny, const double *v0_per_cell, double atol,
+                          double rtol, int use_libc_rand)
 {
+  if (comp < 1 || comp > GR_COMP) return NULL;
   FILE *f = fopen(param_file, "r");
   if (!f) return NULL;
-  static double rad[GR_COMP], len[GR_COMP], Ra[GR_COMP], cmd[GR_COMP], dens[GR_COMP][N_COND_GR];
-  static int l_connect[GR_COMP], compart[GR_COMP];
-  for (int i = 0; i < GR_COMP; i++) {
+  double *rad = dal(comp), *len = dal(comp), *Ra = dal(comp), *cmd = dal(comp);
+  double (*dens)[N_COND_GR] = (double (*)[N_COND_GR])calloc((size_t)comp, sizeof(double[N_COND_GR]));
+  int *l_connect = (int *)calloc((size_t)comp, sizeof(int)), *compart = (int *)calloc((size_t)comp, sizeof(int));
+#define GR_CREATE_FREE() do { free(rad); free(len); free(Ra); free(cmd); free(dens); free(l_connect); free(compart); } while (0)
+  for (int i = 0; i < comp; i++) {
     int i1, i2, i3;
     double d1, d2, d3, d4, x[N_COND_GR];
     if (fscanf(f, "%d %d %lf %lf %lf %lf %d %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf ", &i1, &i2, &d1, &d2, &d3, &d4,
                &i3, &x[0], &x[1], &x[2], &x[3], &x[4], &x[5], &x[6], &x[7], &x[8], &x[9]) != 17 || i1 < 0 ||
-        i1 >= GR_COMP || i3 < 0 || i3 > 8) {
+        i1 >= comp || i3 < 0 || i3 > 8) {
       fclose(f);
+      GR_CREATE_FREE();
       return NULL;
     }
     rad[i1] = 0.5 * d1 * 1e-4;
@@ -280,9 +287,9 @@ or_pop *or_gr_create(const char *param_file, int nx, int ny, const double *v0_pe
   }
   fclose(f);
 
-  const int n = nx * ny, nc = n * GR_COMP;
+  const int n = nx * ny, nc = n * comp;
   or_pop *p = (or_pop *)calloc(1, sizeof(or_pop));
-  p->type = OR_GR; p->n = n; p->nx = nx; p->ny = ny; p->comp = GR_COMP; p->nc = nc;
+  p->type = OR_GR; p->n = n; p->nx = nx; p->ny = ny; p->comp = comp; p->nc = nc;
   p->ncond = N_COND_GR; p->nion = N_ION_GR;
   p->v = dal(nc); p->ca = dal(nc); p->cm = dal(nc); p->area = dal(nc);
   p->iext = dal(nc); p->shell = dal(nc); p->rev = dal(nc);
@@ -296,8 +303,8 @@ or_pop *or_gr_create(const char *param_file, int nx, int ny, const double *v0_pe
 
   /* gr.cu:39-57 */
   for (int c = 0; c < n; c++)
-    for (int k = 0; k < GR_COMP; k++) {
-      const int i = c * GR_COMP + k;
+    for (int k = 0; k < comp; k++) {
+      const int i = c * comp + k;
       const double a = 2.0 * M_PI * rad[k] * len[k];
       p->area[i] = a;
       p->cm[i] = cmd[k] * a;
@@ -306,17 +313,17 @@ or_pop *or_gr_create(const char *param_file, int nx, int ny, const double *v0_pe
     }
 
   /* gr_solve.cu:37-43, then CSR (:73-133) with the RKC sign */
-  double *mat = (double *)calloc((size_t)GR_COMP * GR_COMP, sizeof(double));
-  for (int i = 0; i < GR_COMP; i++) {
+  double *mat = (double *)calloc((size_t)comp * comp, sizeof(double));
+  for (int i = 0; i < comp; i++) {
     int d = l_connect[i];
     if (d >= 0) {
-      mat[d + GR_COMP * i] = (2.0 / ((Ra[i] * len[i]) / (rad[i] * rad[i] * M_PI) + (Ra[d] * len[d]) / (rad[d] * rad[d] * M_PI)));
-      mat[i + GR_COMP * d] = mat[d + GR_COMP * i];
+      mat[d + comp * i] = (2.0 / ((Ra[i] * len[i]) / (rad[i] * rad[i] * M_PI) + (Ra[d] * len[d]) / (rad[d] * rad[d] * M_PI)));
+      mat[i + comp * d] = mat[d + comp * i];
     }
   }
   int nnz1 = 0;
-  for (int r = 0; r < GR_COMP; r++)
-    for (int c = 0; c < GR_COMP; c++) nnz1 += (mat[c + (size_t)GR_COMP * r] > 0.0 || r == c);
+  for (int r = 0; r < comp; r++)
+    for (int c = 0; c < comp; c++) nnz1 += (mat[c + (size_t)comp * r] > 0.0 || r == c);
   p->nnz = nnz1 * n;
   p->row = (int *)malloc(sizeof(int) * (nc + 1));
   p->col = (int *)malloc(sizeof(int) * (p->nnz > 0 ? p->nnz : 1));
@@ -324,25 +331,25 @@ or_pop *or_gr_create(const char *param_file, int nx, int ny, const double *v0_pe
   int k = 0;
   p->row[0] = 0;
   for (int cell = 0; cell < n; cell++)
-    for (int r = 0; r < GR_COMP; r++) {
+    for (int r = 0; r < comp; r++) {
       double dsum = 0.0;
-      for (int c = 0; c < GR_COMP; c++)
-        if (c != r) dsum += mat[c + (size_t)GR_COMP * r];
-      for (int c = 0; c < GR_COMP; c++) {
-        const double m = mat[c + (size_t)GR_COMP * r];
-        if (m > 0.0) { p->col[k] = cell * GR_COMP + c; p->val[k] = m; k++; }
-        else if (c == r) { p->col[k] = cell * GR_COMP + c; p->val[k] = -dsum; k++; }
+      for (int c = 0; c < comp; c++)
+        if (c != r) dsum += mat[c + (size_t)comp * r];
+      for (int c = 0; c < comp; c++) {
+        const double m = mat[c + (size_t)comp * r];
+        if (m > 0.0) { p->col[k] = cell * comp + c; p->val[k] = m; k++; }
+        else if (c == r) { p->col[k] = cell * comp + c; p->val[k] = -dsum; k++; }
       }
-      p->row[cell * GR_COMP + r + 1] = k;
+      p->row[cell * comp + r + 1] = k;
     }
   free(mat);
 
   /* initial state, gr_ion.cu:635-672 */
   double init_v_rand = 0.0;
   for (int i = 0; i < nc; i++) {
-    if (i % GR_COMP == 0 && !v0_per_cell)
+    if (i % comp == 0 && !v0_per_cell)
       init_v_rand = use_libc_rand ? (((double)rand() + 1.0) / ((double)RAND_MAX + 2.0)) - 0.5 : 0.0;
-    const double vv = v0_per_cell ? v0_per_cell[i / GR_COMP] : V_INIT_GR + 2.0 * init_v_rand;
+    const double vv = v0_per_cell ? v0_per_cell[i / comp] : V_INIT_GR + 2.0 * init_v_rand;
     double **g = p->ion;
     p->v[i] = vv;
     p->ca[i] = Ca1_0_GR;
@@ -359,15 +366,23 @@ or_pop *or_gr_create(const char *param_file, int nx, int ny, const double *v0_pe
     g[c1_Na][i] = 1.0;
   }
   /* gr_set_current (gr.cu:245-254): 0.01 nA into compartment 16 over [50, 100) ms */
-  p->stim_comp = 16;
+  p->stim_comp = comp > 16 ? 16 : 0;
   p->stim_t0 = 50.0;
   p->stim_t1 = 100.0;
   p->stim_amp = dal(n);
   for (int c = 0; c < n; c++) p->stim_amp[c] = 0.01e-3;
   p->others[OR_ATOL] = atol;
   p->others[OR_RTOL] = rtol;
+  GR_CREATE_FREE();
+#undef GR_CREATE_FREE
   if (or_rkc_init(p) != 0) { or_pop_free(p); return NULL; }
   return p;
+}
+
+or_pop *or_gr_create(const char *param_file, int nx, int ny, const double *v0_per_cell, double atol, double rtol,
+                     int use_libc_rand)
+{
+  return or_gr_create_comp(param_file, GR_COMP, nx, ny, v0_per_cell, atol, rtol, use_libc_rand);
 }
 
 /* gr_update_matrix, solve_rkc.cu:226-249 (terms in the reference's order) */

@@ -144,6 +144,8 @@ int or_go_kahp(double ca_old, double ca, double dt, double *st6);
 /* ---- granule cells (or_gr.c) ---- */
 or_pop *or_gr_create(const char *param_file, int nx, int ny, const double *v0_per_cell, double atol, double rtol,
                      int use_libc_rand);
+or_pop *or_gr_create_comp(const char *param_file, int comp, int nx, int ny, const double *v0_per_cell, double atol,
+                          double rtol, int use_libc_rand);
 void or_gr_update_b(or_pop *p, const double *vv);
 void or_gr_update_ion(or_pop *p, const double *vnew, double dt);
 double or_gr_rate(int which, double v, double ca);
