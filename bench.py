@@ -2,11 +2,14 @@
 """Benchmark: compartment-steps/s of the batched mixed-precision RKC engine on
 1024 synthetic Purkinje cells (1600 compartments each), with the voltage and
 spike error against the per-cell FP64 engine on the same run, the plain FP64
-population-global RKC beside it, and the CPU oracle on a bounded sample.
+population-global RKC beside it, and the CPU oracle on the 9-cell plain
+workload.
 
-One bench step = one macro step (H_MAX = 0.125 ms, the reference's output
-interval) of all cells.  A compartment-step is one compartment advanced over
-one macro step, independent of the adaptive internal steps.
+One bench step = 80 macro steps (10 ms simulated) of all 1024 cells; the
+default 20 timed steps are the 200 ms precision workload, run as one
+persistent launch.  A compartment-step is one compartment advanced over one
+macro step (H_MAX = 0.125 ms, the reference's output interval), independent
+of the adaptive internal steps.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -34,21 +37,36 @@ import numpy as np  # noqa: E402
 METRIC = "compartment-steps/s @1024 PCs RKC: mixed-FP32 vs FP64, ≤0.1 mV error"
 H_MAX = 0.125
 COMP = 1600
-# Algorithmic work per unit, counted from the method (s = 4 bytes per FP32 value):
-#   compartment-RHS (one RHS evaluation of one compartment, fused with its stage
-#     update): reads Y_{j-1}, Y_{j-2}, Y_0, F_0, b, 1/C, g_parent + parent index,
-#     writes Y_j: 8s + 4 = 36 B; ~16 flop
-#   compartment-accepted-step (error norm): reads y, y_n, f_n, f(y): 4s = 16 B; ~5 flop
-#   compartment-ion-update: reads V_old, V_new, Ca, rev, shell, 18 gates, type
-#     byte; writes 18 gates, Ca, rev, b: 44s + 1 = 177 B; ~335 flop
-BYTES_RHS = 36.0
-BYTES_STEP = 16.0
-BYTES_ION = 177.0
+# Algorithmic work of the persistent per-cell kernel, per unit, counted from
+# the method as this design runs it (s = 4 bytes per FP32 value).  The stage
+# loop and the error estimate run out of registers and LDS, so HBM is touched
+# only by:
+#   compartment-ion-update: reads 18 gates, V_old, rev, area, leak, shell and
+#     the type word (24 x 4 B), Ca (FP64, 8 B); writes 18 gates, V_old, rev,
+#     b (21 x 4 B), Ca (8 B); the A rebuild that follows reads b (4 B): 200 B;
+#     ~335 flop (~45 exp/log, ~40 reciprocals)
+#   compartment-macro-step (Poisson drive): reads the type word and the
+#     conductance, writes the conductance: 12 B
+#   compartment-launch (launch edges): reads y_n hi/lo, F_0, Y_{m-2}, the type
+#     word, g_parent, 1/C, topology (8 x 4 B); writes y_n hi/lo, F_0, Y_{m-2}
+#     (16 B) and V in FP64 (8 B): 56 B
+# On chip, per compartment-RHS (one RHS evaluation fused with its stage
+# update): LDS reads of topology, g_parent, 1/C, the own and ~2 neighbour
+# D values and their conductances plus one LDS write: ~32 B; ~16 flop.
+# Per compartment per accepted step (error estimate): ~5 flop, registers.
+BYTES_ION = 200.0
+BYTES_MACRO = 12.0
+BYTES_LAUNCH = 56.0
+LDS_RHS = 32.0
 FLOP_RHS = 16.0
 FLOP_STEP = 5.0
 FLOP_ION = 335.0
 PEAK_HBM_GBS = 8000.0
-PEAK_FP32_TFLOPS = 157.3
+# FP32 vector peak without packed FP32 instructions (which this build does not
+# use): 256 CUs x 64 lanes x 2 flop (FMA) x 2.4 GHz
+PEAK_FP32_TFLOPS = 78.6
+# LDS: 128 B per clock per CU x 256 CUs x 2.4 GHz
+PEAK_LDS_GBS = 78643.2
 
 
 def philox_params(seed, n):
@@ -124,44 +142,47 @@ def error_report(compare, ref, tr, name, early_ms=25.0):
         "max_abs_dV_mV_all_samples": float(np.max(np.abs(ref - tr)))}
 
 
-def measured_traffic(nt, cell_steps_per_launch):
-    """HBM bytes per launch of the run kernel: the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (scaled by a plain-copy calibration) gives
-    bytes per cell-macro-step for the 512-thread mapping; times this launch's
-    cell-steps.  None for another mapping or when the summary is absent."""
+def measured_traffic(argv_key):
+    """HBM bytes of the timed launch of the run kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this same command (scaled by
+    the plain-copy calibration recorded there); None when the summary was
+    made with other arguments or is absent."""
     f = os.path.join(ROOT, "profiles", "k_cell_run_1024pkj_traffic.md")
-    if nt != 512 or not os.path.exists(f):
+    if not os.path.exists(f):
         return None
+    key, val = None, None
     with open(f) as fh:
         for line in fh:
-            if line.startswith("| HBM bytes per cell-macro-step |"):
-                return round(float(line.split("|")[2]) * cell_steps_per_launch)
-    return None
+            if line.startswith("| bench arguments |"):
+                key = line.split("|")[2].strip().strip("`")
+            if line.startswith("| HBM bytes, timed launch |"):
+                val = float(line.split("|")[2])
+    return round(val) if key == argv_key and val else None
 
 
-def cpu_baseline(morph_file, v0, amp, args):
-    """The oracle (FP64 C restatement, population-global controller, OpenMP) on
-    the first cells of the same population for a bounded wall time."""
+def cpu_baseline(args):
+    """The oracle (FP64 C restatement of the reference RKC with its
+    population-global controller, OpenMP over compartments) on the plain
+    workload: 9 seeded synthetic Purkinje cells, rtol 1e-4 / atol 1e-6,
+    0.5 nA at the soma over [20, 80) ms, 100 ms; on all host threads for up
+    to --cpu-seconds, then on one thread for a third of that."""
+    import cbrkc
     import orc
-    ncell = args.cpu_cells
-    lines = []
-    with open(morph_file) as f:
-        for i, l in enumerate(f):
-            if i >= ncell * COMP:
-                break
-            lines.append(l)
-    sample = morph_file + f".first{ncell}"
-    with open(sample, "w") as f:
-        f.writelines(lines)
+    n, seed = 9, 101
+    morph = f"/tmp/cbrkc_bench_config1_s{seed}.txt"
+    if not os.path.exists(morph):
+        cbrkc.synth_pkj_morph(morph, seed=seed, ncell=n)
+    v0, _ = philox_params(seed, n)
+    steps_total = 800
+
     def timed(threads, seconds):
-        o = orc.Pop.pkj(sample, ncell, 1, v0=v0[:ncell], atol=args.tol, rtol=args.tol, libc_rand=False)
+        o = orc.Pop.pkj(morph, n, 1, v0=v0, atol=1e-6, rtol=1e-4, libc_rand=False)
         o.stable_split()
-        o.enable_poisson(seed=args.seed)
-        o.set_stim(1599, 0.0, 1e9, amp[:ncell])
+        o.set_stim(1599, 20.0, 80.0, np.full(n, 0.5e-3))
         o.threads(threads)
         t = time.perf_counter()
         steps = 0
-        while steps < args.steps and time.perf_counter() - t < seconds:
+        while steps < steps_total and time.perf_counter() - t < seconds:
             o.set_current(steps * H_MAX)
             o.advance()
             steps += 1
@@ -169,33 +190,38 @@ def cpu_baseline(morph_file, v0, amp, args):
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     steps, dt = timed(threads, args.cpu_seconds)
-    steps1, dt1 = timed(1, args.cpu_seconds / 3)       # the same run on one thread
+    steps1, dt1 = timed(1, args.cpu_seconds / 3)
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
             model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
     except OSError:
         pass
-    return {"value": ncell * COMP * steps / dt, "unit": "compartment-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{ncell} of the 1024 cells as one population (reference global controller, FP64), "
-                      f"{steps} macro steps ({steps * H_MAX:g} ms simulated) in {dt:.1f} s",
-            "value_1_thread": ncell * COMP * steps1 / dt1, "sample_1_thread": f"{steps1} macro steps in {dt1:.1f} s",
-            "cpu_model": model, "nproc": os.cpu_count()}
+    return {"value": n * COMP * steps / dt, "unit": "compartment-steps/s", "cores": threads, "kind": "port",
+            "sample": f"plain workload: 9 synthetic Purkinje cells x 1600 compartments, FP64 RKC rtol 1e-4 "
+                      f"atol 1e-6, 0.5 nA over [20, 80) ms; {steps} of its 800 macro steps "
+                      f"({steps * H_MAX:g} ms simulated) in {dt:.1f} s",
+            "value_1_thread": n * COMP * steps1 / dt1,
+            "sample_1_thread": f"{steps1} macro steps ({steps1 * H_MAX:g} ms) in {dt1:.1f} s",
+            "cpu_model": model, "nproc": os.cpu_count(),
+            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1600, help="timed macro steps (1600 = 200 ms)")
-    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=20, help="timed bench steps (20 x 10 ms = 200 ms)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed bench steps before them")
+    ap.add_argument("--step-macro", type=int, default=80,
+                    help="macro steps (0.125 ms each) per bench step (80 = 10 ms simulated)")
+    ap.add_argument("--scale-steps", type=int, default=400, help="scale workload: macro steps (400 = 50 ms)")
     ap.add_argument("--chunk", type=int, default=0,
                     help="macro steps per persistent launch (0: the whole timed run in one launch)")
     ap.add_argument("--cells", type=int, default=1024)
     ap.add_argument("--tol", type=float, default=1e-6, help="atol = rtol (the reference's TOL_PKJ)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--plain-steps", type=int, default=40)
-    ap.add_argument("--cpu-cells", type=int, default=8)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--threads", type=int, default=0,
                     help="threads per cell of the mixed engine (0: autotune among 256/320/512)")
     ap.add_argument("--no-fp64", action="store_true")
@@ -242,50 +268,53 @@ def main():
     if not os.path.exists(morph):
         cbrkc.synth_pkj_morph(morph, seed=args.seed, ncell=n)
     v0, amp = philox_params(args.seed, n)
-    t0 = args.warmup * H_MAX
+    warm = args.warmup * args.step_macro        # macro steps
+    timed = args.steps * args.step_macro
+    t0 = warm * H_MAX
 
     # ---- mixed engine (the measured value) ----
-    print(f"[bench] mixed engine: {n} cells, warmup {args.warmup}, timed {args.steps} macro steps", file=sys.stderr,
-          flush=True)
+    print(f"[bench] mixed engine: {n} cells, warmup {warm}, timed {timed} macro steps", file=sys.stderr, flush=True)
     pm = make_pop(cbrkc, cbrkc.ENGINE_CELL_MIXED, morph, n, v0, amp, args)
     nt = pm.autotune(4, threads=args.threads)
-    timed_run(pm, args.warmup, args.chunk, 0.0, record=False)
+    timed_run(pm, warm, args.chunk, 0.0, record=False)
     pm.kernel_time(reset=True)
     s0 = pm.stats()
-    tr_m, wall_m = timed_run(pm, args.steps, args.chunk, t0)
+    tr_m, wall_m = timed_run(pm, timed, args.chunk, t0)
     kms, nlaunch = pm.kernel_time(reset=True)
     wm = work(s0, pm.stats())
-    value = n * COMP * args.steps / wall_m
+    value = n * COMP * timed / wall_m
 
-    # roofline of the persistent kernel over the timed region: algorithmic
-    # bytes and flops of the work it did, over its event-timed duration
+    # roofline of the persistent kernel over the timed region: the HBM bytes,
+    # LDS bytes and FP32 flops its work needs (per-unit figures above), over
+    # its HIP-event-timed duration on the engine's stream
     rhs = wm["stages"] - wm["steps"] - wm["rejects"] + wm["rho_iters"]
-    bytes_ = COMP * (BYTES_RHS * rhs + BYTES_STEP * wm["steps"] + BYTES_ION * wm["ion_updates"])
+    bytes_ = COMP * (BYTES_ION * wm["ion_updates"] + BYTES_MACRO * n * timed + BYTES_LAUNCH * n * nlaunch)
+    lds = COMP * LDS_RHS * rhs
     flops = COMP * (FLOP_RHS * rhs + FLOP_STEP * wm["steps"] + FLOP_ION * wm["ion_updates"])
     ksec = kms * 1e-3
     gbs = bytes_ / ksec / 1e9
-    tfl = flops / ksec / 1e12
-    traffic = measured_traffic(nt, n * args.steps / nlaunch)
+    argv_key = f"--steps {args.steps} --warmup {args.warmup} --step-macro {args.step_macro} --cells {n}"
+    traffic = measured_traffic(argv_key)
     roof = {"bound": "hbm", "achieved": round(gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(gbs / PEAK_HBM_GBS, 5), "traffic": traffic,
             "kernel": f"k_cell_run<float,{nt}> (persistent, one workgroup per cell)",
             "kernel_ms_per_launch": round(kms / nlaunch, 4), "launches": nlaunch,
-            "macro_steps_per_launch": args.steps / nlaunch,
-            "algorithmic_bytes_per_launch": round(bytes_ / nlaunch),
-            "compartment_rhs_per_launch": round(COMP * rhs / nlaunch),
-            "compartment_ion_updates_per_launch": round(COMP * wm["ion_updates"] / nlaunch)}
-    roof_valu = {"achieved": round(tfl, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                 "frac": round(tfl / PEAK_FP32_TFLOPS, 5),
-                 "algorithmic_flops_per_launch": round(flops / nlaunch)}
-    roof["binding"] = "fp32-valu" if roof_valu["frac"] > roof["frac"] else "hbm"
+            "macro_steps_per_launch": timed / nlaunch,
+            "algorithmic_hbm_bytes_per_launch": round(bytes_ / nlaunch),
+            "compartment_ion_updates_per_launch": round(COMP * wm["ion_updates"] / nlaunch),
+            "compartment_rhs_per_launch": round(COMP * rhs / nlaunch)}
     if traffic:
-        # the bytes that do reach HBM (rocprofv3 FETCH_SIZE / WRITE_SIZE per
-        # cell-step, scaled to this launch) over the same duration: the
-        # algorithmic figure prices the on-chip stage loop as streamed, so it
-        # can exceed the peak while the measured traffic cannot
-        rate = traffic / (kms / nlaunch * 1e-3) / 1e9
-        roof["traffic_rate_GBs"] = round(rate, 1)
-        roof["traffic_frac"] = round(rate / PEAK_HBM_GBS, 4)
+        # HBM bytes that actually moved (rocprofv3 counters of this command):
+        # above the algorithmic bytes means re-reads and spill traffic
+        roof["traffic_rate_GBs"] = round(traffic / (kms / nlaunch * 1e-3) / 1e9, 1)
+        roof["traffic_frac"] = round(roof["traffic_rate_GBs"] / PEAK_HBM_GBS, 4)
+        roof["traffic_over_algorithmic"] = round(traffic / (bytes_ / nlaunch), 3)
+    on_chip = {"lds": {"achieved": round(lds / ksec / 1e9, 1), "peak": PEAK_LDS_GBS, "unit": "GB/s",
+                       "frac": round(lds / ksec / 1e9 / PEAK_LDS_GBS, 5)},
+               "fp32_valu": {"achieved": round(flops / ksec / 1e12, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                             "frac": round(flops / ksec / 1e12 / PEAK_FP32_TFLOPS, 5)}}
+    fr = {"hbm": roof["frac"], "lds": on_chip["lds"]["frac"], "fp32_valu": on_chip["fp32_valu"]["frac"]}
+    roof["binding"] = max(fr, key=fr.get)
 
     out = {
         "metric": METRIC, "value": value, "unit": "compartment-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -295,11 +324,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{n} synthetic Purkinje cells x 1600 compartments (Pkj.txt format, per-cell seeded "
                                f"trees), RKC atol=rtol={args.tol:g}, soma current U(0.2,1.0) nA, dendritic Poisson "
-                               "5 Hz x 1 nS tau 2 ms; safeguards: ion interval 2^-8 ms, fn refresh, spectral-radius floor",
-                   "cells": n, "compartments": n * COMP, "sim_ms": args.steps * H_MAX, "threads_per_cell": nt},
-        "sim_ms_per_s": args.steps * H_MAX / wall_m,
+                               "5 Hz x 1 nS tau 2 ms; safeguards: ion interval 2^-8 ms, fn refresh, spectral-radius "
+                               f"floor; one step = {args.step_macro} macro steps ({args.step_macro * H_MAX:g} ms)",
+                   "cells": n, "compartments": n * COMP, "sim_ms": timed * H_MAX, "macro_steps": timed,
+                   "threads_per_cell": nt},
+        "sim_ms_per_s": timed * H_MAX / wall_m,
         "mixed": {"wall_s": wall_m, "work": wm},
-        "roofline": roof, "roofline_valu": roof_valu,
+        "roofline": roof, "roofline_on_chip": on_chip,
     }
 
     # ---- per-cell FP64 engine: error of the mixed run ----
@@ -307,10 +338,10 @@ def main():
         print("[bench] per-cell FP64 engine on the same run", file=sys.stderr, flush=True)
         pd = make_pop(cbrkc, cbrkc.ENGINE_CELL_FP64, morph, n, v0, amp, args)
         pd.autotune(0, threads=256)
-        timed_run(pd, args.warmup, args.chunk, 0.0, record=False)
+        timed_run(pd, warm, args.chunk, 0.0, record=False)
         s0 = pd.stats()
-        tr_d, wall_d = timed_run(pd, args.steps, args.chunk, t0)
-        out["fp64_cell"] = {"value": n * COMP * args.steps / wall_d, "wall_s": wall_d, "work": work(s0, pd.stats())}
+        tr_d, wall_d = timed_run(pd, timed, args.chunk, t0)
+        out["fp64_cell"] = {"value": n * COMP * timed / wall_d, "wall_s": wall_d, "work": work(s0, pd.stats())}
         out["error_vs_fp64"] = error_report(compare, tr_d, tr_m, "mixed")
         del pd
         if not args.no_noise_floor:
@@ -320,8 +351,8 @@ def main():
             print("[bench] FP64 noise floor (FP64 engine, 320 threads per cell)", file=sys.stderr, flush=True)
             pf = make_pop(cbrkc, cbrkc.ENGINE_CELL_FP64, morph, n, v0, amp, args)
             pf.autotune(0, threads=320)
-            timed_run(pf, args.warmup, args.chunk, 0.0, record=False)
-            tr_f, _ = timed_run(pf, args.steps, args.chunk, t0)
+            timed_run(pf, warm, args.chunk, 0.0, record=False)
+            tr_f, _ = timed_run(pf, timed, args.chunk, t0)
             out["fp64_noise_floor"] = error_report(compare, tr_d, tr_f, "fp64_reordered")
             del pf
 
@@ -341,7 +372,7 @@ def main():
     # ---- CPU oracle ----
     if not args.no_cpu:
         print("[bench] CPU oracle sample", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(morph, v0, amp, args)
+        out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)
 
 
@@ -355,7 +386,7 @@ def scale_stress(cbrkc, compare, args):
     comparison point is the fastest step whose median per-cell subthreshold
     error against the FP64 RKC run is within 0.1 mV."""
     n = 8192 if args.cells == 1024 else args.cells
-    steps = 400 if args.steps == 1600 else args.steps
+    steps = args.scale_steps
     morph = args.morph or f"/tmp/cbrkc_scale_pkj{n}_s{args.seed}.txt"
     if not os.path.exists(morph):
         cbrkc.synth_pkj_morph(morph, seed=args.seed, ncell=n, dscale=philox_dscale(args.seed, n))

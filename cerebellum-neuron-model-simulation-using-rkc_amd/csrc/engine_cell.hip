@@ -229,6 +229,7 @@ struct Lds {
   T *d0, *d1;        // stage buffers
   T *gp;             // gpar
   T *ic;             // invc
+  T *bl;             // frozen current b (a copy of the slab plane, read by every A rebuild)
   uint32_t *tp;      // topology
 };
 
@@ -272,7 +273,8 @@ __device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, doub
   L.d1 = L.d0 + C;
   L.gp = L.d1 + C;
   L.ic = L.gp + C;
-  L.tp = (uint32_t *)(L.ic + C);
+  L.bl = L.ic + C;
+  L.tp = (uint32_t *)(L.bl + C);
   double *red = (double *)(((uintptr_t)(L.tp + C) + 15) & ~(uintptr_t)15);
   __shared__ int s_cell;
   __shared__ double s_cnt[6];   // steps, rejects, stages, rho iterations, ion updates, macro steps
@@ -299,6 +301,7 @@ __device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, doub
       L.gp[i] = a.gpar[base + i];
       L.ic[i] = a.invc[base + i];
       L.tp[i] = a.topo[base + i];
+      L.bl[i] = S.ld(P_B, i);
     }
 
     // register-resident per-compartment state
@@ -360,7 +363,7 @@ __device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, doub
         if (i < C) {
           T s = lap<T>(L, h, i, hi[k]);
           if constexpr (sizeof(T) == 4) s += lap<T>(L, l, i, lo[k]);
-          A[k] = (-S.ld(P_B, i) + s) * L.ic[i];
+          A[k] = (-L.bl[i] + s) * L.ic[i];
         }
       }
       __syncthreads();
@@ -371,69 +374,133 @@ __device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, doub
     // pkj_update_ion_RKC (pkj_ion.cu:304-358) when do_ion, then the frozen
     // current b (pkj_update_matrix, solve_rkc.cu:369-395).  Streams the
     // compartment's gates from HBM; one compartment per iteration so the
-    // channel arithmetic does not multiply register pressure by K.
+    // channel arithmetic does not multiply register pressure by K.  The
+    // mixed engine updates and streams only the channels present in the
+    // compartment's class (a dendrite: 9 of the 18 gates; zero-density
+    // channels add exact zeros to the current and the Ca influx).
     auto ion_and_b = [&](double dt, bool do_ion, bool stim_live) {
       PHASE_T0(tI);
       // the synaptic current of the macro step, decayed over the interval
       // since the last ion update (solve_rkc.cu:344-367)
       const bool net_on = NET && a.net.row != nullptr;
       const T net_decay = net_on ? (T)exp(-dt / 0.6) : T(0);
-      // the voltage for the update goes through LDS: each thread reads back
-      // only what it wrote, so no barrier is needed
+      // the voltage for the update goes through LDS
       T *vyl = L.d1;
 #pragma unroll
       for (int k = 0; k < K; k++) {
         const int i = tid + k * NT;
         if (i < C) vyl[i] = VY[k];
       }
-#pragma unroll 1
-      for (int i = tid; i < C; i += NT) {
+      // FP64 engine: thread t updates its own compartments t, t + NT, ...
+      // and reads back only what it wrote.  Mixed engine: the last, partial
+      // block of compartments goes to the highest threads instead, so wave 0
+      // (which holds the soma and the first dendrite, the compartments with
+      // the long channel lists, in its first block) runs one block fewer;
+      // the vyl exchange then needs a barrier on either side.
+      constexpr bool REMAP = sizeof(T) == 4;
+      const int kl = (C - 1) / NT, rem = C - kl * NT;
+      if constexpr (REMAP) __syncthreads();
+      // One compartment's inputs as streamed from the slab.  The mixed engine
+      // streams the channels every class has (the dendrite's); the soma and
+      // main-dendrite compartments fetch their extra gates when they are
+      // updated.
+      struct In {
+        uint32_t meta;
+        T ar, leak, rev, vo, shell, syn, net;
+        double ca;
+        T g[NGATE];
+      };
+      constexpr unsigned MS = (sizeof(T) == 4) ? channels_of_row(2) : CH_ALL;
+      auto fetch = [&](int i, In &x) {
         // the flag word first: the loads return in order, so the class
         // select waits for it alone
-        const uint32_t meta = S.meta(i);
-        const T ar = S.ld(P_AREA, i);
-        const T leak = S.ld(P_LEAK, i);
+        x.meta = S.meta(i);
+        x.ar = S.ld(P_AREA, i);
+        x.leak = S.ld(P_LEAK, i);
+        x.rev = S.ld(P_REV, i);
+        if (do_ion) {
+          x.vo = S.ld(P_VOLD, i);
+          x.ca = S.ca(i);
+          x.shell = S.ld(P_SHELL, i);
+        }
+        if (a.syn_on) x.syn = S.ld(P_SYN, i);
+        if (net_on) x.net = S.ld(p_nets<T>(), i);
+#pragma unroll
+        for (int q = 0; q < NGATE; q++)
+          if (chan_of_gate(q) & MS) x.g[q] = S.ld(P_GATE + q, i);
+      };
+      auto body = [&](auto Mc, const In &x, int i) {
+        constexpr unsigned M = decltype(Mc)::value;
         T g[NGATE];
 #pragma unroll
-        for (int q = 0; q < NGATE; q++) g[q] = S.ld(P_GATE + q, i);
-        const int og = (int)(meta & 0xFFFFu);
-        const int fl = (int)(meta >> 16);
+        for (int q = 0; q < NGATE; q++)
+          g[q] = (chan_of_gate(q) & M) ? ((chan_of_gate(q) & MS) ? x.g[q] : S.ld(P_GATE + q, i)) : T(0);
+        const int og = (int)(x.meta & 0xFFFFu);
+        const int fl = (int)(x.meta >> 16);
         const int cl = fl & CLS_MASK;
         const int row = (cl == SOMA) ? 0 : (cl == MAIN) ? 1 : 2;
         T c[NCOND];
-        c[g_leak] = (T)(1.0 / ((cl == SOMA) ? RM_S : RM_D)) * leak;
-        densities<T>(row, ar, c, std::make_index_sequence<10>{});
-        T rev = S.ld(P_REV, i);
+        c[g_leak] = (T)(1.0 / ((cl == SOMA) ? RM_S : RM_D)) * x.leak;
+        densities<T>(row, x.ar, c, std::make_index_sequence<10>{});
+        T rev = x.rev;
         const T vn = vyl[i];
         if (do_ion) {
-          const T vo = S.ld(P_VOLD, i);
-          double ca = S.ca(i);
+          const T vo = x.vo;
+          double ca = x.ca;
           if constexpr (sizeof(T) == 4) {
+            T gca = T(0);
+            if constexpr ((M & CH_CAP) != 0) gca = c[g_CaP] * g[m_CaP] * g[h_CaP];
+            if constexpr ((M & CH_CAT) != 0) gca = gca + c[g_CaT] * g[m_CaT] * g[h_CaT];
             // the shell plane holds 1 / (2 F shell) for the mixed engine
-            const double caq = ca_rk4_mixed(vo, ca, rev, (double)S.ld(P_SHELL, i),
-                                            c[g_CaP] * g[m_CaP] * g[h_CaP] + c[g_CaT] * g[m_CaT] * g[h_CaT], dt,
-                                            (fl & FIXED_REV) != 0);
-            gates_cn_fast(0.5f * (vn + vo), (float)caq, (float)dt, g);
+            const double caq = ca_rk4_mixed(vo, ca, rev, (double)x.shell, gca, dt, (fl & FIXED_REV) != 0);
+            gates_cn_fast<M>(0.5f * (vn + vo), (float)caq, (float)dt, g);
           } else {
             double rv = rev;
-            const double caq = ca_rk4(vo, ca, rv, S.ld(P_SHELL, i), c[g_CaP] * g[m_CaP] * g[h_CaP],
+            const double caq = ca_rk4(vo, ca, rv, x.shell, c[g_CaP] * g[m_CaP] * g[h_CaP],
                                       c[g_CaT] * g[m_CaT] * g[h_CaT], dt, (fl & FIXED_REV) != 0);
             rev = rv;
             gates_cn<T>(T(0.5) * (vn + vo), (T)caq, (T)dt, g);
           }
 #pragma unroll
-          for (int q = 0; q < NGATE; q++) S.st(P_GATE + q, i, g[q]);
+          for (int q = 0; q < NGATE; q++)
+            if (chan_of_gate(q) & M) S.st(P_GATE + q, i, g[q]);
           S.set_ca(i, ca);
           S.st(P_REV, i, rev);
           S.st(P_VOLD, i, vn);
         }
         const T iext = (stim_live && og == a.stim_orig && a.stim_t0 <= t_cur && t_cur < a.stim_t1)
                            ? (T)a.amp[cell] : T(0);
-        T bb = current<T>(vn, iext, rev, c, g);
-        if (a.syn_on) bb += S.ld(P_SYN, i) * (vn - a.syn_e);
-        if (net_on) bb += S.ld(p_nets<T>(), i) * net_decay;
+        T bb = current_m<M, T>(vn, iext, rev, c, g);
+        if (a.syn_on) bb += x.syn * (vn - a.syn_e);
+        if (net_on) bb += x.net * net_decay;
         S.st(P_B, i, bb);
+        L.bl[i] = bb;
+      };
+      // compartment of block k for this thread, or -1
+      auto index_of = [&](int k) {
+        int i = tid + k * NT;
+        if (REMAP && k == kl) i = kl * NT + tid - (NT - rem);
+        return (i >= k * NT && i < C) ? i : -1;
+      };
+#pragma unroll 1
+      for (int k = 0; k <= kl; k++) {
+        const int i = index_of(k);
+        if (i < 0) continue;
+        In x;
+        fetch(i, x);
+        if constexpr (sizeof(T) == 4) {
+          const int cl = (int)(x.meta >> 16) & CLS_MASK;
+          if (cl == SOMA)
+            body(std::integral_constant<unsigned, channels_of_row(0)>{}, x, i);
+          else if (cl == MAIN)
+            body(std::integral_constant<unsigned, channels_of_row(1)>{}, x, i);
+          else
+            body(std::integral_constant<unsigned, channels_of_row(2)>{}, x, i);
+        } else {
+          body(std::integral_constant<unsigned, CH_ALL>{}, x, i);
+        }
       }
+      if constexpr (REMAP) __syncthreads();
       a_valid = false;
       PHASE_ADD(PH_ION, tI);
     };
@@ -1386,7 +1453,7 @@ class CellPkj : public Engine {
     return v;
   }
 
-  size_t lds_bytes() const { return (size_t)C_ * (4 * sizeof(T) + 4) + 16 + 2 * 16 * sizeof(double); }
+  size_t lds_bytes() const { return (size_t)C_ * (5 * sizeof(T) + 4) + 16 + 2 * 16 * sizeof(double); }
 
   void choose_default() {
     cur_ = 0;
