@@ -7,7 +7,7 @@ workload.
 
 One bench step = 80 macro steps (10 ms simulated) of all 1024 cells; the
 default 20 timed steps are the 200 ms precision workload, run as one
-persistent launch.  A compartment-step is one compartment advanced over one
+persistent launch after 5 untimed steps (50 ms).  A compartment-step is one compartment advanced over one
 macro step (H_MAX = 0.125 ms, the reference's output interval), independent
 of the adaptive internal steps.
 
@@ -37,30 +37,32 @@ import numpy as np  # noqa: E402
 METRIC = "compartment-steps/s @1024 PCs RKC: mixed-FP32 vs FP64, ≤0.1 mV error"
 H_MAX = 0.125
 COMP = 1600
+PRECISION_STEPS = 1600   # the precision workload: 200 ms of macro steps
 # Algorithmic work of the persistent per-cell kernel, per unit, counted from
 # the method as this design runs it (s = 4 bytes per FP32 value).  The stage
-# loop and the error estimate run out of registers and LDS, so HBM is touched
-# only by:
-#   compartment-ion-update: reads 18 gates, V_old, rev, area, leak, shell and
-#     the type word (24 x 4 B), Ca (FP64, 8 B); writes 18 gates, V_old, rev,
-#     b (21 x 4 B), Ca (8 B); the A rebuild that follows reads b (4 B): 200 B;
-#     ~335 flop (~45 exp/log, ~40 reciprocals)
+# loop, the error estimate and the A rebuilds run out of registers and LDS,
+# so HBM is touched only by:
+#   compartment-ion-update (a dendritic compartment: 9 of the 18 gates belong
+#     to channels present there): reads the type word, area, leak, rev, V_old,
+#     shell, the Poisson conductance and 9 gates (16 x 4 B) and Ca (FP64,
+#     8 B); writes 9 gates, rev, V_old, b (12 x 4 B) and Ca (8 B): 128 B;
+#     ~200 flop (~10 exp, ~15 reciprocals)
 #   compartment-macro-step (Poisson drive): reads the type word and the
 #     conductance, writes the conductance: 12 B
 #   compartment-launch (launch edges): reads y_n hi/lo, F_0, Y_{m-2}, the type
-#     word, g_parent, 1/C, topology (8 x 4 B); writes y_n hi/lo, F_0, Y_{m-2}
-#     (16 B) and V in FP64 (8 B): 56 B
+#     word, b, g_parent, 1/C, topology (9 x 4 B); writes y_n hi/lo, F_0,
+#     Y_{m-2} (16 B) and V in FP64 (8 B): 60 B
 # On chip, per compartment-RHS (one RHS evaluation fused with its stage
 # update): LDS reads of topology, g_parent, 1/C, the own and ~2 neighbour
 # D values and their conductances plus one LDS write: ~32 B; ~16 flop.
 # Per compartment per accepted step (error estimate): ~5 flop, registers.
-BYTES_ION = 200.0
+BYTES_ION = 128.0
 BYTES_MACRO = 12.0
-BYTES_LAUNCH = 56.0
+BYTES_LAUNCH = 60.0
 LDS_RHS = 32.0
 FLOP_RHS = 16.0
 FLOP_STEP = 5.0
-FLOP_ION = 335.0
+FLOP_ION = 200.0
 PEAK_HBM_GBS = 8000.0
 # FP32 vector peak without packed FP32 instructions (which this build does not
 # use): 256 CUs x 64 lanes x 2 flop (FMA) x 2.4 GHz
@@ -142,6 +144,23 @@ def error_report(compare, ref, tr, name, early_ms=25.0):
         "max_abs_dV_mV_all_samples": float(np.max(np.abs(ref - tr)))}
 
 
+def measured_valu_busy(argv_key):
+    """Fraction of SIMD cycles issuing VALU instructions in the timed launch,
+    from the committed rocprofv3 SQ counter summary of this same command;
+    None when made with other arguments or absent."""
+    f = os.path.join(ROOT, "profiles", "k_cell_run_1024pkj_valu_lds.md")
+    if not os.path.exists(f):
+        return None
+    key, val = None, None
+    with open(f) as fh:
+        for line in fh:
+            if line.startswith("| bench arguments |"):
+                key = line.split("|")[2].strip().strip("`")
+            if line.startswith("| VALU busy"):
+                val = float(line.split("|")[2].strip().strip("*"))
+    return val if key == argv_key else None
+
+
 def measured_traffic(argv_key):
     """HBM bytes of the timed launch of the run kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this same command (scaled by
@@ -211,7 +230,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed bench steps (20 x 10 ms = 200 ms)")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed bench steps before them")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed bench steps before them")
     ap.add_argument("--step-macro", type=int, default=80,
                     help="macro steps (0.125 ms each) per bench step (80 = 10 ms simulated)")
     ap.add_argument("--scale-steps", type=int, default=400, help="scale workload: macro steps (400 = 50 ms)")
@@ -276,7 +295,7 @@ def main():
     print(f"[bench] mixed engine: {n} cells, warmup {warm}, timed {timed} macro steps", file=sys.stderr, flush=True)
     pm = make_pop(cbrkc, cbrkc.ENGINE_CELL_MIXED, morph, n, v0, amp, args)
     nt = pm.autotune(4, threads=args.threads)
-    timed_run(pm, warm, args.chunk, 0.0, record=False)
+    tw_m, _ = timed_run(pm, warm, args.chunk, 0.0)
     pm.kernel_time(reset=True)
     s0 = pm.stats()
     tr_m, wall_m = timed_run(pm, timed, args.chunk, t0)
@@ -313,7 +332,14 @@ def main():
                        "frac": round(lds / ksec / 1e9 / PEAK_LDS_GBS, 5)},
                "fp32_valu": {"achieved": round(flops / ksec / 1e12, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                              "frac": round(flops / ksec / 1e12 / PEAK_FP32_TFLOPS, 5)}}
+    busy = measured_valu_busy(argv_key)
+    if busy is not None:
+        # issue-slot occupancy of the vector ALU (all instructions, not only
+        # the algorithmic flops), rocprofv3 SQ counters of this command
+        on_chip["valu_busy_measured"] = busy
     fr = {"hbm": roof["frac"], "lds": on_chip["lds"]["frac"], "fp32_valu": on_chip["fp32_valu"]["frac"]}
+    if busy is not None:
+        fr["valu_issue"] = busy
     roof["binding"] = max(fr, key=fr.get)
 
     out = {
@@ -338,11 +364,16 @@ def main():
         print("[bench] per-cell FP64 engine on the same run", file=sys.stderr, flush=True)
         pd = make_pop(cbrkc, cbrkc.ENGINE_CELL_FP64, morph, n, v0, amp, args)
         pd.autotune(0, threads=256)
-        timed_run(pd, warm, args.chunk, 0.0, record=False)
+        tw_d, _ = timed_run(pd, warm, args.chunk, 0.0)
         s0 = pd.stats()
         tr_d, wall_d = timed_run(pd, timed, args.chunk, t0)
         out["fp64_cell"] = {"value": n * COMP * timed / wall_d, "wall_s": wall_d, "work": work(s0, pd.stats())}
-        out["error_vs_fp64"] = error_report(compare, tr_d, tr_m, "mixed")
+        # the errors are stated over the precision workload itself: the
+        # first 200 ms from t = 0 (warm-up included), whatever the warm-up
+        win = min(PRECISION_STEPS, warm + timed)
+        full_d = np.vstack([tw_d, tr_d])[:win]
+        out["error_window_ms"] = [0.0, win * H_MAX]
+        out["error_vs_fp64"] = error_report(compare, full_d, np.vstack([tw_m, tr_m])[:win], "mixed")
         del pd
         if not args.no_noise_floor:
             # the same FP64 run with another thread mapping: only the order of
@@ -351,9 +382,9 @@ def main():
             print("[bench] FP64 noise floor (FP64 engine, 320 threads per cell)", file=sys.stderr, flush=True)
             pf = make_pop(cbrkc, cbrkc.ENGINE_CELL_FP64, morph, n, v0, amp, args)
             pf.autotune(0, threads=320)
-            timed_run(pf, warm, args.chunk, 0.0, record=False)
+            tw_f, _ = timed_run(pf, warm, args.chunk, 0.0)
             tr_f, _ = timed_run(pf, timed, args.chunk, t0)
-            out["fp64_noise_floor"] = error_report(compare, tr_d, tr_f, "fp64_reordered")
+            out["fp64_noise_floor"] = error_report(compare, full_d, np.vstack([tw_f, tr_f])[:win], "fp64_reordered")
             del pf
 
     # ---- plain FP64 population-global RKC ----

@@ -278,6 +278,7 @@ __device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, doub
   double *red = (double *)(((uintptr_t)(L.tp + C) + 15) & ~(uintptr_t)15);
   __shared__ int s_cell;
   __shared__ double s_cnt[6];   // steps, rejects, stages, rho iterations, ion updates, macro steps
+  __shared__ double s_cs[NT / 64][14];   // per-wave copy of the controller state
 #ifdef CB_PHASE_CLOCKS
   __shared__ double s_ph[8];    // cycles per phase, kept by thread 0
 #endif
@@ -324,11 +325,19 @@ __device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, doub
     __syncthreads();
 
     // ---- controller state (uniform across the workgroup) ----
-    double absh = ctl[C_ABSH], sprad = ctl[C_SPRAD], naccpt = ctl[C_NACCPT], errold = ctl[C_ERROLD],
-           hold = ctl[C_HOLD], lt = ctl[C_LT], tend = ctl[C_TEND], gt = ctl[C_GT], gt_old = ctl[C_GTOLD],
-           sum_dt = ctl[C_SUMDT], mmax = ctl[C_MMAX], hlast = ctl[C_HLAST], ltlast = ctl[C_LTLAST];
+    // Every wave runs the controller on its own copy, held in LDS rather
+    // than in registers: FP64 values have no scalar registers, and kept as
+    // 14 VGPR pairs across the stage loop they pushed the run kernel into
+    // scratch spills that were reloaded at every step.
+    double *cs = s_cs[tid >> 6];
+    double &absh = cs[0], &sprad = cs[1], &naccpt = cs[2], &errold = cs[3], &hold = cs[4], &lt = cs[5],
+           &tend = cs[6], &gt = cs[7], &gt_old = cs[8], &sum_dt = cs[9], &mmax = cs[10], &hlast = cs[11],
+           &ltlast = cs[12], &idid = cs[13];
+    absh = ctl[C_ABSH]; sprad = ctl[C_SPRAD]; naccpt = ctl[C_NACCPT]; errold = ctl[C_ERROLD];
+    hold = ctl[C_HOLD]; lt = ctl[C_LT]; tend = ctl[C_TEND]; gt = ctl[C_GT]; gt_old = ctl[C_GTOLD];
+    sum_dt = ctl[C_SUMDT]; mmax = ctl[C_MMAX]; hlast = ctl[C_HLAST]; ltlast = ctl[C_LTLAST];
+    idid = ctl[C_IDID];
     bool newspc = ctl[C_NEWSPC] != 0.0, jacatt = ctl[C_JACATT] != 0.0, last = ctl[C_LAST] != 0.0;
-    double idid = ctl[C_IDID];
     // work counters live in LDS, kept by thread 0 (they need no registers elsewhere)
     if (tid == 0)
       for (int q = 0; q < 6; q++) s_cnt[q] = ctl[C_STEPS + q];
