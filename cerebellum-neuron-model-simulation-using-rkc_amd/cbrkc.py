@@ -35,7 +35,7 @@ EXPORTS = [
     "cb_pop_run", "cb_pop_run_noalloc", "cb_pop_set_raster", "cb_pop_spikes", "cb_pop_read", "cb_pop_size", "cb_pop_stats", "cb_pop_controller",
     "cb_synth_pkj_morph", "cb_philox4x32_10", "cb_last_error", "cb_pkj_initialize",
     "cb_pkj_solve_update_v", "cb_pkj_set_current", "cb_pkj_output_file", "cb_pkj_finalize",
-    "cb_pop_kernel_time", "cb_pop_autotune", "cb_pkj_cable_spectral_radius",
+    "cb_pop_kernel_time", "cb_pop_autotune", "cb_pop_mapping", "cb_pkj_cable_spectral_radius",
     "cb_io_initialize", "cb_io_solve_update_v", "cb_io_set_current", "cb_io_output_file", "cb_io_finalize",
     "cb_go_initialize", "cb_go_solve_update_v", "cb_go_set_current", "cb_go_output_file", "cb_go_finalize",
     "cb_gr_initialize", "cb_gr_solve_update_v", "cb_gr_set_current", "cb_gr_output_file", "cb_gr_finalize",
@@ -162,6 +162,7 @@ def lib():
         L.cb_pop_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
         L.cb_pop_controller.argtypes = [vp, dp, dp]
         L.cb_pop_kernel_time.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int]
+        L.cb_pop_mapping.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
         L.cb_pop_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.cb_pkj_cable_spectral_radius.argtypes = [ctypes.c_char_p, ctypes.c_int, dp]
         L.cb_synth_pkj_morph.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_double,
@@ -493,6 +494,12 @@ class Population:
         t = ctypes.c_int(threads)
         _check(lib().cb_pop_autotune(self.h, trial_steps, ctypes.byref(t)))
         return t.value
+
+    def mapping(self):
+        """the launch mapping in use (and the last autotune trial), as text"""
+        buf = ctypes.create_string_buffer(4096)
+        _check(lib().cb_pop_mapping(self.h, buf, 4096))
+        return buf.value.decode()
 
     def controller(self):
         o = np.zeros(14)

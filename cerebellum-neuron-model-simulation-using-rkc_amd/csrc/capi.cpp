@@ -637,6 +637,15 @@ int cb_pop_autotune(cb_pop *p, int trial_steps, int *threads) {
   });
 }
 
+int cb_pop_mapping(const cb_pop *p, char *buf, int len) {
+  return guarded([&]() -> int {
+    if (!p || !buf || len <= 0) return fail(CB_E_ARG, "null argument");
+    const std::string m = p->eng->mapping();
+    std::snprintf(buf, (size_t)len, "%s", m.c_str());
+    return (int)m.size() < len ? CB_OK : fail(CB_E_ARG, "buffer too short for the mapping description");
+  });
+}
+
 int cb_pop_controller(const cb_pop *p, double *o, double *hw) {
   return guarded([&]() -> int {
     if (!p || !o || !hw) return fail(CB_E_ARG, "null argument");

@@ -85,6 +85,8 @@ class Engine {
   }
   // selects the mapping with this many threads per cell; false if none
   virtual bool select_mapping(int threads) { return threads == 0; }
+  // a description of the launch mapping in use (and of the last autotune trial)
+  virtual std::string mapping() const { return "one thread per compartment"; }
   // event-timed duration of the main kernel (ms) and its launch count
   virtual void kernel_time(double *ms, long *launches, bool reset) {
     *ms = 0.0;

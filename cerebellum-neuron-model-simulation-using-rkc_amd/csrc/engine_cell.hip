@@ -1406,6 +1406,13 @@ class CellPkj : public Engine {
   std::vector<std::pair<void *, size_t>> state_buffers() { return mutable_bufs(); }
   std::vector<TuneRec> tune_log_;
   int chosen_nt() const { return variants_[cur_].nt; }
+  std::string mapping() const override {
+    std::string m = "one workgroup per cell, " + std::to_string(variants_[cur_].nt) + " threads x " +
+                    std::to_string(variants_[cur_].k) + " compartments per thread";
+    for (const TuneRec &r : tune_log_)
+      m += "; trial " + std::to_string(r.nt) + " x " + std::to_string(r.k) + ": " + std::to_string(r.sec * 1e3) + " ms";
+    return m;
+  }
 
  private:
   struct Snapshot {

@@ -1073,7 +1073,7 @@ class CnEngine : public Engine {
     if (direct_) build_hines(n_, comp_, row, col, val, hh);   // before any device allocation
     create_stream(&st_);
     if (direct_ && !hh.nseg.empty()) {
-      chain_ = true;
+      chain_ = has_chain_ = true;
       Q_.perm = dup(hh.cperm);
       Q_.up = dup(hh.cup);
       Q_.dn = dup(hh.cdn);
@@ -1161,18 +1161,69 @@ class CnEngine : public Engine {
   void set_current(double t) override {
     k_cn_stim<<<(n_ + 127) / 128, 128, 0, st_>>>(S_.iext, amp_, comp_, stim_.comp, stim_.t0, stim_.t1, t, n_);
   }
+  // one exact tree solve of (diag, b) into v: by unbranched runs or level
+  // by level (the same operations per compartment, bit-identical results)
+  void tree_solve(bool by_runs, const double *diag, const double *b, double *v) {
+    if (by_runs) {
+      const size_t lds = (size_t)comp_ * 4 * sizeof(double) + CHAIN_SEGS * sizeof(int);
+      k_hines_chain<<<n_, HINES_WAVE, lds, st_>>>(Q_, diag, b, v);
+    } else if (sym_) {
+      const size_t lds = (size_t)comp_ * (3 * sizeof(double) + sizeof(int));
+      k_hines<true><<<n_, HINES_WAVE, lds, st_>>>(P_, diag, b, v);
+    } else {
+      const size_t lds = (size_t)comp_ * (4 * sizeof(double) + sizeof(int));
+      k_hines<false><<<n_, HINES_WAVE, lds, st_>>>(P_, diag, b, v);
+    }
+  }
+
+  // Tree engine: when every cell is at most CHAIN_SEGS unbranched runs both
+  // solves apply; each runs trial_steps times on a copy of the current system
+  // (the population's state is not touched) and the faster is kept.
+  // Returns the lanes per cell (one wavefront either way).
+  int autotune(int trial_steps, double t0) override {
+    (void)t0;
+    if (!direct_) return 0;
+    if (!has_chain_ || trial_steps <= 0) return HINES_WAVE;
+    double *d = dal<double>(nc_), *b = dal<double>(nc_), *v = dal<double>(nc_);
+    CB_HIP(hipMemcpyAsync(d, S_.diag, sizeof(double) * nc_, hipMemcpyDeviceToDevice, st_));
+    CB_HIP(hipMemcpyAsync(b, S_.b, sizeof(double) * nc_, hipMemcpyDeviceToDevice, st_));
+    hipEvent_t e0, e1;
+    CB_HIP(hipEventCreate(&e0));
+    CB_HIP(hipEventCreate(&e1));
+    float ms[2] = {0.0f, 0.0f};
+    for (int by_runs = 0; by_runs < 2; by_runs++) {
+      tree_solve(by_runs != 0, d, b, v);   // warm
+      CB_HIP(hipEventRecord(e0, st_));
+      for (int k = 0; k < trial_steps; k++) tree_solve(by_runs != 0, d, b, v);
+      CB_HIP(hipEventRecord(e1, st_));
+      CB_HIP(hipEventSynchronize(e1));
+      CB_HIP(hipEventElapsedTime(&ms[by_runs], e0, e1));
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(d);
+    (void)hipFree(b);
+    (void)hipFree(v);
+    chain_ = ms[1] <= ms[0];
+    tune_ms_[0] = ms[0];
+    tune_ms_[1] = ms[1];
+    return HINES_WAVE;
+  }
+  std::string mapping() const override {
+    if (!direct_) return nb_ <= KRY_SMALL_BLOCKS ? "Krylov solve in one workgroup" : "Krylov solve, two launches per iteration";
+    std::string m = chain_ ? "tree solve by unbranched runs, one lane per run, one wavefront per cell"
+                           : "tree solve level by level, one wavefront per cell";
+    if (tune_ms_[0] > 0.0 || tune_ms_[1] > 0.0)
+      m += " (trial: level walk " + std::to_string(tune_ms_[0]) + " ms, run walk " + std::to_string(tune_ms_[1]) +
+           " ms)";
+    return m;
+  }
+
   // one CN (BE) step of CN_DT (BE_DT): xx_solve_update_v with that solver
   void advance() override {
     prep();
-    if (chain_) {
-      const size_t lds = (size_t)comp_ * 4 * sizeof(double) + CHAIN_SEGS * sizeof(int);
-      k_hines_chain<<<n_, HINES_WAVE, lds, st_>>>(Q_, S_.diag, S_.b, S_.v);
-    } else if (direct_ && sym_) {
-      const size_t lds = (size_t)comp_ * (3 * sizeof(double) + sizeof(int));
-      k_hines<true><<<n_, HINES_WAVE, lds, st_>>>(P_, S_.diag, S_.b, S_.v);
-    } else if (direct_) {
-      const size_t lds = (size_t)comp_ * (4 * sizeof(double) + sizeof(int));
-      k_hines<false><<<n_, HINES_WAVE, lds, st_>>>(P_, S_.diag, S_.b, S_.v);
+    if (direct_) {
+      tree_solve(chain_, S_.diag, S_.b, S_.v);
     } else if (nb_ <= KRY_SMALL_BLOCKS) {
       k_kry_small<<<1, NB, 0, st_>>>(S_);
     } else {
@@ -1277,7 +1328,9 @@ class CnEngine : public Engine {
   HinesPlan P_{};
   ChainPlan Q_{};
   bool sym_ = false;     // the level walk stages one off-diagonal plane
-  bool chain_ = false;   // every cell at most CHAIN_SEGS segments: the chain solve
+  bool chain_ = false;   // the solve by unbranched runs is in use
+  bool has_chain_ = false;   // every cell at most CHAIN_SEGS runs: the run walk applies
+  double tune_ms_[2] = {0.0, 0.0};
   double *amp_ = nullptr, *rec_ = nullptr;
   hipStream_t st_{};
   Stats st;

@@ -188,6 +188,16 @@ int cb_pop_kernel_time(cb_pop *p, double *ms, int64_t *launches, int reset);
  * reference. */
 int cb_pop_autotune(cb_pop *p, int trial_steps, int *threads);
 
+/* Tree-solve engine (CB_ENGINE_TREE_FP64): when every cell is at most 64
+ * unbranched runs, both the solve by runs and the level-by-level solve apply;
+ * cb_pop_autotune times both on trial_steps solves of a copy of the current
+ * system and keeps the faster (*threads = 64, one wavefront per cell).
+ *
+ * A text description of the launch mapping in use and of the last autotune
+ * trial, NUL-terminated in buf (CB_E_ARG when len is too short).  No
+ * counterpart in the reference. */
+int cb_pop_mapping(const cb_pop *p, char *buf, int len);
+
 /* Global-controller scalars: h_others[14] and h_work[8] (param.h:22,
  * solve_rkc.cu:453-476).  CB_E_UNSUP for per-cell engines. */
 int cb_pop_controller(const cb_pop *p, double *others14, double *hwork8);

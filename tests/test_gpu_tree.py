@@ -184,3 +184,25 @@ def test_step_validation(ref_data):
     with pytest.raises(cbrkc.CbError):        # RKC chooses its own steps
         cbrkc.Population(cbrkc.GR, 1, 1, solver=cbrkc.RKC, morph=os.path.join(ref_data, "Gr.txt"), v0=[-70.0],
                          step_ms=2.0 ** -7)
+
+
+def test_tree_autotune_picks_a_solve_and_leaves_the_run_unchanged(ref_data):
+    # granule cells are 9 unbranched runs: both the solve by runs and the
+    # level walk apply; the autotuner times both on copies of the current
+    # system, keeps the faster and changes no result (the two are
+    # bit-identical, and the trial does not touch the population's state)
+    kw = dict(solver=cbrkc.CN, engine=cbrkc.ENGINE_TREE_FP64, morph=os.path.join(ref_data, "Gr.txt"),
+              v0=np.array([-70.0, -65.0]), stim=(16, 0.0, 1e9, 0.015e-3))
+    print("case: tree-solve autotune on 2 granule cells, CN, 8 output intervals", flush=True)
+    p = cbrkc.Population(cbrkc.GR, 2, 1, **kw)
+    q = cbrkc.Population(cbrkc.GR, 2, 1, **kw)
+    a = p.run(2, 0.0, 16)
+    b = q.run(2, 0.0, 16)
+    assert q.autotune(16) == 64
+    m = q.mapping()
+    print(m, flush=True)
+    assert "trial" in m and ("by unbranched runs" in m or "level by level" in m)
+    a = np.vstack([a, p.run(6, 0.25, 16)])
+    b = np.vstack([b, q.run(6, 0.25, 16)])
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(p.read(cbrkc.F_V), q.read(cbrkc.F_V))
