@@ -233,16 +233,28 @@ struct Lds {
   uint32_t *tp;      // topology
 };
 
-// sum_nb g (D_nb - D_i) with D in LDS buffer cur
+// sum_nb g (D_nb - D_i) with D in LDS buffer cur, where cur[i] holds this
+// thread's own D_i.  Branch-free: a missing parent or child is read as the
+// compartment itself (D_i - D_i = 0 exactly) with conductance 0, so every
+// LDS read is issued unconditionally and together; the added terms are
+// exact zeros, so the sum equals the one over the real neighbours.
 template <typename T>
 __device__ inline T lap(const Lds<T> &L, const T *cur, int i, T di) {
   const uint32_t tp = L.tp[i];
   const int p = tp & 0x7FF, fc = (tp >> 11) & 0x7FF, nch = (tp >> 22) & 3;
-  T s = T(0);
-  if (p != NO_PARENT) s += L.gp[i] * (cur[p] - di);
+  const int pp = (p != NO_PARENT) ? p : i;
+  const T gi = L.gp[i];
+  const T dp = cur[pp];
+  T gc[3], dc[3];
 #pragma unroll
-  for (int c = 0; c < 3; c++)   // at most 3 children: straight-line, predicated
-    if (c < nch) s += L.gp[fc + c] * (cur[fc + c] - di);
+  for (int c = 0; c < 3; c++) {   // at most 3 children
+    const int j = (c < nch) ? fc + c : i;
+    gc[c] = L.gp[j];
+    dc[c] = cur[j];
+  }
+  T s = ((p != NO_PARENT) ? gi : T(0)) * (dp - di);
+#pragma unroll
+  for (int c = 0; c < 3; c++) s += ((c < nch) ? gc[c] : T(0)) * (dc[c] - di);
   return s;
 }
 
