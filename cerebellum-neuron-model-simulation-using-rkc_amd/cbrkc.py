@@ -44,7 +44,8 @@ EXPORTS = [
     "cb_syn_size", "cb_syn_comp", "cb_syn_read", "cb_syn_build",
     "cb_mix_config_default", "cb_mix_create", "cb_mix_run", "cb_mix_layout", "cb_mix_stats", "cb_mix_autotune",
     "cb_mix_kernel_time", "cb_mix_use_graph", "cb_mix_destroy", "cb_synth_gr_chains", "cb_synth_go_trees",
-    "cb_synth_io_gj",
+    "cb_synth_io_gj", "cb_net_config_default", "cb_net_generate", "cb_net_create", "cb_net_run", "cb_net_spikes",
+    "cb_net_conductance", "cb_net_stats", "cb_net_destroy",
 ]
 
 # synapse kinds and fields (cb_syn_build, cb_syn_read)
@@ -92,6 +93,37 @@ class Config(ctypes.Structure):
         ("device", ctypes.c_int),
         ("step_ms", ctypes.c_double),
         ("hmin_ms", ctypes.c_double),
+        ("comp", ctypes.c_int),
+    ]
+
+
+class NetConfig(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint64),
+        ("deg_mf_gr", ctypes.c_int),
+        ("deg_go_gr", ctypes.c_int),
+        ("deg_gr_go", ctypes.c_int),
+        ("deg_gr_pkj", ctypes.c_int),
+        ("total", ctypes.c_int64),
+        ("w_min_nS", ctypes.c_double),
+        ("w_max_nS", ctypes.c_double),
+        ("tau_ampa_ms", ctypes.c_double),
+        ("tau_gaba_ms", ctypes.c_double),
+        ("e_ampa_mV", ctypes.c_double),
+        ("e_gaba_mV", ctypes.c_double),
+        ("delay_ms", ctypes.c_double),
+        ("mf_rate_hz", ctypes.c_double),
+        ("gr_target_comp", ctypes.c_int),
+        ("go_target_lo", ctypes.c_int),
+        ("go_target_hi", ctypes.c_int),
+        ("pkj_target_lo", ctypes.c_int),
+        ("pkj_target_hi", ctypes.c_int),
+        ("gr_spike_comp", ctypes.c_int),
+        ("go_spike_comp", ctypes.c_int),
+        ("pkj_spike_comp", ctypes.c_int),
+        ("gr_thr_mV", ctypes.c_double),
+        ("go_thr_mV", ctypes.c_double),
+        ("pkj_thr_mV", ctypes.c_double),
     ]
 
 
@@ -205,6 +237,18 @@ def lib():
         L.cb_syn_read.argtypes = [vp, ctypes.c_int, dp]
         L.cb_syn_build.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp,
                                    ctypes.c_uint64, ip, ip, ip, dp]
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        L.cb_net_config_default.argtypes = [ctypes.POINTER(NetConfig)]
+        L.cb_net_generate.argtypes = [ctypes.POINTER(NetConfig), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                      ctypes.POINTER(ctypes.c_int8), ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_int32), dp, i64p]
+        L.cb_net_create.argtypes = [vp, vp, vp, ctypes.POINTER(NetConfig), ctypes.POINTER(vp)]
+        L.cb_net_run.argtypes = [vp, ctypes.c_int]
+        L.cb_net_spikes.argtypes = [vp, ctypes.c_int, dp, ip, ip, ip]
+        L.cb_net_conductance.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.cb_net_stats.argtypes = [vp, i64p]
+        L.cb_net_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -370,7 +414,7 @@ class Population:
 
     def __init__(self, cell_type=PKJ, nx=1, ny=1, solver=RKC, engine=ENGINE_GLOBAL_FP64, morph=None, gj="",
                  atol=None, rtol=None, v0=None, stim=None, poisson=None, ion_interval=None, refresh_fn=False,
-                 rho_floor=False, device=0, step_ms=None, hmin_ms=None):
+                 rho_floor=False, device=0, step_ms=None, hmin_ms=None, comp=None):
         L = lib()
         cfg = Config()
         _check(L.cb_config_default(ctypes.byref(cfg), cell_type, solver, nx, ny))
@@ -394,10 +438,10 @@ class Population:
         if stim == "none":
             cfg.stim_comp = STIM_NONE
         elif stim is not None:
-            comp, t0, t1, amp = stim
+            stim_comp, t0, t1, amp = stim
             amp = np.ascontiguousarray(np.broadcast_to(np.asarray(amp, dtype=np.float64), (nx * ny,)))
             self._keep.append(amp)
-            cfg.stim_comp, cfg.stim_t0, cfg.stim_t1 = comp, t0, t1
+            cfg.stim_comp, cfg.stim_t0, cfg.stim_t1 = stim_comp, t0, t1
             cfg.stim_amp = _dp(amp)
         if poisson is not None:
             cfg.poisson_on = 1
@@ -415,6 +459,8 @@ class Population:
             cfg.step_ms = step_ms
         if hmin_ms:
             cfg.hmin_ms = hmin_ms
+        if comp:
+            cfg.comp = comp
         h = ctypes.c_void_p()
         _check(L.cb_pop_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
@@ -509,6 +555,94 @@ class Population:
 
 
 # ---- reference-named entry points (Purkinje) ----
+
+NET_MFGR, NET_GOGR, NET_GRGO, NET_GRPKJ = range(4)
+NET_GR, NET_GO, NET_PKJ = range(3)
+NET_AMPA, NET_GABA = range(2)
+
+
+def net_config(**kw):
+    """cb_net_config_default, then the given fields"""
+    cfg = NetConfig()
+    _check(lib().cb_net_config_default(ctypes.byref(cfg)))
+    for k, v in kw.items():
+        if not hasattr(cfg, k):
+            raise CbError(f"no network field {k}")
+        setattr(cfg, k, v)
+    return cfg
+
+
+def net_generate(cfg, n, comp):
+    """the synapse list on the host (cb_net_generate): kind, pre, post, w_nS"""
+    L = lib()
+    cnt = ctypes.c_int64()
+    _check(L.cb_net_generate(ctypes.byref(cfg), *n, *comp, 0, None, None, None, None, ctypes.byref(cnt)))
+    m = cnt.value
+    kind = np.zeros(m, np.int8)
+    pre = np.zeros(m, np.int32)
+    post = np.zeros(m, np.int32)
+    w = np.zeros(m, np.float64)
+    _check(L.cb_net_generate(ctypes.byref(cfg), *n, *comp, m, kind.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)),
+                             pre.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                             post.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _dp(w), ctypes.byref(cnt)))
+    return kind, pre, post, w
+
+
+class Network:
+    """A fixed in-degree network over a granule, a Golgi and a Purkinje
+    population (cb_net_*); the populations must outlive it."""
+
+    def __init__(self, gr, go, pkj, cfg=None):
+        self.pops = (gr, go, pkj)
+        cfg = cfg if cfg is not None else net_config()
+        h = ctypes.c_void_p()
+        _check(lib().cb_net_create(gr.h, go.h, pkj.h, ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+
+    def run(self, nint):
+        _check(lib().cb_net_run(self.h, nint))
+
+    def spikes(self):
+        """every raster entry so far as arrays (t [ms], population, cell)"""
+        L = lib()
+        ts, ps, cs = [], [], []
+        while True:
+            cap = 4096
+            t = np.zeros(cap)
+            p = np.zeros(cap, np.int32)
+            c = np.zeros(cap, np.int32)
+            n = ctypes.c_int()
+            _check(L.cb_net_spikes(self.h, cap, _dp(t), p.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                   c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(n)))
+            ts.append(t[:n.value])
+            ps.append(p[:n.value])
+            cs.append(c[:n.value])
+            if n.value < cap:
+                break
+        return np.concatenate(ts), np.concatenate(ps), np.concatenate(cs)
+
+    def conductance(self, pop, receptor):
+        out = np.zeros(self.pops[pop].nc)
+        _check(lib().cb_net_conductance(self.h, pop, receptor, _dp(out)))
+        return out
+
+    def stats(self):
+        out = (ctypes.c_int64 * 9)()
+        _check(lib().cb_net_stats(self.h, out))
+        v = list(out)
+        return dict(synapses=v[0:4], slots=v[4:7], intervals=v[7], delay=v[8])
+
+    def close(self):
+        if getattr(self, "h", None):
+            _check(lib().cb_net_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 def pkj_initialize(nx, ny, solver="RKC"):
     """pkj_initialize + pkj_solve_initialize (pkj.cu:240, pkj_solve.cu:286); reads ./Pkj.txt."""

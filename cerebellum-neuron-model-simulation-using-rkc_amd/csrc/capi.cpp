@@ -13,6 +13,7 @@
 #include "gr_model.hpp"
 #include "io_model.hpp"
 #include "morph.hpp"
+#include "network.hpp"
 #include "philox.hpp"
 #include "pkj_model.hpp"
 
@@ -32,6 +33,8 @@ struct cb_pop {
   // Golgi input) and their CSR by postsynaptic compartment on the device
   std::vector<cb_syn *> syn_in;
   int *syn_row = nullptr, *syn_slot = nullptr;
+  // coupled to a network (cb_net), which holds its engine
+  bool in_net = false;
   ~cb_pop() {
     eng.reset();
     if (syn_row) (void)hipFree(syn_row);
@@ -204,16 +207,21 @@ int create_gr(const cb_pop_config &cfg, cb_pop **out) {
   if (n == 0) return fail(CB_E_ARG, "empty population");
   if (cfg.engine != CB_ENGINE_GLOBAL_FP64 && cfg.engine != CB_ENGINE_TREE_FP64)
     return fail(CB_E_UNSUP, "granule cells: global engine only");
+  // a chain file (cb_synth_gr_chains) has comp rows per cell; Gr.txt 578
+  const int comp = cfg.comp > 0 ? cfg.comp : gr::COMP;
+  if (comp > gr::COMP) return fail(CB_E_ARG, "granule compartment count out of range");
+  if (comp != gr::COMP && cfg.stim_comp == -1)
+    return fail(CB_E_ARG, "a granule chain needs a stimulus compartment or CB_STIM_NONE");
   GrHost host;
   std::string err;
   const std::string path = cfg.morph_path ? cfg.morph_path : "./Gr.txt";
-  if (!build_gr_host(path, cfg.nx, cfg.ny, cfg.v0, host, err)) return fail(CB_E_IO, err);
+  if (!build_gr_host(path, cfg.nx, cfg.ny, cfg.v0, host, err, comp)) return fail(CB_E_IO, err);
   StimSpec stim;   // gr_set_current (gr.cu:245-254): compartment 16, [50, 100) ms
   stim.comp = gr::REC;
   stim.t0 = gr::STIM_T0;
   stim.t1 = gr::STIM_T1;
   if (cfg.stim_comp >= 0) {
-    if (cfg.stim_comp >= gr::COMP) return fail(CB_E_ARG, "stimulus compartment out of range");
+    if (cfg.stim_comp >= comp) return fail(CB_E_ARG, "stimulus compartment out of range");
     stim.comp = cfg.stim_comp;
     stim.t0 = cfg.stim_t0;
     stim.t1 = cfg.stim_t1;
@@ -221,7 +229,10 @@ int create_gr(const cb_pop_config &cfg, cb_pop **out) {
   stim.amp.assign(n, gr::STIM);
   if (cfg.stim_amp)
     for (int c = 0; c < n; c++) stim.amp[c] = cfg.stim_amp[c];
-  if (cfg.stim_comp == CB_STIM_NONE) stim.amp.assign(n, 0.0);
+  if (cfg.stim_comp == CB_STIM_NONE) {
+    stim.amp.assign(n, 0.0);
+    stim.comp = 0;
+  }
   SolverSpec sol;
   const double tol = 1.0e-9;   // TOL_GR, solve_rkc.cuh:28
   sol.atol = cfg.atol > 0 ? cfg.atol : tol;
@@ -235,7 +246,7 @@ int create_gr(const cb_pop_config &cfg, cb_pop **out) {
   p->cell_type = CB_GR;
   p->solver = cfg.solver;
   p->engine = cfg.engine;
-  p->comp = gr::COMP;
+  p->comp = comp;
   p->device = cfg.device;
   CB_HIP(hipSetDevice(cfg.device));
   const bool tree = cfg.engine == CB_ENGINE_TREE_FP64;
@@ -317,6 +328,7 @@ int make_syn(int kind, cb_pop *a, cb_pop *b, cb_pop *pre, cb_pop *post, const do
     s->old_d = dev_copy(thousand);   // old_v = 1000: no spike on the first update
     // the reference couples every set except the inhibitory Purkinje input
     if (kind != SYN_MLIPKJ) {
+      if (post->in_net) return fail(CB_E_UNSUP, "the population is coupled to a network");
       if (post->syn_in.size() >= 2) return fail(CB_E_UNSUP, "a population takes at most two synapse sets");
       post->syn_in.push_back(s.get());
       try {
@@ -379,6 +391,45 @@ int solver_from_name(const char *s) {
 }
 }  // namespace
 
+namespace {
+NetSpec net_spec_of(const cb_net_config &c) {
+  NetSpec s;
+  s.seed = c.seed;
+  s.deg[NET_MFGR] = c.deg_mf_gr;
+  s.deg[NET_GOGR] = c.deg_go_gr;
+  s.deg[NET_GRGO] = c.deg_gr_go;
+  s.deg[NET_GRPKJ] = c.deg_gr_pkj;
+  s.total = c.total;
+  s.w_min_nS = c.w_min_nS;
+  s.w_max_nS = c.w_max_nS;
+  s.tau_ms[0] = c.tau_ampa_ms;
+  s.tau_ms[1] = c.tau_gaba_ms;
+  s.e_mV[0] = c.e_ampa_mV;
+  s.e_mV[1] = c.e_gaba_mV;
+  s.delay_ms = c.delay_ms;
+  s.mf_rate_hz = c.mf_rate_hz;
+  s.tgt_lo[NP_GR] = c.gr_target_comp;
+  s.tgt_hi[NP_GR] = c.gr_target_comp + 1;
+  s.tgt_lo[NP_GO] = c.go_target_lo;
+  s.tgt_hi[NP_GO] = c.go_target_hi;
+  s.tgt_lo[NP_PKJ] = c.pkj_target_lo;
+  s.tgt_hi[NP_PKJ] = c.pkj_target_hi;
+  s.spike_comp[NP_GR] = c.gr_spike_comp;
+  s.spike_comp[NP_GO] = c.go_spike_comp;
+  s.spike_comp[NP_PKJ] = c.pkj_spike_comp;
+  s.thr_mV[NP_GR] = c.gr_thr_mV;
+  s.thr_mV[NP_GO] = c.go_thr_mV;
+  s.thr_mV[NP_PKJ] = c.pkj_thr_mV;
+  return s;
+}
+}  // namespace
+
+struct cb_net {
+  std::unique_ptr<Network> net;
+  cb_pop *pop[3] = {nullptr, nullptr, nullptr};
+  int device = 0;
+};
+
 extern "C" {
 
 const char *cb_last_error(void) { return g_err.c_str(); }
@@ -421,6 +472,8 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
     if (cfg->solver == CB_SOLVER_RKC && cfg->step_ms > 0.0)
       return fail(CB_E_ARG, "step_ms applies to the CN and BE solvers (RKC chooses its own steps)");
     if (cfg->hmin_ms > 0.125) return fail(CB_E_ARG, "hmin_ms must not exceed H_MAX = 0.125 ms");
+    if (cfg->comp > 0 && cfg->cell_type != CB_GR)
+      return fail(CB_E_ARG, "comp applies to granule chains (the other formats fix the compartment count)");
     if (cfg->solver != CB_SOLVER_RKC && cfg->poisson_on)
       return fail(CB_E_UNSUP, "the Poisson conductance drive is provided for the RKC solver");
     // the device is selected right before an engine is built, so argument
@@ -522,6 +575,7 @@ int cb_pop_create(const cb_pop_config *cfg, cb_pop **out) {
 int cb_pop_destroy(cb_pop *p) {
   return guarded([&]() -> int {
     if (!p) return CB_OK;
+    if (p->in_net) return fail(CB_E_ARG, "the population is coupled to a network: destroy the network first");
     CB_HIP(hipSetDevice(p->device));
     delete p;
     return CB_OK;
@@ -1130,6 +1184,142 @@ int cb_mix_destroy(cb_mix *m) {
     if (!m) return CB_OK;
     CB_HIP(hipSetDevice(m->device));
     delete m;
+    return CB_OK;
+  });
+}
+
+int cb_net_config_default(cb_net_config *cfg) {
+  return guarded([&]() -> int {
+    if (!cfg) return fail(CB_E_ARG, "null config");
+    std::memset(cfg, 0, sizeof(*cfg));
+    const NetSpec s;
+    cfg->seed = s.seed;
+    cfg->deg_mf_gr = s.deg[NET_MFGR];
+    cfg->deg_go_gr = s.deg[NET_GOGR];
+    cfg->deg_gr_go = s.deg[NET_GRGO];
+    cfg->deg_gr_pkj = s.deg[NET_GRPKJ];
+    cfg->total = s.total;
+    cfg->w_min_nS = s.w_min_nS;
+    cfg->w_max_nS = s.w_max_nS;
+    cfg->tau_ampa_ms = s.tau_ms[0];
+    cfg->tau_gaba_ms = s.tau_ms[1];
+    cfg->e_ampa_mV = s.e_mV[0];
+    cfg->e_gaba_mV = s.e_mV[1];
+    cfg->delay_ms = s.delay_ms;
+    cfg->mf_rate_hz = s.mf_rate_hz;
+    cfg->gr_target_comp = 0;
+    cfg->go_target_lo = s.tgt_lo[NP_GO];
+    cfg->go_target_hi = s.tgt_hi[NP_GO];
+    cfg->pkj_target_lo = s.tgt_lo[NP_PKJ];
+    cfg->pkj_target_hi = s.tgt_hi[NP_PKJ];
+    cfg->gr_spike_comp = 3;
+    cfg->go_spike_comp = s.spike_comp[NP_GO];
+    cfg->pkj_spike_comp = s.spike_comp[NP_PKJ];
+    cfg->gr_thr_mV = s.thr_mV[NP_GR];
+    cfg->go_thr_mV = s.thr_mV[NP_GO];
+    cfg->pkj_thr_mV = s.thr_mV[NP_PKJ];
+    return CB_OK;
+  });
+}
+
+int cb_net_generate(const cb_net_config *cfg, int n_gr, int n_go, int n_pkj, int comp_gr, int comp_go,
+                    int comp_pkj, int64_t cap, int8_t *kind, int32_t *pre, int32_t *post, double *w_nS,
+                    int64_t *n) {
+  return guarded([&]() -> int {
+    if (!cfg || !n) return fail(CB_E_ARG, "null argument");
+    NetSpec s = net_spec_of(*cfg);
+    const int nn[3] = {n_gr, n_go, n_pkj}, cc[3] = {comp_gr, comp_go, comp_pkj};
+    for (int p = 0; p < 3; p++) {
+      s.n[p] = nn[p];
+      s.comp[p] = cc[p];
+    }
+    const std::string err = net_check(s);
+    if (!err.empty()) return fail(CB_E_ARG, err);
+    *n = net_size(s);
+    if (!kind && !pre && !post && !w_nS) return CB_OK;
+    if (!kind || !pre || !post || !w_nS) return fail(CB_E_ARG, "give every output array or none");
+    if (cap < *n) return fail(CB_E_ARG, "cap is smaller than the synapse count");
+    NetList L;
+    net_generate(s, L);
+    for (size_t j = 0; j < L.kind.size(); j++) {
+      kind[j] = L.kind[j];
+      pre[j] = L.pre[j];
+      post[j] = L.post[j];
+      w_nS[j] = L.w_nS[j];
+    }
+    return CB_OK;
+  });
+}
+
+int cb_net_create(cb_pop *gr, cb_pop *go, cb_pop *pkj, const cb_net_config *cfg, cb_net **out) {
+  return guarded([&]() -> int {
+    if (!gr || !go || !pkj || !cfg || !out) return fail(CB_E_ARG, "null argument");
+    *out = nullptr;
+    if (gr->cell_type != CB_GR || go->cell_type != CB_GO || pkj->cell_type != CB_PKJ)
+      return fail(CB_E_ARG, "cb_net_create takes a granule, a Golgi and a Purkinje population");
+    for (cb_pop *p : {gr, go, pkj}) {
+      if (p->solver != CB_SOLVER_RKC) return fail(CB_E_UNSUP, "the network runs RKC populations");
+      if (p->in_net) return fail(CB_E_ARG, "a population is coupled to another network");
+      if (!p->syn_in.empty()) return fail(CB_E_ARG, "a population already takes synapse sets");
+      if (p->device != gr->device) return fail(CB_E_ARG, "the populations must share one device");
+    }
+    CB_HIP(hipSetDevice(gr->device));
+    std::unique_ptr<cb_net> n(new cb_net);
+    n->device = gr->device;
+    n->net.reset(new Network(gr->eng.get(), go->eng.get(), pkj->eng.get(), net_spec_of(*cfg), gr->device));
+    n->pop[0] = gr;
+    n->pop[1] = go;
+    n->pop[2] = pkj;
+    for (cb_pop *p : n->pop) p->in_net = true;
+    *out = n.release();
+    return CB_OK;
+  });
+}
+
+int cb_net_run(cb_net *net, int nint) {
+  return guarded([&]() -> int {
+    if (!net) return fail(CB_E_ARG, "null network");
+    if (nint < 0) return fail(CB_E_ARG, "negative interval count");
+    CB_HIP(hipSetDevice(net->device));
+    net->net->run(nint);
+    return CB_OK;
+  });
+}
+
+int cb_net_spikes(cb_net *net, int cap, double *t_out, int *pop_out, int *cell_out, int *n_out) {
+  return guarded([&]() -> int {
+    if (!net || !n_out || (cap > 0 && (!t_out || !pop_out || !cell_out))) return fail(CB_E_ARG, "null argument");
+    *n_out = net->net->take_spikes(cap, t_out, pop_out, cell_out);
+    return CB_OK;
+  });
+}
+
+int cb_net_conductance(const cb_net *net, int pop, int receptor, double *dst) {
+  return guarded([&]() -> int {
+    if (!net || !dst) return fail(CB_E_ARG, "null argument");
+    net->net->conductance(pop, receptor, dst);
+    return CB_OK;
+  });
+}
+
+int cb_net_stats(const cb_net *net, int64_t *out9) {
+  return guarded([&]() -> int {
+    if (!net || !out9) return fail(CB_E_ARG, "null argument");
+    for (int k = 0; k < NET_NKIND; k++) out9[k] = net->net->count(k);
+    for (int p = 0; p < NP_N; p++) out9[4 + p] = net->net->slots(p);
+    out9[7] = net->net->intervals();
+    out9[8] = net->net->delay_intervals();
+    return CB_OK;
+  });
+}
+
+int cb_net_destroy(cb_net *net) {
+  return guarded([&]() -> int {
+    if (!net) return CB_OK;
+    CB_HIP(hipSetDevice(net->device));
+    for (cb_pop *p : net->pop)
+      if (p) p->in_net = false;
+    delete net;
     return CB_OK;
   });
 }

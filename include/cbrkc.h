@@ -99,6 +99,10 @@ typedef struct cb_pop_config {
                                counterpart in the reference, whose step is a compile-time constant) */
   double hmin_ms;           /* RKC: the smallest step [ms]; <= 0: the reference's H_MIN of the cell type
                                (2^-10, granule 2^-12, solve_rkc.cuh:24-27); at most H_MAX = 0.125 */
+  int comp;                 /* granule cells: compartments per cell of a chain file (one block of comp
+                               rows per cell, as cb_synth_gr_chains writes); <= 0: the format's 578
+                               (gr.cuh:11).  Must be <= 0 for the other types, whose files fix it.
+                               A chain needs an explicit stimulus compartment or CB_STIM_NONE. */
 } cb_pop_config;
 
 /* Fills a config with the reference defaults for one cell type and solver. */
@@ -346,6 +350,84 @@ int cb_syn_read(const cb_syn *s, int field, double *dst);
  * weight NULL only *n is set; otherwise they receive *n entries. */
 int cb_syn_build(int kind, int nx_a, int ny_a, int nx_b, int ny_b, const double *x1600, uint64_t seed, int *n,
                  int *pre, int *post, double *weight);
+
+/* ---- fixed in-degree network between populations ----
+ * The reference's network run (main.cu:182-311 with network_sim on: per
+ * macro step every population advances, then the synapse sets update from
+ * the presynaptic voltages, syn.cu:822-917) over a granule, a Golgi and a
+ * Purkinje population, with a seeded fixed in-degree connectivity in place of
+ * the reference's geometric rules (syn.cu:144-261, 292-410, 440-644):
+ *   mossy fibre -> granule (AMPA), Golgi -> granule (GABA_A), granule ->
+ *   Golgi (AMPA), granule -> Purkinje (AMPA);
+ * weights U(w_min, w_max) nS; one decaying conductance per (compartment,
+ * receptor), G <- G exp(-dt / tau) + the weights of the spikes arriving in
+ * the interval (the reference's per-synapse traces summed per compartment,
+ * syn.cu:87-111, 264-288, 413-437, 647-668); a spike (an upward crossing of
+ * the spike compartment's threshold between interval ends) arrives delay_ms
+ * later; mossy fibres fire with probability rate x dt per interval
+ * (mfgr_update, syn.cu:87-102; Philox4x32-10 in place of cuRAND).  The
+ * conductances couple into each population's engine as the reference's sets
+ * do (solve_rkc.cu:200-224, 253-287, 318-367).  The interval is the macro
+ * step, H_MAX = 0.125 ms. */
+typedef struct cb_net cb_net;
+typedef struct cb_net_config {
+  uint64_t seed;
+  int deg_mf_gr, deg_go_gr, deg_gr_go, deg_gr_pkj;  /* inputs per postsynaptic cell */
+  int64_t total;            /* > 0: the first (total - the in-degrees' sum) Purkinje cells take one more
+                               granule input each, so the network holds exactly total synapses */
+  double w_min_nS, w_max_nS;
+  double tau_ampa_ms, tau_gaba_ms;
+  double e_ampa_mV, e_gaba_mV;
+  double delay_ms;          /* a whole number (1 to 4096) of 0.125 ms intervals */
+  double mf_rate_hz;
+  int gr_target_comp;       /* the granule compartment of the mossy-fibre and Golgi inputs */
+  int go_target_lo, go_target_hi;    /* [lo, hi): Golgi compartments of granule inputs (uniform) */
+  int pkj_target_lo, pkj_target_hi;  /* [lo, hi): Purkinje compartments of granule inputs (uniform) */
+  int gr_spike_comp, go_spike_comp, pkj_spike_comp;
+  double gr_thr_mV, go_thr_mV, pkj_thr_mV;
+} cb_net_config;
+
+/* 4 mossy-fibre and 4 Golgi inputs per granule cell, 100 granule inputs per
+ * Golgi cell, 1392 per Purkinje cell, total 780404 (2048 / 512 / 512 cells),
+ * w U(0.5, 1.5) nS, AMPA 1.5 ms / 0 mV, GABA_A 6 ms / -82 mV (E_GOGR,
+ * syn.cuh), delay 1 ms, mossy fibres 5 Hz; targets: granule compartment 0
+ * (the chain's dendrite), Golgi 31..130 (the Go_131.txt dendrites), Purkinje
+ * 0..1598; spikes: granule chain end 3 and Golgi 30 through 0 mV (the
+ * reference's pre V > 0 && old_v < 0 test), Purkinje 1599 through -10 mV. */
+int cb_net_config_default(cb_net_config *cfg);
+/* The synapse list alone, on the host (no device), for populations of n_gr,
+ * n_go, n_pkj cells of comp_gr, comp_go, comp_pkj compartments: in
+ * generation order (kind CB_NET_MFGR..CB_NET_GRPKJ, then postsynaptic cell,
+ * then input) the kind, presynaptic cell (mossy fibres: the fibre, one per
+ * synapse), postsynaptic compartment within its population and weight [nS].
+ * With every array NULL only *n is set; otherwise they receive *n <= cap
+ * entries (CB_E_ARG when cap is smaller).  No counterpart in the reference. */
+enum { CB_NET_MFGR = 0, CB_NET_GOGR = 1, CB_NET_GRGO = 2, CB_NET_GRPKJ = 3 };
+int cb_net_generate(const cb_net_config *cfg, int n_gr, int n_go, int n_pkj, int comp_gr, int comp_go,
+                    int comp_pkj, int64_t cap, int8_t *kind, int32_t *pre, int32_t *post, double *w_nS,
+                    int64_t *n);
+/* Builds the network over three populations of one device (RKC; the
+ * Purkinje population on any RKC engine) and couples it into them: the
+ * mfgr / gogr / grgo / grpkj _create calls of main.cu:140-158 in one.  The
+ * populations take no other synapse sets meanwhile and are destroyed after
+ * the network. */
+int cb_net_create(cb_pop *gr, cb_pop *go, cb_pop *pkj, const cb_net_config *cfg, cb_net **out);
+/* nint intervals of 0.125 ms: per interval every population advances
+ * (set_current at the interval's start, then its solve) with the
+ * conductances of the interval, and its spikes enter the ring; the three
+ * populations run concurrently for up to delay + 1 intervals between
+ * exchanges (the network's own delay bounds what an interval can depend on). */
+int cb_net_run(cb_net *net, int nint);
+/* moves up to cap raster entries out of the network: time [ms] at the end of
+ * the interval, population (0 granule, 1 Golgi, 2 Purkinje) and cell, in
+ * time order, populations in that order within a time, cells ascending */
+int cb_net_spikes(cb_net *net, int cap, double *t_out, int *pop_out, int *cell_out, int *n_out);
+/* the conductance [mS] of every compartment of one population (0 granule, 1
+ * Golgi, 2 Purkinje) for one receptor (0 AMPA, 1 GABA_A); 0 where none */
+int cb_net_conductance(const cb_net *net, int pop, int receptor, double *dst);
+/* synapses per kind (4), slots per population (3), intervals run, delay in intervals */
+int cb_net_stats(const cb_net *net, int64_t *out9);
+int cb_net_destroy(cb_net *net);
 
 /* ---- batched engine over an uneven mix of cell types ----
  * Granule, Golgi, Purkinje and inferior-olive cells advanced together, each

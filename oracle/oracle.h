@@ -240,6 +240,34 @@ void or_syn_comp(const or_syn *s, int *pre, int *post);
 /* ---- Philox4x32-10 (or_philox.c), Salmon et al., SC'11 ---- */
 void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
+/* ---- fixed in-degree network (or_net.c) ----
+   kinds 0 mossy fibre -> granule (AMPA), 1 Golgi -> granule (GABA_A),
+   2 granule -> Golgi (AMPA), 3 granule -> Purkinje (AMPA); populations
+   0 granule, 1 Golgi, 2 Purkinje */
+typedef struct or_net_spec {
+  uint64_t seed;
+  int deg[4];
+  long long total;
+  double w_min_nS, w_max_nS;
+  double tau_ampa_ms, tau_gaba_ms, e_ampa_mV, e_gaba_mV;
+  int delay;            /* intervals */
+  double mf_rate_hz, dt_ms;
+  int n[3], comp[3], tgt_lo[3], tgt_hi[3], spike_comp[3];
+  double thr_mV[3];
+} or_net_spec;
+typedef struct or_net or_net;
+long long or_net_size(const or_net_spec *s);
+long long or_net_generate(const or_net_spec *s, int8_t *kind, int32_t *pre, int32_t *post, double *w_nS);
+/* the populations (n and comp taken from them) take the network's sets;
+   free the network before them */
+or_net *or_net_create(or_pop *gr, or_pop *go, or_pop *pkj, const or_net_spec *spec);
+void or_net_deliver(or_net *N);
+void or_net_mark(or_net *N);
+int or_net_run(or_net *N, int nint, unsigned char *spikes_out);
+void or_net_conductance(const or_net *N, int pop, int rec, double *out);
+void or_net_deliver_given(or_net *N, const unsigned char *arrive);
+void or_net_free(or_net *N);
+
 #ifdef __cplusplus
 }
 #endif
