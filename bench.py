@@ -248,13 +248,17 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--morph", default=None, help="reuse an existing morphology file")
-    ap.add_argument("--workload", choices=("precision", "scale", "rows", "batching"), default="precision",
+    ap.add_argument("--workload", choices=("precision", "scale", "rows", "batching", "network"), default="precision",
                     help="precision: the headline 1024-cell run; scale: the 8192-cell scale-stress comparison; "
                          "rows: every cell type x solver on the reference's morphologies, with the oracle beside it")
     ap.add_argument("--rows", default="PKJ,GR,GO,IO", help="cell types of the rows workload")
     ap.add_argument("--mix-scale", type=int, default=1, help="batching: divide every population by this")
     ap.add_argument("--mix-intervals", type=int, default=1000, help="batching: timed 0.1 ms intervals (100 ms)")
     ap.add_argument("--mix-trial", type=int, default=8, help="batching: autotuner trial intervals")
+    ap.add_argument("--net-intervals", type=int, default=800, help="network: timed 0.125 ms intervals (100 ms)")
+    ap.add_argument("--net-warmup", type=int, default=16, help="network: untimed intervals before them")
+    ap.add_argument("--net-scale", type=int, default=1, help="network: divide every population by this")
+    ap.add_argument("--net-cpu-intervals", type=int, default=16, help="network: intervals of the oracle sample")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -282,6 +286,8 @@ def main():
         return rows(cbrkc, args)
     if args.workload == "batching":
         return batching(cbrkc, args)
+    if args.workload == "network":
+        return network(cbrkc, args)
     n = args.cells
     morph = args.morph or f"/tmp/cbrkc_bench_pkj{n}_s{args.seed}.txt"
     if not os.path.exists(morph):
@@ -568,6 +574,136 @@ def batching(cbrkc, args):
     if not args.no_cpu:
         out["cpu_baseline"] = batching_cpu(cbrkc, pops, gjf, interval, args)
     print(json.dumps(out), flush=True)
+
+
+def network(cbrkc, args):
+    """The fixed in-degree network: 2048 granule cells (4-compartment
+    synthetic chains), 512 Golgi cells (synthetic 131-compartment trees), 512
+    Purkinje cells (synthetic 1600-compartment trees), 780404 synapses
+    (4 mossy-fibre and 4 Golgi inputs per granule cell, 100 granule inputs
+    per Golgi cell, 1392-1393 per Purkinje cell), weights U(0.5, 1.5) nS,
+    AMPA 1.5 ms / GABA_A 6 ms, 1 ms delay, mossy fibres at 5 Hz.  Granule and
+    Golgi cells on the global FP64 RKC engine (the reference's tolerances),
+    Purkinje cells on the per-cell mixed engine; the same network with the
+    Purkinje cells on the per-cell FP64 engine is run beside it for the
+    raster comparison."""
+    import net_oracle as NO
+    sc = args.net_scale
+    n = (2048 // sc, 512 // sc, 512 // sc)
+    comp = (4, 131, 1600)
+    d = "/tmp/cbrkc_net"
+    os.makedirs(d, exist_ok=True)
+    files = NO.synth_files(d, *n, seed=args.seed)
+    rng_v0 = (uniform(args.seed, 20, n[0], -72.0, -68.0), uniform(args.seed, 21, n[1], -62.0, -58.0),
+              philox_params(args.seed, n[2])[0])
+    cfg = cbrkc.net_config()
+    if sc != 1:
+        cfg.total = 0
+    nint, warm = args.net_intervals, args.net_warmup
+
+    def build(pkj_engine):
+        gr = cbrkc.Population(cbrkc.GR, n[0], 1, morph=files[0], comp=4, v0=rng_v0[0], stim="none")
+        go = cbrkc.Population(cbrkc.GO, n[1], 1, morph=files[1], v0=rng_v0[1], stim="none")
+        pk = cbrkc.Population(cbrkc.PKJ, n[2], 1, morph=files[2], gj="", v0=rng_v0[2], atol=args.tol, rtol=args.tol,
+                              stim="none", engine=pkj_engine, **cbrkc.STABLE)
+        threads = pk.autotune(4, threads=args.threads)
+        return (gr, go, pk), threads
+
+    res = {}
+    for name, eng in (("mixed", cbrkc.ENGINE_CELL_MIXED), ("fp64", cbrkc.ENGINE_CELL_FP64)):
+        if name == "fp64" and args.no_fp64:
+            continue
+        print(f"[network] {n} cells, Purkinje engine {name}: warm-up {warm}, timed {nint} intervals",
+              file=sys.stderr, flush=True)
+        pops, threads = build(eng)
+        net = cbrkc.Network(*pops, cfg=cfg)
+        st = net.stats()
+        net.run(warm)
+        t0 = net.times()
+        s0 = [p.stats() for p in pops]
+        t = time.perf_counter()
+        net.run(nint)
+        wall = time.perf_counter() - t
+        t1 = net.times()
+        s1 = [p.stats() for p in pops]
+        spikes = net.spikes()
+        net.close()
+        for p in pops:
+            p.close()
+        res[name] = dict(wall=wall, stats=st, threads=threads, spikes=spikes,
+                         pop_ms=[a - b for a, b in zip(t1["populations"], t0["populations"])],
+                         exchange_ms=t1["exchange"] - t0["exchange"],
+                         work=[work(a, b) for a, b in zip(s0, s1)])
+    m = res["mixed"]
+    total_comp = sum(a * b for a, b in zip(n, comp))
+    sim_ms = nint * H_MAX
+    names = ("granule", "Golgi", "Purkinje")
+    t_m, p_m, c_m = m["spikes"]
+    per = {}
+    for q in range(3):
+        per[names[q]] = {"cells": n[q], "compartments": n[q] * comp[q],
+                         "host_ms_in_own_loop": round(m["pop_ms"][q], 1), "spikes": int(np.sum(p_m == q)),
+                         "work": m["work"][q]}
+    out = {"metric": "compartment-steps/s, GR/GO/PKJ network (780404 synapses), 0.125 ms exchange, 1 ms delay",
+           "value": total_comp * nint / m["wall"], "unit": "compartment-steps/s (one compartment over 0.125 ms "
+           "simulated)", "n_gpus": 1, "higher_is_better": True, "data": "synthetic",
+           "dtype": "f32 (Purkinje, mixed) / f64 (granule, Golgi)",
+           "config": {"workload": f"{n[0]} granule (4-compartment chains), {n[1]} Golgi (131), {n[2]} Purkinje "
+                                  f"(1600); fixed in-degree network, {sum(m['stats']['synapses'])} synapses; "
+                                  f"{sim_ms:g} ms timed after {warm * H_MAX:g} ms",
+                      "compartments": total_comp, "intervals": nint, "synapses": m["stats"]["synapses"],
+                      "purkinje_threads_per_cell": m["threads"]},
+           "wall_s": m["wall"], "sim_ms_per_s": sim_ms / m["wall"], "populations": per,
+           "exchange_ms": round(m["exchange_ms"], 1)}
+    if "fp64" in res:
+        f = res["fp64"]
+        a = set(zip(np.rint(f["spikes"][0] / H_MAX).astype(int).tolist(), f["spikes"][1].tolist(),
+                    f["spikes"][2].tolist()))
+        b = set(zip(np.rint(t_m / H_MAX).astype(int).tolist(), p_m.tolist(), c_m.tolist()))
+        small_equal = {s for s in a if s[1] < 2} == {s for s in b if s[1] < 2}
+        pk_a = sorted(s for s in a if s[1] == 2)
+        pk_b = sorted(s for s in b if s[1] == 2)
+        cnt_a = np.bincount([s[2] for s in pk_a], minlength=n[2])
+        cnt_b = np.bincount([s[2] for s in pk_b], minlength=n[2])
+        shifts = []
+        for c in np.nonzero(cnt_a == cnt_b)[0]:
+            ta = [s[0] for s in pk_a if s[2] == c]
+            tb = [s[0] for s in pk_b if s[2] == c]
+            shifts += [abs(x - y) for x, y in zip(ta, tb)]
+        out["vs_fp64_purkinje_run"] = {
+            "fp64_wall_s": f["wall"],
+            "granule_golgi_rasters_identical": bool(small_equal),
+            "purkinje_spikes_fp64": len(pk_a), "purkinje_spikes_mixed": len(pk_b),
+            "purkinje_cells_with_count_mismatch": int(np.sum(cnt_a != cnt_b)),
+            "purkinje_spike_shift_intervals_max": int(max(shifts)) if shifts else 0,
+            "purkinje_spikes_in_the_same_interval": int(sum(1 for x in shifts if x == 0))}
+    if not args.no_cpu:
+        out["cpu_baseline"] = network_cpu(cfg, files, n, rng_v0, args)
+    print(json.dumps(out), flush=True)
+
+
+def network_cpu(cfg, files, n, v0, args):
+    """The oracle's network (FP64 RKC, the reference's population-global
+    controllers; the granule and Golgi oracle readers take cell 0's
+    geometry for all cells) over the same populations for the first
+    --net-cpu-intervals intervals, every population on the host's threads."""
+    import ctypes
+    import net_oracle as NO
+    lib = NO.L()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    opops = NO.oracle_pops(files, v0, pkj_tol=args.tol)
+    for p in opops:
+        p.threads(threads)
+    h = lib.or_net_create(opops[0].h, opops[1].h, opops[2].h, ctypes.byref(NO.spec_of(cfg)))
+    k = args.net_cpu_intervals
+    t = time.perf_counter()
+    rc = lib.or_net_run(h, k, None)
+    wall = time.perf_counter() - t
+    lib.or_net_free(h)
+    total = sum(a * b for a, b in zip(n, (4, 131, 1600)))
+    return {"value": total * k / wall, "unit": "compartment-steps/s", "cores": threads, "kind": "port",
+            "sample": f"the oracle's network over the same {n} cells, first {k} intervals ({k * H_MAX:g} ms) "
+                      f"in {wall:.1f} s, status {rc}"}
 
 
 def batching_cpu(cbrkc, pops, gjf, interval, args):

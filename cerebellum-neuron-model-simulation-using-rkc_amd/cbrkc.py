@@ -45,7 +45,7 @@ EXPORTS = [
     "cb_mix_config_default", "cb_mix_create", "cb_mix_run", "cb_mix_layout", "cb_mix_stats", "cb_mix_autotune",
     "cb_mix_kernel_time", "cb_mix_use_graph", "cb_mix_destroy", "cb_synth_gr_chains", "cb_synth_go_trees",
     "cb_synth_io_gj", "cb_net_config_default", "cb_net_generate", "cb_net_create", "cb_net_run", "cb_net_spikes",
-    "cb_net_conductance", "cb_net_stats", "cb_net_destroy",
+    "cb_net_conductance", "cb_net_stats", "cb_net_time", "cb_net_destroy",
 ]
 
 # synapse kinds and fields (cb_syn_build, cb_syn_read)
@@ -248,6 +248,7 @@ def lib():
         L.cb_net_spikes.argtypes = [vp, ctypes.c_int, dp, ip, ip, ip]
         L.cb_net_conductance.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
         L.cb_net_stats.argtypes = [vp, i64p]
+        L.cb_net_time.argtypes = [vp, dp, dp, dp]
         L.cb_net_destroy.argtypes = [vp]
         _lib = L
     return _lib
@@ -631,6 +632,14 @@ class Network:
         _check(lib().cb_net_stats(self.h, out))
         v = list(out)
         return dict(synapses=v[0:4], slots=v[4:7], intervals=v[7], delay=v[8])
+
+    def times(self):
+        """host wall ms: per population, exchanges, total"""
+        pop = np.zeros(3)
+        ex = ctypes.c_double()
+        tot = ctypes.c_double()
+        _check(lib().cb_net_time(self.h, _dp(pop), ctypes.byref(ex), ctypes.byref(tot)))
+        return dict(populations=pop.tolist(), exchange=ex.value, total=tot.value)
 
     def close(self):
         if getattr(self, "h", None):

@@ -1212,7 +1212,7 @@ int cb_net_config_default(cb_net_config *cfg) {
     cfg->go_target_hi = s.tgt_hi[NP_GO];
     cfg->pkj_target_lo = s.tgt_lo[NP_PKJ];
     cfg->pkj_target_hi = s.tgt_hi[NP_PKJ];
-    cfg->gr_spike_comp = 3;
+    cfg->gr_spike_comp = s.spike_comp[NP_GR];
     cfg->go_spike_comp = s.spike_comp[NP_GO];
     cfg->pkj_spike_comp = s.spike_comp[NP_PKJ];
     cfg->gr_thr_mV = s.thr_mV[NP_GR];
@@ -1309,6 +1309,14 @@ int cb_net_stats(const cb_net *net, int64_t *out9) {
     for (int p = 0; p < NP_N; p++) out9[4 + p] = net->net->slots(p);
     out9[7] = net->net->intervals();
     out9[8] = net->net->delay_intervals();
+    return CB_OK;
+  });
+}
+
+int cb_net_time(const cb_net *net, double *pop_ms3, double *exchange_ms, double *total_ms) {
+  return guarded([&]() -> int {
+    if (!net || !pop_ms3 || !exchange_ms || !total_ms) return fail(CB_E_ARG, "null argument");
+    net->net->times(pop_ms3, exchange_ms, total_ms);
     return CB_OK;
   });
 }

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <exception>
 #include <thread>
@@ -325,8 +326,21 @@ void Network::interval(int p, long k) {
   }
 }
 
+namespace {
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+}  // namespace
+
+void Network::times(double *pop3, double *exchange, double *total) const {
+  for (int p = 0; p < NP_N; p++) pop3[p] = ms_pop_[p];
+  *exchange = ms_exchange_;
+  *total = ms_total_;
+}
+
 void Network::run(int nint) {
   if (nint < 0) throw Error("negative interval count", -2);
+  const auto t_run = std::chrono::steady_clock::now();
   std::vector<uint32_t> ring((size_t)nring_ * words_);
   while (nint > 0) {
     // delay + 1 intervals read only the ring slots of earlier chunks and
@@ -335,7 +349,9 @@ void Network::run(int nint) {
     const int len = std::min(nint, delay_ + 1);
     const long k0 = k_;
     std::exception_ptr fail[NP_N];
+    double ms[NP_N] = {0.0, 0.0, 0.0};
     auto work = [&](int p) {
+      const auto t0 = std::chrono::steady_clock::now();
       try {
         CB_HIP(hipSetDevice(device_));
         for (int i = 0; i < len; i++) interval(p, k0 + i);
@@ -343,13 +359,17 @@ void Network::run(int nint) {
       } catch (...) {
         fail[p] = std::current_exception();
       }
+      ms[p] = ms_since(t0);
     };
     std::thread th[NP_N - 1];
     for (int p = 1; p < NP_N; p++) th[p - 1] = std::thread(work, p);
     work(0);
     for (auto &t : th) t.join();
-    for (int p = 0; p < NP_N; p++)
+    for (int p = 0; p < NP_N; p++) {
       if (fail[p]) std::rethrow_exception(fail[p]);
+      ms_pop_[p] += ms[p];
+    }
+    const auto t_ex = std::chrono::steady_clock::now();
     CB_HIP(hipMemcpy(ring.data(), ring_, sizeof(uint32_t) * ring.size(), hipMemcpyDeviceToHost));
     for (int i = 0; i < len; i++) {
       const long k = k0 + i;
@@ -359,9 +379,11 @@ void Network::run(int nint) {
           if ((slot[word_off_[p] + (c >> 5)] >> (c & 31)) & 1u)
             spikes_.push_back({(k + 1) * spec_.interval_ms, {p, c}});
     }
+    ms_exchange_ += ms_since(t_ex);
     k_ += len;
     nint -= len;
   }
+  ms_total_ += ms_since(t_run);
 }
 
 int Network::take_spikes(int cap, double *t, int *pop, int *cell) {

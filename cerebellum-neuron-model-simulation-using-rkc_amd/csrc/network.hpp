@@ -55,7 +55,7 @@ struct NetSpec {
   double interval_ms = 0.125;        // the populations' macro step (param.h:12)
   int n[NP_N] = {0, 0, 0}, comp[NP_N] = {0, 0, 0};
   int tgt_lo[NP_N] = {0, 31, 0}, tgt_hi[NP_N] = {1, 131, 1599};   // [lo, hi) target compartments
-  int spike_comp[NP_N] = {1, 30, 1599};
+  int spike_comp[NP_N] = {3, 30, 1599};   // granule: the chain end of a 4-compartment chain
   double thr_mV[NP_N] = {0.0, 0.0, -10.0};
 };
 
@@ -98,6 +98,10 @@ class Network {
   long long slots(int pop) const { return post_[pop].nslot; }
   long intervals() const { return k_; }
   int delay_intervals() const { return delay_; }
+  // host wall time [ms] of the runs so far: each population's own interval
+  // loop (its engine's work and waits, in its thread), the exchanges, and the
+  // whole of every run
+  void times(double *pop3, double *exchange, double *total) const;
 
  private:
   struct Post {
@@ -122,6 +126,7 @@ class Network {
   double *prev_[NP_N] = {nullptr, nullptr, nullptr};
   long k_ = 0;
   std::vector<std::pair<double, std::pair<int, int>>> spikes_;
+  double ms_pop_[NP_N] = {0.0, 0.0, 0.0}, ms_exchange_ = 0.0, ms_total_ = 0.0;
 };
 
 }  // namespace cbrkc

@@ -427,6 +427,11 @@ int cb_net_spikes(cb_net *net, int cap, double *t_out, int *pop_out, int *cell_o
 int cb_net_conductance(const cb_net *net, int pop, int receptor, double *dst);
 /* synapses per kind (4), slots per population (3), intervals run, delay in intervals */
 int cb_net_stats(const cb_net *net, int64_t *out9);
+/* host wall time [ms] of the runs so far: each population's own interval loop
+ * (its engine's work and waits; the populations run concurrently), the
+ * exchanges between chunks, and the runs in total.  No counterpart in the
+ * reference. */
+int cb_net_time(const cb_net *net, double *pop_ms3, double *exchange_ms, double *total_ms);
 int cb_net_destroy(cb_net *net);
 
 /* ---- batched engine over an uneven mix of cell types ----
