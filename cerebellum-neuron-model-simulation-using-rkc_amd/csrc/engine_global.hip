@@ -46,6 +46,22 @@ __global__ void __launch_bounds__(256) k_reduce256(const double *__restrict__ in
 
 namespace {
 
+// the last stage of the deterministic reduction (n <= 256 partials, the tree
+// of k_reduce256's single block) written straight into mapped host memory,
+// with the flag word beside it (null: 0), so that a step's read-back is one
+// stream synchronisation and no copy commands
+__global__ void __launch_bounds__(256) k_reduce_to_host(const double *__restrict__ in, int n,
+                                                        const int *__restrict__ bad, double *hout) {
+  __shared__ double s[256];
+  const int t = threadIdx.x;
+  const double x = (t < n) ? in[t] : 0.0;
+  const double r = block_tree_sum(x, s);
+  if (t == 0) {
+    hout[0] = r;
+    hout[1] = bad ? (double)*bad : 0.0;
+  }
+}
+
 constexpr double UROUND = 2.22e-16;   // solve_rkc.cuh:22
 constexpr double H_MAX = 0.125;       // solve_rkc.cuh:23
 
@@ -385,7 +401,9 @@ class GlobalRkc : public Engine {
     red_b_ = dalloc<double>(nblk(nc_) + 1);
     bad_ = dalloc<int>(1);
     CB_HIP(hipMemset(bad_, 0, sizeof(int)));
-    CB_HIP(hipHostMalloc((void **)&hres_, sizeof(double) * 2, hipHostMallocDefault));
+    // coherent mapped host memory: the last reduction block writes the sum here
+    CB_HIP(hipHostMalloc((void **)&hres_, sizeof(double) * 2, hipHostMallocMapped | hipHostMallocCoherent));
+    CB_HIP(hipHostGetDevicePointer((void **)&hres_dev_, hres_, 0));
     amp_ = dalloc<double>(n_); h2d(amp_, stim_.amp);
     if (syn_.on) {
       syn_g_ = dalloc<double>(nc_);
@@ -517,17 +535,20 @@ class GlobalRkc : public Engine {
     if (refresh_fn_) k_stage_rhs(vec_[YN], vec_[FN]);
   }
 
-  // deterministic total of the per-block partials left in red_a_
-  double finish_sum(int nparts) {
+  // the sum of nparts block partials in red_a_ (the deterministic tree), and
+  // with flag the zero-weight flag of the same step, in one read-back
+  double finish_sum(int nparts, int *flag = nullptr) {
     double *in = red_a_, *out = red_b_;
-    while (nparts > 1) {
+    while (nparts > RED_BLOCK) {
       int nb = nblk(nparts);
       k_reduce256<<<nb, 256, 0, st_>>>(in, out, nparts);
       std::swap(in, out);
       nparts = nb;
     }
-    CB_HIP(hipMemcpyAsync(hres_, in, sizeof(double), hipMemcpyDeviceToHost, st_));
+    k_reduce_to_host<<<1, 256, 0, st_>>>(in, nparts, flag ? bad_ : nullptr, hres_dev_);
+    CB_HIP(hipGetLastError());
     CB_HIP(hipStreamSynchronize(st_));
+    if (flag) *flag = hres_[1] != 0.0;
     return hres_[0];
   }
 
@@ -733,8 +754,9 @@ class GlobalRkc : public Engine {
       k_err<<<nb, 256, 0, st_>>>(csr(), b_, cm_, vec_[Y], vec_[YN], vec_[FN], vec_[VT1], o_[ATOL], o_[RTOL], h, red_a_,
                                  bad_, nc_);
       st.stages++;
-      double err = finish_sum(nb);
-      if (read_bad()) {
+      int bad = 0;
+      double err = finish_sum(nb, &bad);
+      if (bad) {
         o_[IDID] = 3;
         return;
       }
@@ -794,6 +816,7 @@ class GlobalRkc : public Engine {
   int *row_, *col_;
   double *vec_[NVEC];
   double *red_a_, *red_b_, *hres_, *amp_;
+  double *hres_dev_ = nullptr;   // hres_ as the device sees it
   double *syn_g_ = nullptr;
   SynIn net_;               // incoming synapses (none: net_.on() false)
   double *net_s_ = nullptr; // Golgi / Purkinje: the macro step's synaptic current
