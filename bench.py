@@ -259,6 +259,9 @@ def main():
     ap.add_argument("--net-warmup", type=int, default=16, help="network: untimed intervals before them")
     ap.add_argument("--net-scale", type=int, default=1, help="network: divide every population by this")
     ap.add_argument("--net-cpu-intervals", type=int, default=16, help="network: intervals of the oracle sample")
+    ap.add_argument("--net-gr", choices=("rkc", "cn-tree"), default="rkc",
+                    help="network: the granule population's solver (the reference's RKC, or Crank-Nicolson with "
+                         "the exact tree solve)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -602,7 +605,12 @@ def network(cbrkc, args):
     nint, warm = args.net_intervals, args.net_warmup
 
     def build(pkj_engine):
-        gr = cbrkc.Population(cbrkc.GR, n[0], 1, morph=files[0], comp=4, v0=rng_v0[0], stim="none")
+        if args.net_gr == "cn-tree":
+            gr = cbrkc.Population(cbrkc.GR, n[0], 1, solver=cbrkc.CN, engine=cbrkc.ENGINE_TREE_FP64, morph=files[0],
+                                  comp=4, v0=rng_v0[0], stim="none")
+            gr.autotune(4)
+        else:
+            gr = cbrkc.Population(cbrkc.GR, n[0], 1, morph=files[0], comp=4, v0=rng_v0[0], stim="none")
         go = cbrkc.Population(cbrkc.GO, n[1], 1, morph=files[1], v0=rng_v0[1], stim="none")
         pk = cbrkc.Population(cbrkc.PKJ, n[2], 1, morph=files[2], gj="", v0=rng_v0[2], atol=args.tol, rtol=args.tol,
                               stim="none", engine=pkj_engine, **cbrkc.STABLE)
@@ -648,9 +656,10 @@ def network(cbrkc, args):
            "value": total_comp * nint / m["wall"], "unit": "compartment-steps/s (one compartment over 0.125 ms "
            "simulated)", "n_gpus": 1, "higher_is_better": True, "data": "synthetic",
            "dtype": "f32 (Purkinje, mixed) / f64 (granule, Golgi)",
-           "config": {"workload": f"{n[0]} granule (4-compartment chains), {n[1]} Golgi (131), {n[2]} Purkinje "
-                                  f"(1600); fixed in-degree network, {sum(m['stats']['synapses'])} synapses; "
-                                  f"{sim_ms:g} ms timed after {warm * H_MAX:g} ms",
+           "config": {"workload": f"{n[0]} granule (4-compartment chains, "
+                                  f"{'RKC' if args.net_gr == 'rkc' else 'CN with the exact tree solve'}), {n[1]} Golgi "
+                                  f"(131), {n[2]} Purkinje (1600); fixed in-degree network, "
+                                  f"{sum(m['stats']['synapses'])} synapses; {sim_ms:g} ms timed after {warm * H_MAX:g} ms",
                       "compartments": total_comp, "intervals": nint, "synapses": m["stats"]["synapses"],
                       "purkinje_threads_per_cell": m["threads"]},
            "wall_s": m["wall"], "sim_ms_per_s": sim_ms / m["wall"], "populations": per,

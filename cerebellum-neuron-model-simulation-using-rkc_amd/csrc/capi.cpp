@@ -1258,7 +1258,6 @@ int cb_net_create(cb_pop *gr, cb_pop *go, cb_pop *pkj, const cb_net_config *cfg,
     if (gr->cell_type != CB_GR || go->cell_type != CB_GO || pkj->cell_type != CB_PKJ)
       return fail(CB_E_ARG, "cb_net_create takes a granule, a Golgi and a Purkinje population");
     for (cb_pop *p : {gr, go, pkj}) {
-      if (p->solver != CB_SOLVER_RKC) return fail(CB_E_UNSUP, "the network runs RKC populations");
       if (p->in_net) return fail(CB_E_ARG, "a population is coupled to another network");
       if (!p->syn_in.empty()) return fail(CB_E_ARG, "a population already takes synapse sets");
       if (p->device != gr->device) return fail(CB_E_ARG, "the populations must share one device");

@@ -315,8 +315,10 @@ void Network::interval(int p, long k) {
     k_net_deliver<<<(unsigned)((P.nslot + 255) / 256), 256, 0, st>>>(D, arrive, (uint32_t)k);
     CB_HIP(hipGetLastError());
   }
-  E->set_current(k * spec_.interval_ms);
-  E->advance();
+  // one output interval of the population's solver: RKC one macro step
+  // (set_current at its start), CN / BE their sub-steps with set_current at
+  // each (main.cu:193-200)
+  E->run(1, k * spec_.interval_ms, -1, nullptr);
   const int n = spec_.n[p];
   if (n > 0) {
     k_net_mark<<<(unsigned)((n + 63) / 64), 64, 0, st>>>(E->device_vnew(), spec_.comp[p], n, spec_.spike_comp[p],

@@ -83,9 +83,9 @@ class Network {
   Network(Engine *gr, Engine *go, Engine *pkj, const NetSpec &s, int device);
   ~Network();
   // nint intervals: per interval and population, the slots' conductances take
-  // the decay and the arriving spikes, the population advances one interval
-  // (set_current at the interval's start, advance), and its spikes go into the
-  // ring; the three populations run concurrently for up to delay + 1
+  // the decay and the arriving spikes, the population advances one output
+  // interval of its solver (Engine::run of one interval), and its spikes go
+  // into the ring; the three populations run concurrently for up to delay + 1
   // intervals between exchanges
   void run(int nint);
   // moves up to cap spikes (time order; granule, Golgi, Purkinje within a

@@ -406,15 +406,18 @@ enum { CB_NET_MFGR = 0, CB_NET_GOGR = 1, CB_NET_GRGO = 2, CB_NET_GRPKJ = 3 };
 int cb_net_generate(const cb_net_config *cfg, int n_gr, int n_go, int n_pkj, int comp_gr, int comp_go,
                     int comp_pkj, int64_t cap, int8_t *kind, int32_t *pre, int32_t *post, double *w_nS,
                     int64_t *n);
-/* Builds the network over three populations of one device (RKC; the
- * Purkinje population on any RKC engine) and couples it into them: the
- * mfgr / gogr / grgo / grpkj _create calls of main.cu:140-158 in one.  The
- * populations take no other synapse sets meanwhile and are destroyed after
- * the network. */
+/* Builds the network over three populations of one device and couples it
+ * into them: the mfgr / gogr / grgo / grpkj _create calls of main.cu:140-158
+ * in one.  Each population keeps its own solver and engine (RKC on any
+ * engine; CN or BE, global Krylov or tree solve) and couples the
+ * conductances as that solver's reference path does (the BE Purkinje path
+ * couples none, as in the reference).  The populations take no other
+ * synapse sets meanwhile and are destroyed after the network. */
 int cb_net_create(cb_pop *gr, cb_pop *go, cb_pop *pkj, const cb_net_config *cfg, cb_net **out);
-/* nint intervals of 0.125 ms: per interval every population advances
- * (set_current at the interval's start, then its solve) with the
- * conductances of the interval, and its spikes enter the ring; the three
+/* nint intervals of 0.125 ms: per interval every population advances one
+ * output interval of its solver (RKC: set_current, one macro step; CN / BE:
+ * the sub-steps of main.cu:193-200) with the conductances of the interval,
+ * and its spikes enter the ring; the three
  * populations run concurrently for up to delay + 1 intervals between
  * exchanges (the network's own delay bounds what an interval can depend on). */
 int cb_net_run(cb_net *net, int nint);

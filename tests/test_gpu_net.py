@@ -227,6 +227,44 @@ def test_network_mixed_purkinje_against_fp64(tmp_path):
     assert ok, why
 
 
+def test_network_with_a_crank_nicolson_granule_population(tmp_path):
+    """Each population keeps its own solver: granule cells on Crank-Nicolson
+    with the exact tree solve (five 0.025 ms sub-steps per interval, the
+    synaptic conductance entering its system as the reference's CN path
+    does), Golgi cells on the global FP64 RKC engine, Purkinje cells on the
+    per-cell mixed engine.  Bars: delivery exact against the oracle's replay
+    of the GPU raster; granule spikes present; voltages finite."""
+    n, nint = (64, 8, 4), 160
+    files = NO.synth_files(tmp_path, *n, seed=41)
+    v0 = v0_of(n, seed=4)
+    cfg = cbrkc.net_config(total=0, mf_rate_hz=40.0)
+    print(f"case: network {n} cells, granule CN tree, {nint} intervals", flush=True)
+    g0, o0, p0 = v0
+    gr = cbrkc.Population(cbrkc.GR, n[0], 1, solver=cbrkc.CN, engine=cbrkc.ENGINE_TREE_FP64, morph=files[0], comp=4,
+                          v0=g0, stim="none")
+    go = cbrkc.Population(cbrkc.GO, n[1], 1, morph=files[1], v0=o0, stim="none")
+    pk = cbrkc.Population(cbrkc.PKJ, n[2], 1, morph=files[2], gj="", v0=p0, atol=1e-6, rtol=1e-6, stim="none",
+                          engine=cbrkc.ENGINE_CELL_MIXED, **cbrkc.STABLE)
+    net = cbrkc.Network(gr, go, pk, cfg=cfg)
+    net.run(nint)
+    t, p, c = net.spikes()
+    g = [[net.conductance(q, r) for r in (0, 1)] for q in range(3)]
+    assert net.stats()["intervals"] == nint
+    assert gr.stats()["macro_steps"] == nint
+    for q in (gr, go, pk):
+        assert np.all(np.isfinite(q.read(cbrkc.F_VNEW)))
+    net.close()
+    for q in (gr, go, pk):
+        q.close()
+    print(f"spikes: granule {int(np.sum(p == 0))}, Golgi {int(np.sum(p == 1))}, Purkinje {int(np.sum(p == 2))}",
+          flush=True)
+    assert np.sum(p == 0) > 0
+    rep = replay_conductances(files, n, cfg, raster_bytes(t, p, c, n, nint), v0)
+    for q in range(3):
+        for r in (0, 1):
+            assert np.array_equal(g[q][r], rep[q][r]), (q, r)
+
+
 def test_network_full_size_780404_synapses(tmp_path):
     n, nint = (2048, 512, 512), 24
     files = NO.synth_files(tmp_path, *n, seed=31)
