@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import numpy as np
+
 import cbrkc
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -112,3 +114,19 @@ def test_failure_paths_before_the_device(tmp_path, monkeypatch, ref_data):
     # reference-named entry point: an unknown solver name (main.cu:60-102 prefix match)
     h = ctypes.c_void_p()
     assert cbrkc.lib().cb_pkj_initialize(1, 1, b"XYZ", ctypes.byref(h)) == CB_E_ARG
+
+
+def test_granule_population_reads_the_reference_file_with_a_stimulus_compartment():
+    """The mirror's stimulus tuple must not leak into cb_pop_config.comp (a
+    stimulus on compartment 16 once set a 16-compartment chain and the
+    reference's Gr.txt failed to read).  Files are read before any device
+    call, so without a GPU the call stops at the HIP layer; with one it
+    succeeds."""
+    gr_txt = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_data", "Gr.txt")
+    try:
+        p = cbrkc.Population(cbrkc.GR, 2, 1, morph=gr_txt, v0=np.array([-70.0, -72.0]),
+                             stim=(16, 2.0, 1e9, np.array([1e-5, 2e-5])))
+        assert p.comp == 578
+        p.close()
+    except cbrkc.CbError as e:
+        assert "READING_ERROR" not in str(e) and "error -2" not in str(e), str(e)
