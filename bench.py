@@ -630,7 +630,13 @@ def network(cbrkc, args):
         t0 = net.times()
         s0 = [p.stats() for p in pops]
         t = time.perf_counter()
-        net.run(nint)
+        done = 0
+        while done < nint:   # in pieces of 100 ms, with a progress line after each
+            k = min(800, nint - done)
+            net.run(k)
+            done += k
+            print(f"[network] {name}: {done} of {nint} intervals, {time.perf_counter() - t:.1f} s",
+                  file=sys.stderr, flush=True)
         wall = time.perf_counter() - t
         t1 = net.times()
         s1 = [p.stats() for p in pops]
