@@ -1,8 +1,9 @@
 """The oracle built with AddressSanitizer and UBSan (oracle/Makefile target
 `asan`) runs a short case of every path -- RKC, CN, BE and the exact tree
-solve for the Purkinje, olive, granule and Golgi models, and a Purkinje
-pair with its gap junctions -- on the reference's morphology files, in
-bounds, with no undefined behaviour and finite voltages."""
+solve for the Purkinje, olive, granule and Golgi models, a Purkinje pair
+with its gap junctions, and the fixed in-degree network over the three
+synaptic populations -- on the reference's morphology files, in bounds,
+with no undefined behaviour and finite voltages."""
 import os
 import shutil
 import subprocess
@@ -21,4 +22,4 @@ def test_oracle_runs_clean_under_asan_and_ubsan(tmp_path, ref_data):
                        env=dict(os.environ, OMP_NUM_THREADS="1", ASAN_OPTIONS="detect_leaks=1"))
     print(r.stdout)
     assert r.returncode == 0, r.stderr[-4000:]
-    assert r.stdout.count(": ok") == 16
+    assert r.stdout.count(": ok") == 20
