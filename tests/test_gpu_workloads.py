@@ -1,4 +1,4 @@
-"""The BASELINE workloads at their stated sizes, on the GPU through the C-ABI.
+"""The plain and the scale-stress workloads at full size, on the GPU through the C-ABI.
 
 * Plain method, small: 9 seeded synthetic Purkinje cells of 1600
   compartments (a 3 x 3 population without gap junctions: Pkj_GJ.csv's
@@ -63,13 +63,13 @@ def oracle_run(morph, n, v0, steps, atol, rtol, stim):
     return out
 
 
-def test_config1_plain_fp64_rkc_against_the_oracle(synth_dir):
+def test_plain_workload_9_cells_fp64_rkc_against_the_oracle(synth_dir):
     n, steps, seed = 9, 800, 101
-    morph = str(synth_dir / "pkj_config1.txt")
+    morph = str(synth_dir / "pkj_plain9.txt")
     cbrkc.synth_pkj_morph(morph, seed=seed, ncell=n)
     v0 = normal_v0(seed, n)
     stim = (1599, 20.0, 80.0, 0.5e-3)
-    print("case: config 1, 9 cells, global FP64 RKC rtol 1e-4 atol 1e-6, 0.5 nA [20, 80) ms, 100 ms", flush=True)
+    print("case: plain workload, 9 cells, global FP64 RKC rtol 1e-4 atol 1e-6, 0.5 nA [20, 80) ms, 100 ms", flush=True)
     p = cbrkc.Population(cbrkc.PKJ, 3, 3, engine=cbrkc.ENGINE_GLOBAL_FP64, morph=morph, v0=v0, atol=1e-6,
                          rtol=1e-4, stim=(stim[0], stim[1], stim[2], np.full(n, stim[3])), **cbrkc.STABLE)
     gpu = p.run(steps, 0.0, 1599)
@@ -97,13 +97,13 @@ def _bench():
     return m
 
 
-def test_config5_scale_stress_properties(synth_dir):
+def test_scale_stress_8192_cells_properties(synth_dir):
     b = _bench()
     n, steps, seed = 8192, 400, 2024
-    morph = str(synth_dir / "pkj_config5.txt")
+    morph = str(synth_dir / "pkj_scale8192.txt")
     cbrkc.synth_pkj_morph(morph, seed=seed, ncell=n, dscale=b.philox_dscale(seed, n))
     v0, amp = b.philox_params(seed, n)
-    print("case: config 5, 8192 cells x 1600 compartments, diameters x U(0.5, 2), 50 ms", flush=True)
+    print("case: scale-stress workload, 8192 cells x 1600 compartments, diameters x U(0.5, 2), 50 ms", flush=True)
 
     def run(engine, threads):
         p = cbrkc.Population(cbrkc.PKJ, n, 1, engine=engine, morph=morph, v0=v0, atol=1e-6, rtol=1e-6,
