@@ -227,19 +227,34 @@ __device__ inline double wg_sum(double x, double *red, int &slot) {
 template <typename T>
 struct Lds {
   T *d0, *d1;        // stage buffers
-  T *gp;             // gpar
+  T *gp;             // gpar (FP64 engine)
   T *ic;             // invc
   T *bl;             // frozen current b (a copy of the slab plane, read by every A rebuild)
-  uint32_t *tp;      // topology
+  uint32_t *tp;      // topology (FP64 engine)
+  float4 *w4;        // FP32 engine: the four edge conductances (parent, children 0..2; 0 where none)
+  uint2 *nb;         // FP32 engine: the four neighbour indices, 16 bits each (the compartment itself where none)
 };
 
 // sum_nb g (D_nb - D_i) with D in LDS buffer cur, where cur[i] holds this
 // thread's own D_i.  Branch-free: a missing parent or child is read as the
 // compartment itself (D_i - D_i = 0 exactly) with conductance 0, so every
 // LDS read is issued unconditionally and together; the added terms are
-// exact zeros, so the sum equals the one over the real neighbours.
+// exact zeros, so the sum equals the one over the real neighbours.  The
+// FP32 engine keeps the neighbour indices and edge conductances decoded in
+// LDS (set once per cell), so a stage reads two words and the four values
+// instead of decoding the topology word; the operands and their order are
+// the same.
 template <typename T>
 __device__ inline T lap(const Lds<T> &L, const T *cur, int i, T di) {
+  if constexpr (sizeof(T) == 4) {
+    const uint2 q = L.nb[i];
+    const float4 w = L.w4[i];
+    T s = w.x * (cur[q.x & 0xFFFFu] - di);
+    s += w.y * (cur[q.x >> 16] - di);
+    s += w.z * (cur[q.y & 0xFFFFu] - di);
+    s += w.w * (cur[q.y >> 16] - di);
+    return s;
+  }
   const uint32_t tp = L.tp[i];
   const int p = tp & 0x7FF, fc = (tp >> 11) & 0x7FF, nch = (tp >> 22) & 3;
   const int pp = (p != NO_PARENT) ? p : i;
@@ -281,13 +296,29 @@ __device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, doub
   extern __shared__ __align__(16) unsigned char smem[];
   const int C = a.comp;
   Lds<T> L;
+  // LDS layout (cell_lds_bytes gives the size):
+  //   FP32: d0 d1 ic bl | w4 (16-byte aligned) | nb | red (16-byte aligned)
+  //   FP64: d0 d1 gp ic bl tp | red (16-byte aligned)
+  double *red;
   L.d0 = (T *)smem;
   L.d1 = L.d0 + C;
-  L.gp = L.d1 + C;
-  L.ic = L.gp + C;
-  L.bl = L.ic + C;
-  L.tp = (uint32_t *)(L.bl + C);
-  double *red = (double *)(((uintptr_t)(L.tp + C) + 15) & ~(uintptr_t)15);
+  if constexpr (sizeof(T) == 4) {
+    L.gp = nullptr;
+    L.tp = nullptr;
+    L.ic = L.d1 + C;
+    L.bl = L.ic + C;
+    L.w4 = (float4 *)(((uintptr_t)(L.bl + C) + 15) & ~(uintptr_t)15);
+    L.nb = (uint2 *)(L.w4 + C);
+    red = (double *)(((uintptr_t)(L.nb + C) + 15) & ~(uintptr_t)15);
+  } else {
+    L.gp = L.d1 + C;
+    L.ic = L.gp + C;
+    L.bl = L.ic + C;
+    L.tp = (uint32_t *)(L.bl + C);
+    L.w4 = nullptr;
+    L.nb = nullptr;
+    red = (double *)(((uintptr_t)(L.tp + C) + 15) & ~(uintptr_t)15);
+  }
   __shared__ int s_cell;
   __shared__ double s_cnt[6];   // steps, rejects, stages, rho iterations, ion updates, macro steps
   __shared__ double s_cs[NT / 64][14];   // per-wave copy of the controller state
@@ -311,10 +342,20 @@ __device__ __forceinline__ void cell_body(const CellArgs<T> &a, int nsteps, doub
     double *ctl = a.ctl + (size_t)cell * NCTL;
 
     for (int i = tid; i < C; i += NT) {
-      L.gp[i] = a.gpar[base + i];
       L.ic[i] = a.invc[base + i];
-      L.tp[i] = a.topo[base + i];
       L.bl[i] = S.ld(P_B, i);
+      const uint32_t tp = a.topo[base + i];
+      if constexpr (sizeof(T) == 4) {
+        const int p = tp & 0x7FF, fc = (tp >> 11) & 0x7FF, nch = (tp >> 22) & 3;
+        const int pp = (p != NO_PARENT) ? p : i;
+        const int j0 = (0 < nch) ? fc : i, j1 = (1 < nch) ? fc + 1 : i, j2 = (2 < nch) ? fc + 2 : i;
+        L.w4[i] = make_float4((p != NO_PARENT) ? a.gpar[base + i] : 0.0f, (0 < nch) ? a.gpar[base + j0] : 0.0f,
+                              (1 < nch) ? a.gpar[base + j1] : 0.0f, (2 < nch) ? a.gpar[base + j2] : 0.0f);
+        L.nb[i] = make_uint2((uint32_t)pp | ((uint32_t)j0 << 16), (uint32_t)j1 | ((uint32_t)j2 << 16));
+      } else {
+        L.gp[i] = a.gpar[base + i];
+        L.tp[i] = tp;
+      }
     }
 
     // register-resident per-compartment state
@@ -1481,7 +1522,19 @@ class CellPkj : public Engine {
     return v;
   }
 
-  size_t lds_bytes() const { return (size_t)C_ * (5 * sizeof(T) + 4) + 16 + 2 * 16 * sizeof(double); }
+  // the kernel's LDS layout (cell_body): FP32 d0 d1 ic bl, the edge
+  // conductances (16-byte aligned) and neighbour indices, the reduction
+  // slots (16-byte aligned); FP64 d0 d1 gp ic bl tp and the reduction slots
+  size_t lds_bytes() const {
+    const size_t red = 16 + 2 * 16 * sizeof(double);
+    if (sizeof(T) == 4) {
+      size_t b = (size_t)C_ * 4 * sizeof(T);
+      b = (b + 15) & ~(size_t)15;
+      b += (size_t)C_ * (sizeof(float4) + sizeof(uint2));
+      return b + red;
+    }
+    return (size_t)C_ * (5 * sizeof(T) + 4) + red;
+  }
 
   void choose_default() {
     cur_ = 0;
