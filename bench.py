@@ -18,7 +18,12 @@ Purkinje cells with per-cell diameter factors U(0.5, 2.0), 50 ms, the mixed
 engine against the per-cell FP64 RKC engine, the plain FP64 global RKC and
 the FP64 Crank-Nicolson engine with the exact per-cell tree solve at several
 steps, each with its error against the FP64 RKC run (one JSON line, not the
-headline metric).
+headline metric).  `--workload rows` times every cell type and solver on the
+reference's own morphology files with the oracle beside each;
+`--workload network` runs the fixed in-degree network (2048 granule, 512
+Golgi, 512 Purkinje cells, 780404 synapses, 1 ms delay, 5 Hz mossy fibres;
+100 ms by default, `--net-intervals 8000` for 1 s) with the per-population
+time split and the mixed-vs-FP64 Purkinje raster comparison.
 """
 import argparse
 import json
